@@ -1,0 +1,275 @@
+/*
+ * mitrace_gpu.h -- C-ABI of the MI355X (gfx950) implementation of MiTrace's Tracer stage.
+ *
+ * The reference has no FFI: its Tracer is a concrete C++ class that is linked statically
+ * (src/Tracer/Tracer.hpp:19-129).  This header is the drop-in boundary that replaces it:
+ *
+ *   reference                                         this ABI
+ *   -----------------------------------------------   -----------------------------------
+ *   Scene::Scene(const Loader::Scene&)                mt_scene_create / mt_scene_destroy
+ *     (src/Scene/Scene.cpp:53-59; BVH build            (host BVH build with the same rules,
+ *      src/Scene/BVH.cpp:117-282, Mesh.cpp:24-41)       flattened into HBM)
+ *   Tracer::Tracer(buf, scene) + StartRender()        mt_render_start
+ *     (src/Tracer/Tracer.cpp:633-640, 718-779)
+ *   Tracer::WaitForRender()  (Tracer.cpp:781-799)     mt_render_wait
+ *   Tracer::KillRender()     (Tracer.cpp:801-804)     mt_render_kill
+ *   Tracer::GetStats()       (Tracer.cpp:806-821)     mt_render_stats
+ *   Tracer::IsDone()         (Tracer.cpp:823-828)     mt_render_is_done
+ *   main.cpp:121-142 (start + wait, synchronous)      mt_render
+ *   IntersectScene (src/Tracer/Intersect.cpp:201-219) mt_trace_closest   (test hook)
+ *   BRDF::EvaluateBRDF / DiffusePdf / SpecularPdf     mt_brdf_eval       (test hook)
+ *     (src/Tracer/BRDF.hpp:52-97)
+ *
+ * Conventions: plain C types only; every pointer argument is a HOST pointer unless its name
+ * ends in _dev; every function returns 0 on success or a negative MT_E* code, and
+ * mt_last_error() gives a message for the calling thread.  No C++ exception crosses the ABI.
+ * Matrices are 4x4 float, column-major (m[col*4 + row]), as glTF and GLM store them.
+ */
+#ifndef MITRACE_GPU_H
+#define MITRACE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MT_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define MT_OK 0
+#define MT_E_INVALID (-1)   /* bad argument or scene description                        */
+#define MT_E_HIP (-2)       /* a HIP runtime call failed                                 */
+#define MT_E_NOMEM (-3)     /* device or host allocation failed                          */
+#define MT_E_UNSUPPORTED (-4)
+#define MT_E_KILLED (-5)    /* render was stopped by mt_render_kill before it finished   */
+
+/* ---- enums mirroring the reference ----------------------------------------------------- */
+/* Scene::Light::LightType (src/Scene/Light.hpp:14-20) */
+#define MT_LIGHT_POINT 0
+#define MT_LIGHT_DIRECTIONAL 1
+#define MT_LIGHT_SPOT 2
+#define MT_LIGHT_AREA 3
+/* Scene::Camera::Type (src/Scene/Camera.hpp:13-17) */
+#define MT_CAMERA_PERSPECTIVE 0
+#define MT_CAMERA_ORTHOGONAL 1
+/* Scene::Image::FilterMode (src/Scene/Image.hpp:15-19) */
+#define MT_FILTER_NEAREST 0
+#define MT_FILTER_LINEAR 1
+/* environment texel formats: the reference loads 8-bit RGB only (Image.cpp:77-104);
+ * RGB32F is the float equirect the large configurations need. */
+#define MT_ENV_NONE 0
+#define MT_ENV_RGB8 1
+#define MT_ENV_RGB32F 2
+/* estimator modes */
+#define MT_MODE_PARITY 0 /* the reference estimator, quirks included (Tracer.cpp:134-348) */
+#define MT_MODE_FIXED 1  /* Russian roulette instead of the energy cut-off                  */
+/* per-sample output */
+#define MT_OUTPUT_TONEMAPPED 0 /* ACES per sample before averaging (Tracer.cpp:212-227,346) */
+#define MT_OUTPUT_LINEAR 1     /* mean of linear radiance                                   */
+
+/* ---- scene description (mirrors Scene::Mesh / MeshInstance / Material / Light / Camera) - */
+
+/* A de-indexed triangle soup (Scene::Mesh, src/Scene/Mesh.hpp:13-42; Q24): corner k of
+ * triangle t is element 3*t+k of every per-corner array. */
+typedef struct mt_mesh_desc {
+    uint32_t tri_count;
+    const float* positions; /* tri_count*9 floats                      */
+    const float* normals;   /* tri_count*9 floats                      */
+    const float* texcoord0; /* tri_count*6 floats, or NULL             */
+} mt_mesh_desc;
+
+/* 8-bit RGB image (Scene::Image, src/Scene/Image.hpp:12-51): row-major, 3 bytes per texel. */
+typedef struct mt_image_desc {
+    uint32_t width;
+    uint32_t height;
+    const uint8_t* rgb;
+} mt_image_desc;
+
+/* Scene::Texture (src/Scene/Texture.hpp:14-40): image index (-1 = no texture), filter and the
+ * KHR_texture_transform offset/scale. */
+typedef struct mt_texture_desc {
+    int32_t image;
+    int32_t filter;
+    float offset[2];
+    float scale[2];
+} mt_texture_desc;
+
+/* Scene::Material (src/Scene/Material.hpp:32-44) */
+typedef struct mt_material_desc {
+    float base_color_factor[4];
+    float emissive_factor[3];
+    float normal_scale;
+    float metallic_factor;
+    float roughness_factor;
+    float occlusion_strength;
+    mt_texture_desc base_color_tex;
+    mt_texture_desc metallic_roughness_tex;
+    mt_texture_desc normal_tex;
+    mt_texture_desc occlusion_tex;
+    mt_texture_desc emissive_tex;
+} mt_material_desc;
+
+/* Scene::MeshInstance (src/Scene/Mesh.hpp:44-72) */
+typedef struct mt_instance_desc {
+    uint32_t mesh;
+    uint32_t material;
+    float transform[16]; /* model-to-world, column-major */
+} mt_instance_desc;
+
+/* Scene::Light (src/Scene/Light.hpp:14-66).  color is already color*intensity. */
+typedef struct mt_light_desc {
+    int32_t type;
+    float position[3];
+    float color[3];
+    float point_size;
+    float spot_inner_cos;
+    float spot_outer_cos;
+    float direction[3]; /* quat * (0,0,-1) for spot/directional lights */
+} mt_light_desc;
+
+/* Scene::Camera (src/Scene/Camera.hpp:10-36) */
+typedef struct mt_camera_desc {
+    int32_t type;
+    float camera_to_world[16];
+    float yfov;
+    float ortho_size[2];
+    float aspect;
+} mt_camera_desc;
+
+/* Environment map, sampled as an equirect (Image::SampleEquirectangular, Image.cpp:63-75). */
+typedef struct mt_env_desc {
+    int32_t format; /* MT_ENV_* */
+    uint32_t width;
+    uint32_t height;
+    const void* texels; /* RGB8: w*h*3 bytes; RGB32F: w*h*3 floats */
+    int32_t filter;     /* the reference always uses MT_FILTER_LINEAR (Texture.cpp:35-50) */
+} mt_env_desc;
+
+/* BVH build options read from the Config singleton by Mesh::Mesh (src/Scene/Mesh.cpp:37-40). */
+typedef struct mt_bvh_options {
+    int32_t disable;        /* --disable-bvh        (default 0)  */
+    uint32_t max_triangles; /* --bvh-max-triangles  (default 32) */
+    uint32_t max_depth;     /* --bvh-max-depth      (default 32) */
+} mt_bvh_options;
+
+typedef struct mt_scene_desc {
+    const mt_mesh_desc* meshes;
+    uint32_t mesh_count;
+    const mt_image_desc* images;
+    uint32_t image_count;
+    const mt_material_desc* materials;
+    uint32_t material_count;
+    const mt_instance_desc* instances;
+    uint32_t instance_count;
+    const mt_light_desc* lights;
+    uint32_t light_count;
+    mt_env_desc env;
+    mt_bvh_options bvh;
+} mt_scene_desc;
+
+/* ---- render settings (the fields of Config::ConfigOptions the path reads, plus GPU ones) - */
+typedef struct mt_settings {
+    uint32_t width;               /* imageWidth             (Config.hpp:50)           */
+    uint32_t height;              /* imageHeight            (Config.hpp:51)           */
+    uint32_t samples;             /* samples                (Config.hpp:52)           */
+    uint32_t bounces;             /* bounces                (Config.hpp:53)           */
+    float ev_exposure;            /* evExposure             (Config.hpp:56)           */
+    float hdri_rotation;          /* degrees; used only when hdri_rotation_apply != 0 */
+    float hdri_secondary_intensity;
+    float hdri_primary_intensity;
+    float emission_base_intensity;
+    int32_t firefly_enable;       /* useStatisticFireflyElimination (Q19: CLI default off) */
+    float firefly_threshold;
+    float terminate_energy;
+    uint32_t image_block_size;    /* tile edge for progress/kill granularity          */
+    /* additions */
+    uint64_t seed;                /* counter-based RNG key                            */
+    int32_t mode;                 /* MT_MODE_*                                        */
+    uint32_t rr_depth;            /* MT_MODE_FIXED: roulette starts after this depth  */
+    int32_t output;               /* MT_OUTPUT_*                                      */
+    int32_t hdri_rotation_apply;  /* 0 = ignore rotation like the reference (Q21)     */
+    uint32_t paths_in_flight;     /* wavefront size; 0 = automatic                    */
+    int32_t count_work;           /* 1 = gather traversal work counters (slower)      */
+} mt_settings;
+
+/* Tracer::Stats (src/Tracer/Tracer.hpp:78-85) plus work counters. */
+typedef struct mt_stats {
+    float progress;
+    float time_elapsed;
+    float estimated_time_remaining;
+    uint64_t rays;    /* camera + extension + shadow rays (Q26) */
+    uint64_t samples; /* camera paths                           */
+    /* traversal work, filled when mt_settings.count_work != 0 */
+    uint64_t inst_aabb_tests;
+    uint64_t inst_entered;
+    uint64_t node_pops;
+    uint64_t tri_tests;
+    uint64_t hits_shaded;
+    uint64_t misses;
+    float kernel_ms_extend;  /* summed HIP-event time of the extend launches   */
+    float kernel_ms_shadow;  /* summed HIP-event time of the shadow launches   */
+    float kernel_ms_shade;   /* summed HIP-event time of the shade launches    */
+    float kernel_ms_total;   /* summed HIP-event time of all render launches   */
+    uint32_t extend_launches;
+    uint32_t shadow_launches;
+} mt_stats;
+
+/* One closest-hit record (Tracer::RayHit, src/Tracer/Ray.hpp:20-39). */
+typedef struct mt_hit {
+    int32_t instance;    /* -1 = miss */
+    uint32_t triangle;   /* triangle index inside the mesh (soup order) */
+    uint32_t bv_index;   /* BVH leaf node that produced the hit (0 for linear meshes) */
+    float u, v;          /* barycentrics of corners 1 and 2 */
+    float distance;      /* world-space |P - O| */
+    float world_pos[3];
+    uint32_t bvh_tests;      /* node pops in the winning instance (Intersect.cpp:71)  */
+    uint32_t triangle_tests; /* triangle tests in the winning instance (Intersect.cpp:93) */
+} mt_hit;
+
+typedef struct mt_scene mt_scene;
+typedef struct mt_job mt_job;
+
+/* ---- functions -------------------------------------------------------------------------- */
+int mt_abi_version(void);
+const char* mt_last_error(void);
+int mt_device_count(int* count);
+
+/* Scene::Scene(loaderScene): builds each mesh's BVH with the reference rules and uploads the
+ * flattened scene to HBM of the current device.  The description is copied; the caller may
+ * free it afterwards. */
+int mt_scene_create(const mt_scene_desc* desc, mt_scene** out);
+int mt_scene_destroy(mt_scene* scene);
+/* Export of mesh `mesh`'s flattened BVH (BVH::Flatten, BVH.cpp:261-272): 8 words per node
+ * {min.xyz, max.xyz, a, b} as raw bits, and the triangle index list.  Pass NULL buffers to
+ * query the sizes. */
+int mt_scene_mesh_bvh(const mt_scene* scene, uint32_t mesh, uint32_t* nodes_out,
+                      uint32_t* node_count, uint32_t* indices_out, uint32_t* index_count);
+
+/* Synchronous render (main.cpp:121-142): writes width*height*3 floats to rgb_out.
+ * sample_rgb_out, when not NULL, receives every per-sample colour ([pixel][sample][3]). */
+int mt_render(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,
+              float* rgb_out, float* sample_rgb_out, mt_stats* stats);
+
+/* Asynchronous render with the Tracer's start/wait/kill/stats surface. */
+int mt_render_start(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,
+                    float* rgb_out, mt_job** job);
+int mt_render_wait(mt_job* job);
+int mt_render_kill(mt_job* job);
+int mt_render_stats(const mt_job* job, mt_stats* stats);
+int mt_render_is_done(const mt_job* job);
+int mt_job_destroy(mt_job* job);
+
+/* IntersectScene over n rays; rays holds 6 floats per ray {origin.xyz, direction.xyz}. */
+int mt_trace_closest(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hits);
+
+/* BRDF::EvaluateBRDF, DiffusePdf, SpecularPdf on the GPU.  in: 14 floats per case
+ * {L.xyz, V.xyz, N.xyz, base.rgb, roughness, metallic}; out: 5 floats per case
+ * {f.rgb, diffuse_pdf, specular_pdf}. */
+int mt_brdf_eval(uint32_t n, const float* in, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MITRACE_GPU_H */

@@ -1,0 +1,151 @@
+// mt_internal.hpp -- internal declarations of the gfx950 Tracer library (not part of the ABI).
+//
+// HBM layout of a scene (all arrays device-resident, built once by mt_scene_create):
+//   instances  DevInstance[I]  192 B: inverse and model matrices, world AABB, mesh/material ids
+//   meshes     DevMesh[M]       32 B: offsets of the mesh's nodes, leaf-ordered triangles and
+//                                     shading attributes, flags
+//   nodes      float4[2*N]      32 B per BVH node, the reference's node (BVH.hpp:74-119) with its
+//                                     fields interleaved as {min.xyz, a} {max.xyz, b}
+//   prims      float4[3*P]      48 B per leaf slot: {v0.xyz, triangle id} {e1.xyz, 0} {e2.xyz, 0}
+//                                     stored in BVH::indices_ order, so a leaf's triangles are
+//                                     contiguous (duplicates of straddling triangles included)
+//   normals    float[9*T]       per-corner normals of every triangle (soup order)
+//   uvs        float[6*T]       per-corner TEXCOORD_0 (meshes without UVs have none)
+//   materials  DevMaterial[]    factors and texture slots
+//   texels     uint32[]         RGBA8 texels of all images; images DevImage[] {offset, w, h}
+//   env        float4[W*H]      equirect texels decoded to float (u8/255 or float input)
+//   lights     DevLight[]
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/mitrace_gpu.h"
+
+namespace mt {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMeshHasBvh = 1u;
+constexpr uint32_t kMeshHasUv = 2u;
+
+struct alignas(16) DevInstance {
+    float inv[16];  // column-major inverse of the model matrix (glm::inverse, Intersect.cpp:152)
+    float M[16];    // column-major model matrix
+    float wmin[3];
+    uint32_t mesh;
+    float wmax[3];
+    uint32_t material;
+    float pad[8];
+};
+static_assert(sizeof(DevInstance) == 192, "DevInstance layout");
+
+struct alignas(16) DevMesh {
+    uint32_t node_base;  // first node (float4 pair) of this mesh in the global node array
+    uint32_t prim_base;  // first leaf slot in the global prim array
+    uint32_t prim_count; // leaf slots (= triangles for a linear mesh)
+    uint32_t flags;      // kMeshHas*
+    uint32_t attr_base;  // first triangle of this mesh in the normals/uvs arrays
+    uint32_t tri_count;
+    uint32_t uv_base;    // first triangle of this mesh in the uvs array
+    uint32_t pad;
+};
+
+struct DevTex {
+    int32_t image;  // -1 = none
+    int32_t filter;
+    float ox, oy, sx, sy;
+    float pad[2];
+};
+
+struct alignas(16) DevMaterial {
+    float base[4];
+    float emissive[3];
+    float normal_scale;
+    float metallic, roughness, pad0, pad1;
+    DevTex tex[4];  // base colour, metallic-roughness, normal, emissive
+};
+
+struct DevImage {
+    uint32_t offset, w, h, pad;
+};
+
+struct alignas(16) DevLight {
+    float pos[3];
+    int32_t type;
+    float color[3];
+    float size;
+    float dir[3];
+    float inner;
+    float outer;
+    float intensity;  // PrepareLights (Tracer.cpp:123-132)
+    float pad[2];
+};
+
+// device pointers and sizes of an uploaded scene, passed to kernels by value
+struct DevScene {
+    const DevInstance* instances;
+    const DevMesh* meshes;
+    const float4* nodes;
+    const float4* prims;
+    const float* normals;
+    const float* uvs;
+    const DevMaterial* materials;
+    const uint32_t* texels;
+    const DevImage* images;
+    const float4* env;
+    const DevLight* lights;
+    uint32_t instance_count;
+    uint32_t light_count;
+    uint32_t env_w, env_h;
+    int32_t env_filter;
+    int32_t has_env;
+};
+
+// host-side product of the BVH build for one mesh (kept for mt_scene_mesh_bvh)
+struct HostNode {
+    float mn[3], mx[3];
+    uint32_t a, b;
+};
+struct HostBvh {
+    std::vector<HostNode> nodes;
+    std::vector<uint32_t> indices;
+    uint32_t depth = 0;  // levels of the flattened tree
+};
+
+// BVH build with the reference's rules (BVH.cpp:13-282); positions is the mesh soup (9 floats
+// per triangle).  max_depth is the value Mesh::Mesh passes (bvhMaxDepth - 1).
+void build_bvh(const float* positions, uint32_t tri_count, uint32_t max_triangles,
+               uint32_t max_depth, HostBvh& out);
+// glm::inverse stand-in; column-major in and out
+void invert4(const float* m, float* out);
+// M * (x, y, z, w) with the library's fixed operation order
+void mul4(const float* m, const float* v, float* out);
+
+}  // namespace mt
+
+// the opaque handle behind mt_scene*
+struct mt_scene {
+    int device = 0;
+    mt::DevScene dev{};
+    std::vector<void*> allocations;
+    std::vector<mt::HostBvh> bvhs;  // per mesh; empty nodes = linear scan
+    std::vector<uint32_t> mesh_has_bvh;
+    uint32_t max_bvh_depth = 0;
+    uint32_t mesh_count = 0;
+    uint32_t instance_count = 0;
+    uint32_t light_count = 0;
+    uint64_t prim_slots = 0;
+    uint64_t node_count = 0;
+    uint64_t triangle_count = 0;
+    ~mt_scene();
+};
+
+namespace mt {
+// error reporting (thread-local last error)
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int upload_scene(const mt_scene_desc* desc, mt_scene* scene);
+}  // namespace mt

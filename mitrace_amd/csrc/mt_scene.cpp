@@ -1,0 +1,485 @@
+// mt_scene.cpp -- host side of mt_scene_create: the reference's per-mesh BVH build and the
+// flattening of the Scene into the HBM layout described in mt_internal.hpp.
+//
+// The BVH rules follow src/Scene/BVH.cpp (Q1-Q3): split plane at the vertex-mean centroid,
+// all three axes tried, score = (|L|+|R|)/|N| (FLT_MAX when one side keeps everything),
+// straddling triangles go to both children, child boxes are the parent box clipped at the
+// plane, children of a node are an adjacent pair in DFS order, node 1 is an unused slot.
+// The tree is built with per-triangle bounds and centroid sums computed once; the float
+// operations and their order are those of BVH.cpp, so the nodes are bit-identical.
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "mt_internal.hpp"
+
+namespace mt {
+
+namespace {
+
+inline float fmin_g(float a, float b) { return (b < a) ? b : a; }  // glm::min scalar form
+inline float fmax_g(float a, float b) { return (a < b) ? b : a; }  // glm::max scalar form
+
+struct TriInfo {
+    float lo[3], hi[3];  // min(v0, min(v1, v2)) and max(...) (BVH.cpp:75-76)
+    float sum[3];        // (v0 + v1) + v2 (BVH.cpp:59)
+};
+
+struct TmpNode {
+    float mn[3], mx[3];
+    int32_t ca = -1, cb = -1;      // children in the temporary array (-1 = leaf)
+    std::vector<uint32_t> tris;    // leaf triangles, split order preserved
+};
+
+class Builder {
+   public:
+    Builder(const std::vector<TriInfo>& info, uint32_t max_tri) : info_(info), max_tri_(max_tri) {}
+
+    // BVHTreeNode::Subdivide (BVH.cpp:117-195); returns the index of the node in nodes_
+    int32_t subdivide(std::vector<uint32_t>&& tris, const float* mn, const float* mx,
+                      uint32_t depth_left) {
+        const int32_t id = static_cast<int32_t>(nodes_.size());
+        nodes_.emplace_back();
+        std::memcpy(nodes_[id].mn, mn, sizeof(float) * 3);
+        std::memcpy(nodes_[id].mx, mx, sizeof(float) * 3);
+        if (tris.size() <= static_cast<size_t>(max_tri_)) {
+            nodes_[id].tris = std::move(tris);
+            return id;
+        }
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t t : tris)
+            for (int k = 0; k < 3; ++k) sum[k] = sum[k] + info_[t].sum[k];
+        const float denom = static_cast<float>(tris.size() * 3);
+        const float com[3] = {sum[0] / denom, sum[1] / denom, sum[2] / denom};
+        // count both sides of every axis first; only the chosen axis is materialised
+        size_t nl[3] = {0, 0, 0}, nr[3] = {0, 0, 0};
+        for (uint32_t t : tris) {
+            const TriInfo& ti = info_[t];
+            for (int k = 0; k < 3; ++k) {
+                nl[k] += ti.lo[k] < com[k];
+                nr[k] += ti.hi[k] >= com[k];
+            }
+        }
+        float score[3];
+        for (int k = 0; k < 3; ++k) {
+            if (nl[k] == tris.size() || nr[k] == tris.size())
+                score[k] = FLT_MAX;
+            else
+                score[k] = static_cast<float>(nl[k] + nr[k]) / static_cast<float>(tris.size());
+        }
+        if (score[0] == FLT_MAX && score[1] == FLT_MAX && score[2] == FLT_MAX) {
+            nodes_[id].tris = std::move(tris);
+            return id;
+        }
+        const int ax = (score[0] < score[1] && score[0] < score[2]) ? 0 : (score[1] < score[2] ? 1 : 2);
+        std::vector<uint32_t> left, right;
+        left.reserve(nl[ax]);
+        right.reserve(nr[ax]);
+        for (uint32_t t : tris) {
+            if (info_[t].lo[ax] < com[ax]) left.push_back(t);
+            if (info_[t].hi[ax] >= com[ax]) right.push_back(t);
+        }
+        std::vector<uint32_t>().swap(tris);
+        float amx[3] = {mx[0], mx[1], mx[2]};
+        float bmn[3] = {mn[0], mn[1], mn[2]};
+        amx[ax] = com[ax];
+        bmn[ax] = com[ax];
+        int32_t a, b;
+        if (depth_left > 0) {
+            a = subdivide(std::move(left), mn, amx, depth_left - 1);
+            b = subdivide(std::move(right), bmn, mx, depth_left - 1);
+        } else {
+            a = leaf(std::move(left), mn, amx);
+            b = leaf(std::move(right), bmn, mx);
+        }
+        nodes_[id].ca = a;
+        nodes_[id].cb = b;
+        return id;
+    }
+
+    int32_t leaf(std::vector<uint32_t>&& tris, const float* mn, const float* mx) {
+        const int32_t id = static_cast<int32_t>(nodes_.size());
+        nodes_.emplace_back();
+        std::memcpy(nodes_[id].mn, mn, sizeof(float) * 3);
+        std::memcpy(nodes_[id].mx, mx, sizeof(float) * 3);
+        nodes_[id].tris = std::move(tris);
+        return id;
+    }
+
+    // BVH::Flatten / FlattenRecursive (BVH.cpp:223-272)
+    void flatten(int32_t root, HostBvh& out) {
+        out.nodes.clear();
+        out.indices.clear();
+        out.nodes.resize(2);
+        std::memset(out.nodes.data(), 0, sizeof(HostNode) * 2);
+        set_box(out.nodes[0], nodes_[root]);
+        uint32_t depth = 1;
+        if (nodes_[root].ca < 0) {
+            // a root that could not be split (the reference would dereference null children)
+            set_leaf(out, 0, nodes_[root]);
+        } else {
+            auto c = pair(out, nodes_[root].ca, nodes_[root].cb, 2, depth);
+            out.nodes[0].a = c.first;
+            out.nodes[0].b = c.second;
+        }
+        out.depth = depth;
+    }
+
+   private:
+    void set_box(HostNode& h, const TmpNode& t) {
+        std::memcpy(h.mn, t.mn, sizeof(h.mn));
+        std::memcpy(h.mx, t.mx, sizeof(h.mx));
+    }
+    void set_leaf(HostBvh& out, uint32_t slot, TmpNode& t) {
+        const uint32_t first = static_cast<uint32_t>(out.indices.size());
+        out.indices.insert(out.indices.end(), t.tris.begin(), t.tris.end());
+        out.nodes[slot].a = first | kLeafBit;
+        out.nodes[slot].b = static_cast<uint32_t>(t.tris.size());
+        std::vector<uint32_t>().swap(t.tris);
+    }
+    std::pair<uint32_t, uint32_t> pair(HostBvh& out, int32_t a, int32_t b, uint32_t level,
+                                       uint32_t& depth) {
+        depth = std::max(depth, level);
+        const uint32_t ia = static_cast<uint32_t>(out.nodes.size()), ib = ia + 1;
+        out.nodes.resize(out.nodes.size() + 2);
+        set_box(out.nodes[ia], nodes_[a]);
+        set_box(out.nodes[ib], nodes_[b]);
+        for (int side = 0; side < 2; ++side) {
+            const int32_t n = side ? b : a;
+            const uint32_t slot = side ? ib : ia;
+            if (nodes_[n].ca < 0) {
+                set_leaf(out, slot, nodes_[n]);
+            } else {
+                auto c = pair(out, nodes_[n].ca, nodes_[n].cb, level + 1, depth);
+                out.nodes[slot].a = c.first;
+                out.nodes[slot].b = c.second;
+            }
+        }
+        return {ia, ib};
+    }
+
+    const std::vector<TriInfo>& info_;
+    uint32_t max_tri_;
+    std::vector<TmpNode> nodes_;
+};
+
+}  // namespace
+
+void build_bvh(const float* P, uint32_t tri_count, uint32_t max_triangles, uint32_t max_depth,
+               HostBvh& out) {
+    std::vector<TriInfo> info(tri_count);
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t t = 0; t < tri_count; ++t) {
+        const float* v0 = P + t * 9;
+        const float* v1 = v0 + 3;
+        const float* v2 = v0 + 6;
+        TriInfo& ti = info[t];
+        for (int k = 0; k < 3; ++k) {
+            ti.lo[k] = fmin_g(v0[k], fmin_g(v1[k], v2[k]));
+            ti.hi[k] = fmax_g(v0[k], fmax_g(v1[k], v2[k]));
+            ti.sum[k] = (v0[k] + v1[k]) + v2[k];
+            // BVHTreeNode::ComputeAABB of the root (BVH.cpp:29-50)
+            mn[k] = fmin_g(mn[k], ti.lo[k]);
+            mx[k] = fmax_g(mx[k], ti.hi[k]);
+        }
+    }
+    std::vector<uint32_t> all(tri_count);
+    for (uint32_t t = 0; t < tri_count; ++t) all[t] = t;
+    Builder b(info, max_triangles);
+    // BVHTree passes maxDepth - 1 to the root's Subdivide (BVH.cpp:204-212), unsigned
+    const int32_t root = b.subdivide(std::move(all), mn, mx, max_depth - 1u);
+    b.flatten(root, out);
+}
+
+void mul4(const float* m, const float* v, float* o) {
+    for (int r = 0; r < 4; ++r)
+        o[r] = (m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1]) + (m[2 * 4 + r] * v[2] + m[3 * 4 + r] * v[3]);
+}
+
+// 4x4 inverse by 2x2 minors of the top and bottom row pairs (Laplace expansion).
+void invert4(const float* m, float* out) {
+    auto A = [&](int r, int c) { return m[c * 4 + r]; };
+    const float s0 = A(0, 0) * A(1, 1) - A(1, 0) * A(0, 1);
+    const float s1 = A(0, 0) * A(1, 2) - A(1, 0) * A(0, 2);
+    const float s2 = A(0, 0) * A(1, 3) - A(1, 0) * A(0, 3);
+    const float s3 = A(0, 1) * A(1, 2) - A(1, 1) * A(0, 2);
+    const float s4 = A(0, 1) * A(1, 3) - A(1, 1) * A(0, 3);
+    const float s5 = A(0, 2) * A(1, 3) - A(1, 2) * A(0, 3);
+    const float c5 = A(2, 2) * A(3, 3) - A(3, 2) * A(2, 3);
+    const float c4 = A(2, 1) * A(3, 3) - A(3, 1) * A(2, 3);
+    const float c3 = A(2, 1) * A(3, 2) - A(3, 1) * A(2, 2);
+    const float c2 = A(2, 0) * A(3, 3) - A(3, 0) * A(2, 3);
+    const float c1 = A(2, 0) * A(3, 2) - A(3, 0) * A(2, 2);
+    const float c0 = A(2, 0) * A(3, 1) - A(3, 0) * A(2, 1);
+    const float det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+    const float id = 1.0f / det;
+    float B[4][4];
+    B[0][0] = (A(1, 1) * c5 - A(1, 2) * c4 + A(1, 3) * c3) * id;
+    B[0][1] = (-A(0, 1) * c5 + A(0, 2) * c4 - A(0, 3) * c3) * id;
+    B[0][2] = (A(3, 1) * s5 - A(3, 2) * s4 + A(3, 3) * s3) * id;
+    B[0][3] = (-A(2, 1) * s5 + A(2, 2) * s4 - A(2, 3) * s3) * id;
+    B[1][0] = (-A(1, 0) * c5 + A(1, 2) * c2 - A(1, 3) * c1) * id;
+    B[1][1] = (A(0, 0) * c5 - A(0, 2) * c2 + A(0, 3) * c1) * id;
+    B[1][2] = (-A(3, 0) * s5 + A(3, 2) * s2 - A(3, 3) * s1) * id;
+    B[1][3] = (A(2, 0) * s5 - A(2, 2) * s2 + A(2, 3) * s1) * id;
+    B[2][0] = (A(1, 0) * c4 - A(1, 1) * c2 + A(1, 3) * c0) * id;
+    B[2][1] = (-A(0, 0) * c4 + A(0, 1) * c2 - A(0, 3) * c0) * id;
+    B[2][2] = (A(3, 0) * s4 - A(3, 1) * s2 + A(3, 3) * s0) * id;
+    B[2][3] = (-A(2, 0) * s4 + A(2, 1) * s2 - A(2, 3) * s0) * id;
+    B[3][0] = (-A(1, 0) * c3 + A(1, 1) * c1 - A(1, 2) * c0) * id;
+    B[3][1] = (A(0, 0) * c3 - A(0, 1) * c1 + A(0, 2) * c0) * id;
+    B[3][2] = (-A(3, 0) * s3 + A(3, 1) * s1 - A(3, 2) * s0) * id;
+    B[3][3] = (A(2, 0) * s3 - A(2, 1) * s1 + A(2, 2) * s0) * id;
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[c * 4 + r] = B[r][c];
+}
+
+namespace {
+
+template <typename T>
+int dev_upload(mt_scene* s, const std::vector<T>& host, const T** out) {
+    *out = nullptr;
+    if (host.empty()) return MT_OK;
+    void* p = nullptr;
+    const size_t bytes = host.size() * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) return fail(MT_E_NOMEM, "hipMalloc failed in scene upload");
+    s->allocations.push_back(p);
+    if (hipMemcpy(p, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(MT_E_HIP, "hipMemcpy failed in scene upload");
+    *out = static_cast<const T*>(p);
+    return MT_OK;
+}
+
+}  // namespace
+
+int upload_scene(const mt_scene_desc* d, mt_scene* s) {
+    if (!d) return fail(MT_E_INVALID, "null scene description");
+    if (d->mesh_count && !d->meshes) return fail(MT_E_INVALID, "meshes is NULL");
+    if (d->instance_count && !d->instances) return fail(MT_E_INVALID, "instances is NULL");
+    if (d->material_count && !d->materials) return fail(MT_E_INVALID, "materials is NULL");
+    if (d->light_count && !d->lights) return fail(MT_E_INVALID, "lights is NULL");
+    if (d->image_count && !d->images) return fail(MT_E_INVALID, "images is NULL");
+    s->mesh_count = d->mesh_count;
+    s->instance_count = d->instance_count;
+    s->light_count = d->light_count;
+
+    std::vector<DevMesh> meshes(d->mesh_count);
+    std::vector<float4> nodes, prims;
+    std::vector<float> normals, uvs;
+    std::vector<std::pair<std::array<float, 3>, std::array<float, 3>>> local_aabb(d->mesh_count);
+    s->bvhs.resize(d->mesh_count);
+    s->mesh_has_bvh.assign(d->mesh_count, 0);
+    uint64_t tri_total = 0;
+    for (uint32_t i = 0; i < d->mesh_count; ++i) {
+        const mt_mesh_desc& md = d->meshes[i];
+        if (md.tri_count && (!md.positions || !md.normals))
+            return fail(MT_E_INVALID, "mesh without positions or normals");
+        const float* P = md.positions;
+        DevMesh& dm = meshes[i];
+        dm.attr_base = static_cast<uint32_t>(tri_total);
+        dm.tri_count = md.tri_count;
+        dm.flags = 0;
+        tri_total += md.tri_count;
+        // Mesh::CalculateAABB (Mesh.cpp:43-54)
+        std::array<float, 3> mn{FLT_MAX, FLT_MAX, FLT_MAX}, mx{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (uint64_t c = 0; c < static_cast<uint64_t>(md.tri_count) * 3; ++c)
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = fmin_g(mn[k], P[c * 3 + k]);
+                mx[k] = fmax_g(mx[k], P[c * 3 + k]);
+            }
+        local_aabb[i] = {mn, mx};
+        // BVH trigger (Mesh.cpp:37-40): max depth passed as bvhMaxDepth - 1 (unsigned)
+        const bool bvh = !d->bvh.disable && md.tri_count > d->bvh.max_triangles;
+        std::vector<uint32_t> order;
+        if (bvh) {
+            build_bvh(P, md.tri_count, d->bvh.max_triangles, d->bvh.max_depth - 1u, s->bvhs[i]);
+            s->mesh_has_bvh[i] = 1;
+            s->max_bvh_depth = std::max(s->max_bvh_depth, s->bvhs[i].depth);
+            dm.flags |= kMeshHasBvh;
+            dm.node_base = static_cast<uint32_t>(nodes.size() / 2);
+            for (const HostNode& h : s->bvhs[i].nodes) {
+                float4 lo, hi;
+                lo.x = h.mn[0];
+                lo.y = h.mn[1];
+                lo.z = h.mn[2];
+                std::memcpy(&lo.w, &h.a, 4);
+                hi.x = h.mx[0];
+                hi.y = h.mx[1];
+                hi.z = h.mx[2];
+                std::memcpy(&hi.w, &h.b, 4);
+                nodes.push_back(lo);
+                nodes.push_back(hi);
+            }
+            order = s->bvhs[i].indices;
+        } else {
+            dm.node_base = 0;
+            order.resize(md.tri_count);
+            for (uint32_t t = 0; t < md.tri_count; ++t) order[t] = t;
+        }
+        dm.prim_base = static_cast<uint32_t>(prims.size() / 3);
+        dm.prim_count = static_cast<uint32_t>(order.size());
+        for (uint32_t t : order) {
+            const float* v = P + static_cast<uint64_t>(t) * 9;
+            float4 a, b, c;
+            a.x = v[0];
+            a.y = v[1];
+            a.z = v[2];
+            std::memcpy(&a.w, &t, 4);
+            b.x = v[3] - v[0];
+            b.y = v[4] - v[1];
+            b.z = v[5] - v[2];
+            b.w = 0.0f;
+            c.x = v[6] - v[0];
+            c.y = v[7] - v[1];
+            c.z = v[8] - v[2];
+            c.w = 0.0f;
+            prims.push_back(a);
+            prims.push_back(b);
+            prims.push_back(c);
+        }
+        normals.insert(normals.end(), md.normals, md.normals + static_cast<uint64_t>(md.tri_count) * 9);
+        dm.uv_base = static_cast<uint32_t>(uvs.size() / 6);
+        if (md.texcoord0) {
+            dm.flags |= kMeshHasUv;
+            uvs.insert(uvs.end(), md.texcoord0, md.texcoord0 + static_cast<uint64_t>(md.tri_count) * 6);
+        }
+    }
+    s->prim_slots = prims.size() / 3;
+    s->node_count = nodes.size() / 2;
+    s->triangle_count = tri_total;
+
+    std::vector<DevInstance> inst(d->instance_count);
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const mt_instance_desc& id = d->instances[i];
+        if (id.mesh >= d->mesh_count || id.material >= d->material_count)
+            return fail(MT_E_INVALID, "instance references a missing mesh or material");
+        DevInstance& di = inst[i];
+        std::memset(&di, 0, sizeof(di));
+        std::memcpy(di.M, id.transform, sizeof(di.M));
+        invert4(di.M, di.inv);
+        di.mesh = id.mesh;
+        di.material = id.material;
+        // MeshInstance::CalculateWorldAABB (Mesh.cpp:56-80)
+        const auto& bb = local_aabb[id.mesh];
+        const float* a = bb.first.data();
+        const float* b = bb.second.data();
+        const float corners[8][3] = {{a[0], a[1], a[2]}, {a[0], a[1], b[2]}, {a[0], b[1], a[2]},
+                                     {a[0], b[1], b[2]}, {b[0], a[1], a[2]}, {b[0], a[1], b[2]},
+                                     {b[0], b[1], a[2]}, {b[0], b[1], b[2]}};
+        for (int k = 0; k < 3; ++k) {
+            di.wmin[k] = FLT_MAX;
+            di.wmax[k] = -FLT_MAX;
+        }
+        for (const auto& c : corners) {
+            const float v[4] = {c[0], c[1], c[2], 1.0f};
+            float t[4];
+            mul4(di.M, v, t);
+            for (int k = 0; k < 3; ++k) {
+                di.wmin[k] = fmin_g(di.wmin[k], t[k]);
+                di.wmax[k] = fmax_g(di.wmax[k], t[k]);
+            }
+        }
+    }
+
+    std::vector<DevImage> images(d->image_count);
+    std::vector<uint32_t> texels;
+    for (uint32_t i = 0; i < d->image_count; ++i) {
+        const mt_image_desc& im = d->images[i];
+        if (!im.rgb || !im.width || !im.height) return fail(MT_E_INVALID, "empty image");
+        images[i] = DevImage{static_cast<uint32_t>(texels.size()), im.width, im.height, 0};
+        const uint64_t n = static_cast<uint64_t>(im.width) * im.height;
+        for (uint64_t k = 0; k < n; ++k)
+            texels.push_back(static_cast<uint32_t>(im.rgb[k * 3]) | (static_cast<uint32_t>(im.rgb[k * 3 + 1]) << 8) |
+                             (static_cast<uint32_t>(im.rgb[k * 3 + 2]) << 16) | 0xFF000000u);
+    }
+    auto tex = [&](const mt_texture_desc& t) {
+        DevTex o{};
+        o.image = (t.image >= 0 && static_cast<uint32_t>(t.image) < d->image_count) ? t.image : -1;
+        o.filter = t.filter;
+        o.ox = t.offset[0];
+        o.oy = t.offset[1];
+        o.sx = t.scale[0];
+        o.sy = t.scale[1];
+        return o;
+    };
+    std::vector<DevMaterial> mats(d->material_count);
+    for (uint32_t i = 0; i < d->material_count; ++i) {
+        const mt_material_desc& m = d->materials[i];
+        DevMaterial& o = mats[i];
+        std::memset(&o, 0, sizeof(o));
+        std::memcpy(o.base, m.base_color_factor, sizeof(o.base));
+        std::memcpy(o.emissive, m.emissive_factor, sizeof(o.emissive));
+        o.normal_scale = m.normal_scale;
+        o.metallic = m.metallic_factor;
+        o.roughness = m.roughness_factor;
+        o.tex[0] = tex(m.base_color_tex);
+        o.tex[1] = tex(m.metallic_roughness_tex);
+        o.tex[2] = tex(m.normal_tex);
+        o.tex[3] = tex(m.emissive_tex);
+    }
+    std::vector<DevLight> lights(d->light_count);
+    for (uint32_t i = 0; i < d->light_count; ++i) {
+        const mt_light_desc& l = d->lights[i];
+        DevLight& o = lights[i];
+        std::memset(&o, 0, sizeof(o));
+        std::memcpy(o.pos, l.position, sizeof(o.pos));
+        o.type = l.type;
+        std::memcpy(o.color, l.color, sizeof(o.color));
+        o.size = l.point_size;
+        std::memcpy(o.dir, l.direction, sizeof(o.dir));
+        o.inner = l.spot_inner_cos;
+        o.outer = l.spot_outer_cos;
+        o.intensity = (l.color[0] + l.color[1] + l.color[2]) / 3.0f;
+    }
+    std::vector<float4> env;
+    s->dev.has_env = 0;
+    if (d->env.format != MT_ENV_NONE && d->env.texels && d->env.width && d->env.height) {
+        const uint64_t n = static_cast<uint64_t>(d->env.width) * d->env.height;
+        env.resize(n);
+        for (uint64_t k = 0; k < n; ++k) {
+            float c[3];
+            if (d->env.format == MT_ENV_RGB8) {
+                const uint8_t* p = static_cast<const uint8_t*>(d->env.texels) + k * 3;
+                for (int j = 0; j < 3; ++j) c[j] = static_cast<float>(p[j]) / 255.0f;  // Image.cpp:11-15
+            } else if (d->env.format == MT_ENV_RGB32F) {
+                const float* p = static_cast<const float*>(d->env.texels) + k * 3;
+                for (int j = 0; j < 3; ++j) c[j] = p[j];
+            } else {
+                return fail(MT_E_INVALID, "unknown environment format");
+            }
+            env[k].x = c[0];
+            env[k].y = c[1];
+            env[k].z = c[2];
+            env[k].w = 1.0f;
+        }
+        s->dev.has_env = 1;
+        s->dev.env_w = d->env.width;
+        s->dev.env_h = d->env.height;
+        s->dev.env_filter = d->env.filter;
+    }
+
+    int rc;
+    if ((rc = dev_upload(s, inst, &s->dev.instances))) return rc;
+    if ((rc = dev_upload(s, meshes, &s->dev.meshes))) return rc;
+    if ((rc = dev_upload(s, nodes, &s->dev.nodes))) return rc;
+    if ((rc = dev_upload(s, prims, &s->dev.prims))) return rc;
+    if ((rc = dev_upload(s, normals, &s->dev.normals))) return rc;
+    if ((rc = dev_upload(s, uvs, &s->dev.uvs))) return rc;
+    if ((rc = dev_upload(s, mats, &s->dev.materials))) return rc;
+    if ((rc = dev_upload(s, texels, &s->dev.texels))) return rc;
+    if ((rc = dev_upload(s, images, &s->dev.images))) return rc;
+    if ((rc = dev_upload(s, env, &s->dev.env))) return rc;
+    if ((rc = dev_upload(s, lights, &s->dev.lights))) return rc;
+    s->dev.instance_count = d->instance_count;
+    s->dev.light_count = d->light_count;
+    return MT_OK;
+}
+
+}  // namespace mt
+
+mt_scene::~mt_scene() {
+    for (void* p : allocations) (void)hipFree(p);
+}

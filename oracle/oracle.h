@@ -1,0 +1,81 @@
+/*
+ * oracle.h -- C API of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * The oracle is a plain C++ restatement of MiTrace's CPU Tracer path
+ * (src/Tracer/*.cpp, src/Scene/*.cpp of the reference).  It is used by the test-suite as the
+ * checker of the GPU library and by bench.py as the CPU baseline.  Nothing in the product
+ * (mitrace_amd/, include/) links or calls it.
+ */
+#ifndef MITRACE_ORACLE_H
+#define MITRACE_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/mitrace_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_RNG_COUNTER 0 /* the keyed counter RNG the GPU uses (per (pixel, sample, dim)) */
+#define ORC_RNG_MT19937 1 /* std::mt19937 per tile, the reference's consumption order (Q23) */
+
+typedef struct orc_render_opts {
+    int32_t threads;          /* worker threads (0 = hardware_concurrency)          */
+    int32_t rng;              /* ORC_RNG_*                                          */
+    int32_t faithful_inverse; /* 1 = glm::inverse per ray and instance, as Intersect.cpp:152 */
+    uint32_t row_begin;       /* render only rows [row_begin, row_end) (0,0 = all)  */
+    uint32_t row_end;
+} orc_render_opts;
+
+typedef struct orc_work {
+    uint64_t rays, samples;
+    uint64_t inst_aabb_tests, inst_entered, node_pops, tri_tests;
+    uint64_t hits_shaded, misses;
+} orc_work;
+
+void* orc_scene_create(const mt_scene_desc* desc);
+void orc_scene_destroy(void* scene);
+const char* orc_last_error(void);
+
+/* flattened BVH of one mesh, same export format as mt_scene_mesh_bvh */
+int orc_scene_mesh_bvh(const void* scene, uint32_t mesh, uint32_t* nodes_out, uint32_t* node_count,
+                       uint32_t* indices_out, uint32_t* index_count);
+/* instance world AABBs: 6 floats per instance */
+int orc_scene_world_aabbs(const void* scene, float* out);
+
+/* IntersectScene for n rays {o.xyz, d.xyz}; work (optional) receives summed counters */
+int orc_trace_closest(const void* scene, uint32_t n, const float* rays, mt_hit* hits, orc_work* work);
+/* exact closest hit in double precision over every triangle (no BVH, no culling slack) */
+int orc_trace_bruteforce(const void* scene, uint32_t n, const float* rays, mt_hit* hits);
+
+/* BRDF: in 14 floats per case, out 5 floats (same layout as mt_brdf_eval) */
+void orc_brdf_eval(uint32_t n, const float* in, float* out);
+/* sampling helpers for distribution tests: in {V.xyz, N.xyz, base.rgb, rough, metal, u0..u4}
+ * (16 floats) -> out {dir.xyz, pdf, energy.rgb, specular(0/1)} (8 floats) */
+void orc_sample_bounce(uint32_t n, const float* in, float* out);
+/* equirect sample of the scene environment, n directions -> rgb (intensity multiplier 1) */
+int orc_env_sample(const void* scene, uint32_t n, const float* dirs, int32_t apply_rotation,
+                   float rotation_deg, float* rgb);
+/* camera rays for (u,v) pairs */
+void orc_camera_rays(const mt_camera_desc* cam, uint32_t n, const float* uv, float aspect,
+                     float* rays);
+/* ACES of L*2^-ev (Tracer.cpp:212-222) */
+void orc_aces(uint32_t n, const float* rgb, float ev, float* out);
+/* CleanupFireflies (Tracer.cpp:490-548) on one pixel's samples; returns the divisor and
+ * zeroes rejected samples in place */
+uint32_t orc_cleanup_fireflies(float* samples_rgb, uint32_t count, float threshold);
+/* the counter RNG: n draws for (seed, pixel, sample, dim) */
+void orc_rng_draws(uint64_t seed, uint32_t n, const uint32_t* pixel, const uint32_t* sample,
+                   const uint32_t* dim, float* out);
+
+/* full render (RenderBlock/ProcessRayForward); rgb_out W*H*3; sample_rgb_out optional
+ * [pixel][sample][3] */
+int orc_render(const void* scene, const mt_camera_desc* cam, const mt_settings* settings,
+               const orc_render_opts* opts, float* rgb_out, float* sample_rgb_out,
+               orc_work* work, double* seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+"""ctypes wrapper of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Imported by tests/ as the checker and by bench.py for the CPU-baseline leg; the product
+package never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from mitrace_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORC_RNG_COUNTER, ORC_RNG_MT19937 = 0, 1
+
+
+class orc_render_opts(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("rng", C.c_int32), ("faithful_inverse", C.c_int32),
+                ("row_begin", C.c_uint32), ("row_end", C.c_uint32)]
+
+
+class orc_work(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("rays", "samples", "inst_aabb_tests", "inst_entered",
+                                         "node_pops", "tri_tests", "hits_shaded", "misses")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+def lib_path(fast=False):
+    return os.path.join(_HERE, "build", "liboracle_fast.so" if fast else "liboracle.so")
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_libs = {}
+
+
+def load(fast=False):
+    if fast in _libs:
+        return _libs[fast]
+    p = lib_path(fast)
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"oracle library {p} is not built (make -C oracle)")
+    L = C.CDLL(p)
+    L.orc_scene_create.restype = C.c_void_p
+    L.orc_scene_create.argtypes = [C.POINTER(abi.mt_scene_desc)]
+    L.orc_scene_destroy.argtypes = [C.c_void_p]
+    L.orc_last_error.restype = C.c_char_p
+    L.orc_scene_mesh_bvh.argtypes = [C.c_void_p, C.c_uint32] + [C.POINTER(C.c_uint32)] * 4
+    L.orc_scene_world_aabbs.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    L.orc_trace_closest.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                    C.POINTER(abi.mt_hit), C.POINTER(orc_work)]
+    L.orc_trace_bruteforce.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                       C.POINTER(abi.mt_hit)]
+    L.orc_brdf_eval.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_sample_bounce.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_env_sample.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_int32,
+                                 C.c_float, C.POINTER(C.c_float)]
+    L.orc_camera_rays.argtypes = [C.POINTER(abi.mt_camera_desc), C.c_uint32, C.POINTER(C.c_float),
+                                  C.c_float, C.POINTER(C.c_float)]
+    L.orc_aces.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_float)]
+    L.orc_cleanup_fireflies.restype = C.c_uint32
+    L.orc_cleanup_fireflies.argtypes = [C.POINTER(C.c_float), C.c_uint32, C.c_float]
+    u32p = C.POINTER(C.c_uint32)
+    L.orc_rng_draws.argtypes = [C.c_uint64, C.c_uint32, u32p, u32p, u32p, C.POINTER(C.c_float)]
+    L.orc_render.argtypes = [C.c_void_p, C.POINTER(abi.mt_camera_desc), C.POINTER(abi.mt_settings),
+                             C.POINTER(orc_render_opts), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                             C.POINTER(orc_work), C.POINTER(C.c_double)]
+    _libs[fast] = L
+    return L
+
+
+HIT_DTYPE = np.dtype([("instance", np.int32), ("triangle", np.uint32), ("bv_index", np.uint32),
+                      ("u", np.float32), ("v", np.float32), ("distance", np.float32),
+                      ("world_pos", np.float32, 3), ("bvh_tests", np.uint32),
+                      ("triangle_tests", np.uint32)])
+assert HIT_DTYPE.itemsize == C.sizeof(abi.mt_hit)
+
+
+class OracleScene:
+    """The oracle's Scene::Scene over a :class:`mitrace_amd.scene.Scene`."""
+
+    def __init__(self, scene, fast=False):
+        self.L = load(fast)
+        self.scene = scene
+        desc, keep = scene.to_desc()
+        self.h = self.L.orc_scene_create(C.byref(desc))
+        if not self.h:
+            raise RuntimeError(self.L.orc_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.L.orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def mesh_bvh(self, mesh):
+        nn, ni = C.c_uint32(), C.c_uint32()
+        self.L.orc_scene_mesh_bvh(self.h, mesh, None, C.byref(nn), None, C.byref(ni))
+        nodes = np.zeros((nn.value, 8), np.uint32)
+        idx = np.zeros(ni.value, np.uint32)
+        self.L.orc_scene_mesh_bvh(self.h, mesh, nodes.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                  C.byref(nn), idx.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(ni))
+        return nodes, idx
+
+    def world_aabbs(self):
+        out = np.zeros((len(self.scene.instances), 6), np.float32)
+        self.L.orc_scene_world_aabbs(self.h, abi.fptr(out))
+        return out
+
+    def trace(self, rays, brute=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        hp = hits.ctypes.data_as(C.POINTER(abi.mt_hit))
+        w = orc_work()
+        if brute:
+            self.L.orc_trace_bruteforce(self.h, len(rays), abi.fptr(rays), hp)
+        else:
+            self.L.orc_trace_closest(self.h, len(rays), abi.fptr(rays), hp, C.byref(w))
+        return hits, w.as_dict()
+
+    def env_sample(self, dirs, apply_rotation=False, rotation_deg=0.0):
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros_like(dirs)
+        self.L.orc_env_sample(self.h, len(dirs), abi.fptr(dirs), int(apply_rotation),
+                              rotation_deg, abi.fptr(out))
+        return out
+
+    def render(self, settings, camera=None, threads=0, rng=ORC_RNG_COUNTER, faithful_inverse=True,
+               rows=None, want_samples=False):
+        cam = (camera or self.scene.camera).to_desc()
+        st = settings.to_desc()
+        W, H, S = settings.width, settings.height, settings.samples
+        rgb = np.zeros((H, W, 3), np.float32)
+        samples = np.zeros((H * W, S, 3), np.float32) if want_samples else None
+        o = orc_render_opts(threads=threads, rng=rng, faithful_inverse=int(faithful_inverse),
+                            row_begin=rows[0] if rows else 0, row_end=rows[1] if rows else 0)
+        w = orc_work()
+        secs = C.c_double()
+        rc = self.L.orc_render(self.h, C.byref(cam), C.byref(st), C.byref(o), abi.fptr(rgb),
+                               abi.fptr(samples) if want_samples else None, C.byref(w), C.byref(secs))
+        if rc != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return rgb, samples, w.as_dict(), secs.value
+
+
+def brdf_eval(inp):
+    L = load()
+    inp = np.ascontiguousarray(inp, np.float32).reshape(-1, 14)
+    out = np.zeros((len(inp), 5), np.float32)
+    L.orc_brdf_eval(len(inp), abi.fptr(inp), abi.fptr(out))
+    return out
+
+
+def sample_bounce(inp):
+    L = load()
+    inp = np.ascontiguousarray(inp, np.float32).reshape(-1, 16)
+    out = np.zeros((len(inp), 8), np.float32)
+    L.orc_sample_bounce(len(inp), abi.fptr(inp), abi.fptr(out))
+    return out
+
+
+def camera_rays(camera, uv, aspect):
+    L = load()
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((len(uv), 6), np.float32)
+    cd = camera.to_desc()
+    L.orc_camera_rays(C.byref(cd), len(uv), abi.fptr(uv), aspect, abi.fptr(out))
+    return out
+
+
+def aces(rgb, ev):
+    L = load()
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    out = np.zeros_like(rgb)
+    L.orc_aces(len(rgb), abi.fptr(rgb), ev, abi.fptr(out))
+    return out
+
+
+def cleanup_fireflies(samples, threshold):
+    L = load()
+    s = np.ascontiguousarray(samples, np.float32).reshape(-1, 3).copy()
+    n = L.orc_cleanup_fireflies(abi.fptr(s), len(s), threshold)
+    return s, n
+
+
+def rng_draws(seed, pixel, sample, dim):
+    L = load()
+    pixel = np.ascontiguousarray(pixel, np.uint32)
+    sample = np.ascontiguousarray(sample, np.uint32)
+    dim = np.ascontiguousarray(dim, np.uint32)
+    out = np.zeros(len(pixel), np.float32)
+    p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    L.orc_rng_draws(seed, len(pixel), p(pixel), p(sample), p(dim), abi.fptr(out))
+    return out
