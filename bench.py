@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 Tracer on the material-spheres grid workload.
+
+Workload: 7x7 UV spheres (64x32 segments) on a 100-tri ground plane under a 2048x1024
+float32 procedural sky rotated 30 deg, BVH leaves of at most 4 triangles, rendered at
+1920x1080 with 256 samples per pixel and the reference's estimator (bounces 5, energy
+cut-off 0.01, per-sample ACES).  One step = one full frame (530,841,600 camera paths).
+
+Printed: one JSON line with Msamples/s (camera paths per second over the timed steps, scene
+build and upload excluded as in main.cpp:125-131), the roofline of the dominant kernel
+(extend = closest-hit traversal) and the CPU baseline (the oracle built with the
+reference's release flags, mt19937 per tile, all worker threads of this host's share).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--spp 256] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s at 1080p, plain vs cached at equal RMSE; fwd+adjoint ms/iter"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of one closest-hit ray: world AABB 24 B per instance tested, inverse
+# matrix 64 B per instance entered, 32 B per BVH node popped, 36 B positions + 4 B index per
+# triangle test
+B_AABB, B_ENTER, B_NODE, B_TRI = 24, 64, 32, 40
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=256)
+    ap.add_argument("--paths-in-flight", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="approximate wall time of the CPU-baseline sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a device-to-device copy of 1 GiB (counter calibration)")
+    return ap.parse_args()
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(scene, settings_fn, seconds):
+    """The oracle (reference release flags, mt19937 per 32x32 tile, faithful per-ray inverse)
+    on full 1080p frames at a reduced spp; per-sample cost does not depend on spp."""
+    from oracle import pyoracle as po
+    o = po.OracleScene(scene, fast=True)
+    threads = cpu_threads()
+    total_s, total_n, total_r, spp_done = 0.0, 0, 0, 0
+    spp = 1
+    while True:
+        st = settings_fn(samples=spp, seed=1000 + spp_done)
+        _, _, w, secs = o.render(st, threads=threads, rng=po.ORC_RNG_MT19937, faithful_inverse=True)
+        total_s += secs
+        total_n += w["samples"]
+        total_r += w["rays"]
+        spp_done += spp
+        if total_s >= seconds * 0.5 or spp_done >= 64:
+            break
+        spp = max(1, min(64 - spp_done, int((seconds - total_s) / max(secs / spp, 1e-3))))
+    o.close()
+    return {"value": total_n / total_s / 1e6, "unit": "Msamples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1920x1080 at {spp_done} spp ({total_n} camera paths, {total_r} rays) in "
+                      f"{total_s:.1f} s; oracle with -Ofast -flto -march=x86-64-v3, mt19937 per "
+                      f"32x32 tile, {threads} threads",
+            "mrays_per_s": total_r / total_s / 1e6}
+
+
+def copy_calibration(nbytes=1 << 30):
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
+        return 2
+    from mitrace_amd import scenegen
+    from mitrace_amd.tracer import GpuScene
+
+    scene = scenegen.sphere_grid(seed=1)
+
+    def settings(**kw):
+        kw.setdefault("samples", args.spp)
+        kw.setdefault("paths_in_flight", args.paths_in_flight)
+        return scenegen.sphere_grid_settings(**kw)
+
+    g = GpuScene(scene)
+    W, H = 1920, 1080
+    # work counts of one frame (same seed => the timed frames do exactly this work)
+    _, _, cnt = g.render(settings(count_work=True))
+    print(f"# counted frame: {cnt['samples']} samples {cnt['rays']} rays", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        g.render(settings())
+        print(f"# warmup {i} done", file=sys.stderr, flush=True)
+    times, stats = [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        _, _, s = g.render(settings())  # returns after the device work and the readback
+        times.append(time.perf_counter() - t0)
+        stats.append(s)
+        print(f"# step {i}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        copy_calibration()
+    total_t = sum(times)
+    samples = sum(s["samples"] for s in stats)
+    rays = sum(s["rays"] for s in stats)
+    ext_ms = sum(s["kernel_ms_extend"] for s in stats)
+    ext_launches = sum(s["extend_launches"] for s in stats)
+    bytes_frame = (B_AABB * cnt["inst_aabb_tests"] + B_ENTER * cnt["inst_entered"] +
+                   B_NODE * cnt["node_pops"] + B_TRI * cnt["tri_tests"])
+    launches_frame = cnt["extend_launches"]
+    bytes_per_launch = bytes_frame / launches_frame
+    avg_launch_s = ext_ms / 1e3 / ext_launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "extend_traffic_sphere_grid_1080p.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": samples / total_t / 1e6,
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total_t / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "material-spheres grid, plain GPU tracer", "width": W, "height": H,
+                   "spp": args.spp, "bounces": 5, "triangles": scene.triangle_count,
+                   "instances": len(scene.instances), "bvh_max_triangles": 4,
+                   "env": "2048x1024 float32 equirect, rotated 30 deg",
+                   "samples_per_step": samples // args.steps},
+        "mrays_per_s": rays / total_t / 1e6,
+        "kernel_ms_per_step": sum(s["kernel_ms_total"] for s in stats) / args.steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_trace<false,0> (extend: closest hit)",
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": avg_launch_s * 1e3,
+                     "per_ray": {k: cnt[k] / cnt["rays"] for k in
+                                 ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(scene, settings, args.cpu_seconds)
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = line["value"] / cb["value"]
+    g.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,234 @@
+// mt_capi.cpp -- the extern "C" entry points of include/mitrace_gpu.h.
+//
+// Errors never cross the ABI as exceptions: every entry point catches, records a message in a
+// thread-local slot (mt_last_error) and returns a negative MT_E* code.
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+
+#include "mt_internal.hpp"
+#include "mt_render.hpp"
+
+namespace mt {
+namespace {
+thread_local std::string g_error;
+}
+void set_error(const std::string& msg) { g_error = msg; }
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+}  // namespace mt
+
+// Tracer state behind mt_job*: one host thread drives the chunk loop on its own HIP stream;
+// kill is checked between chunks (the reference checks it between tiles, Tracer.cpp:620).
+struct mt_job {
+    mt_scene* scene = nullptr;
+    mt_camera_desc cam{};
+    mt_settings st{};
+    float* rgb_out = nullptr;
+    mt::Renderer renderer;
+    std::thread worker;
+    std::atomic<bool> killed{false};
+    std::atomic<bool> done{false};
+    std::atomic<uint32_t> chunks_done{0};
+    uint32_t chunks_total = 0;
+    int status = MT_OK;
+    std::string error;
+    mutable std::mutex mu;
+    mt_stats snapshot{};
+    std::chrono::steady_clock::time_point start;
+};
+
+namespace {
+
+int run_job(mt_job* j, float* sample_rgb_out) {
+    if (hipSetDevice(j->scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+    int rc = j->renderer.setup(j->scene, j->cam, j->st);
+    if (rc) return rc;
+    {
+        std::scoped_lock lk(j->mu);
+        j->chunks_total = j->renderer.chunks();
+    }
+    for (uint32_t c = 0; c < j->renderer.chunks(); ++c) {
+        if (j->killed.load()) return mt::fail(MT_E_KILLED, "render killed");
+        rc = j->renderer.run_chunk(c, sample_rgb_out);
+        if (rc) return rc;
+        j->chunks_done.store(c + 1);
+        std::scoped_lock lk(j->mu);
+        j->snapshot = j->renderer.stats();
+    }
+    rc = j->renderer.finish(j->rgb_out);
+    std::scoped_lock lk(j->mu);
+    j->snapshot = j->renderer.stats();
+    return rc;
+}
+
+void fill_stats(const mt_job* j, mt_stats* out) {
+    std::scoped_lock lk(j->mu);
+    *out = j->snapshot;
+    const float total = j->chunks_total ? static_cast<float>(j->chunks_total) : 1.0f;
+    out->progress = static_cast<float>(j->chunks_done.load()) / total;
+    out->time_elapsed =
+        std::chrono::duration<float>(std::chrono::steady_clock::now() - j->start).count();
+    out->estimated_time_remaining =
+        out->progress > 0.0f ? (out->time_elapsed / out->progress) * (1.0f - out->progress) : 0.0f;
+}
+
+bool valid_args(mt_scene* scene, const mt_camera_desc* cam, const mt_settings* st) {
+    if (!scene || !cam || !st) {
+        mt::set_error("null scene, camera or settings");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mt_abi_version(void) { return MT_ABI_VERSION; }
+
+const char* mt_last_error(void) { return mt::g_error.c_str(); }
+
+int mt_device_count(int* count) {
+    if (!count) return mt::fail(MT_E_INVALID, "null count");
+    if (hipGetDeviceCount(count) != hipSuccess) {
+        *count = 0;
+        return mt::fail(MT_E_HIP, "hipGetDeviceCount failed");
+    }
+    return MT_OK;
+}
+
+int mt_scene_create(const mt_scene_desc* desc, mt_scene** out) {
+    if (!out) return mt::fail(MT_E_INVALID, "null output handle");
+    *out = nullptr;
+    try {
+        auto s = std::make_unique<mt_scene>();
+        if (hipGetDevice(&s->device) != hipSuccess) return mt::fail(MT_E_HIP, "no HIP device");
+        const int rc = mt::upload_scene(desc, s.get());
+        if (rc) return rc;
+        *out = s.release();
+        return MT_OK;
+    } catch (const std::bad_alloc&) {
+        return mt::fail(MT_E_NOMEM, "host allocation failed in mt_scene_create");
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_scene_destroy(mt_scene* scene) {
+    delete scene;
+    return MT_OK;
+}
+
+int mt_scene_mesh_bvh(const mt_scene* scene, uint32_t mesh, uint32_t* nodes_out, uint32_t* node_count,
+                      uint32_t* indices_out, uint32_t* index_count) {
+    if (!scene || mesh >= scene->mesh_count) return mt::fail(MT_E_INVALID, "bad scene or mesh index");
+    const mt::HostBvh& b = scene->bvhs[mesh];
+    if (node_count) *node_count = static_cast<uint32_t>(b.nodes.size());
+    if (index_count) *index_count = static_cast<uint32_t>(b.indices.size());
+    if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(mt::HostNode));
+    if (indices_out) std::memcpy(indices_out, b.indices.data(), b.indices.size() * sizeof(uint32_t));
+    return MT_OK;
+}
+
+int mt_render(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,
+              float* rgb_out, float* sample_rgb_out, mt_stats* stats) {
+    if (!valid_args(scene, camera, settings)) return MT_E_INVALID;
+    try {
+        mt_job j;
+        j.scene = scene;
+        j.cam = *camera;
+        j.st = *settings;
+        j.rgb_out = rgb_out;
+        j.start = std::chrono::steady_clock::now();
+        const int rc = run_job(&j, sample_rgb_out);
+        if (stats) fill_stats(&j, stats);
+        return rc;
+    } catch (const std::bad_alloc&) {
+        return mt::fail(MT_E_NOMEM, "host allocation failed in mt_render");
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_render_start(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,
+                    float* rgb_out, mt_job** job) {
+    if (!job) return mt::fail(MT_E_INVALID, "null job handle");
+    *job = nullptr;
+    if (!valid_args(scene, camera, settings)) return MT_E_INVALID;
+    try {
+        auto j = std::make_unique<mt_job>();
+        j->scene = scene;
+        j->cam = *camera;
+        j->st = *settings;
+        j->rgb_out = rgb_out;
+        j->start = std::chrono::steady_clock::now();
+        mt_job* raw = j.get();
+        raw->worker = std::thread([raw]() {
+            raw->status = run_job(raw, nullptr);
+            if (raw->status) raw->error = mt_last_error();
+            raw->done.store(true);
+        });
+        *job = j.release();
+        return MT_OK;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_render_wait(mt_job* job) {
+    if (!job) return mt::fail(MT_E_INVALID, "null job");
+    if (job->worker.joinable()) job->worker.join();
+    if (job->status) mt::set_error(job->error);
+    return job->status;
+}
+
+int mt_render_kill(mt_job* job) {
+    if (!job) return mt::fail(MT_E_INVALID, "null job");
+    job->killed.store(true);
+    return MT_OK;
+}
+
+int mt_render_stats(const mt_job* job, mt_stats* stats) {
+    if (!job || !stats) return mt::fail(MT_E_INVALID, "null job or stats");
+    fill_stats(job, stats);
+    return MT_OK;
+}
+
+int mt_render_is_done(const mt_job* job) { return job && job->done.load() ? 1 : 0; }
+
+int mt_job_destroy(mt_job* job) {
+    if (!job) return MT_OK;
+    if (job->worker.joinable()) {
+        job->killed.store(true);
+        job->worker.join();
+    }
+    delete job;
+    return MT_OK;
+}
+
+int mt_trace_closest(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hits) {
+    if (!scene || (n && (!rays || !hits))) return mt::fail(MT_E_INVALID, "bad arguments");
+    try {
+        return mt::trace_rays(scene, n, rays, hits);
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_brdf_eval(uint32_t n, const float* in, float* out) {
+    if (n && (!in || !out)) return mt::fail(MT_E_INVALID, "bad arguments");
+    try {
+        return mt::brdf_eval(n, in, out);
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+}  // extern "C"

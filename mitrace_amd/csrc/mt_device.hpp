@@ -1,0 +1,472 @@
+// mt_device.hpp -- device-side arithmetic of the gfx950 Tracer: vector math with a fixed
+// operation order, the counter RNG, texture/environment lookups, the BRDF, BRDF sampling and
+// the closest-hit traversal.  Included by mt_render.hip only.
+//
+// Operation-order contract (the CPU oracle follows the same one, written separately):
+//   float32, no FMA contraction (built with -ffp-contract=off), IEEE-rounded division/sqrt;
+//   dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z; normalize(v) = v * (1/sqrt(dot(v,v)));
+//   min(a,b) = (b < a) ? b : a; max(a,b) = (a < b) ? b : a (GLM's scalar forms);
+//   mat4*vec4 row r = (m[0][r]*x + m[1][r]*y) + (m[2][r]*z + m[3][r]*w).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mt_internal.hpp"
+
+namespace mt {
+namespace dev {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kFltMax = 3.402823466e+38f;
+constexpr float kFltEps = 1.192092896e-07f;
+constexpr float kMinDelta = 0.1f;       // traversal slack, Intersect.cpp:58
+constexpr float kPullout = 0.0001f;     // Tracer.cpp:24
+constexpr uint32_t kLinearTag = 0xFFFFFFFFu;
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float a) { return {a, a, a}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 mul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 divv(V3 a, V3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ V3 scale(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 divs(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ V3 neg(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float gmin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float gmax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ V3 vmin(V3 a, V3 b) { return {gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)}; }
+__device__ __forceinline__ V3 vmax(V3 a, V3 b) { return {gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)}; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ float len(V3 v) { return __builtin_sqrtf(dot(v, v)); }
+__device__ __forceinline__ V3 normalize(V3 v) { return scale(v, 1.0f / __builtin_sqrtf(dot(v, v))); }
+__device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// column-major 4x4 times (p, w)
+__device__ __forceinline__ V3 xform(const float* m, V3 p, float w) {
+    V3 o;
+    o.x = (m[0] * p.x + m[4] * p.y) + (m[8] * p.z + m[12] * w);
+    o.y = (m[1] * p.x + m[5] * p.y) + (m[9] * p.z + m[13] * w);
+    o.z = (m[2] * p.x + m[6] * p.y) + (m[10] * p.z + m[14] * w);
+    return o;
+}
+
+// ------------------------------------------------------------------ counter RNG
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t path_key(uint32_t seed_key, uint32_t pixel, uint32_t sample) {
+    return mix32(mix32(seed_key ^ pixel) + sample * 0x9E3779B9u);
+}
+__device__ __forceinline__ float draw(uint32_t key, uint32_t dim) {
+    const uint32_t x = mix32(key ^ mix32(dim + 0x7F4A7C15u));
+    return static_cast<float>(x >> 8) * (1.0f / 16777216.0f);
+}
+// per-bounce dimension offsets; bounce k starts at 2 + k * (8 + light_count)
+enum : uint32_t { kDimLobe = 0, kDimDir = 1, kDimFallback = 3, kDimLightPt = 5, kDimRR = 7,
+                  kDimLightSel = 8 };
+
+// ------------------------------------------------------------------ textures (Image.cpp)
+struct Tex8 {
+    const uint32_t* texels;
+    int w, h;
+    __device__ __forceinline__ float4 operator()(int x, int y) const {
+        const uint32_t t = texels[y * w + x];
+        return make_float4(static_cast<float>(t & 0xFFu) / 255.0f,
+                           static_cast<float>((t >> 8) & 0xFFu) / 255.0f,
+                           static_cast<float>((t >> 16) & 0xFFu) / 255.0f, 1.0f);
+    }
+};
+struct TexF {
+    const float4* texels;
+    int w, h;
+    __device__ __forceinline__ float4 operator()(int x, int y) const { return texels[y * w + x]; }
+};
+__device__ __forceinline__ float4 mix4(float4 a, float4 b, float t) {
+    return make_float4(mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t), mixf(a.w, b.w, t));
+}
+template <typename F>
+__device__ float4 sample_image(const F& f, float uvx, float uvy, int filter) {
+    const int w = f.w, h = f.h;
+    const float x = uvx - floorf(uvx);
+    const float y = uvy - floorf(uvy);
+    if (filter == MT_FILTER_NEAREST) {
+        int ix = static_cast<int>(x * static_cast<float>(w)) % w;
+        int iy = static_cast<int>(y * static_cast<float>(h)) % h;
+        if (ix < 0) ix += w;
+        if (iy < 0) iy += h;
+        ix = ix > 0 ? ix : 0;
+        iy = iy > 0 ? iy : 0;
+        return f(ix, iy);
+    }
+    // weights in uv units (Image.cpp:26-44); first texel clamped where x*W rounds up to W
+    int x0 = static_cast<int>(floorf(x * static_cast<float>(w)));
+    int y0 = static_cast<int>(floorf(y * static_cast<float>(h)));
+    x0 = x0 < w - 1 ? x0 : w - 1;
+    y0 = y0 < h - 1 ? y0 : h - 1;
+    x0 = x0 > 0 ? x0 : 0;
+    y0 = y0 > 0 ? y0 : 0;
+    const int x1 = x0 + 1 < w - 1 ? x0 + 1 : w - 1;
+    const int y1 = y0 + 1 < h - 1 ? y0 + 1 : h - 1;
+    const float sx = x - static_cast<float>(x0) / static_cast<float>(w);
+    const float sy = y - static_cast<float>(y0) / static_cast<float>(h);
+    const float4 c0 = mix4(f(x0, y0), f(x1, y0), sx);
+    const float4 c1 = mix4(f(x0, y1), f(x1, y1), sx);
+    return mix4(c0, c1, sy);
+}
+__device__ __forceinline__ float4 sample_tex(const DevScene& S, const DevTex& t, float u, float v) {
+    const DevImage im = S.images[t.image];
+    Tex8 f{S.texels + im.offset, static_cast<int>(im.w), static_cast<int>(im.h)};
+    return sample_image(f, u * t.sx + t.ox, v * t.sy + t.oy, t.filter);
+}
+// Scene::SampleSkybox without the intensity (Scene.cpp:61-73 -> Image.cpp:63-75)
+__device__ V3 sample_env(const DevScene& S, V3 d, int apply_rot, float rot_offset) {
+    if (!S.has_env) return mk(0.0f);
+    const float theta = atan2f(d.z, d.x);
+    const float phi = acosf(d.y / len(d));
+    float u = (theta + kPi) / (2.0f * kPi);
+    const float v = phi / kPi;
+    if (apply_rot) u = u * 1.0f + rot_offset;
+    TexF f{S.env, static_cast<int>(S.env_w), static_cast<int>(S.env_h)};
+    const float4 c = sample_image(f, u, v, S.env_filter);
+    return {c.x, c.y, c.z};
+}
+
+// ------------------------------------------------------------------ BRDF (BRDF.hpp)
+__device__ __forceinline__ V3 fresnel(float c, V3 f0) {
+    c = clampf(c, 0.0f, 1.0f);
+    const float p = powf(1.0f - c, 5.0f);
+    return {f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
+}
+__device__ __forceinline__ V3 base_refl(V3 base, float m) {
+    m = clampf(m, 0.0f, 1.0f);
+    return {mixf(0.04f, base.x, m), mixf(0.04f, base.y, m), mixf(0.04f, base.z, m)};
+}
+__device__ __forceinline__ float ggx_d(float nh, float r) {
+    nh = clampf(nh, 0.0f, 1.0f);
+    const float a = r * r;
+    const float a2 = a * a;
+    float den = (nh * nh) * (a2 - 1.0f) + 1.0f;
+    den = kPi * den * den;
+    return a2 / den;
+}
+__device__ __forceinline__ float smith_g1(float nv, float r) {
+    nv = clampf(nv, 0.0f, 1.0f);
+    const float rr = r + 1.0f;
+    const float k = (rr * rr) / 8.0f;
+    return nv / (nv * (1.0f - k) + k);
+}
+__device__ V3 eval_brdf(V3 L, V3 V, V3 N, V3 base, float rough, float metal) {
+    rough = gmax(rough, 0.05f);
+    const float nl = gmax(dot(N, L), 0.0f);
+    const float nv = gmax(dot(N, V), 0.0f);
+    if (nl <= 0.0f || nv <= 0.0f) return mk(0.0f);
+    const V3 h = normalize(add(V, L));
+    const float nh = gmax(dot(N, h), 0.0f);
+    const float hv = gmax(dot(h, V), 0.0f);
+    const V3 f = fresnel(hv, base_refl(base, metal));
+    const float d = ggx_d(nh, rough);
+    const float g = smith_g1(nv, rough) * smith_g1(nl, rough);
+    const V3 kd = scale(sub(mk(1.0f), f), 1.0f - metal);
+    const V3 diffuse = divs(mul(kd, base), kPi);
+    const V3 spec = divs(scale(f, d * g), gmax(4.0f * nv * nl, 0.001f));
+    return vmax(add(diffuse, spec), mk(0.0f));
+}
+__device__ __forceinline__ float diffuse_pdf(V3 L, V3 N) { return gmax(dot(N, L), 0.0f) / kPi; }
+__device__ float specular_pdf(V3 L, V3 V, V3 N, float rough) {
+    const float nl = gmax(dot(N, L), 0.0f);
+    const float nv = gmax(dot(N, V), 0.0f);
+    if (nl <= 0.0f || nv <= 0.0f) return 0.0f;
+    const V3 h = normalize(add(V, L));
+    const float nh = gmax(dot(N, h), 0.0f);
+    const float vh = gmax(dot(V, h), 0.0f);
+    if (vh <= 0.0f) return 0.0f;
+    return ggx_d(nh, gmax(rough, 0.05f)) * nh / (4.0f * vh);
+}
+
+// ------------------------------------------------------------------ sampling (Tracer.cpp:26-121)
+__device__ __forceinline__ void tbn(V3 n, V3& t, V3& b) {
+    const V3 up = (fabsf(n.z) < 0.999f) ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+    t = normalize(cross(up, n));
+    b = cross(n, t);
+}
+__device__ V3 cosine_dir(float u1, float u2, V3 n) {
+    const float r = __builtin_sqrtf(u1);
+    const float phi = 2.0f * kPi * u2;
+    V3 t, b;
+    tbn(n, t, b);
+    float s, c;
+    sincosf(phi, &s, &c);
+    const float lz = __builtin_sqrtf(gmax(0.0f, 1.0f - u1));
+    return normalize(add(add(scale(t, r * c), scale(b, r * s)), scale(n, lz)));
+}
+__device__ bool ggx_dir(float u1, float u2, V3 incident, V3 n, float rough, V3& out) {
+    const float alpha = gmax(rough * rough, 0.0025f);
+    const float alpha2 = alpha * alpha;
+    const float phi = 2.0f * kPi * u2;
+    const float tan2 = alpha2 * u1 / gmax(1.0f - u1, 0.0001f);
+    const float ct = 1.0f / __builtin_sqrtf(1.0f + tan2);
+    const float st = __builtin_sqrtf(gmax(0.0f, 1.0f - ct * ct));
+    V3 t, b;
+    tbn(n, t, b);
+    float s, c;
+    sincosf(phi, &s, &c);
+    V3 h = normalize(add(add(scale(t, st * c), scale(b, st * s)), scale(n, ct)));
+    if (dot(h, neg(incident)) < 0.0f) h = neg(h);
+    const V3 refl = sub(incident, scale(scale(h, dot(h, incident)), 2.0f));  // glm::reflect
+    if (dot(refl, n) > 0.0f) {
+        out = normalize(refl);
+        return true;
+    }
+    return false;
+}
+__device__ __forceinline__ float spec_probability(V3 view, V3 n, V3 base, float metal) {
+    const V3 f = fresnel(gmax(dot(n, view), 0.0f), base_refl(base, metal));
+    return clampf((f.x + f.y + f.z) / 3.0f, 0.05f, 0.95f);
+}
+__device__ V3 shading_normal(V3 surf, V3 texn) {  // Tracer.cpp:95-105
+    const V3 n = normalize(surf);
+    const V3 up = (fabsf(n.y) < 0.9f) ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+    const V3 t = normalize(cross(n, up));
+    const V3 b = normalize(cross(t, n));
+    return normalize(add(add(scale(t, texn.x), scale(b, texn.y)), scale(n, texn.z)));
+}
+__device__ __forceinline__ V3 aces(V3 c, float expo) {  // Tracer.cpp:212-222
+    const V3 x = scale(c, expo);
+    const V3 num = mul(x, add(scale(x, 2.51f), mk(0.03f)));
+    const V3 den = add(mul(x, add(scale(x, 2.43f), mk(0.59f))), mk(0.14f));
+    return divv(num, den);
+}
+
+// ------------------------------------------------------------------ traversal (Intersect.cpp)
+__device__ __forceinline__ float slab(V3 o, V3 inv, V3 mn, V3 mx) {
+    const V3 t1 = mul(sub(mn, o), inv);
+    const V3 t2 = mul(sub(mx, o), inv);
+    const V3 lo = vmin(t1, t2), hi = vmax(t1, t2);
+    const float tmin = gmax(gmax(lo.x, lo.y), lo.z);
+    const float tmax = gmin(gmin(hi.x, hi.y), hi.z);
+    return tmin <= tmax ? tmax : -1.0f;
+}
+
+// two-sided Moller-Trumbore with the bounds tested against det before scaling
+__device__ __forceinline__ bool tri_test(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float& u, float& v, float& t) {
+    const V3 p = cross(d, e2);
+    const float det = dot(e1, p);
+    V3 q;
+    if (det > kFltEps) {
+        const V3 s = sub(o, v0);
+        u = dot(s, p);
+        if (u < 0.0f || u > det) return false;
+        q = cross(s, e1);
+        v = dot(d, q);
+        if (v < 0.0f || u + v > det) return false;
+    } else if (det < -kFltEps) {
+        const V3 s = sub(o, v0);
+        u = dot(s, p);
+        if (u > 0.0f || u < det) return false;
+        q = cross(s, e1);
+        v = dot(d, q);
+        if (v > 0.0f || u + v < det) return false;
+    } else {
+        return false;
+    }
+    const float id = 1.0f / det;
+    t = dot(e2, q) * id;
+    u = u * id;
+    v = v * id;
+    return true;
+}
+
+struct Hit {
+    int inst;
+    uint32_t tri, bv;
+    float u, v, dist;
+    V3 wpos;
+    uint32_t bvh_tests, tri_tests;  // of the winning instance
+};
+
+struct Work {
+    uint32_t aabb, enter, pops, tris;
+};
+
+// Traversal stack: entries [0, lds_depth) in LDS (lds[i * lds_stride]), deeper entries in a
+// global spill area (spill[(i - lds_depth) * spill_stride]).  cap is the allocated depth; an
+// overflow sets *err and drops the entry (never writes out of bounds).
+struct Stack {
+    uint2* lds;
+    uint32_t lds_stride, lds_depth;
+    uint2* spill;
+    uint32_t spill_stride;
+    uint32_t cap;
+    uint32_t* err;
+    uint32_t sp;
+    __device__ __forceinline__ void push(uint32_t node, float d) {
+        const uint2 e = make_uint2(node, __float_as_uint(d));
+        if (sp < lds_depth) {
+            lds[sp * lds_stride] = e;
+        } else if (sp < cap) {
+            spill[(sp - lds_depth) * spill_stride] = e;
+        } else {
+            atomicOr(err, 1u);
+            return;
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ uint2 pop() {
+        --sp;
+        return sp < lds_depth ? lds[sp * lds_stride] : spill[(sp - lds_depth) * spill_stride];
+    }
+};
+
+// IntersectScene (Intersect.cpp:201-219) with IntersectMeshInstance (143-199),
+// IntersectTrianglesBVH (47-141) and IntersectTrianglesLinear (20-45).  Instances are
+// culled by their world AABB in a wave-uniform pass over groups of 64; each lane then walks
+// its own entered instances in index order inside one loop, so lanes on small instances do
+// not wait for lanes on large ones.  Per-instance results and the strict '<' across
+// instances in index order are those of the reference.
+template <bool kCount>
+__device__ void trace(const DevScene& S, V3 o, V3 d, Hit& R, Stack& st, Work& w) {
+    const V3 invd = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    R.inst = -1;
+    R.dist = kFltMax;
+    R.tri = R.bv = 0;
+    R.u = R.v = 0.0f;
+    R.bvh_tests = R.tri_tests = 0;
+    for (uint32_t cb = 0; cb < S.instance_count; cb += 64) {
+        const uint32_t nch = (S.instance_count - cb) < 64u ? (S.instance_count - cb) : 64u;
+        uint64_t mask = 0;
+        for (uint32_t j = 0; j < nch; ++j) {
+            const DevInstance* I = S.instances + cb + j;
+            const V3 mn = {I->wmin[0], I->wmin[1], I->wmin[2]};
+            const V3 mx = {I->wmax[0], I->wmax[1], I->wmax[2]};
+            if (slab(o, invd, mn, mx) >= 0.0f) mask |= (1ull << j);
+        }
+        if (kCount) w.aabb += nch;
+        int cur = -1;
+        V3 lo{}, ld{}, linv{};
+        float ldist = kFltMax, bu = 0.0f, bvv = 0.0f;
+        uint32_t btri = 0, bbv = 0, btests = 0, ttests = 0;
+        uint32_t node_base = 0, prim_base = 0, prim_count = 0;
+        st.sp = 0;
+        while (true) {
+            if (st.sp == 0) {
+                if (cur >= 0) {
+                    if (kCount) {
+                        w.pops += btests;
+                        w.tris += ttests;
+                    }
+                    if (ldist < kFltMax) {
+                        const float* M = S.instances[cb + cur].M;
+                        const V3 lp = add(lo, scale(ld, ldist));
+                        const V3 wp = xform(M, lp, 1.0f);
+                        const float wd = len(sub(wp, o));
+                        if (wd < R.dist) {
+                            R.dist = wd;
+                            R.inst = static_cast<int>(cb) + cur;
+                            R.tri = btri;
+                            R.bv = bbv;
+                            R.u = bu;
+                            R.v = bvv;
+                            R.wpos = wp;
+                            R.bvh_tests = btests;
+                            R.tri_tests = ttests;
+                        }
+                    }
+                }
+                if (mask == 0) break;
+                cur = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const DevInstance* I = S.instances + cb + cur;
+                float inv[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) inv[k] = I->inv[k];
+                lo = xform(inv, o, 1.0f);
+                ld = normalize(xform(inv, d, 0.0f));
+                linv = {1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z};
+                ldist = kFltMax;
+                bu = bvv = 0.0f;
+                btri = bbv = btests = ttests = 0;
+                const DevMesh m = S.meshes[I->mesh];
+                node_base = m.node_base;
+                prim_base = m.prim_base;
+                prim_count = m.prim_count;
+                if (kCount) w.enter++;
+                if (m.flags & kMeshHasBvh) {
+                    const float4 a = S.nodes[2 * node_base], b = S.nodes[2 * node_base + 1];
+                    st.push(0u, slab(lo, linv, xyz(a), xyz(b)));
+                } else {
+                    st.push(kLinearTag, 0.0f);
+                }
+                continue;
+            }
+            const uint2 e = st.pop();
+            uint32_t first, count;
+            if (e.x == kLinearTag) {
+                first = 0;
+                count = prim_count;
+            } else {
+                btests++;
+                const float nd = __uint_as_float(e.y);
+                if (nd < 0.0f) continue;
+                if (nd > ldist + kMinDelta) continue;
+                const float4 na = S.nodes[2 * (node_base + e.x)];
+                const float4 nb = S.nodes[2 * (node_base + e.x) + 1];
+                const uint32_t ca = __float_as_uint(na.w), cbi = __float_as_uint(nb.w);
+                if (!(ca & kLeafBit)) {
+                    const float4 a0 = S.nodes[2 * (node_base + ca)], a1 = S.nodes[2 * (node_base + ca) + 1];
+                    const float4 b0 = S.nodes[2 * (node_base + cbi)], b1 = S.nodes[2 * (node_base + cbi) + 1];
+                    const float da = slab(lo, linv, xyz(a0), xyz(a1));
+                    const float db = slab(lo, linv, xyz(b0), xyz(b1));
+                    const bool pa = (da >= 0.0f && da <= ldist + kMinDelta);
+                    const bool pb = (db >= 0.0f && db <= ldist + kMinDelta);
+                    if (pa && pb) {
+                        if (da < db) {
+                            st.push(cbi, db);
+                            st.push(ca, da);
+                        } else {
+                            st.push(ca, da);
+                            st.push(cbi, db);
+                        }
+                    } else if (pa) {
+                        st.push(ca, da);
+                    } else if (pb) {
+                        st.push(cbi, db);
+                    }
+                    continue;
+                }
+                first = ca & ~kLeafBit;
+                count = cbi & ~kLeafBit;
+            }
+            for (uint32_t i = first; i < first + count; ++i) {
+                const float4* P = S.prims + 3ull * (prim_base + i);
+                const float4 p0 = P[0], p1 = P[1], p2 = P[2];
+                float u, v, t;
+                ttests++;
+                if (tri_test(lo, ld, xyz(p0), xyz(p1), xyz(p2), u, v, t)) {
+                    if (t > 0.0f && t < ldist) {
+                        ldist = t;
+                        bu = u;
+                        bvv = v;
+                        bbv = (e.x == kLinearTag) ? 0u : e.x;
+                        btri = __float_as_uint(p0.w);
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace dev
+}  // namespace mt

@@ -1,0 +1,740 @@
+// mt_render.hip -- wavefront path tracing kernels for gfx950 and the render driver.
+//
+// One render = a sequence of chunks.  A chunk is a set of pixels with all their samples
+// (npix * spp paths, path p = sample * npix + local pixel, so a wave holds 64 neighbouring
+// pixels of one sample index).  Per chunk:
+//   generate                camera rays, path state init             (Tracer.cpp:556-586)
+//   for bounce k < bounces:
+//     extend   closest hit of every active path                      (Intersect.cpp:201-219)
+//     shade    miss -> sky; hit -> material, BRDF sampling, emission,
+//              light choice, shadow-ray record, termination          (Tracer.cpp:134-340)
+//     shadow   closest hit of every shadow ray, visible -> add light  (Tracer.cpp:317-336)
+//   finalize  per-sample total and ACES                               (Tracer.cpp:344-346)
+//   resolve   per-pixel mean (with firefly rejection when enabled)    (Tracer.cpp:490-601)
+// Active paths and shadow rays live in SoA queues compacted with wave ballots and one atomic
+// per wave.  extend/shadow are persistent kernels: each wave takes 64 queue entries at a time
+// from an atomic work counter until the queue is drained.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mt_device.hpp"
+#include "mt_render.hpp"
+
+namespace mt {
+
+using namespace dev;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kLdsDepth = 16;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// wave-aggregated queue slot: one atomic per wave, ballot + prefix for the lane offsets
+__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0;
+    const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(lane_id()) == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(m)));
+    base = __shfl(base, leader);
+    const uint64_t below = m & ((1ull << lane_id()) - 1ull);
+    return base + static_cast<uint32_t>(__popcll(below));
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
+    // wave reduction then one atomic
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (lane_id() == 0 && x) atomicAdd(dst, x);
+}
+
+__device__ __forceinline__ void path_coords(const ChunkParams& c, uint32_t p, uint32_t& pix,
+                                            uint32_t& s) {
+    s = p / c.npix;
+    pix = c.pix_begin + (p - s * c.npix);
+}
+
+__global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        for (uint32_t k = 0; k < kMaxCounters; ++k) b.counters[k] = 0;
+        b.counters[kCntExtend] = c.paths;
+    }
+    if (p >= c.paths) return;
+    uint32_t pix, s;
+    path_coords(c, p, pix, s);
+    const uint32_t x = pix % c.width, y = pix / c.width;
+    const uint32_t key = path_key(c.seed_key, pix, s);
+    // jitter: u = x/W + U[0, 1/W) (Tracer.cpp:567-586)
+    const float u = static_cast<float>(x) * c.px_w + (draw(key, 0) * (c.px_w - 0.0f) + 0.0f);
+    const float v = static_cast<float>(y) * c.px_h + (draw(key, 1) * (c.px_h - 0.0f) + 0.0f);
+    V3 o, d;
+    if (c.cam_type == MT_CAMERA_PERSPECTIVE) {  // Tracer/Camera.cpp:17-29
+        const float px = (2.0f * u - 1.0f) * c.fov_scale * c.aspect;
+        const float py = (2.0f * v - 1.0f) * c.fov_scale;
+        const V3 dc = normalize(V3{px, -py, -1.0f});
+        o = xform(c.cam, mk(0.0f), 1.0f);
+        d = normalize(xform(c.cam, dc, 0.0f));
+    } else {  // Camera.cpp:31-44
+        const float px = (2.0f * u - 1.0f) * c.ortho_x * c.aspect / c.cam_aspect;
+        const float py = (2.0f * v - 1.0f) * c.ortho_y;
+        const V3 dc = normalize(V3{0.0f, 0.0f, -1.0f});
+        o = xform(c.cam, V3{px, -py, 0.0f}, 1.0f);
+        d = normalize(xform(c.cam, dc, 0.0f));
+    }
+    b.ray_o[p] = f4(o, 0.0f);
+    b.ray_d[p] = f4(d, 0.0f);
+    b.t_rad[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.t_term[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.prev_base[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.acc_e[p] = z;
+    b.acc_d[p] = z;
+    b.acc_i[p] = z;
+    b.acc_s[p] = z;
+    b.queue[0][p] = p;
+}
+
+template <bool kCount>
+__device__ __forceinline__ void trace_one(const DevScene& S, V3 o, V3 d, Hit& h, Stack& st,
+                                          Work& w) {
+    trace<kCount>(S, o, d, h, st, w);
+}
+
+// persistent closest-hit over a queue: mode 0 = extend (queue of path ids), 1 = shadow records
+template <bool kCount, int kMode>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, ChunkParams c, PathBuffers b,
+                                                  uint32_t bounce) {
+    __shared__ uint2 lds_stack[kLdsDepth * kBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st{lds_stack + threadIdx.x, kBlock, kLdsDepth, b.spill + gtid,
+             gridDim.x * blockDim.x, c.stack_cap, b.counters + kCntError, 0};
+    const uint32_t count = kMode == 0 ? b.counters[kCntExtend + bounce] : b.counters[kCntShadow + bounce];
+    uint32_t* fetch = b.counters + (kMode == 0 ? kCntFetchExt : kCntFetchSh) + bounce;
+    Work w{0, 0, 0, 0};
+    const uint32_t* q = b.queue[bounce & 1];
+    while (true) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        const uint32_t idx = base + lane_id();
+        if (idx < count) {
+            if (kMode == 0) {
+                const uint32_t p = q[idx];
+                const V3 o = xyz(b.ray_o[p]), d = xyz(b.ray_d[p]);
+                Hit h;
+                trace_one<kCount>(S, o, d, h, st, w);
+                b.hit_inst[p] = h.inst;
+                b.hit_tri[p] = h.tri;
+                b.hit_uv[p] = make_float2(h.u, h.v);
+                b.hit_pos[p] = f4(h.wpos, h.dist);
+            } else {
+                const float4 so = b.sh_o[idx], sd = b.sh_d[idx];
+                Hit h;
+                trace_one<kCount>(S, xyz(so), xyz(sd), h, st, w);
+                if (h.inst < 0 || h.dist > so.w) {  // Tracer.cpp:322
+                    const uint32_t p = __float_as_uint(sd.w);
+                    const float4 cc = b.sh_c[idx];
+                    float4* acc = (__float_as_uint(cc.w) == 0u) ? b.acc_d : b.acc_i;
+                    const float4 a = acc[p];
+                    acc[p] = make_float4(a.x + cc.x, a.y + cc.y, a.z + cc.z, 0.0f);
+                }
+            }
+        }
+    }
+    if (kCount) {
+        wave_add_u64(b.work + 0, w.aabb);
+        wave_add_u64(b.work + 1, w.enter);
+        wave_add_u64(b.work + 2, w.pops);
+        wave_add_u64(b.work + 3, w.tris);
+    }
+}
+
+__device__ void eval_material(const DevScene& S, const DevMaterial& m, float u, float v, V3& base,
+                              float& metal, float& rough, V3& ntex, V3& emission) {
+    // Material::SampleMaterial (Material.hpp:82-135)
+    base = V3{m.base[0], m.base[1], m.base[2]};
+    if (m.tex[0].image >= 0) {
+        const float4 t = sample_tex(S, m.tex[0], u, v);
+        base = V3{t.x * m.base[0], t.y * m.base[1], t.z * m.base[2]};
+    }
+    if (m.tex[1].image >= 0) {
+        const float4 t = sample_tex(S, m.tex[1], u, v);
+        metal = t.z * m.metallic;
+        rough = t.y * m.roughness;
+    } else {
+        metal = m.metallic;
+        rough = m.roughness;
+    }
+    if (m.tex[2].image >= 0) {
+        const float4 t = sample_tex(S, m.tex[2], u, v);
+        const float ns = m.normal_scale;
+        ntex = normalize(V3{t.x * ns * 2.0f - 1.0f, t.y * ns * 2.0f - 1.0f, t.z * ns * 2.0f - 1.0f});
+    } else {
+        ntex = V3{0.0f, 0.0f, 1.0f};
+    }
+    const V3 em = {m.emissive[0], m.emissive[1], m.emissive[2]};
+    if (m.tex[3].image >= 0) {
+        const float4 t = sample_tex(S, m.tex[3], u, v);
+        emission = mul(V3{t.x, t.y, t.z}, em);
+    } else {
+        emission = em;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, PathBuffers b,
+                                                  uint32_t k) {
+    const uint32_t count = b.counters[kCntExtend + k];
+    const uint32_t* qin = b.queue[k & 1];
+    uint32_t* qout = b.queue[(k + 1) & 1];
+    const bool last = (k + 1 >= c.bounces);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t hits = 0, misses = 0;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t idx = base + threadIdx.x;
+        const bool valid = idx < count;
+        bool cont = false, shadow = false;
+        float4 sho{}, shd{}, shc{};
+        uint32_t p = 0;
+        if (valid) {
+            p = qin[idx];
+            const V3 d = xyz(b.ray_d[p]);
+            const int inst = b.hit_inst[p];
+            const V3 trad = xyz(b.t_rad[p]);
+            if (inst < 0) {  // miss: sky term (Tracer.cpp:244-252)
+                misses++;
+                const V3 col = (k > 0) ? xyz(b.prev_base[p]) : mk(1.0f);
+                const float mult = (k == 0) ? c.hdri_primary : c.hdri_secondary;
+                const V3 sk = scale(sample_env(S, d, c.hdri_rot_apply, c.hdri_rot_offset), mult);
+                b.acc_s[p] = f4(mul(mul(col, trad), sk), 0.0f);
+            } else {
+                hits++;
+                uint32_t pix, s;
+                path_coords(c, p, pix, s);
+                const uint32_t key = path_key(c.seed_key, pix, s);
+                const uint32_t dim0 = 2u + k * c.light_stride;
+                const V3 o = xyz(b.ray_o[p]);
+                const float4 hp = b.hit_pos[p];
+                const V3 wpos = xyz(hp);
+                const float dist = hp.w;
+                const float2 huv = b.hit_uv[p];
+                const uint32_t tri = b.hit_tri[p];
+                const DevInstance* I = S.instances + inst;
+                const DevMesh m = S.meshes[I->mesh];
+                // RayHitGeometryInfo (Ray.cpp:7-48)
+                const float w = 1.0f - huv.x - huv.y;
+                const float* nn = S.normals + 9ull * (m.attr_base + tri);
+                const V3 n0 = {nn[0], nn[1], nn[2]}, n1 = {nn[3], nn[4], nn[5]}, n2 = {nn[6], nn[7], nn[8]};
+                const V3 ni = normalize(add(add(scale(n0, w), scale(n1, huv.x)), scale(n2, huv.y)));
+                float M[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) M[j] = I->M[j];
+                const V3 gN = normalize(xform(M, ni, 0.0f));
+                float tu = 0.0f, tv = 0.0f;
+                if (m.flags & kMeshHasUv) {
+                    const float* uv = S.uvs + 6ull * (m.uv_base + tri);
+                    tu = (w * uv[0] + huv.x * uv[2]) + huv.y * uv[4];
+                    tv = (w * uv[1] + huv.x * uv[3]) + huv.y * uv[5];
+                }
+                V3 base, ntex, emission;
+                float metal, rough;
+                eval_material(S, S.materials[I->material], tu, tv, base, metal, rough, ntex, emission);
+                base = add(scale(base, 0.96f), mk(0.04f));  // Tracer.cpp:159
+                const V3 N = shading_normal(gN, ntex);
+                const V3 view = neg(d);
+                // BRDF sampling (Tracer.cpp:177-200)
+                const float ps = spec_probability(view, N, base, metal);
+                V3 nd;
+                if (draw(key, dim0 + kDimLobe) < ps) {
+                    const float u1 = draw(key, dim0 + kDimDir), u2 = draw(key, dim0 + kDimDir + 1);
+                    if (!ggx_dir(u1, u2, d, N, rough, nd))
+                        nd = cosine_dir(draw(key, dim0 + kDimFallback), draw(key, dim0 + kDimFallback + 1), N);
+                } else {
+                    nd = cosine_dir(draw(key, dim0 + kDimDir), draw(key, dim0 + kDimDir + 1), N);
+                }
+                const float pdf = (1.0f - ps) * diffuse_pdf(nd, N) + ps * specular_pdf(nd, view, N, rough);
+                const V3 fb = eval_brdf(nd, view, N, base, rough, metal);
+                const float nl = gmax(dot(N, nd), 0.0f);
+                V3 energy = (pdf > 0.0f) ? scale(fb, nl / pdf) : mk(0.0f);
+                // termination bookkeeping of GeneratePath (Tracer.cpp:206-208)
+                V3 tterm = mul(xyz(b.t_term[p]), energy);
+                bool stop;
+                if (c.mode == MT_MODE_FIXED) {
+                    stop = false;
+                    if (k + 1 >= c.rr_depth) {
+                        const float q = clampf(gmax(gmax(tterm.x, tterm.y), tterm.z), 0.05f, 1.0f);
+                        if (draw(key, dim0 + kDimRR) >= q) {
+                            stop = true;
+                        } else {
+                            energy = divs(energy, q);
+                            tterm = divs(tterm, q);
+                        }
+                    }
+                } else {
+                    stop = len(tterm) < c.terminate_energy;
+                }
+                // ProcessRayForward for this step (Tracer.cpp:254-339)
+                {
+                    const float4 ae = b.acc_e[p];
+                    const V3 e = mul(scale(trad, c.emission_base), emission);
+                    b.acc_e[p] = make_float4(ae.x + e.x, ae.y + e.y, ae.z + e.z, 0.0f);
+                }
+                int chosen = -1;
+                float chosen_pdf = 0.0f, total = 0.0f;
+                for (uint32_t li = 0; li < S.light_count; ++li) {
+                    const DevLight* L = S.lights + li;
+                    const V3 lp = sub(V3{L->pos[0], L->pos[1], L->pos[2]}, wpos);
+                    const float score = L->intensity / (dot(lp, lp) + 1.0f);
+                    total += score;
+                    if (draw(key, dim0 + kDimLightSel + li) * total < score) {
+                        chosen = static_cast<int>(li);
+                        chosen_pdf = score;
+                    }
+                }
+                chosen_pdf /= total;
+                V3 trad_next = trad;
+                if (chosen >= 0) {
+                    const DevLight L = S.lights[chosen];
+                    V3 lpos = {L.pos[0], L.pos[1], L.pos[2]};
+                    if (L.type == MT_LIGHT_POINT || L.type == MT_LIGHT_SPOT) {  // Tracer.cpp:84-93
+                        const float ph = draw(key, dim0 + kDimLightPt) * (2.0f * kPi - 0.0f) + 0.0f;
+                        const float th = draw(key, dim0 + kDimLightPt + 1) * (kPi * 0.5f - (-kPi * 0.5f)) + (-kPi * 0.5f);
+                        const float ct = cosf(th);
+                        const V3 rd = normalize(V3{cosf(ph) * ct, sinf(th), sinf(ph) * ct});
+                        lpos = add(lpos, scale(rd, L.size));
+                    }
+                    const V3 lvec = sub(lpos, wpos);
+                    const V3 ldir = normalize(lvec);
+                    V3 lcol = {L.color[0], L.color[1], L.color[2]};
+                    if (L.type == MT_LIGHT_SPOT) {  // Light.hpp:58-63
+                        const float dd = dot(neg(ldir), V3{L.dir[0], L.dir[1], L.dir[2]});
+                        lcol = scale(lcol, clampf((dd - L.outer) / (L.inner - L.outer), 0.0f, 1.0f));
+                    }
+                    const float llen = len(lvec);
+                    const V3 fl = eval_brdf(ldir, view, N, base, rough, metal);
+                    const float nll = gmax(dot(N, ldir), 0.0f);
+                    const float fall = 1.0f / (llen * llen + 1.0f);
+                    const V3 le = scale(scale(fl, nll), fall);
+                    const V3 contrib = divs(mul(mul(trad, le), lcol), chosen_pdf);
+                    shadow = true;
+                    sho = f4(add(wpos, scale(lvec, kPullout)), llen);
+                    shd = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(p));
+                    shc = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(k == 0 ? 0u : 1u));
+                    trad_next = mul(trad, energy);
+                }
+                if (!stop && !last) {
+                    cont = true;
+                    const V3 no = add(add(o, scale(d, dist)), scale(N, kPullout));
+                    b.ray_o[p] = f4(no, 0.0f);
+                    b.ray_d[p] = f4(nd, 0.0f);
+                    b.t_rad[p] = f4(trad_next, 0.0f);
+                    b.t_term[p] = f4(tterm, 0.0f);
+                    b.prev_base[p] = f4(base, 0.0f);
+                }
+            }
+        }
+        const uint32_t ss = wave_push(b.counters + kCntShadow + k, shadow);
+        if (shadow) {
+            b.sh_o[ss] = sho;
+            b.sh_d[ss] = shd;
+            b.sh_c[ss] = shc;
+        }
+        const uint32_t cs = wave_push(b.counters + kCntExtend + k + 1, cont);
+        if (cont) qout[cs] = p;
+    }
+    if (c.count_work) {
+        wave_add_u64(b.work + 4, hits);
+        wave_add_u64(b.work + 5, misses);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= c.paths) return;
+    const V3 e = xyz(b.acc_e[p]), d = xyz(b.acc_d[p]), i = xyz(b.acc_i[p]), s = xyz(b.acc_s[p]);
+    const V3 L = add(add(add(e, d), i), s);  // Tracer.cpp:344-345
+    const V3 col = c.output_linear ? L : aces(L, c.exposure);
+    b.sample_rgb[p] = f4(col, 0.0f);
+}
+
+// per-pixel mean of the samples in sample order, with CleanupFireflies (Tracer.cpp:490-548)
+__global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b, float* img) {
+    const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lp >= c.npix) return;
+    const uint32_t n = c.spp;
+    const float4* sp = b.sample_rgb + lp;
+    uint32_t cnt = n;
+    float avg = 0.0f, thr = 0.0f;
+    if (c.firefly) {
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < n; ++s) {
+            const float4 v = sp[static_cast<size_t>(s) * c.npix];
+            sum = sum + (v.x + v.y + v.z) / 3.0f;
+        }
+        avg = sum / static_cast<float>(n);
+        float var = 0.0f;
+        for (uint32_t s = 0; s < n; ++s) {
+            const float4 v = sp[static_cast<size_t>(s) * c.npix];
+            const float dd = (v.x + v.y + v.z) / 3.0f - avg;
+            var = var + dd * dd;
+        }
+        var = var / static_cast<float>(n);
+        thr = c.firefly_threshold * __builtin_sqrtf(var);
+    }
+    V3 acc = mk(0.0f);
+    for (uint32_t s = 0; s < n; ++s) {
+        const float4 v = sp[static_cast<size_t>(s) * c.npix];
+        V3 col = xyz(v);
+        if (c.firefly && fabsf((v.x + v.y + v.z) / 3.0f - avg) > thr) {
+            col = mk(0.0f);
+            cnt--;
+        }
+        acc = add(acc, col);
+    }
+    if (cnt == 0) cnt = 1;
+    const V3 r = divs(acc, static_cast<float>(cnt));
+    const uint32_t pix = c.pix_begin + lp;
+    img[pix * 3 + 0] = r.x;
+    img[pix * 3 + 1] = r.y;
+    img[pix * 3 + 2] = r.z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, uint32_t n, const float* rays,
+                                                        mt_hit* out, uint2* spill, uint32_t cap,
+                                                        uint32_t* err) {
+    __shared__ uint2 lds_stack[kLdsDepth * kBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Stack st{lds_stack + threadIdx.x, kBlock, kLdsDepth, spill + i, n, cap, err, 0};
+    const V3 o = {rays[i * 6 + 0], rays[i * 6 + 1], rays[i * 6 + 2]};
+    const V3 d = {rays[i * 6 + 3], rays[i * 6 + 4], rays[i * 6 + 5]};
+    Hit h;
+    Work w{0, 0, 0, 0};
+    trace<false>(S, o, d, h, st, w);
+    mt_hit r;
+    r.instance = h.inst;
+    r.triangle = h.tri;
+    r.bv_index = h.bv;
+    r.u = h.u;
+    r.v = h.v;
+    r.distance = h.inst >= 0 ? h.dist : 0.0f;
+    r.world_pos[0] = h.inst >= 0 ? h.wpos.x : 0.0f;
+    r.world_pos[1] = h.inst >= 0 ? h.wpos.y : 0.0f;
+    r.world_pos[2] = h.inst >= 0 ? h.wpos.z : 0.0f;
+    r.bvh_tests = h.bvh_tests;
+    r.triangle_tests = h.tri_tests;
+    out[i] = r;
+}
+
+__global__ void k_brdf(uint32_t n, const float* in, float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = in + i * 14;
+    const V3 L = {q[0], q[1], q[2]}, V = {q[3], q[4], q[5]}, N = {q[6], q[7], q[8]};
+    const V3 base = {q[9], q[10], q[11]};
+    const V3 f = eval_brdf(L, V, N, base, q[12], q[13]);
+    float* o = out + i * 5;
+    o[0] = f.x;
+    o[1] = f.y;
+    o[2] = f.z;
+    o[3] = diffuse_pdf(L, N);
+    o[4] = specular_pdf(L, V, N, q[12]);
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(MT_E_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+int device_cu_count() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+template <typename T>
+int alloc(std::vector<void*>& owned, T** p, size_t n) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+        return fail(MT_E_NOMEM, "hipMalloc of path buffers failed");
+    owned.push_back(q);
+    *p = static_cast<T*>(q);
+    return MT_OK;
+}
+
+}  // namespace
+
+uint32_t stack_capacity(const mt_scene* s) { return std::max<uint32_t>(s->max_bvh_depth + 1u, 2u); }
+
+int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits) {
+    if (n == 0) return MT_OK;
+    std::vector<void*> owned;
+    struct Guard {
+        std::vector<void*>& v;
+        ~Guard() {
+            for (void* p : v) (void)hipFree(p);
+        }
+    } guard{owned};
+    float* drays;
+    mt_hit* dhits;
+    uint2* spill;
+    uint32_t* err;
+    const uint32_t cap = stack_capacity(s);
+    const uint32_t spill_depth = cap > kLdsDepth ? cap - kLdsDepth : 1;
+    int rc;
+    if ((rc = alloc(owned, &drays, static_cast<size_t>(n) * 6))) return rc;
+    if ((rc = alloc(owned, &dhits, n))) return rc;
+    if ((rc = alloc(owned, &spill, static_cast<size_t>(spill_depth) * n))) return rc;
+    if ((rc = alloc(owned, &err, 1))) return rc;
+    HIP_TRY(hipMemcpy(drays, rays, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(err, 0, sizeof(uint32_t)));
+    hipLaunchKernelGGL(k_trace_rays, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, 0, s->dev, n,
+                       drays, dhits, spill, cap, err);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    uint32_t herr = 0;
+    HIP_TRY(hipMemcpy(&herr, err, sizeof(herr), hipMemcpyDeviceToHost));
+    if (herr) return fail(MT_E_INVALID, "traversal stack overflow");
+    HIP_TRY(hipMemcpy(hits, dhits, sizeof(mt_hit) * n, hipMemcpyDeviceToHost));
+    return MT_OK;
+}
+
+int brdf_eval(uint32_t n, const float* in, float* out) {
+    if (n == 0) return MT_OK;
+    std::vector<void*> owned;
+    struct Guard {
+        std::vector<void*>& v;
+        ~Guard() {
+            for (void* p : v) (void)hipFree(p);
+        }
+    } guard{owned};
+    float *din, *dout;
+    int rc;
+    if ((rc = alloc(owned, &din, static_cast<size_t>(n) * 14))) return rc;
+    if ((rc = alloc(owned, &dout, static_cast<size_t>(n) * 5))) return rc;
+    HIP_TRY(hipMemcpy(din, in, sizeof(float) * 14 * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_brdf, dim3((n + 255) / 256), dim3(256), 0, 0, n, din, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, dout, sizeof(float) * 5 * n, hipMemcpyDeviceToHost));
+    return MT_OK;
+}
+
+// ------------------------------------------------------------------ render driver
+
+Renderer::~Renderer() { release(); }
+
+void Renderer::release() {
+    for (void* p : owned_) (void)hipFree(p);
+    owned_.clear();
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+    events_.clear();
+    if (stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
+    if (host_counters_) (void)hipHostFree(host_counters_);
+    host_counters_ = nullptr;
+}
+
+int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st) {
+    scene_ = scene;
+    cam_ = cam;
+    st_ = st;
+    if (st.width == 0 || st.height == 0 || st.samples == 0)
+        return fail(MT_E_INVALID, "width, height and samples must be positive");
+    if (st.bounces > kMaxBounces) return fail(MT_E_UNSUPPORTED, "bounces above 64 are not supported");
+    const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
+    if (npix_total * 3 > 0xFFFFFFFFull) return fail(MT_E_UNSUPPORTED, "image too large");
+    uint64_t target = st.paths_in_flight ? st.paths_in_flight : (1ull << 22);
+    target = std::max<uint64_t>(target, st.samples);
+    target = std::min<uint64_t>(target, 1ull << 26);
+    npix_chunk_ = static_cast<uint32_t>(std::max<uint64_t>(1, target / st.samples));
+    npix_chunk_ = static_cast<uint32_t>(std::min<uint64_t>(npix_chunk_, npix_total));
+    const uint64_t P = static_cast<uint64_t>(npix_chunk_) * st.samples;
+    if (P > 0xFFFFFFFFull / 2) return fail(MT_E_UNSUPPORTED, "too many paths in flight");
+    capacity_ = static_cast<uint32_t>(P);
+    chunks_ = static_cast<uint32_t>((npix_total + npix_chunk_ - 1) / npix_chunk_);
+
+    HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    const int cus = device_cu_count();
+    trace_blocks_ = static_cast<uint32_t>(cus * 8);
+    shade_blocks_ = static_cast<uint32_t>(std::min<uint64_t>((P + kBlock - 1) / kBlock, cus * 16ull));
+    const uint32_t cap = stack_capacity(scene);
+    const uint32_t spill_depth = cap > kLdsDepth ? cap - kLdsDepth : 1;
+    const size_t trace_threads = static_cast<size_t>(trace_blocks_) * kBlock;
+    PathBuffers& b = buf_;
+    int rc;
+    if ((rc = alloc(owned_, &b.ray_o, P)) || (rc = alloc(owned_, &b.ray_d, P)) ||
+        (rc = alloc(owned_, &b.t_rad, P)) || (rc = alloc(owned_, &b.t_term, P)) ||
+        (rc = alloc(owned_, &b.prev_base, P)) || (rc = alloc(owned_, &b.acc_e, P)) ||
+        (rc = alloc(owned_, &b.acc_d, P)) || (rc = alloc(owned_, &b.acc_i, P)) ||
+        (rc = alloc(owned_, &b.acc_s, P)) || (rc = alloc(owned_, &b.hit_inst, P)) ||
+        (rc = alloc(owned_, &b.hit_tri, P)) || (rc = alloc(owned_, &b.hit_uv, P)) ||
+        (rc = alloc(owned_, &b.hit_pos, P)) || (rc = alloc(owned_, &b.queue[0], P)) ||
+        (rc = alloc(owned_, &b.queue[1], P)) || (rc = alloc(owned_, &b.sh_o, P)) ||
+        (rc = alloc(owned_, &b.sh_d, P)) || (rc = alloc(owned_, &b.sh_c, P)) ||
+        (rc = alloc(owned_, &b.sample_rgb, P)) ||
+        (rc = alloc(owned_, &b.spill, spill_depth * trace_threads)) ||
+        (rc = alloc(owned_, &b.counters, kMaxCounters)) || (rc = alloc(owned_, &b.work, 8)) ||
+        (rc = alloc(owned_, &img_, npix_total * 3)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(b.work, 0, sizeof(unsigned long long) * 8, stream_));
+    HIP_TRY(hipMemsetAsync(img_, 0, sizeof(float) * npix_total * 3, stream_));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_counters_), sizeof(uint32_t) * kMaxCounters));
+
+    ChunkParams& c = params_;
+    std::memset(&c, 0, sizeof(c));
+    c.width = st.width;
+    c.spp = st.samples;
+    c.bounces = st.bounces;
+    c.px_w = 1.0f / static_cast<float>(st.width);
+    c.px_h = 1.0f / static_cast<float>(st.height);
+    c.aspect = static_cast<float>(st.width) / static_cast<float>(st.height);
+    c.cam_type = cam.type;
+    std::memcpy(c.cam, cam.camera_to_world, sizeof(c.cam));
+    c.fov_scale = std::tan(cam.yfov * 0.5f);
+    c.ortho_x = cam.ortho_size[0];
+    c.ortho_y = cam.ortho_size[1];
+    c.cam_aspect = cam.aspect;
+    const uint32_t k0 = seed_mix(static_cast<uint32_t>(st.seed) ^ 0xA511E9B3u);
+    c.seed_key = seed_mix(k0 ^ static_cast<uint32_t>(st.seed >> 32) ^ 0x63D83595u);
+    c.light_stride = 8u + scene->light_count;
+    c.mode = st.mode;
+    c.rr_depth = st.rr_depth;
+    c.terminate_energy = st.terminate_energy;
+    c.emission_base = st.emission_base_intensity;
+    c.hdri_primary = st.hdri_primary_intensity;
+    c.hdri_secondary = st.hdri_secondary_intensity;
+    c.hdri_rot_apply = st.hdri_rotation_apply;
+    c.hdri_rot_offset = st.hdri_rotation / 360.0f;
+    c.exposure = std::pow(2.0f, -st.ev_exposure);
+    c.output_linear = st.output == MT_OUTPUT_LINEAR;
+    c.firefly = st.firefly_enable;
+    c.firefly_threshold = st.firefly_threshold;
+    c.count_work = st.count_work;
+    c.stack_cap = cap;
+    for (uint32_t i = 0; i < 2 + 4 * st.bounces; ++i) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        events_.push_back(e);
+    }
+    return MT_OK;
+}
+
+uint32_t seed_mix(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
+    const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
+    ChunkParams c = params_;
+    c.pix_begin = chunk * npix_chunk_;
+    c.npix = static_cast<uint32_t>(std::min<uint64_t>(npix_chunk_, npix_total - c.pix_begin));
+    c.paths = c.npix * st_.samples;
+    const uint32_t gen_blocks = (c.paths + kBlock - 1) / kBlock;
+    hipStream_t s = stream_;
+    // events: [0] chunk start, [1] chunk end, then per bounce 4: before extend, after extend,
+    // after shade, after shadow (all on this stream, read after the chunk's sync)
+    HIP_TRY(hipEventRecord(events_[0], s));
+    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    for (uint32_t k = 0; k < st_.bounces; ++k) {
+        hipEvent_t* ev = &events_[2 + 4 * k];
+        HIP_TRY(hipEventRecord(ev[0], s));
+        if (c.count_work)
+            hipLaunchKernelGGL((k_trace<true, 0>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        else
+            hipLaunchKernelGGL((k_trace<false, 0>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        HIP_TRY(hipEventRecord(ev[1], s));
+        hipLaunchKernelGGL(k_shade, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        HIP_TRY(hipEventRecord(ev[2], s));
+        if (scene_->light_count) {
+            if (c.count_work)
+                hipLaunchKernelGGL((k_trace<true, 1>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+            else
+                hipLaunchKernelGGL((k_trace<false, 1>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        }
+        HIP_TRY(hipEventRecord(ev[3], s));
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    hipLaunchKernelGGL(k_resolve, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, buf_, img_);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(events_[1], s));
+    HIP_TRY(hipMemcpyAsync(host_counters_, buf_.counters, sizeof(uint32_t) * kMaxCounters,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float total = 0, ms_ext = 0, ms_shade = 0, ms_sh = 0;
+    HIP_TRY(hipEventElapsedTime(&total, events_[0], events_[1]));
+    for (uint32_t k = 0; k < st_.bounces; ++k) {
+        hipEvent_t* ev = &events_[2 + 4 * k];
+        float a = 0, b2 = 0, d = 0;
+        HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b2, ev[1], ev[2]));
+        HIP_TRY(hipEventElapsedTime(&d, ev[2], ev[3]));
+        ms_ext += a;
+        ms_shade += b2;
+        ms_sh += d;
+    }
+    stats_.extend_launches += st_.bounces;
+    if (scene_->light_count) stats_.shadow_launches += st_.bounces;
+    if (host_counters_[kCntError]) return fail(MT_E_INVALID, "traversal stack overflow");
+    uint64_t rays = 0;
+    for (uint32_t k = 0; k < st_.bounces; ++k)
+        rays += static_cast<uint64_t>(host_counters_[kCntExtend + k]) + host_counters_[kCntShadow + k];
+    stats_.rays += rays;
+    stats_.samples += c.paths;
+    stats_.kernel_ms_total += total;
+    stats_.kernel_ms_extend += ms_ext;
+    stats_.kernel_ms_shade += ms_shade;
+    stats_.kernel_ms_shadow += ms_sh;
+    if (sample_rgb_out) {
+        std::vector<float4> tmp(c.paths);
+        HIP_TRY(hipMemcpy(tmp.data(), buf_.sample_rgb, sizeof(float4) * c.paths, hipMemcpyDeviceToHost));
+        for (uint32_t p = 0; p < c.paths; ++p) {
+            const uint32_t smp = p / c.npix, lp = p % c.npix;
+            float* q = sample_rgb_out + (static_cast<size_t>(c.pix_begin + lp) * st_.samples + smp) * 3;
+            q[0] = tmp[p].x;
+            q[1] = tmp[p].y;
+            q[2] = tmp[p].z;
+        }
+    }
+    return MT_OK;
+}
+
+int Renderer::finish(float* rgb_out) {
+    const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
+    if (st_.count_work) {
+        unsigned long long w[8];
+        HIP_TRY(hipMemcpy(w, buf_.work, sizeof(w), hipMemcpyDeviceToHost));
+        stats_.inst_aabb_tests = w[0];
+        stats_.inst_entered = w[1];
+        stats_.node_pops = w[2];
+        stats_.tri_tests = w[3];
+        stats_.hits_shaded = w[4];
+        stats_.misses = w[5];
+    }
+    if (rgb_out) HIP_TRY(hipMemcpy(rgb_out, img_, sizeof(float) * npix_total * 3, hipMemcpyDeviceToHost));
+    return MT_OK;
+}
+
+}  // namespace mt

@@ -1,0 +1,91 @@
+// mt_render.hpp -- render driver of the gfx950 Tracer (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mt_internal.hpp"
+
+namespace mt {
+
+constexpr uint32_t kMaxBounces = 64;
+// layout of the per-chunk device counter array
+constexpr uint32_t kCntExtend = 0;                              // [kMaxBounces + 1] queue sizes
+constexpr uint32_t kCntShadow = kCntExtend + kMaxBounces + 1;   // [kMaxBounces] shadow queue sizes
+constexpr uint32_t kCntFetchExt = kCntShadow + kMaxBounces;     // [kMaxBounces] work fetch
+constexpr uint32_t kCntFetchSh = kCntFetchExt + kMaxBounces;    // [kMaxBounces] work fetch
+constexpr uint32_t kCntError = kCntFetchSh + kMaxBounces;       // traversal stack overflow flag
+constexpr uint32_t kMaxCounters = kCntError + 1;
+
+// per-chunk constants, passed to kernels by value
+struct ChunkParams {
+    uint32_t width, spp, bounces;
+    uint32_t pix_begin, npix, paths;
+    float px_w, px_h, aspect;
+    int32_t cam_type;
+    float cam[16];
+    float fov_scale, ortho_x, ortho_y, cam_aspect;
+    uint32_t seed_key, light_stride;
+    int32_t mode;
+    uint32_t rr_depth;
+    float terminate_energy, emission_base, hdri_primary, hdri_secondary;
+    int32_t hdri_rot_apply;
+    float hdri_rot_offset;
+    float exposure;
+    int32_t output_linear, firefly;
+    float firefly_threshold;
+    int32_t count_work;
+    uint32_t stack_cap;
+};
+
+// SoA path state, queues and scratch of one render (capacity = paths per chunk)
+struct PathBuffers {
+    float4 *ray_o, *ray_d;
+    float4 *t_rad, *t_term, *prev_base;
+    float4 *acc_e, *acc_d, *acc_i, *acc_s;
+    int32_t* hit_inst;
+    uint32_t* hit_tri;
+    float2* hit_uv;
+    float4* hit_pos;  // world position, world distance
+    uint32_t* queue[2];
+    float4 *sh_o, *sh_d, *sh_c;  // shadow rays: {o, max dist} {d, path} {contribution, slot}
+    float4* sample_rgb;
+    uint2* spill;
+    uint32_t* counters;
+    unsigned long long* work;  // aabb, entered, pops, tris, hits, misses
+};
+
+class Renderer {
+   public:
+    ~Renderer();
+    int setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st);
+    int run_chunk(uint32_t chunk, float* sample_rgb_out);
+    int finish(float* rgb_out);
+    uint32_t chunks() const { return chunks_; }
+    const mt_stats& stats() const { return stats_; }
+    mt_stats& stats() { return stats_; }
+
+   private:
+    void release();
+    mt_scene* scene_ = nullptr;
+    mt_camera_desc cam_{};
+    mt_settings st_{};
+    ChunkParams params_{};
+    PathBuffers buf_{};
+    std::vector<void*> owned_;
+    std::vector<hipEvent_t> events_;
+    hipStream_t stream_ = nullptr;
+    uint32_t* host_counters_ = nullptr;
+    float* img_ = nullptr;
+    uint32_t capacity_ = 0, npix_chunk_ = 0, chunks_ = 0;
+    uint32_t trace_blocks_ = 0, shade_blocks_ = 0;
+    mt_stats stats_{};
+};
+
+uint32_t seed_mix(uint32_t x);
+uint32_t stack_capacity(const mt_scene* s);
+int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits);
+int brdf_eval(uint32_t n, const float* in, float* out);
+
+}  // namespace mt

@@ -561,14 +561,16 @@ RGBA sampleImage(const Image& im, float uvx, float uvy, int filter) {  // Image.
         int iy = static_cast<int>(y * static_cast<float>(im.h)) % im.h;
         if (ix < 0) ix += im.w;
         if (iy < 0) iy += im.h;
+        ix = std::max(ix, 0);  // a NaN coordinate converts to INT_MIN on x86
+        iy = std::max(iy, 0);
         return texel(im, ix, iy);
     }
     // "linear" with weights in uv units (Image.cpp:26-44, Q20); x0/y0 are clamped to the last
     // texel where the reference would read past the row (x*W rounding up to W).
     int x0 = static_cast<int>(std::floor(x * static_cast<float>(im.w)));
     int y0 = static_cast<int>(std::floor(y * static_cast<float>(im.h)));
-    x0 = std::min(x0, im.w - 1);
-    y0 = std::min(y0, im.h - 1);
+    x0 = std::max(std::min(x0, im.w - 1), 0);
+    y0 = std::max(std::min(y0, im.h - 1), 0);
     const int x1 = std::min(x0 + 1, im.w - 1);
     const int y1 = std::min(y0 + 1, im.h - 1);
     const float sx = x - static_cast<float>(x0) / static_cast<float>(im.w);
