@@ -214,6 +214,10 @@ typedef struct mt_stats {
     float kernel_ms_total;   /* summed HIP-event time of all render launches   */
     uint32_t extend_launches;
     uint32_t shadow_launches;
+    /* traversal diagnostics, filled when count_work != 0: [0] instance-culling rounds,
+     * lane steps [1] set-up/finish [2] culled pop [3] inner node [4] leaf, wave iterations
+     * [5] all [6] with a set-up lane [7] with an inner-node lane [8] with a leaf lane */
+    uint64_t diag[9];
 } mt_stats;
 
 /* One closest-hit record (Tracer::RayHit, src/Tracer/Ray.hpp:20-39). */

@@ -116,7 +116,8 @@ class mt_stats(C.Structure):
                 ("hits_shaded", C.c_uint64), ("misses", C.c_uint64),
                 ("kernel_ms_extend", C.c_float), ("kernel_ms_shadow", C.c_float),
                 ("kernel_ms_shade", C.c_float), ("kernel_ms_total", C.c_float),
-                ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32)]
+                ("extend_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
+                ("diag", C.c_uint64 * 9)]
 
 
 class mt_hit(C.Structure):

@@ -299,6 +299,9 @@ struct Hit {
 
 struct Work {
     uint32_t aabb, enter, pops, tris;
+    // traversal diagnostics (counting kernels only): lane steps by kind, wave-level iterations
+    // and iterations in which any lane took each kind of step
+    uint32_t st_setup, st_cull, st_inner, st_leaf, it_wave, it_setup, it_inner, it_leaf, it_mask;
 };
 
 // Traversal stack: entries [0, lds_depth) in LDS (lds[i * lds_stride]), deeper entries in a
@@ -330,142 +333,183 @@ struct Stack {
     }
 };
 
-// IntersectScene (Intersect.cpp:201-219) with IntersectMeshInstance (143-199),
-// IntersectTrianglesBVH (47-141) and IntersectTrianglesLinear (20-45).  Instances are
-// culled by their world AABB in a wave-uniform pass over groups of 64; each lane then walks
-// its own entered instances in index order inside one loop, so lanes on small instances do
-// not wait for lanes on large ones.  Per-instance results and the strict '<' across
-// instances in index order are those of the reference.
+// Per-lane state of the closest-hit engine: IntersectScene (Intersect.cpp:201-219) over
+// IntersectMeshInstance (143-199), IntersectTrianglesBVH (47-141) and
+// IntersectTrianglesLinear (20-45).  Instances are culled by their world AABB in groups of
+// 64 (instance_mask); a lane then walks its entered instances in index order.  Work is split
+// into small steps -- instance set-up/finish, one node pop, one triangle test -- so the
+// kernel can batch lanes that need the same kind of step and retire/refill lanes
+// independently.  Each lane performs exactly the reference's sequence of node pops and
+// triangle tests; the per-instance results and the strict '<' across instances in index
+// order are those of the reference.
+constexpr uint32_t kNoLeaf = 0xFFFFFFFFu;
+enum : uint32_t { kStepSetup = 0, kStepCull = 1, kStepInner = 2, kStepLeaf = 3 };
+
+struct Trav {
+    V3 o, d, invd;
+    Hit R;
+    uint64_t mask;
+    uint32_t cb;   // first instance of the current group of 64
+    int cur;       // instance (within the group) being traversed, -1 = none
+    uint32_t leaf; // next prim slot of the leaf being tested, kNoLeaf = none
+    V3 lo, ld, linv;
+    float ldist, bu, bvv;
+    uint32_t btri, bbv, btests, ttests, pair_base, prim_base;
+};
+
+__device__ __forceinline__ void trav_begin(Trav& t, V3 o, V3 d) {
+    t.o = o;
+    t.d = d;
+    t.invd = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    t.R.inst = -1;
+    t.R.dist = kFltMax;
+    t.R.tri = t.R.bv = 0;
+    t.R.u = t.R.v = 0.0f;
+    t.R.wpos = mk(0.0f);
+    t.R.bvh_tests = t.R.tri_tests = 0;
+    t.mask = 0;
+    t.cb = 0;
+    t.cur = -1;
+    t.leaf = kNoLeaf;
+}
+
+// world-AABB test of instances [cb, cb + 64) (Intersect.cpp:147); cb must be wave-uniform so
+// the instance records are read with scalar loads
 template <bool kCount>
-__device__ void trace(const DevScene& S, V3 o, V3 d, Hit& R, Stack& st, Work& w) {
-    const V3 invd = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    R.inst = -1;
-    R.dist = kFltMax;
-    R.tri = R.bv = 0;
-    R.u = R.v = 0.0f;
-    R.bvh_tests = R.tri_tests = 0;
-    for (uint32_t cb = 0; cb < S.instance_count; cb += 64) {
-        const uint32_t nch = (S.instance_count - cb) < 64u ? (S.instance_count - cb) : 64u;
-        uint64_t mask = 0;
-        for (uint32_t j = 0; j < nch; ++j) {
-            const DevInstance* I = S.instances + cb + j;
-            const V3 mn = {I->wmin[0], I->wmin[1], I->wmin[2]};
-            const V3 mx = {I->wmax[0], I->wmax[1], I->wmax[2]};
-            if (slab(o, invd, mn, mx) >= 0.0f) mask |= (1ull << j);
+__device__ __forceinline__ uint64_t instance_mask(const DevScene& S, uint32_t cb, V3 o, V3 invd,
+                                                  Work& w) {
+    const uint32_t nch = (S.instance_count - cb) < 64u ? (S.instance_count - cb) : 64u;
+    uint64_t mask = 0;
+    for (uint32_t j = 0; j < nch; ++j) {
+        const DevInstance* I = S.instances + cb + j;
+        const V3 mn = {I->wmin[0], I->wmin[1], I->wmin[2]};
+        const V3 mx = {I->wmax[0], I->wmax[1], I->wmax[2]};
+        if (slab(o, invd, mn, mx) >= 0.0f) mask |= (1ull << j);
+    }
+    if (kCount) w.aabb += nch;
+    return mask;
+}
+
+// The lane needs a set-up step: its stack is empty and no leaf is open.
+__device__ __forceinline__ bool trav_needs_setup(const Trav& t, const Stack& st) {
+    return st.sp == 0 && t.leaf == kNoLeaf;
+}
+
+// Set-up step: finish the current instance (combine its hit), then start the next entered
+// instance.  Returns true when the ray is finished; sets need_mask when the lane moved on to
+// the next group of 64 instances.
+template <bool kCount>
+__device__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool& need_mask) {
+    if (t.cur >= 0) {
+        if (kCount) {
+            w.pops += t.btests;
+            w.tris += t.ttests;
         }
-        if (kCount) w.aabb += nch;
-        int cur = -1;
-        V3 lo{}, ld{}, linv{};
-        float ldist = kFltMax, bu = 0.0f, bvv = 0.0f;
-        uint32_t btri = 0, bbv = 0, btests = 0, ttests = 0;
-        uint32_t node_base = 0, prim_base = 0, prim_count = 0;
-        st.sp = 0;
-        while (true) {
-            if (st.sp == 0) {
-                if (cur >= 0) {
-                    if (kCount) {
-                        w.pops += btests;
-                        w.tris += ttests;
-                    }
-                    if (ldist < kFltMax) {
-                        const float* M = S.instances[cb + cur].M;
-                        const V3 lp = add(lo, scale(ld, ldist));
-                        const V3 wp = xform(M, lp, 1.0f);
-                        const float wd = len(sub(wp, o));
-                        if (wd < R.dist) {
-                            R.dist = wd;
-                            R.inst = static_cast<int>(cb) + cur;
-                            R.tri = btri;
-                            R.bv = bbv;
-                            R.u = bu;
-                            R.v = bvv;
-                            R.wpos = wp;
-                            R.bvh_tests = btests;
-                            R.tri_tests = ttests;
-                        }
-                    }
-                }
-                if (mask == 0) break;
-                cur = __builtin_ctzll(mask);
-                mask &= mask - 1;
-                const DevInstance* I = S.instances + cb + cur;
-                float inv[16];
+        if (t.ldist < kFltMax) {
+            const float* M = S.instances[t.cb + t.cur].M;
+            const V3 lp = add(t.lo, scale(t.ld, t.ldist));
+            const V3 wp = xform(M, lp, 1.0f);
+            const float wd = len(sub(wp, t.o));
+            if (wd < t.R.dist) {
+                t.R.dist = wd;
+                t.R.inst = static_cast<int>(t.cb) + t.cur;
+                t.R.tri = t.btri;
+                t.R.bv = t.bbv;
+                t.R.u = t.bu;
+                t.R.v = t.bvv;
+                t.R.wpos = wp;
+                t.R.bvh_tests = t.btests;
+                t.R.tri_tests = t.ttests;
+            }
+        }
+        t.cur = -1;
+    }
+    if (t.mask == 0) {
+        if (t.cb + 64u < S.instance_count) {
+            t.cb += 64u;
+            need_mask = true;
+            return false;
+        }
+        return true;
+    }
+    t.cur = __builtin_ctzll(t.mask);
+    t.mask &= t.mask - 1;
+    const DevInstance* I = S.instances + t.cb + t.cur;
+    float inv[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) inv[k] = I->inv[k];
-                lo = xform(inv, o, 1.0f);
-                ld = normalize(xform(inv, d, 0.0f));
-                linv = {1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z};
-                ldist = kFltMax;
-                bu = bvv = 0.0f;
-                btri = bbv = btests = ttests = 0;
-                const DevMesh m = S.meshes[I->mesh];
-                node_base = m.node_base;
-                prim_base = m.prim_base;
-                prim_count = m.prim_count;
-                if (kCount) w.enter++;
-                if (m.flags & kMeshHasBvh) {
-                    const float4 a = S.nodes[2 * node_base], b = S.nodes[2 * node_base + 1];
-                    st.push(0u, slab(lo, linv, xyz(a), xyz(b)));
-                } else {
-                    st.push(kLinearTag, 0.0f);
-                }
-                continue;
-            }
-            const uint2 e = st.pop();
-            uint32_t first, count;
-            if (e.x == kLinearTag) {
-                first = 0;
-                count = prim_count;
-            } else {
-                btests++;
-                const float nd = __uint_as_float(e.y);
-                if (nd < 0.0f) continue;
-                if (nd > ldist + kMinDelta) continue;
-                const float4 na = S.nodes[2 * (node_base + e.x)];
-                const float4 nb = S.nodes[2 * (node_base + e.x) + 1];
-                const uint32_t ca = __float_as_uint(na.w), cbi = __float_as_uint(nb.w);
-                if (!(ca & kLeafBit)) {
-                    const float4 a0 = S.nodes[2 * (node_base + ca)], a1 = S.nodes[2 * (node_base + ca) + 1];
-                    const float4 b0 = S.nodes[2 * (node_base + cbi)], b1 = S.nodes[2 * (node_base + cbi) + 1];
-                    const float da = slab(lo, linv, xyz(a0), xyz(a1));
-                    const float db = slab(lo, linv, xyz(b0), xyz(b1));
-                    const bool pa = (da >= 0.0f && da <= ldist + kMinDelta);
-                    const bool pb = (db >= 0.0f && db <= ldist + kMinDelta);
-                    if (pa && pb) {
-                        if (da < db) {
-                            st.push(cbi, db);
-                            st.push(ca, da);
-                        } else {
-                            st.push(ca, da);
-                            st.push(cbi, db);
-                        }
-                    } else if (pa) {
-                        st.push(ca, da);
-                    } else if (pb) {
-                        st.push(cbi, db);
-                    }
-                    continue;
-                }
-                first = ca & ~kLeafBit;
-                count = cbi & ~kLeafBit;
-            }
-            for (uint32_t i = first; i < first + count; ++i) {
-                const float4* P = S.prims + 3ull * (prim_base + i);
-                const float4 p0 = P[0], p1 = P[1], p2 = P[2];
-                float u, v, t;
-                ttests++;
-                if (tri_test(lo, ld, xyz(p0), xyz(p1), xyz(p2), u, v, t)) {
-                    if (t > 0.0f && t < ldist) {
-                        ldist = t;
-                        bu = u;
-                        bvv = v;
-                        bbv = (e.x == kLinearTag) ? 0u : e.x;
-                        btri = __float_as_uint(p0.w);
-                    }
-                }
-            }
+    for (int k = 0; k < 16; ++k) inv[k] = I->inv[k];
+    t.lo = xform(inv, t.o, 1.0f);
+    t.ld = normalize(xform(inv, t.d, 0.0f));
+    t.linv = {1.0f / t.ld.x, 1.0f / t.ld.y, 1.0f / t.ld.z};
+    t.ldist = kFltMax;
+    t.bu = t.bvv = 0.0f;
+    t.btri = t.bbv = t.btests = t.ttests = 0;
+    const DevMesh& m = S.meshes[I->mesh];
+    t.pair_base = m.pair_base;
+    t.prim_base = m.prim_base;
+    if (kCount) w.enter++;
+    if (m.flags & kMeshHasBvh) {
+        const V3 rmn = {m.root_min[0], m.root_min[1], m.root_min[2]};
+        const V3 rmx = {m.root_max[0], m.root_max[1], m.root_max[2]};
+        st.push(m.root_word, slab(t.lo, t.linv, rmn, rmx));
+    } else {
+        t.leaf = 0;  // linear scan: the whole mesh is one leaf, no node pops counted
+    }
+    return false;
+}
+
+// One node pop (Intersect.cpp:67-78, 109-139).  Returns false when the pop was culled.
+template <bool kCount>
+__device__ bool trav_node(const DevScene& S, Trav& t, Stack& st) {
+    const uint2 e = st.pop();
+    t.btests++;
+    const float nd = __uint_as_float(e.y);
+    if (nd < 0.0f) return false;
+    if (nd > t.ldist + kMinDelta) return false;
+    if (e.x & kLeafBit) {
+        t.leaf = e.x & ~kLeafBit;
+        return true;
+    }
+    // the children's boxes and words in one 64-byte record
+    const float4* q = S.pairs + 4ull * (t.pair_base + e.x);
+    const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
+    const float da = slab(t.lo, t.linv, xyz(a0), xyz(a1));
+    const float db = slab(t.lo, t.linv, xyz(b0), xyz(b1));
+    const uint32_t wa = __float_as_uint(a0.w), wb = __float_as_uint(a1.w);
+    const bool pa = (da >= 0.0f && da <= t.ldist + kMinDelta);
+    const bool pb = (db >= 0.0f && db <= t.ldist + kMinDelta);
+    if (pa && pb) {
+        if (da < db) {
+            st.push(wb, db);
+            st.push(wa, da);
+        } else {
+            st.push(wa, da);
+            st.push(wb, db);
+        }
+    } else if (pa) {
+        st.push(wa, da);
+    } else if (pb) {
+        st.push(wb, db);
+    }
+    return true;
+}
+
+// One triangle test of the open leaf (Intersect.cpp:84-105 / 24-44).
+__device__ __forceinline__ void trav_tri(const DevScene& S, Trav& t) {
+    const float4* P = S.prims + 3ull * (t.prim_base + t.leaf);
+    const float4 p0 = P[0], p1 = P[1], p2 = P[2];
+    float u, v, tt;
+    t.ttests++;
+    if (tri_test(t.lo, t.ld, xyz(p0), xyz(p1), xyz(p2), u, v, tt)) {
+        if (tt > 0.0f && tt < t.ldist) {
+            t.ldist = tt;
+            t.bu = u;
+            t.bvv = v;
+            t.bbv = __float_as_uint(p1.w);
+            t.btri = __float_as_uint(p0.w);
         }
     }
+    t.leaf = (__float_as_uint(p2.w) & kPrimLast) ? kNoLeaf : t.leaf + 1;
 }
 
 }  // namespace dev

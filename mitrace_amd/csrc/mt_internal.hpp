@@ -2,13 +2,16 @@
 //
 // HBM layout of a scene (all arrays device-resident, built once by mt_scene_create):
 //   instances  DevInstance[I]  192 B: inverse and model matrices, world AABB, mesh/material ids
-//   meshes     DevMesh[M]       32 B: offsets of the mesh's nodes, leaf-ordered triangles and
-//                                     shading attributes, flags
-//   nodes      float4[2*N]      32 B per BVH node, the reference's node (BVH.hpp:74-119) with its
-//                                     fields interleaved as {min.xyz, a} {max.xyz, b}
-//   prims      float4[3*P]      48 B per leaf slot: {v0.xyz, triangle id} {e1.xyz, 0} {e2.xyz, 0}
-//                                     stored in BVH::indices_ order, so a leaf's triangles are
-//                                     contiguous (duplicates of straddling triangles included)
+//   meshes     DevMesh[M]       64 B: root box and word, offsets of the mesh's pair records,
+//                                     leaf-ordered triangles and shading attributes, flags
+//   pairs      float4[4*Q]      64 B per sibling pair of BVH nodes (the reference keeps siblings
+//                                     adjacent, BVH.cpp:223-259): {A.min, A.word} {A.max, B.word}
+//                                     {B.min, 0} {B.max, 0}; word = child pair index of an inner
+//                                     node, or first leaf slot | kLeafBit; popping a node needs
+//                                     only the record of its children
+//   prims      float4[3*P]      48 B per leaf slot: {v0.xyz, triangle id} {e1.xyz, leaf node}
+//                                     {e2.xyz, last-in-leaf flag}, in BVH::indices_ order, so a
+//                                     leaf's triangles are contiguous (straddling duplicates kept)
 //   normals    float[9*T]       per-corner normals of every triangle (soup order)
 //   uvs        float[6*T]       per-corner TEXCOORD_0 (meshes without UVs have none)
 //   materials  DevMaterial[]    factors and texture slots
@@ -30,6 +33,7 @@ namespace mt {
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kMeshHasBvh = 1u;
 constexpr uint32_t kMeshHasUv = 2u;
+constexpr uint32_t kPrimLast = 1u;  // prims[3i+2].w: last slot of its leaf
 
 struct alignas(16) DevInstance {
     float inv[16];  // column-major inverse of the model matrix (glm::inverse, Intersect.cpp:152)
@@ -43,15 +47,19 @@ struct alignas(16) DevInstance {
 static_assert(sizeof(DevInstance) == 192, "DevInstance layout");
 
 struct alignas(16) DevMesh {
-    uint32_t node_base;  // first node (float4 pair) of this mesh in the global node array
+    float root_min[3];
+    uint32_t root_word;  // the root node's word (pair index or leaf slot | kLeafBit)
+    float root_max[3];
+    uint32_t flags;      // kMeshHas*
+    uint32_t pair_base;  // first sibling-pair record of this mesh
     uint32_t prim_base;  // first leaf slot in the global prim array
     uint32_t prim_count; // leaf slots (= triangles for a linear mesh)
-    uint32_t flags;      // kMeshHas*
-    uint32_t attr_base;  // first triangle of this mesh in the normals/uvs arrays
+    uint32_t attr_base;  // first triangle of this mesh in the normals array
     uint32_t tri_count;
     uint32_t uv_base;    // first triangle of this mesh in the uvs array
-    uint32_t pad;
+    uint32_t pad[2];
 };
+static_assert(sizeof(DevMesh) == 64, "DevMesh layout");
 
 struct DevTex {
     int32_t image;  // -1 = none
@@ -88,7 +96,7 @@ struct alignas(16) DevLight {
 struct DevScene {
     const DevInstance* instances;
     const DevMesh* meshes;
-    const float4* nodes;
+    const float4* pairs;
     const float4* prims;
     const float* normals;
     const float* uvs;

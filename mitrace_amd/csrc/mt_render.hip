@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -33,6 +34,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kLdsDepth = 16;
+constexpr uint32_t kRefillMin = 16;  // idle lanes that trigger a refill
+constexpr uint32_t kSetupMin = 8;    // lanes needing an instance set-up that trigger one
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -102,52 +105,162 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
     b.queue[0][p] = p;
 }
 
-template <bool kCount>
-__device__ __forceinline__ void trace_one(const DevScene& S, V3 o, V3 d, Hit& h, Stack& st,
-                                          Work& w) {
-    trace<kCount>(S, o, d, h, st, w);
-}
+// rays of the test hook (mode 2)
+struct TestIO {
+    const float* rays;
+    mt_hit* out;
+    uint32_t n;
+    uint32_t* fetch;
+};
 
-// persistent closest-hit over a queue: mode 0 = extend (queue of path ids), 1 = shadow records
-template <bool kCount, int kMode>
+// Persistent closest-hit kernel.  mode 0: extend (queue of path ids), 1: shadow records,
+// 2: test rays.  A wave keeps 64 lanes busy: whenever at least refill_min lanes are idle it
+// takes that many new work items with one atomic (ballot + prefix for the lane slots), runs
+// the instance culling for them together, and every lane then advances its own ray by one
+// unit of work per loop iteration.  Waves leave the loop when the queue is drained.
+template <bool kCount, int kMode, uint32_t kDepth>
 __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, ChunkParams c, PathBuffers b,
-                                                  uint32_t bounce) {
-    __shared__ uint2 lds_stack[kLdsDepth * kBlock];
+                                                  uint32_t bounce, TestIO io) {
+    __shared__ uint2 lds_stack[kDepth * kBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{lds_stack + threadIdx.x, kBlock, kLdsDepth, b.spill + gtid,
+    Stack st{lds_stack + threadIdx.x, kBlock, kDepth, b.spill + gtid,
              gridDim.x * blockDim.x, c.stack_cap, b.counters + kCntError, 0};
-    const uint32_t count = kMode == 0 ? b.counters[kCntExtend + bounce] : b.counters[kCntShadow + bounce];
-    uint32_t* fetch = b.counters + (kMode == 0 ? kCntFetchExt : kCntFetchSh) + bounce;
-    Work w{0, 0, 0, 0};
+    uint32_t count;
+    uint32_t* fetch;
+    if (kMode == 0) {
+        count = b.counters[kCntExtend + bounce];
+        fetch = b.counters + kCntFetchExt + bounce;
+    } else if (kMode == 1) {
+        count = b.counters[kCntShadow + bounce];
+        fetch = b.counters + kCntFetchSh + bounce;
+    } else {
+        count = io.n;
+        fetch = io.fetch;
+    }
     const uint32_t* q = b.queue[bounce & 1];
+    const uint32_t refill_min = c.refill_min;
+    const uint64_t lanes_below = (1ull << lane_id()) - 1ull;
+    Work w{};
+    Trav t;
+    bool active = false, need = false, exhausted = false;
+    uint32_t item = 0;
     while (true) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        const uint32_t idx = base + lane_id();
-        if (idx < count) {
-            if (kMode == 0) {
-                const uint32_t p = q[idx];
-                const V3 o = xyz(b.ray_o[p]), d = xyz(b.ray_d[p]);
-                Hit h;
-                trace_one<kCount>(S, o, d, h, st, w);
-                b.hit_inst[p] = h.inst;
-                b.hit_tri[p] = h.tri;
-                b.hit_uv[p] = make_float2(h.u, h.v);
-                b.hit_pos[p] = f4(h.wpos, h.dist);
-            } else {
-                const float4 so = b.sh_o[idx], sd = b.sh_d[idx];
-                Hit h;
-                trace_one<kCount>(S, xyz(so), xyz(sd), h, st, w);
-                if (h.inst < 0 || h.dist > so.w) {  // Tracer.cpp:322
-                    const uint32_t p = __float_as_uint(sd.w);
-                    const float4 cc = b.sh_c[idx];
-                    float4* acc = (__float_as_uint(cc.w) == 0u) ? b.acc_d : b.acc_i;
-                    const float4 a = acc[p];
-                    acc[p] = make_float4(a.x + cc.x, a.y + cc.y, a.z + cc.z, 0.0f);
+        const uint64_t idle = __ballot(!active);
+        const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
+        if (!exhausted && n_idle >= refill_min) {
+            uint32_t base = 0;
+            if (lane_id() == 0) base = atomicAdd(fetch, n_idle);
+            base = __shfl(base, 0);
+            if (base + n_idle >= count) exhausted = true;
+            if (!active) {
+                const uint32_t my = base + static_cast<uint32_t>(__popcll(idle & lanes_below));
+                if (my < count) {
+                    item = my;
+                    V3 o, d;
+                    if (kMode == 0) {
+                        const uint32_t p = q[my];
+                        o = xyz(b.ray_o[p]);
+                        d = xyz(b.ray_d[p]);
+                    } else if (kMode == 1) {
+                        o = xyz(b.sh_o[my]);
+                        d = xyz(b.sh_d[my]);
+                    } else {
+                        o = V3{io.rays[my * 6 + 0], io.rays[my * 6 + 1], io.rays[my * 6 + 2]};
+                        d = V3{io.rays[my * 6 + 3], io.rays[my * 6 + 4], io.rays[my * 6 + 5]};
+                    }
+                    trav_begin(t, o, d);
+                    active = true;
+                    need = true;
                 }
             }
+        }
+        // instance culling for lanes that start a (group of 64) instances, one group at a time
+        while (true) {
+            const uint64_t nm = __ballot(need);
+            if (nm == 0) break;
+            if (kCount) w.it_mask++;
+            const int l = __ffsll(static_cast<unsigned long long>(nm)) - 1;
+            const uint32_t ucb = __builtin_amdgcn_readfirstlane(__shfl(t.cb, l));
+            if (need && t.cb == ucb) {
+                t.mask = instance_mask<kCount>(S, ucb, t.o, t.invd, w);
+                t.cur = -1;
+                t.leaf = kNoLeaf;
+                st.sp = 0;
+                need = false;
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        // one step per lane; set-up steps (instance finish/start, the costliest branch) run only
+        // when at least setup_min lanes need one or no lane can do anything else
+        const bool wants_setup = active && trav_needs_setup(t, st);
+        const uint32_t n_setup = static_cast<uint32_t>(__popcll(__ballot(wants_setup)));
+        const uint32_t n_active = static_cast<uint32_t>(__popcll(__ballot(active)));
+        const bool run_setup = n_setup >= c.setup_min || n_setup == n_active;
+        uint32_t kind = 0xFFu;
+        bool finished = false;
+        bool nm = false;
+        if (active) {
+            if (wants_setup) {
+                if (run_setup) {
+                    kind = kStepSetup;
+                    finished = trav_setup<kCount>(S, t, st, w, nm);
+                }
+            } else if (t.leaf != kNoLeaf) {
+                kind = kStepLeaf;
+                trav_tri(S, t);
+            } else {
+                kind = trav_node<kCount>(S, t, st) ? kStepInner : kStepCull;
+            }
+        }
+        if (kCount) {
+            w.it_wave++;
+            w.st_setup += kind == kStepSetup;
+            w.st_cull += kind == kStepCull;
+            w.st_inner += kind == kStepInner;
+            w.st_leaf += kind == kStepLeaf;
+            w.it_setup += __ballot(kind == kStepSetup) != 0;
+            w.it_inner += __ballot(kind == kStepInner || kind == kStepCull) != 0;
+            w.it_leaf += __ballot(kind == kStepLeaf) != 0;
+        }
+        if (active) {
+            if (finished) {
+                active = false;
+                const Hit& h = t.R;
+                if (kMode == 0) {
+                    const uint32_t p = q[item];
+                    b.hit_inst[p] = h.inst;
+                    b.hit_tri[p] = h.tri;
+                    b.hit_uv[p] = make_float2(h.u, h.v);
+                    b.hit_pos[p] = f4(h.wpos, h.dist);
+                } else if (kMode == 1) {
+                    const float4 so = b.sh_o[item];
+                    if (h.inst < 0 || h.dist > so.w) {  // Tracer.cpp:322
+                        const uint32_t p = __float_as_uint(b.sh_d[item].w);
+                        const float4 cc = b.sh_c[item];
+                        float4* acc = (__float_as_uint(cc.w) == 0u) ? b.acc_d : b.acc_i;
+                        const float4 a = acc[p];
+                        acc[p] = make_float4(a.x + cc.x, a.y + cc.y, a.z + cc.z, 0.0f);
+                    }
+                } else {
+                    mt_hit r;
+                    r.instance = h.inst;
+                    r.triangle = h.tri;
+                    r.bv_index = h.bv;
+                    r.u = h.u;
+                    r.v = h.v;
+                    r.distance = h.inst >= 0 ? h.dist : 0.0f;
+                    r.world_pos[0] = h.wpos.x;
+                    r.world_pos[1] = h.wpos.y;
+                    r.world_pos[2] = h.wpos.z;
+                    r.bvh_tests = h.bvh_tests;
+                    r.triangle_tests = h.tri_tests;
+                    io.out[item] = r;
+                }
+            }
+            need = nm;
         }
     }
     if (kCount) {
@@ -155,7 +268,38 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, ChunkParams c, Pat
         wave_add_u64(b.work + 1, w.enter);
         wave_add_u64(b.work + 2, w.pops);
         wave_add_u64(b.work + 3, w.tris);
+        wave_add_u64(b.work + 8, w.st_setup);
+        wave_add_u64(b.work + 9, w.st_cull);
+        wave_add_u64(b.work + 10, w.st_inner);
+        wave_add_u64(b.work + 11, w.st_leaf);
+        if (lane_id() == 0) {  // wave-level counts are identical on every lane
+            atomicAdd(b.work + 12, static_cast<unsigned long long>(w.it_wave));
+            atomicAdd(b.work + 13, static_cast<unsigned long long>(w.it_setup));
+            atomicAdd(b.work + 14, static_cast<unsigned long long>(w.it_inner));
+            atomicAdd(b.work + 15, static_cast<unsigned long long>(w.it_leaf));
+            atomicAdd(b.work + 7, static_cast<unsigned long long>(w.it_mask));
+        }
     }
+}
+
+template <int kMode>
+void launch_trace(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, const DevScene& S,
+                  const ChunkParams& c, const PathBuffers& b, uint32_t k, TestIO io = {}) {
+#define MT_TRACE_CASE(D)                                                                          \
+    case D:                                                                                       \
+        if (count)                                                                                \
+            hipLaunchKernelGGL((k_trace<true, kMode, D>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io); \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_trace<false, kMode, D>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io); \
+        break;
+    switch (depth) {
+        MT_TRACE_CASE(4)
+        MT_TRACE_CASE(8)
+        MT_TRACE_CASE(12)
+        default:
+        MT_TRACE_CASE(16)
+    }
+#undef MT_TRACE_CASE
 }
 
 __device__ void eval_material(const DevScene& S, const DevMaterial& m, float u, float v, V3& base,
@@ -407,33 +551,6 @@ __global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b
     img[pix * 3 + 2] = r.z;
 }
 
-__global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, uint32_t n, const float* rays,
-                                                        mt_hit* out, uint2* spill, uint32_t cap,
-                                                        uint32_t* err) {
-    __shared__ uint2 lds_stack[kLdsDepth * kBlock];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Stack st{lds_stack + threadIdx.x, kBlock, kLdsDepth, spill + i, n, cap, err, 0};
-    const V3 o = {rays[i * 6 + 0], rays[i * 6 + 1], rays[i * 6 + 2]};
-    const V3 d = {rays[i * 6 + 3], rays[i * 6 + 4], rays[i * 6 + 5]};
-    Hit h;
-    Work w{0, 0, 0, 0};
-    trace<false>(S, o, d, h, st, w);
-    mt_hit r;
-    r.instance = h.inst;
-    r.triangle = h.tri;
-    r.bv_index = h.bv;
-    r.u = h.u;
-    r.v = h.v;
-    r.distance = h.inst >= 0 ? h.dist : 0.0f;
-    r.world_pos[0] = h.inst >= 0 ? h.wpos.x : 0.0f;
-    r.world_pos[1] = h.inst >= 0 ? h.wpos.y : 0.0f;
-    r.world_pos[2] = h.inst >= 0 ? h.wpos.z : 0.0f;
-    r.bvh_tests = h.bvh_tests;
-    r.triangle_tests = h.tri_tests;
-    out[i] = r;
-}
-
 __global__ void k_brdf(uint32_t n, const float* in, float* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -455,6 +572,13 @@ __global__ void k_brdf(uint32_t n, const float* in, float* out) {
         if (e_ != hipSuccess)                                                          \
             return fail(MT_E_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
     } while (0)
+
+uint32_t env_u32(const char* name, uint32_t def) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return def;
+    const long x = std::strtol(v, nullptr, 10);
+    return x > 0 ? static_cast<uint32_t>(x) : def;
+}
 
 int device_cu_count() {
     static int cus = 0;
@@ -493,23 +617,29 @@ int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits) {
     } guard{owned};
     float* drays;
     mt_hit* dhits;
-    uint2* spill;
-    uint32_t* err;
+    PathBuffers b{};
+    const uint32_t depth = kLdsDepth;
     const uint32_t cap = stack_capacity(s);
-    const uint32_t spill_depth = cap > kLdsDepth ? cap - kLdsDepth : 1;
+    const uint32_t spill_depth = cap > depth ? cap - depth : 1;
+    const uint32_t blocks = std::max<uint32_t>(
+        1u, std::min<uint32_t>((n + kBlock - 1) / kBlock, static_cast<uint32_t>(device_cu_count() * 8)));
     int rc;
     if ((rc = alloc(owned, &drays, static_cast<size_t>(n) * 6))) return rc;
     if ((rc = alloc(owned, &dhits, n))) return rc;
-    if ((rc = alloc(owned, &spill, static_cast<size_t>(spill_depth) * n))) return rc;
-    if ((rc = alloc(owned, &err, 1))) return rc;
+    if ((rc = alloc(owned, &b.spill, static_cast<size_t>(spill_depth) * blocks * kBlock))) return rc;
+    if ((rc = alloc(owned, &b.counters, kMaxCounters + 1))) return rc;
     HIP_TRY(hipMemcpy(drays, rays, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(err, 0, sizeof(uint32_t)));
-    hipLaunchKernelGGL(k_trace_rays, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, 0, s->dev, n,
-                       drays, dhits, spill, cap, err);
+    HIP_TRY(hipMemset(b.counters, 0, sizeof(uint32_t) * (kMaxCounters + 1)));
+    ChunkParams c{};
+    c.stack_cap = cap;
+    c.refill_min = kRefillMin;
+    c.setup_min = kSetupMin;
+    TestIO io{drays, dhits, n, b.counters + kMaxCounters};
+    launch_trace<2>(depth, false, blocks, nullptr, s->dev, c, b, 0, io);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     uint32_t herr = 0;
-    HIP_TRY(hipMemcpy(&herr, err, sizeof(herr), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&herr, b.counters + kCntError, sizeof(herr), hipMemcpyDeviceToHost));
     if (herr) return fail(MT_E_INVALID, "traversal stack overflow");
     HIP_TRY(hipMemcpy(hits, dhits, sizeof(mt_hit) * n, hipMemcpyDeviceToHost));
     return MT_OK;
@@ -572,10 +702,12 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
 
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     const int cus = device_cu_count();
-    trace_blocks_ = static_cast<uint32_t>(cus * 8);
+    lds_depth_ = env_u32("MT_LDS_DEPTH", kLdsDepth);
+    if (lds_depth_ != 4 && lds_depth_ != 8 && lds_depth_ != 12) lds_depth_ = 16;
+    trace_blocks_ = static_cast<uint32_t>(cus * env_u32("MT_TRACE_BLOCKS_PER_CU", 8));
     shade_blocks_ = static_cast<uint32_t>(std::min<uint64_t>((P + kBlock - 1) / kBlock, cus * 16ull));
     const uint32_t cap = stack_capacity(scene);
-    const uint32_t spill_depth = cap > kLdsDepth ? cap - kLdsDepth : 1;
+    const uint32_t spill_depth = cap > lds_depth_ ? cap - lds_depth_ : 1;
     const size_t trace_threads = static_cast<size_t>(trace_blocks_) * kBlock;
     PathBuffers& b = buf_;
     int rc;
@@ -590,10 +722,10 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         (rc = alloc(owned_, &b.sh_d, P)) || (rc = alloc(owned_, &b.sh_c, P)) ||
         (rc = alloc(owned_, &b.sample_rgb, P)) ||
         (rc = alloc(owned_, &b.spill, spill_depth * trace_threads)) ||
-        (rc = alloc(owned_, &b.counters, kMaxCounters)) || (rc = alloc(owned_, &b.work, 8)) ||
+        (rc = alloc(owned_, &b.counters, kMaxCounters)) || (rc = alloc(owned_, &b.work, 16)) ||
         (rc = alloc(owned_, &img_, npix_total * 3)))
         return rc;
-    HIP_TRY(hipMemsetAsync(b.work, 0, sizeof(unsigned long long) * 8, stream_));
+    HIP_TRY(hipMemsetAsync(b.work, 0, sizeof(unsigned long long) * 16, stream_));
     HIP_TRY(hipMemsetAsync(img_, 0, sizeof(float) * npix_total * 3, stream_));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_counters_), sizeof(uint32_t) * kMaxCounters));
 
@@ -628,6 +760,8 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     c.firefly_threshold = st.firefly_threshold;
     c.count_work = st.count_work;
     c.stack_cap = cap;
+    c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
+    c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
     for (uint32_t i = 0; i < 2 + 4 * st.bounces; ++i) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
@@ -660,19 +794,12 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
     for (uint32_t k = 0; k < st_.bounces; ++k) {
         hipEvent_t* ev = &events_[2 + 4 * k];
         HIP_TRY(hipEventRecord(ev[0], s));
-        if (c.count_work)
-            hipLaunchKernelGGL((k_trace<true, 0>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
-        else
-            hipLaunchKernelGGL((k_trace<false, 0>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        launch_trace<0>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[1], s));
         hipLaunchKernelGGL(k_shade, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[2], s));
-        if (scene_->light_count) {
-            if (c.count_work)
-                hipLaunchKernelGGL((k_trace<true, 1>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
-            else
-                hipLaunchKernelGGL((k_trace<false, 1>), dim3(trace_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
-        }
+        if (scene_->light_count)
+            launch_trace<1>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(hipGetLastError());
     }
@@ -724,8 +851,9 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
 int Renderer::finish(float* rgb_out) {
     const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
     if (st_.count_work) {
-        unsigned long long w[8];
+        unsigned long long w[16];
         HIP_TRY(hipMemcpy(w, buf_.work, sizeof(w), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 9; ++i) stats_.diag[i] = w[7 + i];
         stats_.inst_aabb_tests = w[0];
         stats_.inst_entered = w[1];
         stats_.node_pops = w[2];

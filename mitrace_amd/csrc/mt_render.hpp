@@ -37,6 +37,8 @@ struct ChunkParams {
     float firefly_threshold;
     int32_t count_work;
     uint32_t stack_cap;
+    uint32_t refill_min;  // idle lanes of a wave that trigger a refill of the traversal
+    uint32_t setup_min;   // lanes needing an instance set-up that trigger a set-up step
 };
 
 // SoA path state, queues and scratch of one render (capacity = paths per chunk)
@@ -79,7 +81,7 @@ class Renderer {
     uint32_t* host_counters_ = nullptr;
     float* img_ = nullptr;
     uint32_t capacity_ = 0, npix_chunk_ = 0, chunks_ = 0;
-    uint32_t trace_blocks_ = 0, shade_blocks_ = 0;
+    uint32_t trace_blocks_ = 0, shade_blocks_ = 0, lds_depth_ = 16;
     mt_stats stats_{};
 };
 

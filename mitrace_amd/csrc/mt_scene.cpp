@@ -267,7 +267,8 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     s->light_count = d->light_count;
 
     std::vector<DevMesh> meshes(d->mesh_count);
-    std::vector<float4> nodes, prims;
+    std::vector<float4> pairs, prims;
+    std::vector<uint8_t> last_flags;
     std::vector<float> normals, uvs;
     std::vector<std::pair<std::array<float, 3>, std::array<float, 3>>> local_aabb(d->mesh_count);
     s->bvhs.resize(d->mesh_count);
@@ -279,6 +280,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             return fail(MT_E_INVALID, "mesh without positions or normals");
         const float* P = md.positions;
         DevMesh& dm = meshes[i];
+        std::memset(&dm, 0, sizeof(dm));
         dm.attr_base = static_cast<uint32_t>(tri_total);
         dm.tri_count = md.tri_count;
         dm.flags = 0;
@@ -294,35 +296,60 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         // BVH trigger (Mesh.cpp:37-40): max depth passed as bvhMaxDepth - 1 (unsigned)
         const bool bvh = !d->bvh.disable && md.tri_count > d->bvh.max_triangles;
         std::vector<uint32_t> order;
+        std::vector<uint32_t> leaf_of_slot;  // reference node index of every leaf slot
         if (bvh) {
             build_bvh(P, md.tri_count, d->bvh.max_triangles, d->bvh.max_depth - 1u, s->bvhs[i]);
             s->mesh_has_bvh[i] = 1;
             s->max_bvh_depth = std::max(s->max_bvh_depth, s->bvhs[i].depth);
+            const HostBvh& hb = s->bvhs[i];
             dm.flags |= kMeshHasBvh;
-            dm.node_base = static_cast<uint32_t>(nodes.size() / 2);
-            for (const HostNode& h : s->bvhs[i].nodes) {
-                float4 lo, hi;
-                lo.x = h.mn[0];
-                lo.y = h.mn[1];
-                lo.z = h.mn[2];
-                std::memcpy(&lo.w, &h.a, 4);
-                hi.x = h.mx[0];
-                hi.y = h.mx[1];
-                hi.z = h.mx[2];
-                std::memcpy(&hi.w, &h.b, 4);
-                nodes.push_back(lo);
-                nodes.push_back(hi);
+            // a node's word: inner -> index of its children's pair record, leaf -> first slot
+            auto word = [&](const HostNode& h) {
+                return (h.a & kLeafBit) ? h.a : (h.a >> 1);
+            };
+            for (int k = 0; k < 3; ++k) {
+                dm.root_min[k] = hb.nodes[0].mn[k];
+                dm.root_max[k] = hb.nodes[0].mx[k];
             }
-            order = s->bvhs[i].indices;
+            dm.root_word = word(hb.nodes[0]);
+            dm.pair_base = static_cast<uint32_t>(pairs.size() / 4);
+            for (size_t q = 0; q < hb.nodes.size() / 2; ++q) {
+                const HostNode& A = hb.nodes[2 * q];
+                const HostNode& B = hb.nodes[2 * q + 1];
+                const uint32_t wa = word(A), wb = word(B), zero = 0;
+                float4 r[4];
+                r[0].x = A.mn[0]; r[0].y = A.mn[1]; r[0].z = A.mn[2]; std::memcpy(&r[0].w, &wa, 4);
+                r[1].x = A.mx[0]; r[1].y = A.mx[1]; r[1].z = A.mx[2]; std::memcpy(&r[1].w, &wb, 4);
+                r[2].x = B.mn[0]; r[2].y = B.mn[1]; r[2].z = B.mn[2]; std::memcpy(&r[2].w, &zero, 4);
+                r[3].x = B.mx[0]; r[3].y = B.mx[1]; r[3].z = B.mx[2]; std::memcpy(&r[3].w, &zero, 4);
+                pairs.insert(pairs.end(), r, r + 4);
+            }
+            order = hb.indices;
+            leaf_of_slot.assign(order.size(), 0);
+            std::vector<uint8_t> last(order.size(), 0);
+            for (size_t n = 0; n < hb.nodes.size(); ++n) {
+                const HostNode& h = hb.nodes[n];
+                if (!(h.a & kLeafBit) || (n == 1)) continue;
+                const uint32_t first = h.a & ~kLeafBit, count = h.b & ~kLeafBit;
+                for (uint32_t k = first; k < first + count; ++k) leaf_of_slot[k] = static_cast<uint32_t>(n);
+                if (count) last[first + count - 1] = 1;
+            }
+            last_flags.swap(last);
         } else {
-            dm.node_base = 0;
+            dm.pair_base = 0;
+            dm.root_word = 0;
             order.resize(md.tri_count);
             for (uint32_t t = 0; t < md.tri_count; ++t) order[t] = t;
+            leaf_of_slot.assign(order.size(), 0);
+            last_flags.assign(order.size(), 0);
+            if (!order.empty()) last_flags.back() = 1;
         }
         dm.prim_base = static_cast<uint32_t>(prims.size() / 3);
         dm.prim_count = static_cast<uint32_t>(order.size());
-        for (uint32_t t : order) {
+        for (size_t k = 0; k < order.size(); ++k) {
+            const uint32_t t = order[k];
             const float* v = P + static_cast<uint64_t>(t) * 9;
+            const uint32_t lf = last_flags[k] ? kPrimLast : 0u;
             float4 a, b, c;
             a.x = v[0];
             a.y = v[1];
@@ -331,11 +358,11 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             b.x = v[3] - v[0];
             b.y = v[4] - v[1];
             b.z = v[5] - v[2];
-            b.w = 0.0f;
+            std::memcpy(&b.w, &leaf_of_slot[k], 4);
             c.x = v[6] - v[0];
             c.y = v[7] - v[1];
             c.z = v[8] - v[2];
-            c.w = 0.0f;
+            std::memcpy(&c.w, &lf, 4);
             prims.push_back(a);
             prims.push_back(b);
             prims.push_back(c);
@@ -348,7 +375,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         }
     }
     s->prim_slots = prims.size() / 3;
-    s->node_count = nodes.size() / 2;
+    s->node_count = pairs.size() / 2;
     s->triangle_count = tri_total;
 
     std::vector<DevInstance> inst(d->instance_count);
@@ -464,7 +491,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     int rc;
     if ((rc = dev_upload(s, inst, &s->dev.instances))) return rc;
     if ((rc = dev_upload(s, meshes, &s->dev.meshes))) return rc;
-    if ((rc = dev_upload(s, nodes, &s->dev.nodes))) return rc;
+    if ((rc = dev_upload(s, pairs, &s->dev.pairs))) return rc;
     if ((rc = dev_upload(s, prims, &s->dev.prims))) return rc;
     if ((rc = dev_upload(s, normals, &s->dev.normals))) return rc;
     if ((rc = dev_upload(s, uvs, &s->dev.uvs))) return rc;

@@ -70,7 +70,9 @@ class GpuScene:
 
 
 def stats_dict(s):
-    return {n: getattr(s, n) for n, _ in abi.mt_stats._fields_}
+    d = {n: getattr(s, n) for n, _ in abi.mt_stats._fields_}
+    d["diag"] = list(s.diag)
+    return d
 
 
 def brdf_eval(inp):
