@@ -1171,6 +1171,31 @@ int orc_trace_closest(const void* sp, uint32_t n, const float* rays, mt_hit* hit
     return MT_OK;
 }
 
+float orc_slab(const float* ray, const float* box) {
+    Ray r;
+    r.o = V3{ray[0], ray[1], ray[2]};
+    r.d = V3{ray[3], ray[4], ray[5]};
+    return slab(r, V3{box[0], box[1], box[2]}, V3{box[3], box[4], box[5]});
+}
+
+int orc_triangle_test(const float* ray, const float* tri, float* uvt) {
+    float u = 0, v = 0, t = 0;
+    const bool hit = triangleTest(V3{ray[0], ray[1], ray[2]}, V3{ray[3], ray[4], ray[5]},
+                                  V3{tri[0], tri[1], tri[2]}, V3{tri[3], tri[4], tri[5]},
+                                  V3{tri[6], tri[7], tri[8]}, u, v, t);
+    uvt[0] = u;
+    uvt[1] = v;
+    uvt[2] = t;
+    return hit ? 1 : 0;
+}
+
+void orc_invert4(const float* m, float* out) {
+    M4 M;
+    std::memcpy(M.m, m, sizeof(M.m));
+    const M4 r = invert4(M);
+    std::memcpy(out, r.m, sizeof(r.m));
+}
+
 int orc_trace_bruteforce(const void* sp, uint32_t n, const float* rays, mt_hit* hits) {
     const Scene& S = *static_cast<const Scene*>(sp);
     for (uint32_t i = 0; i < n; ++i) {

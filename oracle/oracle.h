@@ -46,6 +46,12 @@ int orc_scene_world_aabbs(const void* scene, float* out);
 
 /* IntersectScene for n rays {o.xyz, d.xyz}; work (optional) receives summed counters */
 int orc_trace_closest(const void* scene, uint32_t n, const float* rays, mt_hit* hits, orc_work* work);
+/* IntersectRayAABB (Intersect.cpp:9-18): ray {o, d}, box {min, max} -> exit t or -1 */
+float orc_slab(const float* ray, const float* box);
+/* glm::intersectRayTriangle contract: ray {o, d}, tri {v0, v1, v2} -> hit, uvt {u, v, t} */
+int orc_triangle_test(const float* ray, const float* tri, float* uvt);
+/* the 4x4 inverse used for the per-ray local transform (column-major) */
+void orc_invert4(const float* m, float* out);
 /* exact closest hit in double precision over every triangle (no BVH, no culling slack) */
 int orc_trace_bruteforce(const void* scene, uint32_t n, const float* rays, mt_hit* hits);
 

@@ -56,6 +56,10 @@ def load(fast=False):
                                     C.POINTER(abi.mt_hit), C.POINTER(orc_work)]
     L.orc_trace_bruteforce.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
                                        C.POINTER(abi.mt_hit)]
+    L.orc_slab.restype = C.c_float
+    L.orc_slab.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_triangle_test.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_invert4.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_brdf_eval.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_sample_bounce.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_env_sample.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_int32,
@@ -198,4 +202,25 @@ def rng_draws(seed, pixel, sample, dim):
     out = np.zeros(len(pixel), np.float32)
     p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
     L.orc_rng_draws(seed, len(pixel), p(pixel), p(sample), p(dim), abi.fptr(out))
+    return out
+
+
+def slab(ray, box):
+    ray = np.ascontiguousarray(ray, np.float32)
+    box = np.ascontiguousarray(box, np.float32)
+    return float(load().orc_slab(abi.fptr(ray), abi.fptr(box)))
+
+
+def triangle_test(ray, tri):
+    ray = np.ascontiguousarray(ray, np.float32)
+    tri = np.ascontiguousarray(tri, np.float32).reshape(9)
+    uvt = np.zeros(3, np.float32)
+    hit = load().orc_triangle_test(abi.fptr(ray), abi.fptr(tri), abi.fptr(uvt))
+    return bool(hit), uvt
+
+
+def invert4(m_colmajor):
+    m = np.ascontiguousarray(m_colmajor, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    load().orc_invert4(abi.fptr(m), abi.fptr(out))
     return out

@@ -1,0 +1,102 @@
+"""CPU checks of the boundary: the built library exports every function include/mitrace_gpu.h
+declares with matching struct layouts, and the glTF reader reproduces the reference loader's
+view of the reference's own test scenes (no GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from helpers import fixture_path, render_params
+from mitrace_amd import abi, gltf, _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mitrace_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mt_[a-z0-9_]+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_function():
+    L = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(abi.SIGNATURES), set(names) ^ set(abi.SIGNATURES)
+    assert L.mt_abi_version() == 1
+
+
+def test_struct_layouts_match_the_header():
+    structs = ["mt_mesh_desc", "mt_image_desc", "mt_texture_desc", "mt_material_desc",
+               "mt_instance_desc", "mt_light_desc", "mt_camera_desc", "mt_env_desc",
+               "mt_bvh_options", "mt_scene_desc", "mt_settings", "mt_stats", "mt_hit"]
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "%s"\nint main(void){\n' % HEADER
+    for s in structs:
+        src += f'printf("{s} %zu\\n", sizeof({s}));\n'
+    src += 'printf("mt_stats.diag %zu\\n", offsetof(mt_stats, diag));\nreturn 0;}\n'
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "sizes.c")
+        exe = os.path.join(d, "sizes")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    sizes = dict(line.split() for line in out if line)
+    for s in structs:
+        assert int(sizes[s]) == C.sizeof(getattr(abi, s)), s
+    assert int(sizes["mt_stats.diag"]) == abi.mt_stats.diag.offset
+
+
+def test_errors_without_device_are_reported_not_raised():
+    L = _lib.load()
+    n = C.c_int(-1)
+    rc = L.mt_device_count(C.byref(n))
+    assert rc in (0, abi.MT_E_HIP)
+    if n.value == 0 or rc != 0:
+        h = C.c_void_p()
+        assert L.mt_scene_create(None, C.byref(h)) != 0
+        assert L.mt_last_error()
+
+
+def test_cornell_fixture_loads_like_the_reference():
+    sc = gltf.load(fixture_path("cornell_box"))
+    # 3 glTF meshes; the walls mesh has 4 primitives -> 6 instances
+    assert len(sc.instances) == 6 and len(sc.meshes) == 6
+    assert sum(m.tri_count for m in sc.meshes) == 42
+    # EXT_light_attributes area light -> a point light of zero jitter at the node (Q16)
+    (l,) = sc.lights
+    assert l.type == abi.MT_LIGHT_AREA
+    np.testing.assert_allclose(l.position, (0, 1.99, 1.0550744533538818), rtol=1e-6)
+    assert l.color == (543.0, 543.0, 543.0)
+    cam = sc.camera
+    assert cam.type == abi.MT_CAMERA_PERSPECTIVE and cam.yfov == pytest.approx(0.6911112070083618)
+    np.testing.assert_allclose(cam.camera_to_world[:3, 3], (0, 1, -3))
+    # material defaults (Q25): present without textures
+    red = [m for m in sc.materials if m.name == "WallRed"][0]
+    assert red.metallic_factor == 0 and red.roughness_factor == 0.5
+    assert render_params("cornell_box") == {"width": 512, "height": 512, "samples": 512, "ev-exposure": 6}
+
+
+def test_material_spheres_fixture():
+    sc = gltf.load(fixture_path("material_spheres"))
+    assert len(sc.instances) == 17 and len(sc.lights) == 4 and len(sc.images) == 1
+    assert sc.images[0].shape == (256, 256, 3)
+    assert sum(sc.meshes[i.mesh].tri_count for i in sc.instances) == 7214
+    m = sc.materials[3]  # metallicFactor absent -> 1.0 (GLTF.cpp:468)
+    assert m.metallic_factor == 1.0 and m.roughness_factor == 0.0
+    tex = sc.materials[0].base_color_tex
+    assert tex.image == 0 and tex.filter == abi.MT_FILTER_LINEAR  # minFilter 9987
+    assert sc.lights[0].color == pytest.approx((2717.5706532941126, 2717.5706532941126 * 0.85, 2717.5706532941126 * 0.7), rel=1e-6)
+    assert sc.lights[0].point_size == 0.1  # KHR_lights_punctual "size" default (Q16)
+
+
+def test_light_creep_orthographic_camera():
+    sc = gltf.load(fixture_path("light_creep"))
+    assert sc.camera.type == abi.MT_CAMERA_ORTHOGONAL
+    assert sc.camera.ortho_size == pytest.approx((0.85, 0.478125), rel=1e-6)
+    assert sc.camera.aspect == pytest.approx(0.85 / 0.478125, rel=1e-6)

@@ -1,0 +1,139 @@
+"""The oracle's estimator (GeneratePath + ProcessRayForward, Tracer.cpp:134-348) pinned by
+scenes whose samples have closed forms, by determinism, and by an unbiasedness check of the
+Russian-roulette mode (no GPU)."""
+import numpy as np
+import pytest
+
+from mitrace_amd import abi, scenegen
+from mitrace_amd.scene import Camera, Environment, Instance, Light, Material, Scene, Settings, trs
+from oracle import pyoracle as po
+
+
+def floor_scene(base=(0.5, 0.3, 0.2), rough=0.7, metal=0.0, lights=(), env_value=1.0, ortho=False):
+    mesh = scenegen.ground_plane(nx=2, nz=2, x0=-50, x1=50, z0=-50, z1=50)
+    env = Environment(texels=np.full((8, 16, 3), env_value, np.float32))
+    if ortho:
+        M = np.array([[1, 0, 0, 0], [0, 0, 1, 5], [0, -1, 0, 0], [0, 0, 0, 1]], np.float32)  # looks down -y
+        cam = Camera(type=abi.MT_CAMERA_ORTHOGONAL, camera_to_world=M, ortho_size=(1e-6, 1e-6), aspect=1.0)
+    else:
+        M = np.array([[1, 0, 0, 0], [0, 0, 1, 5], [0, -1, 0, 0], [0, 0, 0, 1]], np.float32)
+        cam = Camera(type=abi.MT_CAMERA_PERSPECTIVE, camera_to_world=M, yfov=0.5, aspect=1.0)
+    mat = Material("floor", (*base, 1.0), metallic_factor=metal, roughness_factor=rough)
+    return Scene(meshes=[mesh], materials=[mat], instances=[Instance(0, 0, np.eye(4, dtype=np.float32))],
+                 lights=list(lights), camera=cam, env=env)
+
+
+def settings(**kw):
+    s = Settings(width=8, height=8, samples=16, bounces=2, output=abi.MT_OUTPUT_LINEAR, seed=3)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def test_sky_term_uses_previous_base_colour_and_no_throughput():
+    """Q10 + Q14: with no lights the radiance throughput stays 1, so a path floor -> sky is
+    worth exactly base' * sky * secondary intensity with base' = 0.96 base + 0.04."""
+    base = np.array([0.5, 0.3, 0.2], np.float32)
+    o = po.OracleScene(floor_scene(base=tuple(base)))
+    _, s, _, _ = o.render(settings(), want_samples=True)
+    bp = base * np.float32(0.96) + np.float32(0.04)
+    expect = bp * np.float32(350.0)
+    assert np.all(s == expect)
+    # a zero-intensity light is never chosen (score 0): the same samples
+    o2 = po.OracleScene(floor_scene(base=tuple(base), lights=[Light(abi.MT_LIGHT_POINT, (0, 3, 0), (0, 0, 0), 0.0)]))
+    _, s2, _, _ = o2.render(settings(), want_samples=True)
+    assert np.array_equal(s, s2)
+
+
+def test_primary_miss_uses_primary_intensity():
+    sc = floor_scene()
+    sc.camera.camera_to_world = np.array([[1, 0, 0, 0], [0, 0, -1, 5], [0, 1, 0, 0], [0, 0, 0, 1]], np.float32)  # looks up
+    o = po.OracleScene(sc)
+    _, s, w, _ = o.render(settings(hdri_primary_intensity=100.0), want_samples=True)
+    assert np.all(s == 100.0) and w["rays"] == w["samples"]
+
+
+def test_energy_cutoff_and_bounce_limit():
+    o = po.OracleScene(floor_scene())
+    _, s, w, _ = o.render(settings(terminate_energy=1e9), want_samples=True)
+    assert np.all(s == 0) and w["rays"] == w["samples"]  # Q13: stops after the first hit
+    _, s, w, _ = o.render(settings(bounces=1), want_samples=True)
+    assert np.all(s == 0) and w["rays"] == w["samples"]
+    _, s, w, _ = o.render(settings(bounces=0), want_samples=True)
+    assert np.all(s == 0) and w["rays"] == 0
+
+
+def test_direct_light_closed_form():
+    """Q15-Q17 with one point light of radius 0: chosen with pdf 1, contribution
+    f(L, V, N) * NL * lightColour / (d^2 + 1) at the (single) hit point."""
+    light = Light(abi.MT_LIGHT_POINT, (1.0, 2.0, 0.5), (30.0, 20.0, 10.0), point_size=0.0)
+    sc = floor_scene(base=(0.6, 0.6, 0.6), rough=0.4, metal=0.2, lights=[light], env_value=0.0, ortho=True)
+    o = po.OracleScene(sc)
+    _, s, _, _ = o.render(settings(bounces=1, samples=4), want_samples=True)
+    P = np.array([0, 0, 0], np.float64)
+    lv = np.array(light.position) - P
+    d = np.linalg.norm(lv)
+    L = lv / d
+    V = np.array([0, 1.0, 0])
+    N = np.array([0, 1.0, 0])
+    base = np.array([0.6, 0.6, 0.6]) * 0.96 + 0.04
+    f = po.brdf_eval(np.concatenate([L, V, N, base, [0.4, 0.2]]).astype(np.float32)[None])[0, :3]
+    expect = f * L[1] / (d * d + 1) * np.array(light.color)
+    np.testing.assert_allclose(s.reshape(-1, 3), np.tile(expect, (s.shape[0] * s.shape[1], 1)), rtol=2e-4)
+
+
+def test_occluded_light_contributes_nothing():
+    """Q17: a shadow ray blocked by geometry closer than the light adds nothing."""
+    light = Light(abi.MT_LIGHT_POINT, (0.0, 2.0, 0.0), (30.0, 30.0, 30.0), point_size=0.0)
+    sc = floor_scene(lights=[light], env_value=0.0, ortho=True)
+    blocker = scenegen.unit_cube("blocker")
+    sc.meshes.append(blocker)
+    sc.instances.append(Instance(1, 0, trs((0.0, 1.0, 0.0), 0.0, (0.2, 0.05, 0.2))))
+    # the camera sits above the blocker: move it to look at the floor next to it, light path blocked
+    sc.camera.camera_to_world = np.array([[1, 0, 0, 0], [0, 0, 1, 0.5], [0, -1, 0, 0], [0, 0, 0, 1]], np.float32)
+    sc.lights = [Light(abi.MT_LIGHT_POINT, (0.0, 2.0, 0.0), (30.0, 30.0, 30.0), point_size=0.0)]
+    sc.camera.camera_to_world[0, 3] = 0.0
+    sc.camera.camera_to_world[1, 3] = 0.5  # between floor and blocker, looking down
+    o = po.OracleScene(sc)
+    _, s, w, _ = o.render(settings(bounces=1, samples=2), want_samples=True)
+    assert np.all(s == 0)
+    assert w["rays"] == 2 * w["samples"]  # camera + shadow ray per sample
+
+
+def test_counter_rng_is_thread_and_tile_invariant():
+    sc = scenegen.cornell()
+    o = po.OracleScene(sc)
+    st = scenegen.cornell_settings(width=24, height=20, samples=4, ev_exposure=2.0)
+    a, _, _, _ = o.render(st, threads=1)
+    b, _, _, _ = o.render(st, threads=5)
+    st.image_block_size = 7
+    c, _, _, _ = o.render(st, threads=3)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    st.seed = 99
+    d, _, _, _ = o.render(st, threads=3)
+    assert not np.array_equal(a, d)
+
+
+def test_mt19937_mode_matches_counter_mode_statistically():
+    sc = scenegen.cornell()
+    o = po.OracleScene(sc)
+    st = scenegen.cornell_settings(width=16, height=16, samples=64, ev_exposure=2.0)
+    a, sa, _, _ = o.render(st, want_samples=True)
+    b, sb, _, _ = o.render(st, rng=po.ORC_RNG_MT19937, want_samples=True)
+    se = np.sqrt(sa.var(axis=1) / 64 + sb.var(axis=1) / 64) + 1e-6
+    z = np.abs(a.reshape(-1, 3) - b.reshape(-1, 3)) / se
+    assert (z < 4).mean() > 0.99
+
+
+def test_russian_roulette_is_unbiased():
+    """N1: the fixed mode's roulette keeps the mean of the unterminated estimator."""
+    sc = scenegen.cornell()
+    o = po.OracleScene(sc)
+    base = dict(width=12, height=12, samples=256, bounces=8, output=abi.MT_OUTPUT_LINEAR)
+    st_ref = scenegen.cornell_settings(terminate_energy=0.0, **base)
+    st_rr = scenegen.cornell_settings(mode=abi.MT_MODE_FIXED, rr_depth=3, seed=11, **base)
+    a, sa, _, _ = o.render(st_ref, want_samples=True)
+    b, sb, _, _ = o.render(st_rr, want_samples=True)
+    ma, mb = sa.mean(axis=(0, 1)), sb.mean(axis=(0, 1))
+    se = np.sqrt(sa.reshape(-1, 3).var(0) / sa.size * 3 + sb.reshape(-1, 3).var(0) / sb.size * 3)
+    assert np.all(np.abs(ma - mb) < 4 * se), (ma, mb, se)
