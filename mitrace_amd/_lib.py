@@ -7,7 +7,8 @@ import os
 
 from . import abi
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmitrace_gpu.so")
+LIB_PATH = os.environ.get("MITRACE_LIB_PATH") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "lib", "libmitrace_gpu.so")
 
 _lib = None
 

@@ -30,7 +30,7 @@ def flags():
             "-Wall", "-Wno-unused-result"]
 
 
-def build(verbose=False, jobs=4):
+def build(verbose=False, jobs=4, extra_flags=(), lib_path=None):
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
@@ -40,7 +40,7 @@ def build(verbose=False, jobs=4):
     for src in SOURCES:
         obj = os.path.join(objdir, src.rsplit(".", 1)[0] + ".o")
         objs.append(obj)
-        cmd = [cc] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc] + flags() + list(extra_flags) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -53,13 +53,14 @@ def build(verbose=False, jobs=4):
             print(out.decode(errors="replace"))
     if errors:
         raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
-    tmp = LIB + ".tmp"
+    out_lib = lib_path or LIB
+    tmp = out_lib + ".tmp"
     cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out_lib)
+    return out_lib
 
 
 if __name__ == "__main__":

@@ -260,6 +260,22 @@ __device__ __forceinline__ float slab(V3 o, V3 inv, V3 mn, V3 mx) {
     return tmin <= tmax ? tmax : -1.0f;
 }
 
+// Slab test with hardware min/max.  For finite t values it makes the same decisions as
+// slab(): v_min/v_max and the GLM ternaries differ only in the sign of a zero result, and
+// every consumer of the exit distance compares it (>= 0, <= best + slack, <, > best + slack),
+// where -0 and +0 are equal.  NaN t values occur only for an infinite inverse direction
+// component (a direction component of exactly 0), and callers use slab() for those rays.
+__device__ __forceinline__ float slab_fast(V3 o, V3 inv, V3 mn, V3 mx) {
+    const V3 t1 = mul(sub(mn, o), inv);
+    const V3 t2 = mul(sub(mx, o), inv);
+    const float tmin = fmaxf(fmaxf(fminf(t1.x, t2.x), fminf(t1.y, t2.y)), fminf(t1.z, t2.z));
+    const float tmax = fminf(fminf(fmaxf(t1.x, t2.x), fmaxf(t1.y, t2.y)), fmaxf(t1.z, t2.z));
+    return tmin <= tmax ? tmax : -1.0f;
+}
+__device__ __forceinline__ bool finite3(V3 v) {
+    return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
+}
+
 // two-sided Moller-Trumbore with the bounds tested against det before scaling
 __device__ __forceinline__ bool tri_test(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float& u, float& v, float& t) {
     const V3 p = cross(d, e2);
@@ -352,6 +368,10 @@ struct Trav {
     uint32_t cb;   // first instance of the current group of 64
     int cur;       // instance (within the group) being traversed, -1 = none
     uint32_t leaf; // next prim slot of the leaf being tested, kNoLeaf = none
+    uint32_t nxt_word;  // the node to pop next, kept in registers (has_next)
+    float nxt_dist;
+    bool has_next;
+    bool fast;     // local ray has a finite inverse direction: slab_fast is exact
     V3 lo, ld, linv;
     float ldist, bu, bvv;
     uint32_t btri, bbv, btests, ttests, pair_base, prim_base;
@@ -371,11 +391,13 @@ __device__ __forceinline__ void trav_begin(Trav& t, V3 o, V3 d) {
     t.cb = 0;
     t.cur = -1;
     t.leaf = kNoLeaf;
+    t.has_next = false;
+    t.fast = false;
 }
 
 // world-AABB test of instances [cb, cb + 64) (Intersect.cpp:147); cb must be wave-uniform so
 // the instance records are read with scalar loads
-template <bool kCount>
+template <bool kCount, bool kFast>
 __device__ __forceinline__ uint64_t instance_mask(const DevScene& S, uint32_t cb, V3 o, V3 invd,
                                                   Work& w) {
     const uint32_t nch = (S.instance_count - cb) < 64u ? (S.instance_count - cb) : 64u;
@@ -384,15 +406,16 @@ __device__ __forceinline__ uint64_t instance_mask(const DevScene& S, uint32_t cb
         const DevInstance* I = S.instances + cb + j;
         const V3 mn = {I->wmin[0], I->wmin[1], I->wmin[2]};
         const V3 mx = {I->wmax[0], I->wmax[1], I->wmax[2]};
-        if (slab(o, invd, mn, mx) >= 0.0f) mask |= (1ull << j);
+        const float te = kFast ? slab_fast(o, invd, mn, mx) : slab(o, invd, mn, mx);
+        if (te >= 0.0f) mask |= (1ull << j);
     }
     if (kCount) w.aabb += nch;
     return mask;
 }
 
-// The lane needs a set-up step: its stack is empty and no leaf is open.
+// The lane needs a set-up step: no node is pending and no leaf is open.
 __device__ __forceinline__ bool trav_needs_setup(const Trav& t, const Stack& st) {
-    return st.sp == 0 && t.leaf == kNoLeaf;
+    return st.sp == 0 && !t.has_next && t.leaf == kNoLeaf;
 }
 
 // Set-up step: finish the current instance (combine its hit), then start the next entered
@@ -448,48 +471,67 @@ __device__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool&
     t.pair_base = m.pair_base;
     t.prim_base = m.prim_base;
     if (kCount) w.enter++;
+    t.fast = finite3(t.linv) && finite3(t.lo);
     if (m.flags & kMeshHasBvh) {
         const V3 rmn = {m.root_min[0], m.root_min[1], m.root_min[2]};
         const V3 rmx = {m.root_max[0], m.root_max[1], m.root_max[2]};
-        st.push(m.root_word, slab(t.lo, t.linv, rmn, rmx));
+        t.nxt_word = m.root_word;
+        t.nxt_dist = t.fast ? slab_fast(t.lo, t.linv, rmn, rmx) : slab(t.lo, t.linv, rmn, rmx);
+        t.has_next = true;
     } else {
         t.leaf = 0;  // linear scan: the whole mesh is one leaf, no node pops counted
     }
     return false;
 }
 
-// One node pop (Intersect.cpp:67-78, 109-139).  Returns false when the pop was culled.
-template <bool kCount>
+// One node pop (Intersect.cpp:67-78, 109-139).  The nearer accepted child is kept in
+// registers as the next node instead of being pushed and popped again, which is the same
+// visiting order.  Returns false when the pop was culled.
+template <bool kFast>
 __device__ bool trav_node(const DevScene& S, Trav& t, Stack& st) {
-    const uint2 e = st.pop();
+    uint32_t word;
+    float nd;
+    if (t.has_next) {
+        word = t.nxt_word;
+        nd = t.nxt_dist;
+        t.has_next = false;
+    } else {
+        const uint2 e = st.pop();
+        word = e.x;
+        nd = __uint_as_float(e.y);
+    }
     t.btests++;
-    const float nd = __uint_as_float(e.y);
     if (nd < 0.0f) return false;
     if (nd > t.ldist + kMinDelta) return false;
-    if (e.x & kLeafBit) {
-        t.leaf = e.x & ~kLeafBit;
+    if (word & kLeafBit) {
+        t.leaf = word & ~kLeafBit;
         return true;
     }
     // the children's boxes and words in one 64-byte record
-    const float4* q = S.pairs + 4ull * (t.pair_base + e.x);
+    const float4* q = S.pairs + 4ull * (t.pair_base + word);
     const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
-    const float da = slab(t.lo, t.linv, xyz(a0), xyz(a1));
-    const float db = slab(t.lo, t.linv, xyz(b0), xyz(b1));
+    float da, db;
+    if (kFast) {
+        da = slab_fast(t.lo, t.linv, xyz(a0), xyz(a1));
+        db = slab_fast(t.lo, t.linv, xyz(b0), xyz(b1));
+    } else {
+        da = slab(t.lo, t.linv, xyz(a0), xyz(a1));
+        db = slab(t.lo, t.linv, xyz(b0), xyz(b1));
+    }
     const uint32_t wa = __float_as_uint(a0.w), wb = __float_as_uint(a1.w);
     const bool pa = (da >= 0.0f && da <= t.ldist + kMinDelta);
     const bool pb = (db >= 0.0f && db <= t.ldist + kMinDelta);
     if (pa && pb) {
-        if (da < db) {
-            st.push(wb, db);
-            st.push(wa, da);
-        } else {
-            st.push(wa, da);
-            st.push(wb, db);
-        }
-    } else if (pa) {
-        st.push(wa, da);
-    } else if (pb) {
-        st.push(wb, db);
+        // the reference pushes the farther child first and pops the nearer one next
+        const bool a_first = da < db;
+        st.push(a_first ? wb : wa, a_first ? db : da);
+        t.nxt_word = a_first ? wa : wb;
+        t.nxt_dist = a_first ? da : db;
+        t.has_next = true;
+    } else if (pa || pb) {
+        t.nxt_word = pa ? wa : wb;
+        t.nxt_dist = pa ? da : db;
+        t.has_next = true;
     }
     return true;
 }

@@ -1,8 +1,9 @@
 // mt_render.hip -- wavefront path tracing kernels for gfx950 and the render driver.
 //
 // One render = a sequence of chunks.  A chunk is a set of pixels with all their samples
-// (npix * spp paths, path p = sample * npix + local pixel, so a wave holds 64 neighbouring
-// pixels of one sample index).  Per chunk:
+// (npix * spp paths, path p = local pixel * spp + sample, so a wave holds the samples of one
+// pixel -- or of 64/spp neighbouring pixels -- and its rays are as coherent as possible).
+// Per chunk:
 //   generate                camera rays, path state init             (Tracer.cpp:556-586)
 //   for bounce k < bounces:
 //     extend   closest hit of every active path                      (Intersect.cpp:201-219)
@@ -60,8 +61,9 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
 
 __device__ __forceinline__ void path_coords(const ChunkParams& c, uint32_t p, uint32_t& pix,
                                             uint32_t& s) {
-    s = p / c.npix;
-    pix = c.pix_begin + (p - s * c.npix);
+    const uint32_t lp = p / c.spp;  // pixel-major: a wave holds the samples of one pixel
+    s = p - lp * c.spp;
+    pix = c.pix_begin + lp;
 }
 
 __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers b) {
@@ -118,8 +120,12 @@ struct TestIO {
 // takes that many new work items with one atomic (ballot + prefix for the lane slots), runs
 // the instance culling for them together, and every lane then advances its own ray by one
 // unit of work per loop iteration.  Waves leave the loop when the queue is drained.
+#ifndef MT_TRACE_WAVES_PER_SIMD
+#define MT_TRACE_WAVES_PER_SIMD 6
+#endif
 template <bool kCount, int kMode, uint32_t kDepth>
-__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, ChunkParams c, PathBuffers b,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(DevScene S, ChunkParams c, PathBuffers b,
                                                   uint32_t bounce, TestIO io) {
     __shared__ uint2 lds_stack[kDepth * kBlock];
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -181,10 +187,17 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, ChunkParams c, Pat
             if (kCount) w.it_mask++;
             const int l = __ffsll(static_cast<unsigned long long>(nm)) - 1;
             const uint32_t ucb = __builtin_amdgcn_readfirstlane(__shfl(t.cb, l));
-            if (need && t.cb == ucb) {
-                t.mask = instance_mask<kCount>(S, ucb, t.o, t.invd, w);
+            const bool mine = need && t.cb == ucb;
+            const bool fast = finite3(t.invd) && finite3(t.o);
+            if (__ballot(mine && !fast) == 0) {
+                if (mine) t.mask = instance_mask<kCount, true>(S, ucb, t.o, t.invd, w);
+            } else {
+                if (mine) t.mask = instance_mask<kCount, false>(S, ucb, t.o, t.invd, w);
+            }
+            if (mine) {
                 t.cur = -1;
                 t.leaf = kNoLeaf;
+                t.has_next = false;
                 st.sp = 0;
                 need = false;
             }
@@ -211,8 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, ChunkParams c, Pat
             } else if (t.leaf != kNoLeaf) {
                 kind = kStepLeaf;
                 trav_tri(S, t);
+            } else if (t.fast) {
+                kind = trav_node<true>(S, t, st) ? kStepInner : kStepCull;
             } else {
-                kind = trav_node<kCount>(S, t, st) ? kStepInner : kStepCull;
+                kind = trav_node<false>(S, t, st) ? kStepInner : kStepCull;
             }
         }
         if (kCount) {
@@ -514,19 +529,19 @@ __global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b
     const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
     if (lp >= c.npix) return;
     const uint32_t n = c.spp;
-    const float4* sp = b.sample_rgb + lp;
+    const float4* sp = b.sample_rgb + static_cast<size_t>(lp) * c.spp;
     uint32_t cnt = n;
     float avg = 0.0f, thr = 0.0f;
     if (c.firefly) {
         float sum = 0.0f;
         for (uint32_t s = 0; s < n; ++s) {
-            const float4 v = sp[static_cast<size_t>(s) * c.npix];
+            const float4 v = sp[s];
             sum = sum + (v.x + v.y + v.z) / 3.0f;
         }
         avg = sum / static_cast<float>(n);
         float var = 0.0f;
         for (uint32_t s = 0; s < n; ++s) {
-            const float4 v = sp[static_cast<size_t>(s) * c.npix];
+            const float4 v = sp[s];
             const float dd = (v.x + v.y + v.z) / 3.0f - avg;
             var = var + dd * dd;
         }
@@ -535,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b
     }
     V3 acc = mk(0.0f);
     for (uint32_t s = 0; s < n; ++s) {
-        const float4 v = sp[static_cast<size_t>(s) * c.npix];
+        const float4 v = sp[s];
         V3 col = xyz(v);
         if (c.firefly && fabsf((v.x + v.y + v.z) / 3.0f - avg) > thr) {
             col = mk(0.0f);
@@ -838,7 +853,7 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
         std::vector<float4> tmp(c.paths);
         HIP_TRY(hipMemcpy(tmp.data(), buf_.sample_rgb, sizeof(float4) * c.paths, hipMemcpyDeviceToHost));
         for (uint32_t p = 0; p < c.paths; ++p) {
-            const uint32_t smp = p / c.npix, lp = p % c.npix;
+            const uint32_t lp = p / c.spp, smp = p % c.spp;
             float* q = sample_rgb_out + (static_cast<size_t>(c.pix_begin + lp) * st_.samples + smp) * 3;
             q[0] = tmp[p].x;
             q[1] = tmp[p].y;
