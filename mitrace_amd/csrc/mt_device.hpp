@@ -372,6 +372,8 @@ struct Trav {
     float nxt_dist;
     bool has_next;
     bool fast;     // local ray has a finite inverse direction: slab_fast is exact
+    bool tl;       // instances come from the top-level BVH (else groups of 64 in index order)
+    uint32_t base; // stack depth below which the entries belong to the top-level BVH
     V3 lo, ld, linv;
     float ldist, bu, bvv;
     uint32_t btri, bbv, btests, ttests, pair_base, prim_base;
@@ -393,6 +395,27 @@ __device__ __forceinline__ void trav_begin(Trav& t, V3 o, V3 d) {
     t.leaf = kNoLeaf;
     t.has_next = false;
     t.fast = false;
+    t.tl = false;
+    t.base = 0;
+}
+
+__device__ __forceinline__ uint32_t trav_instance(const DevScene& S, const Trav& t, int j) {
+    return t.tl ? S.tlas_inst[t.cb + j] : t.cb + static_cast<uint32_t>(j);
+}
+
+// Start the top-level traversal of a ray whose inverse direction is finite: push the root.
+__device__ __forceinline__ void trav_tlas_begin(const DevScene& S, Trav& t, Stack& st) {
+    t.tl = true;
+    t.mask = 0;
+    t.cur = -1;
+    t.leaf = kNoLeaf;
+    t.has_next = false;
+    t.base = 0;
+    st.sp = 0;
+    const V3 mn = {S.tlas_root_min[0], S.tlas_root_min[1], S.tlas_root_min[2]};
+    const V3 mx = {S.tlas_root_max[0], S.tlas_root_max[1], S.tlas_root_max[2]};
+    if (slab_fast(t.o, t.invd, mn, mx) >= 0.0f) st.push(S.tlas_root_word, 0.0f);
+    t.base = st.sp;  // the lane starts with a set-up step, which walks the top level
 }
 
 // world-AABB test of instances [cb, cb + 64) (Intersect.cpp:147); cb must be wave-uniform so
@@ -415,7 +438,7 @@ __device__ __forceinline__ uint64_t instance_mask(const DevScene& S, uint32_t cb
 
 // The lane needs a set-up step: no node is pending and no leaf is open.
 __device__ __forceinline__ bool trav_needs_setup(const Trav& t, const Stack& st) {
-    return st.sp == 0 && !t.has_next && t.leaf == kNoLeaf;
+    return st.sp == t.base && !t.has_next && t.leaf == kNoLeaf;
 }
 
 // Set-up step: finish the current instance (combine its hit), then start the next entered
@@ -429,13 +452,16 @@ __device__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool&
             w.tris += t.ttests;
         }
         if (t.ldist < kFltMax) {
-            const float* M = S.instances[t.cb + t.cur].M;
+            const uint32_t ii = trav_instance(S, t, t.cur);
+            const float* M = S.instances[ii].M;
             const V3 lp = add(t.lo, scale(t.ld, t.ldist));
             const V3 wp = xform(M, lp, 1.0f);
             const float wd = len(sub(wp, t.o));
-            if (wd < t.R.dist) {
+            // strict '<' in instance order (Intersect.cpp:210); the top level visits
+            // instances out of order, so an equal distance goes to the lower index
+            if (wd < t.R.dist || (wd == t.R.dist && static_cast<int>(ii) < t.R.inst)) {
                 t.R.dist = wd;
-                t.R.inst = static_cast<int>(t.cb) + t.cur;
+                t.R.inst = static_cast<int>(ii);
                 t.R.tri = t.btri;
                 t.R.bv = t.bbv;
                 t.R.u = t.bu;
@@ -448,16 +474,46 @@ __device__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool&
         t.cur = -1;
     }
     if (t.mask == 0) {
-        if (t.cb + 64u < S.instance_count) {
-            t.cb += 64u;
-            need_mask = true;
-            return false;
+        if (!t.tl) {
+            if (t.cb + 64u < S.instance_count) {
+                t.cb += 64u;
+                need_mask = true;
+                return false;
+            }
+            return true;
         }
-        return true;
+        // top level: pop nodes until a leaf yields instances whose world box the ray crosses
+        // (the reference's own test, Intersect.cpp:147)
+        while (true) {
+            if (st.sp == 0) return true;
+            const uint32_t word = st.pop().x;
+            if (word & kLeafBit) {
+                const uint32_t first = word & 0x00FFFFFFu, cnt = (word >> 24) & 0x7Fu;
+                uint64_t m = 0;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const DevInstance* I = S.instances + S.tlas_inst[first + j];
+                    const V3 mn = {I->wmin[0], I->wmin[1], I->wmin[2]};
+                    const V3 mx = {I->wmax[0], I->wmax[1], I->wmax[2]};
+                    if (slab_fast(t.o, t.invd, mn, mx) >= 0.0f) m |= (1ull << j);
+                }
+                if (kCount) w.aabb += cnt;
+                if (m) {
+                    t.cb = first;
+                    t.mask = m;
+                    break;
+                }
+            } else {
+                const float4* q = S.tlas_pairs + 4ull * word;
+                const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
+                if (slab_fast(t.o, t.invd, xyz(a0), xyz(a1)) >= 0.0f) st.push(__float_as_uint(a0.w), 0.0f);
+                if (slab_fast(t.o, t.invd, xyz(b0), xyz(b1)) >= 0.0f) st.push(__float_as_uint(a1.w), 0.0f);
+            }
+        }
     }
     t.cur = __builtin_ctzll(t.mask);
     t.mask &= t.mask - 1;
-    const DevInstance* I = S.instances + t.cb + t.cur;
+    t.base = st.sp;
+    const DevInstance* I = S.instances + trav_instance(S, t, t.cur);
     float inv[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) inv[k] = I->inv[k];

@@ -107,6 +107,13 @@ struct DevScene {
     const DevLight* lights;
     uint32_t instance_count;
     uint32_t light_count;
+    // top-level BVH over instance world boxes (used when instance_count > 64): pair records
+    // like the meshes', leaf word = kLeafBit | count << 24 | first into tlas_inst
+    const float4* tlas_pairs;
+    const uint32_t* tlas_inst;
+    float tlas_root_min[3], tlas_root_max[3];
+    uint32_t tlas_root_word;
+    uint32_t use_tlas;
     uint32_t env_w, env_h;
     int32_t env_filter;
     int32_t has_env;
@@ -142,6 +149,7 @@ struct mt_scene {
     std::vector<mt::HostBvh> bvhs;  // per mesh; empty nodes = linear scan
     std::vector<uint32_t> mesh_has_bvh;
     uint32_t max_bvh_depth = 0;
+    uint32_t tlas_depth = 0;
     uint32_t mesh_count = 0;
     uint32_t instance_count = 0;
     uint32_t light_count = 0;
@@ -156,4 +164,9 @@ namespace mt {
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int upload_scene(const mt_scene_desc* desc, mt_scene* scene);
+// top-level BVH over boxes (6 floats each): SAH over 12 bins per axis, up to 4 boxes a leaf.
+// pairs/indices use the device format of DevScene::tlas_*; returns the number of levels.
+uint32_t build_tlas(const std::vector<float>& boxes, std::vector<float4>& pairs,
+                    std::vector<uint32_t>& indices, float root_min[3], float root_max[3],
+                    uint32_t& root_word);
 }  // namespace mt

@@ -176,7 +176,12 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                     }
                     trav_begin(t, o, d);
                     active = true;
-                    need = true;
+                    if (S.use_tlas && finite3(t.invd) && finite3(o)) {
+                        trav_tlas_begin(S, t, st);
+                        need = false;
+                    } else {
+                        need = true;
+                    }
                 }
             }
         }
@@ -619,7 +624,10 @@ int alloc(std::vector<void*>& owned, T** p, size_t n) {
 
 }  // namespace
 
-uint32_t stack_capacity(const mt_scene* s) { return std::max<uint32_t>(s->max_bvh_depth + 1u, 2u); }
+uint32_t stack_capacity(const mt_scene* s) {
+    const uint32_t blas = std::max<uint32_t>(s->max_bvh_depth + 1u, 2u);
+    return s->dev.use_tlas ? blas + s->tlas_depth + 1u : blas;
+}
 
 int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits) {
     if (n == 0) return MT_OK;

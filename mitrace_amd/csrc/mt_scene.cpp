@@ -237,6 +237,168 @@ void invert4(const float* m, float* out) {
         for (int c = 0; c < 4; ++c) out[c * 4 + r] = B[r][c];
 }
 
+
+namespace {
+
+struct TBuild {
+    const std::vector<float>& boxes;
+    std::vector<uint32_t> ids;
+    struct Node {
+        float mn[3], mx[3];
+        int32_t a = -1, b = -1;  // children, -1 = leaf
+        uint32_t first = 0, count = 0;
+    };
+    std::vector<Node> nodes;
+    explicit TBuild(const std::vector<float>& b) : boxes(b) {}
+
+    void bounds(uint32_t lo, uint32_t hi, float* mn, float* mx) const {
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = FLT_MAX;
+            mx[k] = -FLT_MAX;
+        }
+        for (uint32_t i = lo; i < hi; ++i) {
+            const float* bx = &boxes[6 * ids[i]];
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = std::min(mn[k], bx[k]);
+                mx[k] = std::max(mx[k], bx[3 + k]);
+            }
+        }
+    }
+    static float area(const float* mn, const float* mx) {
+        const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+    int32_t build(uint32_t lo, uint32_t hi, uint32_t level, uint32_t& depth) {
+        depth = std::max(depth, level);
+        const int32_t id = static_cast<int32_t>(nodes.size());
+        nodes.emplace_back();
+        float mn[3], mx[3];
+        bounds(lo, hi, mn, mx);
+        std::memcpy(nodes[id].mn, mn, sizeof(mn));
+        std::memcpy(nodes[id].mx, mx, sizeof(mx));
+        const uint32_t n = hi - lo;
+        if (n <= 4) {
+            nodes[id].first = lo;
+            nodes[id].count = n;
+            return id;
+        }
+        // binned SAH on box centres
+        constexpr int kBins = 12;
+        float best = FLT_MAX;
+        int best_ax = -1, best_bin = 0;
+        float cmin[3], cmax[3];
+        for (int k = 0; k < 3; ++k) {
+            cmin[k] = FLT_MAX;
+            cmax[k] = -FLT_MAX;
+        }
+        for (uint32_t i = lo; i < hi; ++i)
+            for (int k = 0; k < 3; ++k) {
+                const float c = 0.5f * (boxes[6 * ids[i] + k] + boxes[6 * ids[i] + 3 + k]);
+                cmin[k] = std::min(cmin[k], c);
+                cmax[k] = std::max(cmax[k], c);
+            }
+        for (int ax = 0; ax < 3; ++ax) {
+            const float ext = cmax[ax] - cmin[ax];
+            if (!(ext > 0.0f)) continue;
+            float bmn[kBins][3], bmx[kBins][3];
+            uint32_t cnt[kBins] = {};
+            for (int b = 0; b < kBins; ++b)
+                for (int k = 0; k < 3; ++k) {
+                    bmn[b][k] = FLT_MAX;
+                    bmx[b][k] = -FLT_MAX;
+                }
+            for (uint32_t i = lo; i < hi; ++i) {
+                const float* bx = &boxes[6 * ids[i]];
+                const float c = 0.5f * (bx[ax] + bx[3 + ax]);
+                int b = static_cast<int>((c - cmin[ax]) / ext * kBins);
+                b = std::min(std::max(b, 0), kBins - 1);
+                cnt[b]++;
+                for (int k = 0; k < 3; ++k) {
+                    bmn[b][k] = std::min(bmn[b][k], bx[k]);
+                    bmx[b][k] = std::max(bmx[b][k], bx[3 + k]);
+                }
+            }
+            for (int s = 1; s < kBins; ++s) {
+                float lmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+                float rmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, rmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+                uint32_t nl = 0, nr = 0;
+                for (int b = 0; b < kBins; ++b) {
+                    float* tmn = b < s ? lmn : rmn;
+                    float* tmx = b < s ? lmx : rmx;
+                    (b < s ? nl : nr) += cnt[b];
+                    for (int k = 0; k < 3; ++k) {
+                        tmn[k] = std::min(tmn[k], bmn[b][k]);
+                        tmx[k] = std::max(tmx[k], bmx[b][k]);
+                    }
+                }
+                if (!nl || !nr) continue;
+                const float cost = nl * area(lmn, lmx) + nr * area(rmn, rmx);
+                if (cost < best) {
+                    best = cost;
+                    best_ax = ax;
+                    best_bin = s;
+                }
+            }
+        }
+        uint32_t mid;
+        if (best_ax < 0) {
+            mid = lo + n / 2;  // coincident centres: split the list
+        } else {
+            const float ext = cmax[best_ax] - cmin[best_ax];
+            auto it = std::partition(ids.begin() + lo, ids.begin() + hi, [&](uint32_t id2) {
+                const float c = 0.5f * (boxes[6 * id2 + best_ax] + boxes[6 * id2 + 3 + best_ax]);
+                int b = static_cast<int>((c - cmin[best_ax]) / ext * kBins);
+                b = std::min(std::max(b, 0), kBins - 1);
+                return b < best_bin;
+            });
+            mid = static_cast<uint32_t>(it - ids.begin());
+            if (mid == lo || mid == hi) mid = lo + n / 2;
+        }
+        const int32_t a = build(lo, mid, level + 1, depth);
+        const int32_t b = build(mid, hi, level + 1, depth);
+        nodes[id].a = a;
+        nodes[id].b = b;
+        return id;
+    }
+};
+
+}  // namespace
+
+uint32_t build_tlas(const std::vector<float>& boxes, std::vector<float4>& pairs,
+                    std::vector<uint32_t>& indices, float root_min[3], float root_max[3],
+                    uint32_t& root_word) {
+    TBuild tb(boxes);
+    const uint32_t n = static_cast<uint32_t>(boxes.size() / 6);
+    tb.ids.resize(n);
+    for (uint32_t i = 0; i < n; ++i) tb.ids[i] = i;
+    uint32_t depth = 1;
+    const int32_t root = tb.build(0, n, 1, depth);
+    indices = tb.ids;
+    // flatten: every inner node's children become one pair record, pair index = word
+    pairs.clear();
+    auto word_of = [&](int32_t id, auto& self) -> uint32_t {
+        const TBuild::Node& nd = tb.nodes[id];
+        if (nd.a < 0) return kLeafBit | (nd.count << 24) | nd.first;
+        const uint32_t q = static_cast<uint32_t>(pairs.size() / 4);
+        pairs.resize(pairs.size() + 4);
+        const uint32_t wa = self(nd.a, self), wb = self(nd.b, self);
+        const TBuild::Node& A = tb.nodes[nd.a];
+        const TBuild::Node& B = tb.nodes[nd.b];
+        float4* r = &pairs[4 * q];
+        r[0] = make_float4(A.mn[0], A.mn[1], A.mn[2], 0.0f);
+        r[1] = make_float4(A.mx[0], A.mx[1], A.mx[2], 0.0f);
+        r[2] = make_float4(B.mn[0], B.mn[1], B.mn[2], 0.0f);
+        r[3] = make_float4(B.mx[0], B.mx[1], B.mx[2], 0.0f);
+        std::memcpy(&r[0].w, &wa, 4);
+        std::memcpy(&r[1].w, &wb, 4);
+        return q;
+    };
+    root_word = word_of(root, word_of);
+    std::memcpy(root_min, tb.nodes[root].mn, sizeof(float) * 3);
+    std::memcpy(root_max, tb.nodes[root].mx, sizeof(float) * 3);
+    return depth;
+}
+
 namespace {
 
 template <typename T>
@@ -261,6 +423,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     if (d->instance_count && !d->instances) return fail(MT_E_INVALID, "instances is NULL");
     if (d->material_count && !d->materials) return fail(MT_E_INVALID, "materials is NULL");
     if (d->light_count && !d->lights) return fail(MT_E_INVALID, "lights is NULL");
+    if (d->instance_count >= (1u << 24)) return fail(MT_E_UNSUPPORTED, "more than 2^24 instances");
     if (d->image_count && !d->images) return fail(MT_E_INVALID, "images is NULL");
     s->mesh_count = d->mesh_count;
     s->instance_count = d->instance_count;
@@ -411,6 +574,20 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         }
     }
 
+    // top-level BVH for scenes with more instances than one wave-wide mask covers
+    std::vector<float4> tlas_pairs;
+    std::vector<uint32_t> tlas_inst;
+    s->dev.use_tlas = d->instance_count > 64 ? 1u : 0u;
+    if (s->dev.use_tlas) {
+        std::vector<float> boxes(static_cast<size_t>(d->instance_count) * 6);
+        for (uint32_t i = 0; i < d->instance_count; ++i) {
+            std::memcpy(&boxes[6 * i], inst[i].wmin, sizeof(float) * 3);
+            std::memcpy(&boxes[6 * i + 3], inst[i].wmax, sizeof(float) * 3);
+        }
+        s->tlas_depth = build_tlas(boxes, tlas_pairs, tlas_inst, s->dev.tlas_root_min,
+                                   s->dev.tlas_root_max, s->dev.tlas_root_word);
+    }
+
     std::vector<DevImage> images(d->image_count);
     std::vector<uint32_t> texels;
     for (uint32_t i = 0; i < d->image_count; ++i) {
@@ -500,6 +677,8 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     if ((rc = dev_upload(s, images, &s->dev.images))) return rc;
     if ((rc = dev_upload(s, env, &s->dev.env))) return rc;
     if ((rc = dev_upload(s, lights, &s->dev.lights))) return rc;
+    if ((rc = dev_upload(s, tlas_pairs, &s->dev.tlas_pairs))) return rc;
+    if ((rc = dev_upload(s, tlas_inst, &s->dev.tlas_inst))) return rc;
     s->dev.instance_count = d->instance_count;
     s->dev.light_count = d->light_count;
     return MT_OK;

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import abi
 from .scene import (BvhOptions, Camera, Environment, Instance, Light, Material, Mesh, Scene,
-                    Settings, look_at, trs)
+                    Settings, Texture, look_at, trs)
 
 
 def _quad(p0, p1, p2, p3, n):
@@ -172,6 +172,149 @@ def sphere_grid(seed=1, env_width=2048, env_height=1024, segments=64, rings=32):
 
 def sphere_grid_settings(**kw):
     s = Settings(width=1920, height=1080, samples=256, bounces=5, hdri_rotation=30.0,
+                 hdri_rotation_apply=True)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+# ------------------------------------------------------------------ large terrain scene
+def _value_noise(res, cells, rng):
+    """bilinearly interpolated random lattice of cells x cells values, sampled at res x res."""
+    g = rng.random((cells + 1, cells + 1))
+    t = np.linspace(0, cells, res, endpoint=False)
+    i = t.astype(np.int64)
+    f = t - i
+    f = f * f * (3 - 2 * f)
+    a = g[np.ix_(i, i)]
+    b = g[np.ix_(i, i + 1)]
+    c = g[np.ix_(i + 1, i)]
+    d = g[np.ix_(i + 1, i + 1)]
+    fx = f[None, :]
+    fy = f[:, None]
+    return (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy
+
+
+def fbm(res, octaves, rng, base_cells=4):
+    h = np.zeros((res, res))
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        cells = min(base_cells * (2 ** o), res - 1)
+        h += amp * _value_noise(res, cells, rng)
+        total += amp
+        amp *= 0.5
+    return h / total
+
+
+def terrain_mesh(res, extent, height, rng, uv_repeat=8.0):
+    """heightfield of res x res vertices over [-extent, extent]^2: 2 (res-1)^2 triangles."""
+    h = (fbm(res, 6, rng) - 0.5) * height
+    xs = np.linspace(-extent, extent, res)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    P = np.stack([X, h, Z], -1)
+    dx = (xs[1] - xs[0])
+    gx = np.gradient(h, dx, axis=1)
+    gz = np.gradient(h, dx, axis=0)
+    N = np.stack([-gx, np.ones_like(h), -gz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    U = np.stack([(X + extent) / (2 * extent) * uv_repeat, (Z + extent) / (2 * extent) * uv_repeat], -1)
+    i0 = np.arange(res - 1)
+    I, J = np.meshgrid(i0, i0, indexing="ij")
+    a = (I * res + J).ravel()
+    b = a + 1
+    c = a + res
+    d = c + 1
+    idx = np.stack([np.stack([a, c, b], 1), np.stack([b, c, d], 1)], 1).reshape(-1)
+    Pf, Nf, Uf = P.reshape(-1, 3), N.reshape(-1, 3), U.reshape(-1, 2)
+    return Mesh(name="terrain", positions=Pf[idx].astype(np.float32), normals=Nf[idx].astype(np.float32),
+                texcoord0=Uf[idx].astype(np.float32)), h, xs
+
+
+def icosphere(subdiv=3, name="icosphere"):
+    """subdivided icosahedron on the unit sphere: 20 * 4^subdiv triangles, spherical UVs."""
+    t = (1 + 5 ** 0.5) / 2
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+         (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    tris = np.array([[V[i] for i in f] for f in F], np.float64)
+    tris /= np.linalg.norm(tris, axis=2, keepdims=True)
+    for _ in range(subdiv):
+        a, b, c = tris[:, 0], tris[:, 1], tris[:, 2]
+        ab, bc, ca = a + b, b + c, c + a
+        ab /= np.linalg.norm(ab, axis=1, keepdims=True)
+        bc /= np.linalg.norm(bc, axis=1, keepdims=True)
+        ca /= np.linalg.norm(ca, axis=1, keepdims=True)
+        tris = np.stack([np.stack([a, ab, ca], 1), np.stack([ab, b, bc], 1),
+                         np.stack([ca, bc, c], 1), np.stack([ab, bc, ca], 1)], 1).reshape(-1, 3, 3)
+    P = tris.reshape(-1, 3)
+    uv = np.stack([np.arctan2(P[:, 2], P[:, 0]) / (2 * np.pi) + 0.5, np.arccos(np.clip(P[:, 1], -1, 1)) / np.pi], 1)
+    return Mesh(name=name, positions=P.astype(np.float32), normals=P.astype(np.float32),
+                texcoord0=uv.astype(np.float32))
+
+
+def noise_texture(res, rng, tint):
+    n = fbm(res, 5, rng, base_cells=8)[..., None]
+    img = np.clip(n * np.asarray(tint) * 1.4, 0, 1)
+    return (img * 255).astype(np.uint8)
+
+
+def large_scene(seed=2, terrain_res=2048, n_spheres=5000, tex_res=2048, env_width=2048,
+                env_height=1024, n_lights=16, extent=100.0):
+    """The large-scene workload: fBm terrain (2 (res-1)^2 triangles), icospheres of 1280
+    triangles at seeded positions with log-uniform radii in [0.2, 3], per-object roughness
+    U[0.05, 1] and metallic Bernoulli(0.3), four value-noise base-colour textures, a
+    metallic-roughness map and a tangent-space normal map, 16 point lights and the procedural
+    sky; BVH leaves of at most 4 triangles, depth 32."""
+    rng = np.random.default_rng(seed)
+    terrain, h, xs = terrain_mesh(terrain_res, extent, height=20.0, rng=rng)
+    sphere = icosphere(3)
+    tints = [(0.6, 0.5, 0.35), (0.8, 0.3, 0.2), (0.3, 0.5, 0.8), (0.5, 0.8, 0.4)]
+    images = [noise_texture(tex_res, rng, t) for t in tints]
+    mr = np.zeros((tex_res, tex_res, 3), np.uint8)
+    mr[..., 1] = (fbm(tex_res, 4, rng, 8) * 255).astype(np.uint8)   # roughness in G
+    mr[..., 2] = (fbm(tex_res, 4, rng, 8) > 0.6).astype(np.uint8) * 255  # metallic in B
+    images.append(mr)
+    hn = fbm(tex_res, 5, rng, 16)
+    gy, gx = np.gradient(hn)
+    nrm = np.stack([-gx * 40, -gy * 40, np.ones_like(hn)], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    images.append(((nrm * 0.5 + 0.5) * 255).astype(np.uint8))
+    materials = [Material("terrain", (1.0, 1.0, 1.0, 1.0), metallic_factor=1.0, roughness_factor=1.0,
+                          base_color_tex=Texture(0), metallic_roughness_tex=Texture(4), normal_tex=Texture(5))]
+    instances = [Instance(0, 0, np.eye(4, dtype=np.float32))]
+    pos = rng.uniform(-extent * 0.9, extent * 0.9, (n_spheres, 2))
+    radii = np.exp(rng.uniform(np.log(0.2), np.log(3.0), n_spheres))
+    rough = rng.uniform(0.05, 1.0, n_spheres)
+    metal = (rng.random(n_spheres) < 0.3).astype(np.float64)
+    cols = rng.uniform(0.3, 1.0, (n_spheres, 3))
+    step = xs[1] - xs[0]
+    for i in range(n_spheres):
+        gi = int(np.clip((pos[i, 1] + extent) / step, 0, terrain_res - 1))
+        gj = int(np.clip((pos[i, 0] + extent) / step, 0, terrain_res - 1))
+        y = h[gi, gj] + radii[i] * 0.6
+        materials.append(Material(f"sphere_{i}", (float(cols[i, 0]), float(cols[i, 1]), float(cols[i, 2]), 1.0),
+                                  metallic_factor=float(metal[i]), roughness_factor=float(rough[i]),
+                                  base_color_tex=Texture(1 + i % 3)))
+        instances.append(Instance(1, len(materials) - 1,
+                                  trs((pos[i, 0], y, pos[i, 1]), float(rng.uniform(0, 360)), (radii[i],) * 3)))
+    lights = []
+    for i in range(n_lights):
+        lx, lz = rng.uniform(-extent * 0.7, extent * 0.7, 2)
+        col = rng.uniform(0.6, 1.0, 3) * 5000.0
+        lights.append(Light(type=abi.MT_LIGHT_POINT, position=(float(lx), float(rng.uniform(15, 30)), float(lz)),
+                            color=tuple(float(c) for c in col), point_size=0.5))
+    cam = Camera(type=abi.MT_CAMERA_PERSPECTIVE,
+                 camera_to_world=look_at((0.0, 35.0, extent * 1.05), (0.0, 0.0, 0.0)),
+                 yfov=float(np.radians(50.0)), aspect=16.0 / 9.0)
+    env = Environment(texels=procedural_sky(env_width, env_height), filter=abi.MT_FILTER_LINEAR)
+    return Scene(meshes=[terrain, sphere], images=images, materials=materials, instances=instances,
+                 lights=lights, camera=cam, env=env, bvh=BvhOptions(False, 4, 32))
+
+
+def large_scene_settings(**kw):
+    s = Settings(width=1920, height=1080, samples=64, bounces=5, hdri_rotation=30.0,
                  hdri_rotation_apply=True)
     for k, v in kw.items():
         setattr(s, k, v)

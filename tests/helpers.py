@@ -37,6 +37,9 @@ def load_scene(name, bvh=None):
         return scenegen.cornell()
     if name == "sphere_grid":
         return scenegen.sphere_grid(env_width=256, env_height=128)
+    if name == "large_small":
+        return scenegen.large_scene(terrain_res=96, n_spheres=300, tex_res=64, env_width=64,
+                                    env_height=32)
     if name == "sphere_grid_coarse":
         return scenegen.sphere_grid(env_width=128, env_height=64, segments=16, rings=8)
     raise KeyError(name)

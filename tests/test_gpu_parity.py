@@ -18,7 +18,7 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 SCENES = ["cornell_gen", "cornell_box", "light_creep", "material_spheres",
-          "material_spheres_bvh4", "sphere_grid"]
+          "material_spheres_bvh4", "sphere_grid", "large_small"]
 
 
 @pytest.fixture(scope="module")
@@ -113,6 +113,7 @@ def _render_pair(scenes, name, settings):
     ("material_spheres", dict(ev_exposure=9.0)),
     ("light_creep", dict(ev_exposure=5.5)),
     ("sphere_grid", dict(hdri_rotation=30.0, hdri_rotation_apply=True)),
+    ("large_small", dict(hdri_rotation=30.0, hdri_rotation_apply=True)),
 ])
 def test_render_matches_oracle(scenes, name, kw):
     st = small_settings(width=40, height=30, samples=16, **kw)
