@@ -232,8 +232,34 @@ typedef struct mt_hit {
     uint32_t triangle_tests; /* triangle tests in the winning instance (Intersect.cpp:93) */
 } mt_hit;
 
+/* Differentiable-rendering parameters (no reference counterpart: the inverse-rendering form).
+ * All pointers are DEVICE pointers (e.g. torch tensors' data_ptr()); NULL keeps the scene's
+ * value.  Materials [tex_material_first, tex_material_first + tex_material_count) take their
+ * base colour from base_texture (RGB float, sampled like the reference's "linear" filter). */
+typedef struct mt_diff_params {
+    const float* base_texture_dev; /* tex_height*tex_width*3 */
+    uint32_t tex_width;
+    uint32_t tex_height;
+    uint32_t tex_material_first;
+    uint32_t tex_material_count;
+    const float* roughness_dev; /* material_count */
+    const float* metallic_dev;  /* material_count */
+    const float* emission_dev;  /* material_count*3 */
+    uint32_t material_count;
+} mt_diff_params;
+
+/* Gradient accumulators (DEVICE pointers, same layouts as mt_diff_params; NULL = skip).
+ * mt_diff_backward ADDS to them. */
+typedef struct mt_diff_grads {
+    float* base_texture_dev;
+    float* roughness_dev;
+    float* metallic_dev;
+    float* emission_dev;
+} mt_diff_grads;
+
 typedef struct mt_scene mt_scene;
 typedef struct mt_job mt_job;
+typedef struct mt_diff mt_diff;
 
 /* ---- functions -------------------------------------------------------------------------- */
 int mt_abi_version(void);
@@ -272,6 +298,24 @@ int mt_trace_closest(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hit
  * {L.xyz, V.xyz, N.xyz, base.rgb, roughness, metallic}; out: 5 floats per case
  * {f.rgb, diffuse_pdf, specular_pdf}. */
 int mt_brdf_eval(uint32_t n, const float* in, float* out);
+
+/* Path-replay adjoint (no reference counterpart; for inverse rendering).  mt_diff_create
+ * sizes per-path storage (16 bytes per camera path: linear radiance and shadow visibility
+ * bits) for settings->width*height*samples paths; the estimator is the one of *settings
+ * with MT_OUTPUT_LINEAR and no firefly rejection.  mt_diff_forward writes the linear image
+ * (width*height*3 floats, device) and stores the per-path data; mt_diff_backward replays every
+ * path from its counter-RNG key and accumulates d(loss)/d(params) given grad_image_dev =
+ * d(loss)/d(image).  seed replaces settings->seed (a backward call must use the seed, camera
+ * and params of the forward call it follows).  stream: a hipStream_t to run on (0 = the
+ * library's own stream, after a device synchronisation); both calls return after their work
+ * on that stream has completed. */
+int mt_diff_create(mt_scene* scene, const mt_settings* settings, mt_diff** out);
+int mt_diff_forward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_params* params,
+                    uint64_t seed, float* image_dev, uint64_t stream);
+int mt_diff_backward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_params* params,
+                     uint64_t seed, const float* grad_image_dev, const mt_diff_grads* grads,
+                     uint64_t stream);
+int mt_diff_destroy(mt_diff* diff);
 
 #ifdef __cplusplus
 }

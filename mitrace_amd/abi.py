@@ -127,6 +127,20 @@ class mt_hit(C.Structure):
                 ("bvh_tests", C.c_uint32), ("triangle_tests", C.c_uint32)]
 
 
+class mt_diff_params(C.Structure):
+    """device pointers (integers from torch's data_ptr()); 0 keeps the scene's value"""
+    _fields_ = [("base_texture_dev", C.c_void_p), ("tex_width", C.c_uint32),
+                ("tex_height", C.c_uint32), ("tex_material_first", C.c_uint32),
+                ("tex_material_count", C.c_uint32), ("roughness_dev", C.c_void_p),
+                ("metallic_dev", C.c_void_p), ("emission_dev", C.c_void_p),
+                ("material_count", C.c_uint32)]
+
+
+class mt_diff_grads(C.Structure):
+    _fields_ = [("base_texture_dev", C.c_void_p), ("roughness_dev", C.c_void_p),
+                ("metallic_dev", C.c_void_p), ("emission_dev", C.c_void_p)]
+
+
 # every exported symbol of the library and its signature: (restype, argtypes)
 SIGNATURES = {
     "mt_abi_version": (C.c_int, []),
@@ -149,6 +163,14 @@ SIGNATURES = {
     "mt_trace_closest": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
                                    C.POINTER(mt_hit)]),
     "mt_brdf_eval": (C.c_int, [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mt_diff_create": (C.c_int, [C.c_void_p, C.POINTER(mt_settings), C.POINTER(C.c_void_p)]),
+    "mt_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(mt_camera_desc),
+                                  C.POINTER(mt_diff_params), C.c_uint64, C.c_void_p,
+                                  C.c_uint64]),
+    "mt_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(mt_camera_desc),
+                                   C.POINTER(mt_diff_params), C.c_uint64, C.c_void_p,
+                                   C.POINTER(mt_diff_grads), C.c_uint64]),
+    "mt_diff_destroy": (C.c_int, [C.c_void_p]),
 }
 
 

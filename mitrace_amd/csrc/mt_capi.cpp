@@ -44,6 +44,14 @@ struct mt_job {
     std::chrono::steady_clock::time_point start;
 };
 
+// Differentiable renderer behind mt_diff*: a Renderer set up once with the per-path store.
+struct mt_diff {
+    mt_scene* scene = nullptr;
+    mt_settings st{};
+    mt::Renderer renderer;
+    bool ready = false;
+};
+
 namespace {
 
 int run_job(mt_job* j, float* sample_rgb_out) {
@@ -85,6 +93,31 @@ bool valid_args(mt_scene* scene, const mt_camera_desc* cam, const mt_settings* s
         return false;
     }
     return true;
+}
+
+int diff_params(const mt_diff* d, const mt_diff_params* p, mt::DiffParams& out) {
+    out = mt::DiffParams{};
+    if (!p) return MT_OK;
+    const uint32_t nm = d->scene->material_count;
+    if ((p->roughness_dev || p->metallic_dev || p->emission_dev) && p->material_count != nm)
+        return mt::fail(MT_E_INVALID, "mt_diff_params.material_count does not match the scene");
+    if (p->base_texture_dev) {
+        if (p->tex_width == 0 || p->tex_height == 0)
+            return mt::fail(MT_E_INVALID, "base texture needs a positive width and height");
+        if (p->tex_material_first > nm || p->tex_material_count > nm - p->tex_material_first)
+            return mt::fail(MT_E_INVALID, "textured material range exceeds the scene's materials");
+        if (static_cast<uint64_t>(p->tex_width) * p->tex_height * 3 > 0xFFFFFFFFull)
+            return mt::fail(MT_E_INVALID, "base texture too large");
+    }
+    out.tex = p->base_texture_dev;
+    out.tex_w = p->tex_width;
+    out.tex_h = p->tex_height;
+    out.tex_first = p->tex_material_first;
+    out.tex_count = p->base_texture_dev ? p->tex_material_count : 0;
+    out.rough = p->roughness_dev;
+    out.metal = p->metallic_dev;
+    out.emis = p->emission_dev;
+    return MT_OK;
 }
 
 }  // namespace
@@ -229,6 +262,85 @@ int mt_brdf_eval(uint32_t n, const float* in, float* out) {
     } catch (const std::exception& e) {
         return mt::fail(MT_E_INVALID, e.what());
     }
+}
+
+int mt_diff_create(mt_scene* scene, const mt_settings* settings, mt_diff** out) {
+    if (!out) return mt::fail(MT_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (!scene || !settings) return mt::fail(MT_E_INVALID, "null scene or settings");
+    if (settings->bounces > 32)
+        return mt::fail(MT_E_UNSUPPORTED, "differentiable rendering supports at most 32 bounces");
+    try {
+        auto d = std::make_unique<mt_diff>();
+        d->scene = scene;
+        d->st = *settings;
+        if (hipSetDevice(scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+        mt_camera_desc cam{};
+        cam.type = MT_CAMERA_PERSPECTIVE;
+        cam.yfov = 1.0f;
+        cam.aspect = 1.0f;
+        for (int i = 0; i < 4; ++i) cam.camera_to_world[i * 5] = 1.0f;
+        const int rc = d->renderer.setup(scene, cam, d->st, true);
+        if (rc) return rc;
+        d->ready = true;
+        *out = d.release();
+        return MT_OK;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_NOMEM, e.what());
+    }
+}
+
+int mt_diff_forward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_params* params,
+                    uint64_t seed, float* image_dev, uint64_t stream) {
+    if (!diff || !diff->ready || !camera || !image_dev)
+        return mt::fail(MT_E_INVALID, "null diff handle, camera or image");
+    try {
+        mt::DiffParams dp;
+        int rc = diff_params(diff, params, dp);
+        if (rc) return rc;
+        if (hipSetDevice(diff->scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+        if (!stream && hipDeviceSynchronize() != hipSuccess)
+            return mt::fail(MT_E_HIP, "hipDeviceSynchronize failed");
+        diff->renderer.set_camera(*camera);
+        diff->renderer.set_seed(seed);
+        rc = diff->renderer.diff_forward(dp, image_dev, reinterpret_cast<hipStream_t>(stream));
+        return rc;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_diff_backward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_params* params,
+                     uint64_t seed, const float* grad_image_dev, const mt_diff_grads* grads,
+                     uint64_t stream) {
+    if (!diff || !diff->ready || !camera || !grad_image_dev || !grads)
+        return mt::fail(MT_E_INVALID, "null diff handle, camera, gradient image or gradients");
+    try {
+        mt::DiffParams dp;
+        int rc = diff_params(diff, params, dp);
+        if (rc) return rc;
+        mt::DiffGradPtrs gr{};
+        gr.tex = dp.tex ? grads->base_texture_dev : nullptr;
+        gr.rough = grads->roughness_dev;
+        gr.metal = grads->metallic_dev;
+        gr.emis = grads->emission_dev;
+        if ((gr.rough || gr.metal || gr.emis) && (!params || params->material_count != diff->scene->material_count))
+            return mt::fail(MT_E_INVALID, "per-material gradients need material_count in the params");
+        if (hipSetDevice(diff->scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+        if (!stream && hipDeviceSynchronize() != hipSuccess)
+            return mt::fail(MT_E_HIP, "hipDeviceSynchronize failed");
+        diff->renderer.set_camera(*camera);
+        diff->renderer.set_seed(seed);
+        rc = diff->renderer.diff_adjoint(dp, gr, grad_image_dev, reinterpret_cast<hipStream_t>(stream));
+        return rc;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_diff_destroy(mt_diff* diff) {
+    delete diff;
+    return MT_OK;
 }
 
 }  // extern "C"

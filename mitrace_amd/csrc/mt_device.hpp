@@ -184,6 +184,69 @@ __device__ V3 eval_brdf(V3 L, V3 V, V3 N, V3 base, float rough, float metal) {
     const V3 spec = divs(scale(f, d * g), gmax(4.0f * nv * nl, 0.001f));
     return vmax(add(diffuse, spec), mk(0.0f));
 }
+// EvaluateBRDF with its partial derivatives (for the adjoint): df_dbase[c] = d f_c / d base_c
+// (channels are independent), df_drough, df_dmetal per channel.  A clamp contributes the
+// derivative of the branch it selects: zero for roughness < 0.05, metallic outside [0, 1],
+// a negative sum in the final max(., 0), and NL/NV <= 0.
+struct BrdfGrad {
+    V3 f, df_dbase, df_drough, df_dmetal;
+};
+__device__ BrdfGrad eval_brdf_grad(V3 L, V3 V, V3 N, V3 base, float rough_in, float metal) {
+    BrdfGrad g;
+    g.f = g.df_dbase = g.df_drough = g.df_dmetal = mk(0.0f);
+    const float rough = gmax(rough_in, 0.05f);
+    const float nl = gmax(dot(N, L), 0.0f);
+    const float nv = gmax(dot(N, V), 0.0f);
+    if (nl <= 0.0f || nv <= 0.0f) return g;
+    const V3 h = normalize(add(V, L));
+    const float nh = clampf(gmax(dot(N, h), 0.0f), 0.0f, 1.0f);
+    const float hv = clampf(gmax(dot(h, V), 0.0f), 0.0f, 1.0f);
+    const float mc = clampf(metal, 0.0f, 1.0f);
+    const float p5 = powf(1.0f - hv, 5.0f);
+    const float a = rough * rough;
+    const float a2 = a * a;
+    const float den = (nh * nh) * (a2 - 1.0f) + 1.0f;
+    const float D = a2 / (kPi * den * den);
+    const float k = ((rough + 1.0f) * (rough + 1.0f)) / 8.0f;
+    const float q1 = nv * (1.0f - k) + k, q2 = nl * (1.0f - k) + k;
+    const float g1 = nv / q1, g2 = nl / q2;
+    const float G = g1 * g2;
+    const float den4 = gmax(4.0f * nv * nl, 0.001f);
+    // roughness derivatives (zero below the 0.05 clamp)
+    float dD = 0.0f, dG = 0.0f;
+    if (!(rough_in < 0.05f)) {  // the max(rough, 0.05) passes rough through
+        const float dD_da2 = (den - 2.0f * a2 * nh * nh) / (kPi * den * den * den);
+        dD = dD_da2 * 4.0f * rough * rough * rough;
+        const float dk = (rough + 1.0f) / 4.0f;
+        const float dg1 = -nv * (1.0f - nv) / (q1 * q1) * dk;
+        const float dg2 = -nl * (1.0f - nl) / (q2 * q2) * dk;
+        dG = dg1 * g2 + g1 * dg2;
+    }
+    const float bc[3] = {base.x, base.y, base.z};
+    float fo[3], db[3], dr[3], dm[3];
+    for (int c = 0; c < 3; ++c) {
+        const float f0 = 0.04f * (1.0f - mc) + bc[c] * mc;
+        const float F = f0 + (1.0f - f0) * p5;
+        const float kd = (1.0f - F) * (1.0f - metal);
+        const float diffuse = kd * bc[c] / kPi;
+        const float spec = D * G * F / den4;
+        fo[c] = gmax(diffuse + spec, 0.0f);
+        const bool live = !((diffuse + spec) < 0.0f);  // max(x, 0) passes x through
+        const float dF_db = mc * (1.0f - p5);
+        const float dkd_db = -dF_db * (1.0f - metal);
+        db[c] = live ? (dkd_db * bc[c] + kd) / kPi + D * G * dF_db / den4 : 0.0f;
+        dr[c] = live ? (dD * G + D * dG) * F / den4 : 0.0f;
+        const float dF_dm = (metal >= 0.0f && metal <= 1.0f) ? (bc[c] - 0.04f) * (1.0f - p5) : 0.0f;
+        const float dkd_dm = -dF_dm * (1.0f - metal) - (1.0f - F);
+        dm[c] = live ? dkd_dm * bc[c] / kPi + D * G * dF_dm / den4 : 0.0f;
+    }
+    g.f = {fo[0], fo[1], fo[2]};
+    g.df_dbase = {db[0], db[1], db[2]};
+    g.df_drough = {dr[0], dr[1], dr[2]};
+    g.df_dmetal = {dm[0], dm[1], dm[2]};
+    return g;
+}
+
 __device__ __forceinline__ float diffuse_pdf(V3 L, V3 N) { return gmax(dot(N, L), 0.0f) / kPi; }
 __device__ float specular_pdf(V3 L, V3 V, V3 N, float rough) {
     const float nl = gmax(dot(N, L), 0.0f);

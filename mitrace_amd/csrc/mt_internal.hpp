@@ -153,6 +153,7 @@ struct mt_scene {
     uint32_t mesh_count = 0;
     uint32_t instance_count = 0;
     uint32_t light_count = 0;
+    uint32_t material_count = 0;
     uint64_t prim_slots = 0;
     uint64_t node_count = 0;
     uint64_t triangle_count = 0;

@@ -104,6 +104,11 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
     b.acc_d[p] = z;
     b.acc_i[p] = z;
     b.acc_s[p] = z;
+    if (c.diff) {
+        b.vis[p] = 0u;
+        b.adj_s[p] = z;
+        b.adj_prev[p] = make_float4(0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu), 0.0f);
+    }
     b.queue[0][p] = p;
 }
 
@@ -263,6 +268,8 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                         float4* acc = (__float_as_uint(cc.w) == 0u) ? b.acc_d : b.acc_i;
                         const float4 a = acc[p];
                         acc[p] = make_float4(a.x + cc.x, a.y + cc.y, a.z + cc.z, 0.0f);
+                        // one shadow ray per path and bounce: no other lane writes this word
+                        if (c.diff) b.vis[p] |= 1u << bounce;
                     }
                 } else {
                     mt_hit r;
@@ -322,38 +329,132 @@ void launch_trace(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, co
 #undef MT_TRACE_CASE
 }
 
-__device__ void eval_material(const DevScene& S, const DevMaterial& m, float u, float v, V3& base,
-                              float& metal, float& rough, V3& ntex, V3& emission) {
-    // Material::SampleMaterial (Material.hpp:82-135)
-    base = V3{m.base[0], m.base[1], m.base[2]};
-    if (m.tex[0].image >= 0) {
-        const float4 t = sample_tex(S, m.tex[0], u, v);
-        base = V3{t.x * m.base[0], t.y * m.base[1], t.z * m.base[2]};
+// Material::SampleMaterial (Material.hpp:82-135) plus the differentiable overrides.
+struct MatSample {
+    V3 base, ntex, emission;
+    float metal, rough;
+    // adjoint bookkeeping: scale of each parameter in the sampled value and the texel taps of
+    // the base-colour texture override
+    float rough_scale, metal_scale;
+    V3 em_scale;
+    int tex;            // 1 when the base colour comes from the differentiable texture
+    uint32_t tap[4];    // texel indices
+    float tw[4];        // bilinear weights (Image.cpp:26-44 weights in uv units)
+};
+
+__device__ __forceinline__ void sample_param_texture(const DiffParams& dp, float uvx, float uvy,
+                                                     MatSample& ms) {
+    const int w = static_cast<int>(dp.tex_w), h = static_cast<int>(dp.tex_h);
+    const float x = uvx - floorf(uvx);
+    const float y = uvy - floorf(uvy);
+    int x0 = static_cast<int>(floorf(x * static_cast<float>(w)));
+    int y0 = static_cast<int>(floorf(y * static_cast<float>(h)));
+    x0 = x0 < w - 1 ? x0 : w - 1;
+    y0 = y0 < h - 1 ? y0 : h - 1;
+    x0 = x0 > 0 ? x0 : 0;
+    y0 = y0 > 0 ? y0 : 0;
+    const int x1 = x0 + 1 < w - 1 ? x0 + 1 : w - 1;
+    const int y1 = y0 + 1 < h - 1 ? y0 + 1 : h - 1;
+    const float sx = x - static_cast<float>(x0) / static_cast<float>(w);
+    const float sy = y - static_cast<float>(y0) / static_cast<float>(h);
+    ms.tap[0] = y0 * w + x0;
+    ms.tap[1] = y0 * w + x1;
+    ms.tap[2] = y1 * w + x0;
+    ms.tap[3] = y1 * w + x1;
+    ms.tw[0] = (1.0f - sx) * (1.0f - sy);
+    ms.tw[1] = sx * (1.0f - sy);
+    ms.tw[2] = (1.0f - sx) * sy;
+    ms.tw[3] = sx * sy;
+    float c[3];
+    for (int ch = 0; ch < 3; ++ch) {
+        const float c00 = dp.tex[ms.tap[0] * 3 + ch], c10 = dp.tex[ms.tap[1] * 3 + ch];
+        const float c01 = dp.tex[ms.tap[2] * 3 + ch], c11 = dp.tex[ms.tap[3] * 3 + ch];
+        const float c0 = mixf(c00, c10, sx), c1 = mixf(c01, c11, sx);
+        c[ch] = mixf(c0, c1, sy);
     }
+    ms.base = {c[0], c[1], c[2]};
+    ms.tex = 1;
+}
+
+template <int kDiff>
+__device__ void eval_material(const DevScene& S, const DiffParams& dp, uint32_t mi, float u,
+                              float v, MatSample& ms) {
+    const DevMaterial& m = S.materials[mi];
+    ms.tex = 0;
+    ms.base = V3{m.base[0], m.base[1], m.base[2]};
+    if (kDiff && dp.tex && mi >= dp.tex_first && mi - dp.tex_first < dp.tex_count) {
+        sample_param_texture(dp, u, v, ms);
+    } else if (m.tex[0].image >= 0) {
+        const float4 t = sample_tex(S, m.tex[0], u, v);
+        ms.base = V3{t.x * m.base[0], t.y * m.base[1], t.z * m.base[2]};
+    }
+    const float rf = (kDiff && dp.rough) ? dp.rough[mi] : m.roughness;
+    const float mf = (kDiff && dp.metal) ? dp.metal[mi] : m.metallic;
     if (m.tex[1].image >= 0) {
         const float4 t = sample_tex(S, m.tex[1], u, v);
-        metal = t.z * m.metallic;
-        rough = t.y * m.roughness;
+        ms.metal = t.z * mf;
+        ms.rough = t.y * rf;
+        ms.metal_scale = t.z;
+        ms.rough_scale = t.y;
     } else {
-        metal = m.metallic;
-        rough = m.roughness;
+        ms.metal = mf;
+        ms.rough = rf;
+        ms.metal_scale = ms.rough_scale = 1.0f;
     }
     if (m.tex[2].image >= 0) {
         const float4 t = sample_tex(S, m.tex[2], u, v);
         const float ns = m.normal_scale;
-        ntex = normalize(V3{t.x * ns * 2.0f - 1.0f, t.y * ns * 2.0f - 1.0f, t.z * ns * 2.0f - 1.0f});
+        ms.ntex = normalize(V3{t.x * ns * 2.0f - 1.0f, t.y * ns * 2.0f - 1.0f, t.z * ns * 2.0f - 1.0f});
     } else {
-        ntex = V3{0.0f, 0.0f, 1.0f};
+        ms.ntex = V3{0.0f, 0.0f, 1.0f};
     }
-    const V3 em = {m.emissive[0], m.emissive[1], m.emissive[2]};
+    const V3 em = (kDiff && dp.emis) ? V3{dp.emis[mi * 3], dp.emis[mi * 3 + 1], dp.emis[mi * 3 + 2]}
+                                     : V3{m.emissive[0], m.emissive[1], m.emissive[2]};
     if (m.tex[3].image >= 0) {
         const float4 t = sample_tex(S, m.tex[3], u, v);
-        emission = mul(V3{t.x, t.y, t.z}, em);
+        ms.em_scale = V3{t.x, t.y, t.z};
+        ms.emission = mul(ms.em_scale, em);
     } else {
-        emission = em;
+        ms.em_scale = mk(1.0f);
+        ms.emission = em;
     }
 }
 
+// per-wave reduction of a float addend per key (a material id), then one atomic per key
+__device__ __forceinline__ void wave_atomic_by_key(float* base, uint32_t key, float v, bool pred) {
+    uint64_t todo = __ballot(pred);
+    while (todo) {
+        const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+        const uint32_t lk = __shfl(key, leader);
+        const bool mine = pred && key == lk;
+        float x = mine ? v : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if (static_cast<int>(lane_id()) == leader && x != 0.0f) atomicAdd(base + lk, x);
+        todo &= ~__ballot(mine);
+    }
+}
+
+// adjoint of the base colour b' = 0.96 * texel + 0.04 at a hit: d/d(texel) = 0.96 * tap weight
+__device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const MatSample& ms, V3 g) {
+    if (!gr.tex || !ms.tex) return;
+    const float gc[3] = {g.x * 0.96f, g.y * 0.96f, g.z * 0.96f};
+    for (int t = 0; t < 4; ++t) {
+        if (ms.tw[t] == 0.0f) continue;
+        for (int ch = 0; ch < 3; ++ch)
+            if (gc[ch] != 0.0f) atomicAdd(gr.tex + ms.tap[t] * 3 + ch, gc[ch] * ms.tw[t]);
+    }
+}
+
+// The shading of one bounce for every path in the extension queue (Tracer.cpp:134-340).
+// kDiff 0: plain render; 1: differentiable forward (parameter overrides, shadow visibility
+// bits); 2: adjoint replay -- the same decisions, no shadow rays (visibility comes from the
+// forward pass), and gradients of the per-path radiance accumulated with atomics:
+//   emission term  T*ebi*e          -> d/d(emission)
+//   light term     T*f_l*NL*fall*C/p -> d f_l / d(base, roughness, metallic)
+//   throughput w = f*cos/pdf        -> (L - S) * (dw/dtheta) / w with the sampling detached
+//   sky term       base'_{k-1}*T*sky -> d/d(base of the previous hit)
+// where T is the radiance throughput of the reference and S the radiance collected so far.
+template <int kDiff>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, PathBuffers b,
                                                   uint32_t k) {
     const uint32_t count = b.counters[kCntExtend + k];
@@ -362,14 +463,33 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
     const bool last = (k + 1 >= c.bounces);
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t hits = 0, misses = 0;
+    const DiffParams& dp = c.dp;
+    const DiffGradPtrs& gr = c.gr;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t idx = base + threadIdx.x;
         const bool valid = idx < count;
         bool cont = false, shadow = false;
         float4 sho{}, shd{}, shc{};
         uint32_t p = 0;
+        // adjoint addends for the per-material wave reductions
+        uint32_t gmat = 0;
+        float g_r = 0.0f, g_m = 0.0f;
+        V3 g_e = mk(0.0f);
+        bool gvalid = false;
         if (valid) {
             p = qin[idx];
+            // adjoint replay: dLoss/dL of this path and what the forward pass recorded for it
+            V3 dl = mk(0.0f), ltot = mk(0.0f);
+            uint32_t visbits = 0;
+            if (kDiff == 2) {
+                uint32_t px, sm;
+                path_coords(c, p, px, sm);
+                const float4 rec = c.path_store[static_cast<size_t>(px) * c.spp + sm];
+                ltot = xyz(rec);
+                visbits = __float_as_uint(rec.w);
+                const float* gi = c.grad_img + static_cast<size_t>(px) * 3;
+                dl = scale(V3{gi[0], gi[1], gi[2]}, c.inv_spp);
+            }
             const V3 d = xyz(b.ray_d[p]);
             const int inst = b.hit_inst[p];
             const V3 trad = xyz(b.t_rad[p]);
@@ -379,6 +499,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 const float mult = (k == 0) ? c.hdri_primary : c.hdri_secondary;
                 const V3 sk = scale(sample_env(S, d, c.hdri_rot_apply, c.hdri_rot_offset), mult);
                 b.acc_s[p] = f4(mul(mul(col, trad), sk), 0.0f);
+                if (kDiff == 2 && k > 0) {
+                    const float4 pv = b.adj_prev[p];
+                    const uint32_t pm = __float_as_uint(pv.z);
+                    if (pm != 0xFFFFFFFFu) {
+                        MatSample pms;
+                        eval_material<2>(S, dp, pm, pv.x, pv.y, pms);
+                        add_texel_grad(gr, pms, mul(dl, mul(trad, sk)));
+                    }
+                }
             } else {
                 hits++;
                 uint32_t pix, s;
@@ -408,14 +537,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     tu = (w * uv[0] + huv.x * uv[2]) + huv.y * uv[4];
                     tv = (w * uv[1] + huv.x * uv[3]) + huv.y * uv[5];
                 }
-                V3 base, ntex, emission;
-                float metal, rough;
-                eval_material(S, S.materials[I->material], tu, tv, base, metal, rough, ntex, emission);
-                base = add(scale(base, 0.96f), mk(0.04f));  // Tracer.cpp:159
-                const V3 N = shading_normal(gN, ntex);
+                const uint32_t mi = I->material;
+                MatSample ms;
+                eval_material<kDiff>(S, dp, mi, tu, tv, ms);
+                const V3 basec = add(scale(ms.base, 0.96f), mk(0.04f));  // Tracer.cpp:159
+                const float metal = ms.metal, rough = ms.rough;
+                const V3 N = shading_normal(gN, ms.ntex);
                 const V3 view = neg(d);
                 // BRDF sampling (Tracer.cpp:177-200)
-                const float ps = spec_probability(view, N, base, metal);
+                const float ps = spec_probability(view, N, basec, metal);
                 V3 nd;
                 if (draw(key, dim0 + kDimLobe) < ps) {
                     const float u1 = draw(key, dim0 + kDimDir), u2 = draw(key, dim0 + kDimDir + 1);
@@ -425,12 +555,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     nd = cosine_dir(draw(key, dim0 + kDimDir), draw(key, dim0 + kDimDir + 1), N);
                 }
                 const float pdf = (1.0f - ps) * diffuse_pdf(nd, N) + ps * specular_pdf(nd, view, N, rough);
-                const V3 fb = eval_brdf(nd, view, N, base, rough, metal);
+                const V3 fb = eval_brdf(nd, view, N, basec, rough, metal);
                 const float nl = gmax(dot(N, nd), 0.0f);
                 V3 energy = (pdf > 0.0f) ? scale(fb, nl / pdf) : mk(0.0f);
                 // termination bookkeeping of GeneratePath (Tracer.cpp:206-208)
                 V3 tterm = mul(xyz(b.t_term[p]), energy);
                 bool stop;
+                float rr_q = 1.0f;
                 if (c.mode == MT_MODE_FIXED) {
                     stop = false;
                     if (k + 1 >= c.rr_depth) {
@@ -440,16 +571,24 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                         } else {
                             energy = divs(energy, q);
                             tterm = divs(tterm, q);
+                            rr_q = q;
                         }
                     }
                 } else {
                     stop = len(tterm) < c.terminate_energy;
                 }
                 // ProcessRayForward for this step (Tracer.cpp:254-339)
+                const V3 e_term = mul(scale(trad, c.emission_base), ms.emission);
                 {
                     const float4 ae = b.acc_e[p];
-                    const V3 e = mul(scale(trad, c.emission_base), emission);
-                    b.acc_e[p] = make_float4(ae.x + e.x, ae.y + e.y, ae.z + e.z, 0.0f);
+                    b.acc_e[p] = make_float4(ae.x + e_term.x, ae.y + e_term.y, ae.z + e_term.z, 0.0f);
+                }
+                V3 ssum = mk(0.0f);
+                if (kDiff == 2) {
+                    ssum = add(xyz(b.adj_s[p]), e_term);
+                    gmat = mi;
+                    gvalid = true;
+                    g_e = mul(mul(dl, scale(trad, c.emission_base)), ms.em_scale);
                 }
                 int chosen = -1;
                 float chosen_pdf = 0.0f, total = 0.0f;
@@ -465,6 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 }
                 chosen_pdf /= total;
                 V3 trad_next = trad;
+                V3 g_base = mk(0.0f);
                 if (chosen >= 0) {
                     const DevLight L = S.lights[chosen];
                     V3 lpos = {L.pos[0], L.pos[1], L.pos[2]};
@@ -483,16 +623,53 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                         lcol = scale(lcol, clampf((dd - L.outer) / (L.inner - L.outer), 0.0f, 1.0f));
                     }
                     const float llen = len(lvec);
-                    const V3 fl = eval_brdf(ldir, view, N, base, rough, metal);
+                    const V3 fl = eval_brdf(ldir, view, N, basec, rough, metal);
                     const float nll = gmax(dot(N, ldir), 0.0f);
                     const float fall = 1.0f / (llen * llen + 1.0f);
                     const V3 le = scale(scale(fl, nll), fall);
                     const V3 contrib = divs(mul(mul(trad, le), lcol), chosen_pdf);
-                    shadow = true;
-                    sho = f4(add(wpos, scale(lvec, kPullout)), llen);
-                    shd = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(p));
-                    shc = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(k == 0 ? 0u : 1u));
+                    if (kDiff == 2) {
+                        if ((visbits >> k) & 1u) {  // the forward pass found the light visible
+                            ssum = add(ssum, contrib);
+                            const BrdfGrad bg = eval_brdf_grad(ldir, view, N, basec, rough, metal);
+                            const V3 wgt = divs(mul(mul(dl, trad), lcol), chosen_pdf);
+                            const float sc = nll * fall;
+                            g_base = add(g_base, scale(mul(wgt, bg.df_dbase), sc));
+                            g_r += dot(scale(wgt, sc), bg.df_drough) * ms.rough_scale;
+                            g_m += dot(scale(wgt, sc), bg.df_dmetal) * ms.metal_scale;
+                        }
+                        // throughput factor w = f*cos/pdf of this step scales everything after it
+                        const V3 rem = sub(ltot, ssum);
+                        const BrdfGrad bb = eval_brdf_grad(nd, view, N, basec, rough, metal);
+                        if (pdf > 0.0f) {
+                            const float sc = nl / pdf / rr_q;
+                            const float cw[3] = {energy.x, energy.y, energy.z};
+                            const float rm[3] = {rem.x, rem.y, rem.z}, dd3[3] = {dl.x, dl.y, dl.z};
+                            const float db[3] = {bb.df_dbase.x, bb.df_dbase.y, bb.df_dbase.z};
+                            const float dr[3] = {bb.df_drough.x, bb.df_drough.y, bb.df_drough.z};
+                            const float dm[3] = {bb.df_dmetal.x, bb.df_dmetal.y, bb.df_dmetal.z};
+                            float gb[3] = {0, 0, 0};
+                            for (int ch = 0; ch < 3; ++ch) {
+                                if (!(cw[ch] > 0.0f)) continue;
+                                const float a = dd3[ch] * rm[ch] / cw[ch] * sc;
+                                gb[ch] = a * db[ch];
+                                g_r += a * dr[ch] * ms.rough_scale;
+                                g_m += a * dm[ch] * ms.metal_scale;
+                            }
+                            g_base = add(g_base, V3{gb[0], gb[1], gb[2]});
+                        }
+                    } else {
+                        shadow = true;
+                        sho = f4(add(wpos, scale(lvec, kPullout)), llen);
+                        shd = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(p));
+                        shc = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(k == 0 ? 0u : 1u));
+                    }
                     trad_next = mul(trad, energy);
+                }
+                if (kDiff == 2) {
+                    add_texel_grad(gr, ms, g_base);
+                    b.adj_s[p] = f4(ssum, 0.0f);
+                    b.adj_prev[p] = make_float4(tu, tv, __uint_as_float(ms.tex ? mi : 0xFFFFFFFFu), 0.0f);
                 }
                 if (!stop && !last) {
                     cont = true;
@@ -501,15 +678,25 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     b.ray_d[p] = f4(nd, 0.0f);
                     b.t_rad[p] = f4(trad_next, 0.0f);
                     b.t_term[p] = f4(tterm, 0.0f);
-                    b.prev_base[p] = f4(base, 0.0f);
+                    b.prev_base[p] = f4(basec, 0.0f);
                 }
             }
         }
-        const uint32_t ss = wave_push(b.counters + kCntShadow + k, shadow);
-        if (shadow) {
-            b.sh_o[ss] = sho;
-            b.sh_d[ss] = shd;
-            b.sh_c[ss] = shc;
+        if (kDiff == 2) {
+            if (gr.rough) wave_atomic_by_key(gr.rough, gmat, g_r, gvalid && g_r != 0.0f);
+            if (gr.metal) wave_atomic_by_key(gr.metal, gmat, g_m, gvalid && g_m != 0.0f);
+            if (gr.emis) {
+                wave_atomic_by_key(gr.emis, gmat * 3 + 0, g_e.x, gvalid && g_e.x != 0.0f);
+                wave_atomic_by_key(gr.emis, gmat * 3 + 1, g_e.y, gvalid && g_e.y != 0.0f);
+                wave_atomic_by_key(gr.emis, gmat * 3 + 2, g_e.z, gvalid && g_e.z != 0.0f);
+            }
+        } else {
+            const uint32_t ss = wave_push(b.counters + kCntShadow + k, shadow);
+            if (shadow) {
+                b.sh_o[ss] = sho;
+                b.sh_d[ss] = shd;
+                b.sh_c[ss] = shc;
+            }
         }
         const uint32_t cs = wave_push(b.counters + kCntExtend + k + 1, cont);
         if (cont) qout[cs] = p;
@@ -527,6 +714,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers 
     const V3 L = add(add(add(e, d), i), s);  // Tracer.cpp:344-345
     const V3 col = c.output_linear ? L : aces(L, c.exposure);
     b.sample_rgb[p] = f4(col, 0.0f);
+    if (c.diff) {
+        uint32_t pix, s;
+        path_coords(c, p, pix, s);
+        c.path_store[static_cast<size_t>(pix) * c.spp + s] = make_float4(L.x, L.y, L.z, __uint_as_float(b.vis[p]));
+    }
 }
 
 // per-pixel mean of the samples in sample order, with CleanupFireflies (Tracer.cpp:490-548)
@@ -704,7 +896,24 @@ void Renderer::release() {
     host_counters_ = nullptr;
 }
 
-int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st) {
+void Renderer::set_camera(const mt_camera_desc& cam) {
+    cam_ = cam;
+    ChunkParams& c = params_;
+    c.cam_type = cam.type;
+    std::memcpy(c.cam, cam.camera_to_world, sizeof(c.cam));
+    c.fov_scale = std::tan(cam.yfov * 0.5f);
+    c.ortho_x = cam.ortho_size[0];
+    c.ortho_y = cam.ortho_size[1];
+    c.cam_aspect = cam.aspect;
+}
+
+void Renderer::set_seed(uint64_t seed) {
+    st_.seed = seed;
+    const uint32_t k0 = seed_mix(static_cast<uint32_t>(seed) ^ 0xA511E9B3u);
+    params_.seed_key = seed_mix(k0 ^ static_cast<uint32_t>(seed >> 32) ^ 0x63D83595u);
+}
+
+int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff) {
     scene_ = scene;
     cam_ = cam;
     st_ = st;
@@ -748,6 +957,10 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         (rc = alloc(owned_, &b.counters, kMaxCounters)) || (rc = alloc(owned_, &b.work, 16)) ||
         (rc = alloc(owned_, &img_, npix_total * 3)))
         return rc;
+    if (diff && ((rc = alloc(owned_, &b.vis, P)) || (rc = alloc(owned_, &b.adj_s, P)) ||
+                 (rc = alloc(owned_, &b.adj_prev, P)) ||
+                 (rc = alloc(owned_, &store_, npix_total * st.samples))))
+        return rc;
     HIP_TRY(hipMemsetAsync(b.work, 0, sizeof(unsigned long long) * 16, stream_));
     HIP_TRY(hipMemsetAsync(img_, 0, sizeof(float) * npix_total * 3, stream_));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_counters_), sizeof(uint32_t) * kMaxCounters));
@@ -760,14 +973,8 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     c.px_w = 1.0f / static_cast<float>(st.width);
     c.px_h = 1.0f / static_cast<float>(st.height);
     c.aspect = static_cast<float>(st.width) / static_cast<float>(st.height);
-    c.cam_type = cam.type;
-    std::memcpy(c.cam, cam.camera_to_world, sizeof(c.cam));
-    c.fov_scale = std::tan(cam.yfov * 0.5f);
-    c.ortho_x = cam.ortho_size[0];
-    c.ortho_y = cam.ortho_size[1];
-    c.cam_aspect = cam.aspect;
-    const uint32_t k0 = seed_mix(static_cast<uint32_t>(st.seed) ^ 0xA511E9B3u);
-    c.seed_key = seed_mix(k0 ^ static_cast<uint32_t>(st.seed >> 32) ^ 0x63D83595u);
+    set_camera(cam);
+    set_seed(st.seed);
     c.light_stride = 8u + scene->light_count;
     c.mode = st.mode;
     c.rr_depth = st.rr_depth;
@@ -785,6 +992,12 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     c.stack_cap = cap;
     c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
     c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
+    c.inv_spp = 1.0f / static_cast<float>(st.samples);
+    if (diff) {  // the differentiable image is the linear radiance, no firefly rejection
+        c.output_linear = 1;
+        c.firefly = 0;
+        c.path_store = store_;
+    }
     for (uint32_t i = 0; i < 2 + 4 * st.bounces; ++i) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
@@ -819,7 +1032,7 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
         HIP_TRY(hipEventRecord(ev[0], s));
         launch_trace<0>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[1], s));
-        hipLaunchKernelGGL(k_shade, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        hipLaunchKernelGGL(k_shade<0>, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[2], s));
         if (scene_->light_count)
             launch_trace<1>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
@@ -867,6 +1080,72 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
             q[1] = tmp[p].y;
             q[2] = tmp[p].z;
         }
+    }
+    return MT_OK;
+}
+
+// one chunk of the differentiable forward (mode 1) or of the adjoint replay (mode 2)
+int Renderer::run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s) {
+    const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
+    ChunkParams c = params_;
+    c.diff = mode;
+    c.pix_begin = chunk * npix_chunk_;
+    c.npix = static_cast<uint32_t>(std::min<uint64_t>(npix_chunk_, npix_total - c.pix_begin));
+    c.paths = c.npix * st_.samples;
+    const uint32_t gen_blocks = (c.paths + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    for (uint32_t k = 0; k < st_.bounces; ++k) {
+        launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
+        if (mode == 1) {
+            hipLaunchKernelGGL(k_shade<1>, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+            if (scene_->light_count)
+                launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
+        } else {
+            hipLaunchKernelGGL(k_shade<2>, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    if (mode == 1) {
+        hipLaunchKernelGGL(k_finalize, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+        hipLaunchKernelGGL(k_resolve, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, buf_, img);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host_counters_, buf_.counters, sizeof(uint32_t) * kMaxCounters,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (host_counters_[kCntError]) return fail(MT_E_INVALID, "traversal stack overflow");
+    uint64_t rays = 0;
+    for (uint32_t k = 0; k < st_.bounces; ++k)
+        rays += static_cast<uint64_t>(host_counters_[kCntExtend + k]) +
+                (mode == 1 ? host_counters_[kCntShadow + k] : 0u);
+    stats_.rays += rays;
+    stats_.samples += c.paths;
+    return MT_OK;
+}
+
+int Renderer::diff_forward(const DiffParams& dp, float* img_dev, hipStream_t s) {
+    if (!store_) return fail(MT_E_INVALID, "renderer was not set up for differentiable rendering");
+    params_.dp = dp;
+    params_.gr = DiffGradPtrs{};
+    params_.grad_img = nullptr;
+    hipStream_t st = s ? s : stream_;
+    for (uint32_t ch = 0; ch < chunks_; ++ch) {
+        const int rc = run_diff_chunk(ch, 1, img_dev, st);
+        if (rc) return rc;
+    }
+    return MT_OK;
+}
+
+int Renderer::diff_adjoint(const DiffParams& dp, const DiffGradPtrs& gr, const float* grad_img,
+                           hipStream_t s) {
+    if (!store_) return fail(MT_E_INVALID, "renderer was not set up for differentiable rendering");
+    params_.dp = dp;
+    params_.gr = gr;
+    params_.grad_img = grad_img;
+    hipStream_t st = s ? s : stream_;
+    for (uint32_t ch = 0; ch < chunks_; ++ch) {
+        const int rc = run_diff_chunk(ch, 2, nullptr, st);
+        if (rc) return rc;
     }
     return MT_OK;
 }

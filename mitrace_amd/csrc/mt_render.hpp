@@ -18,6 +18,24 @@ constexpr uint32_t kCntFetchSh = kCntFetchExt + kMaxBounces;    // [kMaxBounces]
 constexpr uint32_t kCntError = kCntFetchSh + kMaxBounces;       // traversal stack overflow flag
 constexpr uint32_t kMaxCounters = kCntError + 1;
 
+// differentiable parameters (device pointers, null = the scene's value): a float RGB texture that
+// replaces the base colour of materials [tex_first, tex_first + tex_count), and per-material
+// roughness, metallic and RGB emission
+struct DiffParams {
+    const float* tex;
+    uint32_t tex_w, tex_h, tex_first, tex_count;
+    const float* rough;
+    const float* metal;
+    const float* emis;
+};
+// gradient accumulators of the same layout (null = not wanted)
+struct DiffGradPtrs {
+    float* tex;
+    float* rough;
+    float* metal;
+    float* emis;
+};
+
 // per-chunk constants, passed to kernels by value
 struct ChunkParams {
     uint32_t width, spp, bounces;
@@ -39,6 +57,13 @@ struct ChunkParams {
     uint32_t stack_cap;
     uint32_t refill_min;  // idle lanes of a wave that trigger a refill of the traversal
     uint32_t setup_min;   // lanes needing an instance set-up that trigger a set-up step
+    // differentiable rendering: 0 plain, 1 forward with per-path store, 2 adjoint replay
+    int32_t diff;
+    DiffParams dp;
+    DiffGradPtrs gr;
+    float4* path_store;    // [npix_total * spp] {linear radiance, shadow visibility bits}
+    const float* grad_img; // [npix_total * 3] dLoss/dImage (adjoint replay)
+    float inv_spp;
 };
 
 // SoA path state, queues and scratch of one render (capacity = paths per chunk)
@@ -56,13 +81,25 @@ struct PathBuffers {
     uint2* spill;
     uint32_t* counters;
     unsigned long long* work;  // aabb, entered, pops, tris, hits, misses
+    // differentiable rendering
+    uint32_t* vis;      // bit k: the shadow ray of bounce k reached its light
+    float4* adj_s;      // radiance collected so far during the replay
+    float4* adj_prev;   // {u, v, material or ~0u} of the previous hit (sky term adjoint)
 };
 
 class Renderer {
    public:
     ~Renderer();
-    int setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st);
+    int setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff = false);
+    void set_camera(const mt_camera_desc& cam);
+    void set_seed(uint64_t seed);
     int run_chunk(uint32_t chunk, float* sample_rgb_out);
+    // differentiable rendering over the whole image on the caller's stream (0 = own stream):
+    // forward writes the linear image (mean radiance per pixel) to img_dev and the per-path
+    // store; adjoint replays the same paths and accumulates into the gradient buffers
+    int diff_forward(const DiffParams& dp, float* img_dev, hipStream_t s);
+    int diff_adjoint(const DiffParams& dp, const DiffGradPtrs& gr, const float* grad_img,
+                     hipStream_t s);
     int finish(float* rgb_out);
     uint32_t chunks() const { return chunks_; }
     const mt_stats& stats() const { return stats_; }
@@ -70,6 +107,7 @@ class Renderer {
 
    private:
     void release();
+    int run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s);
     mt_scene* scene_ = nullptr;
     mt_camera_desc cam_{};
     mt_settings st_{};
@@ -80,6 +118,7 @@ class Renderer {
     hipStream_t stream_ = nullptr;
     uint32_t* host_counters_ = nullptr;
     float* img_ = nullptr;
+    float4* store_ = nullptr;
     uint32_t capacity_ = 0, npix_chunk_ = 0, chunks_ = 0;
     uint32_t trace_blocks_ = 0, shade_blocks_ = 0, lds_depth_ = 16;
     mt_stats stats_{};

@@ -609,6 +609,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         o.sy = t.scale[1];
         return o;
     };
+    s->material_count = d->material_count;
     std::vector<DevMaterial> mats(d->material_count);
     for (uint32_t i = 0; i < d->material_count; ++i) {
         const mt_material_desc& m = d->materials[i];

@@ -10,6 +10,9 @@
   for rows 4-6, seeded base colours in [0.2, 0.9]^3, lit by a procedural 2048x1024 float32
   equirect sky (vertical gradient plus a 0.5 deg sun disk of radiance 5e4), BVH leaves of at
   most 4 triangles.
+* :func:`inverse_scene` -- the sphere grid with one shared float texture as every sphere's
+  base colour, scalar roughness/metallic per sphere and four point lights; 8 camera poses from
+  :func:`inverse_poses`.
 """
 import numpy as np
 
@@ -82,28 +85,31 @@ def cornell_settings(**kw):
     return s
 
 
-def uv_sphere(segments=64, rings=32, radius=1.0, name="uv_sphere"):
+def uv_sphere(segments=64, rings=32, radius=1.0, name="uv_sphere", with_uv=False):
     """UV sphere with smooth normals: 2*segments cap triangles + 2*segments*(rings-2) quads
-    split in two (3968 triangles for 64x32)."""
+    split in two (3968 triangles for 64x32).  with_uv adds texcoords u = phi/2pi, v = theta/pi."""
     th = np.linspace(0.0, np.pi, rings + 1)
     ph = np.linspace(0.0, 2.0 * np.pi, segments + 1)
     T, P = np.meshgrid(th, ph, indexing="ij")
     V = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], axis=-1)
-    tris = []
+    UV = np.stack([P / (2.0 * np.pi), T / np.pi], axis=-1)
+    idx = []
     for r in range(rings):
         for s in range(segments):
-            a, b = V[r, s], V[r, s + 1]
-            c, d = V[r + 1, s], V[r + 1, s + 1]
+            a, b, c, d = (r, s), (r, s + 1), (r + 1, s), (r + 1, s + 1)
             if r == 0:
-                tris.append((a, d, c))
+                idx.append((a, d, c))
             elif r == rings - 1:
-                tris.append((a, b, c))
+                idx.append((a, b, c))
             else:
-                tris.append((a, b, d))
-                tris.append((a, d, c))
-    n = np.array(tris, np.float64).reshape(-1, 3)
+                idx.append((a, b, d))
+                idx.append((a, d, c))
+    ij = np.array(idx, np.int64).reshape(-1, 2)
+    n = V[ij[:, 0], ij[:, 1]].astype(np.float64)
     n /= np.linalg.norm(n, axis=1, keepdims=True)
-    return Mesh(name=name, positions=(n * radius).astype(np.float32), normals=n.astype(np.float32))
+    uv = UV[ij[:, 0], ij[:, 1]].astype(np.float32) if with_uv else None
+    return Mesh(name=name, positions=(n * radius).astype(np.float32), normals=n.astype(np.float32),
+                texcoord0=uv)
 
 
 def ground_plane(nx=10, nz=5, x0=-3.0, x1=9.0, z0=-3.0, z1=9.0, y=0.0):
@@ -168,6 +174,74 @@ def sphere_grid(seed=1, env_width=2048, env_height=1024, segments=64, rings=32):
     env = Environment(texels=procedural_sky(env_width, env_height), filter=abi.MT_FILTER_LINEAR)
     return Scene(meshes=meshes, materials=mats, instances=inst, lights=[], camera=cam, env=env,
                  bvh=BvhOptions(disable=False, max_triangles=4, max_depth=32))
+
+
+def smooth_noise_texture(res, seed):
+    """seeded smooth RGB noise in [0.1, 0.9]: three octaves of value noise per channel"""
+    rng = np.random.default_rng(seed)
+    ch = [0.1 + 0.8 * fbm(res, 3, rng, base_cells=4) for _ in range(3)]
+    return np.ascontiguousarray(np.stack(ch, -1), np.float32)
+
+
+def inverse_scene(seed=4, tex_res=512, env_width=2048, env_height=1024, segments=64, rings=32):
+    """The sphere grid for inverse rendering: every sphere takes its base colour from one shared
+    float RGB texture (seeded smooth noise, spherical UVs) and has scalar roughness and
+    metallic.  Metallic is 0.1 / 0.9 (rows 0-3 / 4-6) rather than 0 / 1 so that both sides of
+    every scalar are inside the BRDF's clamps and central differences exist.  Four point lights
+    above the grid: without a light the reference's path throughput is never updated and the
+    estimate would not depend on roughness or metallic except through the sampled directions.
+
+    Returns (scene, params) with params = {tex, tex_first, tex_count, rough, metal, emis}
+    (the ground-truth parameters; the spheres are materials 0..48, the ground is 49)."""
+    rng = np.random.default_rng(seed)
+    sc = sphere_grid(seed=1, env_width=env_width, env_height=env_height, segments=segments,
+                     rings=rings)
+    sc.meshes[0] = uv_sphere(segments, rings, 1.0, with_uv=True)
+    for i in range(49):
+        m = sc.materials[i]
+        m.metallic_factor = 0.1 if i // 7 < 4 else 0.9
+        m.base_color_factor = (1.0, 1.0, 1.0, 1.0)
+    for x, z in [(0.5, 0.5), (5.5, 0.5), (0.5, 5.5), (5.5, 5.5)]:
+        sc.lights.append(Light(type=abi.MT_LIGHT_POINT, position=(x, 3.0, z),
+                               color=(40.0, 38.0, 36.0), point_size=0.1))
+    # a wider, dimmer sun (3 deg, radiance 100): the 0.5 deg / 5e4 disk of the throughput
+    # workload makes 32-sample estimates dominated by rare sun hits
+    sc.env = Environment(texels=procedural_sky(env_width, env_height, sun_radius_deg=3.0,
+                                               sun_radiance=100.0), filter=abi.MT_FILTER_LINEAR)
+    tex = smooth_noise_texture(tex_res, int(rng.integers(1 << 31)))
+    params = {
+        "tex": tex, "tex_first": 0, "tex_count": 49,
+        "rough": np.array([m.roughness_factor for m in sc.materials], np.float32),
+        "metal": np.array([m.metallic_factor for m in sc.materials], np.float32),
+        "emis": np.array([m.emissive_factor for m in sc.materials], np.float32).reshape(-1),
+    }
+    return sc, params
+
+
+def inverse_poses(n=8, seed=4):
+    """n seeded cameras on a ring around the grid centre, looking at it (square aspect)"""
+    rng = np.random.default_rng(seed + 1000)
+    cams = []
+    for i in range(n):
+        az = 2.0 * np.pi * (i + rng.uniform(-0.3, 0.3)) / n
+        el = rng.uniform(np.radians(25.0), np.radians(55.0))
+        r = rng.uniform(8.0, 10.0)
+        c = np.array([3.0, 0.2, 3.0])
+        eye = c + r * np.array([np.cos(el) * np.cos(az), np.sin(el), np.cos(el) * np.sin(az)])
+        cams.append(Camera(type=abi.MT_CAMERA_PERSPECTIVE, camera_to_world=look_at(eye, c),
+                           yfov=float(np.radians(40.0)), aspect=1.0))
+    return cams
+
+
+def inverse_settings(**kw):
+    """512x512, linear radiance, Russian roulette from depth 3, 5 bounces"""
+    s = Settings(width=512, height=512, samples=32, bounces=5, hdri_rotation=30.0,
+                 hdri_rotation_apply=True, hdri_primary_intensity=1.0,
+                 hdri_secondary_intensity=1.0, mode=abi.MT_MODE_FIXED, rr_depth=3,
+                 output=abi.MT_OUTPUT_LINEAR)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
 
 
 def sphere_grid_settings(**kw):

@@ -927,6 +927,284 @@ struct Tracer {
     }
 };
 
+// ---------------------------------------------------------------- differentiable estimator
+// Forward-mode (tangent) derivative of the per-sample radiance along one direction in parameter
+// space, with the sampling detached: every decision (lobe, direction, light, termination) is
+// taken from the primal values and the derivative flows only through the BRDF values, the
+// base colour, the emission and the throughput products.  This is an automatic-differentiation
+// restatement, independent of the hand-derived reverse-mode (adjoint) kernels it checks.
+struct Dv {
+    float v, d;
+};
+inline Dv dv(float v) { return {v, 0.0f}; }
+inline Dv operator+(Dv a, Dv b) { return {a.v + b.v, a.d + b.d}; }
+inline Dv operator-(Dv a, Dv b) { return {a.v - b.v, a.d - b.d}; }
+inline Dv operator*(Dv a, Dv b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dv operator/(Dv a, Dv b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+inline Dv operator+(Dv a, float b) { return a + dv(b); }
+inline Dv operator-(float a, Dv b) { return dv(a) - b; }
+inline Dv operator*(Dv a, float b) { return a * dv(b); }
+inline Dv operator*(float a, Dv b) { return dv(a) * b; }
+inline Dv operator/(Dv a, float b) { return a / dv(b); }
+inline Dv dmax(Dv a, float b) { return (a.v < b) ? dv(b) : a; }  // same selection as gmax
+inline Dv dmin(Dv a, float b) { return (b < a.v) ? dv(b) : a; }  // same selection as gmin
+struct D3 {
+    Dv c[3];
+};
+inline D3 d3(V3 v) { return {{dv(v.x), dv(v.y), dv(v.z)}}; }
+inline D3 d3(V3 v, V3 t) { return {{{v.x, t.x}, {v.y, t.y}, {v.z, t.z}}}; }
+inline D3 operator+(const D3& a, const D3& b) { return {{a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]}}; }
+inline D3 operator*(const D3& a, const D3& b) { return {{a.c[0] * b.c[0], a.c[1] * b.c[1], a.c[2] * b.c[2]}}; }
+inline D3 operator*(const D3& a, float s) { return {{a.c[0] * s, a.c[1] * s, a.c[2] * s}}; }
+inline V3 tangent(const D3& a) { return {a.c[0].d, a.c[1].d, a.c[2].d}; }
+
+// EvaluateBRDF (BRDF.hpp:52-77) over tangent numbers in base, roughness and metallic
+D3 evalBrdfDv(V3 L, V3 V, V3 N, const D3& base, Dv rough, Dv metal) {
+    D3 out = d3(v3(0.0f));
+    rough = dmax(rough, 0.05f);
+    const float nl = gmax(dot(N, L), 0.0f);
+    const float nv = gmax(dot(N, V), 0.0f);
+    if (nl <= 0.0f || nv <= 0.0f) return out;
+    const V3 h = normalize(V + L);
+    const float nh = clampf(gmax(dot(N, h), 0.0f), 0.0f, 1.0f);
+    const float hv = clampf(gmax(dot(h, V), 0.0f), 0.0f, 1.0f);
+    const Dv mc = dmin(dmax(metal, 0.0f), 1.0f);
+    const float p5 = std::pow(1.0f - hv, 5.0f);
+    const Dv a = rough * rough;
+    const Dv a2 = a * a;
+    Dv den = (a2 - dv(1.0f)) * (nh * nh) + 1.0f;
+    den = (den * den) * kPi;
+    const Dv D = a2 / den;
+    const Dv rr = rough + 1.0f;
+    const Dv k = rr * rr / 8.0f;
+    const Dv g1 = dv(nv) / ((1.0f - k) * nv + k);
+    const Dv g2 = dv(nl) / ((1.0f - k) * nl + k);
+    const float den4 = gmax(4.0f * nv * nl, 0.001f);
+    for (int c = 0; c < 3; ++c) {
+        const Dv f0 = (1.0f - mc) * 0.04f + base.c[c] * mc;
+        const Dv F = f0 + (1.0f - f0) * p5;
+        const Dv kd = (1.0f - F) * (1.0f - metal);
+        const Dv diffuse = kd * base.c[c] / kPi;
+        const Dv spec = D * (g1 * g2) * F / den4;
+        out.c[c] = dmax(diffuse + spec, 0.0f);
+    }
+    return out;
+}
+
+struct DiffSet {  // host copies of the differentiable parameters (value or tangent)
+    bool hasTex = false;
+    Image tex;
+    uint32_t texFirst = 0, texCount = 0;
+    std::vector<float> rough, metal, emis;
+};
+
+struct DiffTracer {
+    const Scene& S;
+    mt_settings st;
+    uint32_t lightStride;
+    const DiffSet& P;  // values
+    const DiffSet& T;  // tangents (same shapes, empty = zero)
+
+    struct MatD {
+        D3 base, emission;
+        Dv rough, metal;
+        V3 ntex;
+    };
+
+    // Material::SampleMaterial (Material.hpp:82-135) with the parameter overrides
+    MatD material(uint32_t mi, float u, float v) const {
+        const Material& m = S.materials[mi];
+        MatD o;
+        if (P.hasTex && mi >= P.texFirst && mi - P.texFirst < P.texCount) {
+            const RGBA c = sampleImage(P.tex, u, v, MT_FILTER_LINEAR);
+            V3 t = v3(0.0f);
+            if (T.hasTex) {
+                const RGBA ct = sampleImage(T.tex, u, v, MT_FILTER_LINEAR);
+                t = V3{ct.r, ct.g, ct.b};
+            }
+            o.base = d3(V3{c.r, c.g, c.b}, t);
+        } else if (m.baseTex.image >= 0) {
+            const RGBA c = sampleTex(S, m.baseTex, u, v);
+            o.base = d3(V3{c.r * m.bcf[0], c.g * m.bcf[1], c.b * m.bcf[2]});
+        } else {
+            o.base = d3(V3{m.bcf[0], m.bcf[1], m.bcf[2]});
+        }
+        const Dv rf = P.rough.empty() ? dv(m.roughness) : Dv{P.rough[mi], T.rough.empty() ? 0.0f : T.rough[mi]};
+        const Dv mf = P.metal.empty() ? dv(m.metallic) : Dv{P.metal[mi], T.metal.empty() ? 0.0f : T.metal[mi]};
+        if (m.mrTex.image >= 0) {
+            const RGBA c = sampleTex(S, m.mrTex, u, v);
+            o.metal = mf * c.b;
+            o.rough = rf * c.g;
+        } else {
+            o.metal = mf;
+            o.rough = rf;
+        }
+        if (m.nrmTex.image >= 0) {
+            const RGBA c = sampleTex(S, m.nrmTex, u, v);
+            o.ntex = normalize(V3{c.r * m.normalScale * 2.0f - 1.0f, c.g * m.normalScale * 2.0f - 1.0f,
+                                  c.b * m.normalScale * 2.0f - 1.0f});
+        } else {
+            o.ntex = V3{0.0f, 0.0f, 1.0f};
+        }
+        D3 em = d3(m.emissive);
+        if (!P.emis.empty()) {
+            for (int c = 0; c < 3; ++c)
+                em.c[c] = Dv{P.emis[mi * 3 + c], T.emis.empty() ? 0.0f : T.emis[mi * 3 + c]};
+        }
+        if (m.emTex.image >= 0) {
+            const RGBA c = sampleTex(S, m.emTex, u, v);
+            o.emission = em * d3(V3{c.r, c.g, c.b});
+        } else {
+            o.emission = em;
+        }
+        return o;
+    }
+
+    struct StepD {
+        Ray ray;
+        V3 hitPos, normal;
+        D3 base, energy, emission;
+        Dv rough, metal;
+        bool didHit;
+    };
+
+    // GeneratePath + ProcessRayForward (Tracer.cpp:134-348) for one camera ray, tangent numbers
+    D3 forward(const Ray& camRay, Rng& rng, std::vector<StepD>& path) const {
+        path.clear();
+        Ray cur = camRay;
+        V3 total = v3(1.0f);
+        while (path.size() < st.bounces) {
+            const uint32_t k = static_cast<uint32_t>(path.size());
+            rng.base = 2 + k * lightStride;
+            StepD step;
+            const Hit h = intersectScene(S, cur, false, nullptr);
+            if (!h.valid) {
+                step.ray = cur;
+                step.didHit = false;
+                path.push_back(step);
+                break;
+            }
+            const Instance& I = S.instances[h.inst];
+            const Mesh& m = S.meshes[I.mesh];
+            const float w = 1.0f - h.u - h.v;
+            const uint32_t t3 = h.tri * 3;
+            const V3 ni = normalize(w * m.nrm[t3] + h.u * m.nrm[t3 + 1] + h.v * m.nrm[t3 + 2]);
+            const V3 gN = normalize(xfDir(I.M, ni));
+            float tu = 0.0f, tv = 0.0f;
+            if (!m.uv.empty()) {
+                tu = w * m.uv[t3 * 2] + h.u * m.uv[t3 * 2 + 2] + h.v * m.uv[t3 * 2 + 4];
+                tv = w * m.uv[t3 * 2 + 1] + h.u * m.uv[t3 * 2 + 3] + h.v * m.uv[t3 * 2 + 5];
+            }
+            const MatD md = material(I.material, tu, tv);
+            step.base = md.base * 0.96f;
+            for (int c = 0; c < 3; ++c) step.base.c[c] = step.base.c[c] + 0.04f;  // Tracer.cpp:159
+            step.ray = cur;
+            step.hitPos = h.wpos;
+            step.rough = md.rough;
+            step.metal = md.metal;
+            step.normal = computeNormal(gN, md.ntex);
+            step.emission = md.emission;
+            step.didHit = true;
+            const V3 view = -cur.d;
+            const V3 basev{step.base.c[0].v, step.base.c[1].v, step.base.c[2].v};
+            const ShadeOut so = sampleBounce(view, step.normal, basev, md.rough.v, md.metal.v, rng);
+            // d(energy) = d(f) * nl / pdf with the sampled direction and its pdf held fixed
+            const D3 fd = evalBrdfDv(so.dir, view, step.normal, step.base, md.rough, md.metal);
+            const float nl = gmax(dot(step.normal, so.dir), 0.0f);
+            const V3 ed = (so.pdf > 0.0f) ? tangent(fd) * (nl / so.pdf) : v3(0.0f);
+            step.energy = d3(so.energy, ed);
+            Ray nr;
+            nr.o = h.o + h.d * h.dist + 0.0001f * step.normal;
+            nr.d = so.dir;
+            path.push_back(step);
+            cur = nr;
+            total = total * so.energy;
+            if (st.mode == MT_MODE_FIXED) {
+                if (k + 1 >= st.rr_depth) {
+                    const float q = clampf(gmax(gmax(total.x, total.y), total.z), 0.05f, 1.0f);
+                    if (rng.u(kDimRR) >= q) break;
+                    path.back().energy = path.back().energy * (1.0f / q);
+                    for (int c = 0; c < 3; ++c) path.back().energy.c[c].v = get(so.energy, c) / q;
+                    total = total / q;
+                }
+            } else if (length(total) < st.terminate_energy) {
+                break;
+            }
+        }
+        D3 cE = d3(v3(1.0f));
+        D3 L = d3(v3(0.0f));
+        for (size_t si = 0; si < path.size(); ++si) {
+            const StepD& step = path[si];
+            rng.base = 2 + static_cast<uint32_t>(si) * lightStride;
+            if (!step.didHit) {
+                const D3 col = (si > 0) ? path[si - 1].base : d3(v3(1.0f));
+                const float mult = si == 0 ? st.hdri_primary_intensity : st.hdri_secondary_intensity;
+                const V3 sk = sampleEnv(S, step.ray.d, st.hdri_rotation_apply != 0, st.hdri_rotation) * mult;
+                L = L + col * cE * d3(sk);
+                break;
+            }
+            L = L + cE * step.emission * st.emission_base_intensity;
+            int chosen = -1;
+            float chosenPdf = 0.0f, totalScore = 0.0f;
+            for (size_t i = 0; i < S.lights.size(); ++i) {
+                const V3 lp = S.lights[i].pos - step.hitPos;
+                const float score = S.lights[i].intensity / (dot(lp, lp) + 1.0f);
+                totalScore += score;
+                if (rng.u(kDimLightSel + static_cast<uint32_t>(i)) * totalScore < score) {
+                    chosen = static_cast<int>(i);
+                    chosenPdf = score;
+                }
+            }
+            chosenPdf /= totalScore;
+            if (chosen == -1) continue;
+            const Light& Lt = S.lights[chosen];
+            V3 lpos = Lt.pos;
+            if (Lt.type == MT_LIGHT_POINT || Lt.type == MT_LIGHT_SPOT) {
+                const float phi = rng.uRange(kDimLightPt, 0.0f, 2.0f * kPi);
+                const float th = rng.uRange(kDimLightPt + 1, -kPi * 0.5f, kPi * 0.5f);
+                const V3 rd = normalize(V3{std::cos(phi) * std::cos(th), std::sin(th),
+                                           std::sin(phi) * std::cos(th)});
+                lpos = Lt.pos + rd * Lt.size;
+            }
+            const V3 lvec = lpos - step.hitPos;
+            const V3 ldir = normalize(lvec);
+            V3 lcol = Lt.color;
+            if (Lt.type == MT_LIGHT_SPOT)
+                lcol = Lt.color * clampf((dot(-ldir, Lt.dir) - Lt.outer) / (Lt.inner - Lt.outer), 0.0f, 1.0f);
+            Ray sr;
+            sr.o = step.hitPos + lvec * 0.0001f;
+            sr.d = ldir;
+            const Hit sh = intersectScene(S, sr, false, nullptr);
+            const float llen = length(lvec);
+            if (!sh.valid || sh.dist > llen) {
+                const D3 f = evalBrdfDv(ldir, -step.ray.d, step.normal, step.base, step.rough, step.metal);
+                const float nl = gmax(dot(step.normal, ldir), 0.0f);
+                const float fall = 1.0f / (llen * llen + 1.0f);
+                L = L + cE * f * d3(lcol * (nl * fall / chosenPdf));
+            }
+            cE = cE * step.energy;
+        }
+        return L;
+    }
+};
+
+DiffSet makeDiffSet(const orc_diff_params* p, uint32_t nmat) {
+    DiffSet d;
+    if (!p) return d;
+    if (p->tex) {
+        d.hasTex = true;
+        d.tex.w = static_cast<int>(p->tex_w);
+        d.tex.h = static_cast<int>(p->tex_h);
+        d.tex.rgb.assign(p->tex, p->tex + static_cast<size_t>(p->tex_w) * p->tex_h * 3);
+        d.texFirst = p->tex_first;
+        d.texCount = p->tex_count;
+    }
+    if (p->rough) d.rough.assign(p->rough, p->rough + nmat);
+    if (p->metal) d.metal.assign(p->metal, p->metal + nmat);
+    if (p->emis) d.emis.assign(p->emis, p->emis + nmat * 3);
+    return d;
+}
+
 // the camera (Tracer/Camera.cpp:17-50)
 Ray cameraRay(const mt_camera_desc& cam, float u, float v, float aspect) {
     M4 M;
@@ -1456,6 +1734,82 @@ int orc_render(const void* sp, const mt_camera_desc* cam, const mt_settings* st,
         work->hits_shaded = hitsN;
         work->misses = missN;
     }
+    return MT_OK;
+}
+
+void orc_brdf_tangent(uint32_t n, const float* in, const float* tin, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + i * 14;
+        const float* t = tin + i * 5;
+        const D3 base = d3(V3{q[9], q[10], q[11]}, V3{t[0], t[1], t[2]});
+        const D3 f = evalBrdfDv(V3{q[0], q[1], q[2]}, V3{q[3], q[4], q[5]}, V3{q[6], q[7], q[8]},
+                                base, Dv{q[12], t[3]}, Dv{q[13], t[4]});
+        float* o = out + i * 6;
+        for (int c = 0; c < 3; ++c) {
+            o[c] = f.c[c].v;
+            o[3 + c] = f.c[c].d;
+        }
+    }
+}
+
+int orc_render_diff(const void* sp, const mt_camera_desc* cam, const mt_settings* st,
+                    const orc_diff_params* params, const orc_diff_params* tangent_dir,
+                    int32_t threads, float* image, float* dimage) {
+    const Scene& S = *static_cast<const Scene*>(sp);
+    if (!cam || !st || !image || st->width == 0 || st->height == 0 || st->samples == 0) {
+        g_err = "invalid render arguments";
+        return MT_E_INVALID;
+    }
+    const uint32_t nm = static_cast<uint32_t>(S.materials.size());
+    const DiffSet P = makeDiffSet(params, nm);
+    DiffSet T = makeDiffSet(tangent_dir, nm);
+    T.texFirst = P.texFirst;
+    T.texCount = P.texCount;
+    if (T.hasTex && (!P.hasTex || T.tex.w != P.tex.w || T.tex.h != P.tex.h)) {
+        g_err = "tangent texture does not match the parameter texture";
+        return MT_E_INVALID;
+    }
+    const DiffTracer tr{S, *st, 8u + static_cast<uint32_t>(S.lights.size()), P, T};
+    const uint32_t W = st->width, H = st->height, spp = st->samples;
+    const float aspect = static_cast<float>(W) / static_cast<float>(H);
+    const float pxW = 1.0f / static_cast<float>(W), pxH = 1.0f / static_cast<float>(H);
+    const uint32_t sk = seedKey(st->seed);
+    std::atomic<uint32_t> nextRow{0};
+    auto worker = [&]() {
+        std::vector<DiffTracer::StepD> path;
+        Rng rng;
+        while (true) {
+            const uint32_t y = nextRow++;
+            if (y >= H) break;
+            for (uint32_t x = 0; x < W; ++x) {
+                const uint32_t pix = y * W + x;
+                V3 acc = v3(0.0f), dacc = v3(0.0f);
+                for (uint32_t s = 0; s < spp; ++s) {
+                    rng.key = pathKey(sk, pix, s);
+                    rng.base = 0;
+                    const float ju = static_cast<float>(x) * pxW + rng.uRange(0, 0.0f, pxW);
+                    const float jv = static_cast<float>(y) * pxH + rng.uRange(1, 0.0f, pxH);
+                    const D3 L = tr.forward(cameraRay(*cam, ju, jv, aspect), rng, path);
+                    acc = acc + V3{L.c[0].v, L.c[1].v, L.c[2].v};
+                    dacc = dacc + tangent(L);
+                }
+                const float inv = 1.0f / static_cast<float>(spp);
+                image[pix * 3] = acc.x * inv;
+                image[pix * 3 + 1] = acc.y * inv;
+                image[pix * 3 + 2] = acc.z * inv;
+                if (dimage) {
+                    dimage[pix * 3] = dacc.x * inv;
+                    dimage[pix * 3 + 1] = dacc.y * inv;
+                    dimage[pix * 3 + 2] = dacc.z * inv;
+                }
+            }
+        }
+    };
+    int nt = threads > 0 ? threads : static_cast<int>(std::thread::hardware_concurrency());
+    if (nt < 1) nt = 1;
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) th.emplace_back(worker);
+    for (auto& t : th) t.join();
     return MT_OK;
 }
 

@@ -81,6 +81,28 @@ int orc_render(const void* scene, const mt_camera_desc* cam, const mt_settings* 
                const orc_render_opts* opts, float* rgb_out, float* sample_rgb_out,
                orc_work* work, double* seconds);
 
+/* differentiable parameters for orc_render_diff (HOST pointers, NULL = the scene's value);
+ * same meaning as mt_diff_params */
+typedef struct orc_diff_params {
+    const float* tex; /* tex_h*tex_w*3 RGB float, replaces the base colour of materials
+                         [tex_first, tex_first + tex_count) */
+    uint32_t tex_w, tex_h, tex_first, tex_count;
+    const float* rough; /* per material */
+    const float* metal; /* per material */
+    const float* emis;  /* per material, RGB */
+} orc_diff_params;
+
+/* EvaluateBRDF over tangent numbers: in as orc_brdf_eval (14 floats), tin 5 floats
+ * {d base.rgb, d rough, d metal} -> out {f.rgb, df.rgb} */
+void orc_brdf_tangent(uint32_t n, const float* in, const float* tin, float* out);
+
+/* The differentiable estimator (linear radiance, no firefly rejection, counter RNG):
+ * image = mean radiance per pixel; dimage = its directional derivative along tangent_dir
+ * (same layout as params, NULL members = 0) with the sampling detached (forward mode). */
+int orc_render_diff(const void* scene, const mt_camera_desc* cam, const mt_settings* settings,
+                    const orc_diff_params* params, const orc_diff_params* tangent_dir,
+                    int32_t threads, float* image, float* dimage);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,30 @@ class orc_work(C.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
+class orc_diff_params(C.Structure):
+    _fields_ = [("tex", C.c_void_p), ("tex_w", C.c_uint32), ("tex_h", C.c_uint32),
+                ("tex_first", C.c_uint32), ("tex_count", C.c_uint32), ("rough", C.c_void_p),
+                ("metal", C.c_void_p), ("emis", C.c_void_p)]
+
+
+def _diff_struct(d, keep):
+    """dict {tex (H,W,3), tex_first, tex_count, rough, metal, emis} -> orc_diff_params"""
+    out = orc_diff_params()
+    if d is None:
+        return out
+    for key in ("tex", "rough", "metal", "emis"):
+        a = d.get(key)
+        if a is not None:
+            a = np.ascontiguousarray(a, np.float32)
+            keep.append(a)
+            setattr(out, key, a.ctypes.data)
+            if key == "tex":
+                out.tex_h, out.tex_w = a.shape[0], a.shape[1]
+    out.tex_first = d.get("tex_first", 0)
+    out.tex_count = d.get("tex_count", 0)
+    return out
+
+
 def lib_path(fast=False):
     return os.path.join(_HERE, "build", "liboracle_fast.so" if fast else "liboracle.so")
 
@@ -74,6 +98,12 @@ def load(fast=False):
     L.orc_render.argtypes = [C.c_void_p, C.POINTER(abi.mt_camera_desc), C.POINTER(abi.mt_settings),
                              C.POINTER(orc_render_opts), C.POINTER(C.c_float), C.POINTER(C.c_float),
                              C.POINTER(orc_work), C.POINTER(C.c_double)]
+    L.orc_brdf_tangent.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float)]
+    L.orc_render_diff.argtypes = [C.c_void_p, C.POINTER(abi.mt_camera_desc),
+                                  C.POINTER(abi.mt_settings), C.POINTER(orc_diff_params),
+                                  C.POINTER(orc_diff_params), C.c_int32, C.POINTER(C.c_float),
+                                  C.POINTER(C.c_float)]
     _libs[fast] = L
     return L
 
@@ -154,12 +184,41 @@ class OracleScene:
         return rgb, samples, w.as_dict(), secs.value
 
 
+    def render_diff(self, settings, params, tangent=None, camera=None, threads=0):
+        """The differentiable estimator: (mean linear radiance, directional derivative along
+        `tangent`) per pixel.  params/tangent: dicts {tex, tex_first, tex_count, rough, metal,
+        emis} of numpy arrays (missing = the scene's value / zero tangent)."""
+        cam = (camera or self.scene.camera).to_desc()
+        st = settings.to_desc()
+        W, H = settings.width, settings.height
+        img = np.zeros((H, W, 3), np.float32)
+        dimg = np.zeros((H, W, 3), np.float32)
+        keep = []
+        p = _diff_struct(params, keep)
+        t = _diff_struct(tangent, keep)
+        rc = self.L.orc_render_diff(self.h, C.byref(cam), C.byref(st), C.byref(p), C.byref(t),
+                                    threads, abi.fptr(img), abi.fptr(dimg))
+        if rc != 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return img, dimg
+
+
 def brdf_eval(inp):
     L = load()
     inp = np.ascontiguousarray(inp, np.float32).reshape(-1, 14)
     out = np.zeros((len(inp), 5), np.float32)
     L.orc_brdf_eval(len(inp), abi.fptr(inp), abi.fptr(out))
     return out
+
+
+def brdf_tangent(inp, tin):
+    """(f, df) of EvaluateBRDF for inputs (n, 14) along tangents (n, 5) {dbase, drough, dmetal}"""
+    L = load()
+    inp = np.ascontiguousarray(inp, np.float32).reshape(-1, 14)
+    tin = np.ascontiguousarray(tin, np.float32).reshape(-1, 5)
+    out = np.zeros((len(inp), 6), np.float32)
+    L.orc_brdf_tangent(len(inp), abi.fptr(inp), abi.fptr(tin), abi.fptr(out))
+    return out[:, :3], out[:, 3:]
 
 
 def sample_bounce(inp):

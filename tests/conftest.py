@@ -4,6 +4,11 @@ import sys
 
 import pytest
 
+try:  # before anything loads the HIP library, so that it shares torch's HIP runtime
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -1,0 +1,198 @@
+"""Path-replay adjoint on the GPU (mt_diff_*) against the oracle's forward-mode tangents and
+against central finite differences with common random numbers.
+
+Tolerances: the adjoint and the oracle differentiate the same estimator on the same counter-RNG
+paths, so their directional derivatives agree to float32 accumulation error except on the few
+paths whose decisions differ by libm ULPs (a few percent of samples in the image parity tests);
+the comparison allows 3 % on a sum over every pixel.  Finite differences see the reparameterised
+sampling as well, which the detached-sampling adjoint leaves out, so they agree only in
+expectation: 5 % at 2048 samples per pixel.
+"""
+import numpy as np
+import pytest
+
+from helpers import load_scene
+from mitrace_amd import abi, scenegen
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+GROUPS = ("tex", "rough", "metal", "emis")
+
+
+@pytest.fixture(scope="module")
+def torch_mod(gpu):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("torch sees no GPU")
+    return torch
+
+
+def _scene(segments=16, rings=8, tex_res=64):
+    return scenegen.inverse_scene(tex_res=tex_res, env_width=256, env_height=128,
+                                  segments=segments, rings=rings)
+
+
+def _dev(torch, params):
+    out = {}
+    for k in GROUPS:
+        out[k] = torch.tensor(np.ascontiguousarray(params[k], np.float32), device="cuda")
+    return out
+
+
+def _renderer(sc, st, params):
+    from mitrace_amd.diff import DiffRenderer
+    from mitrace_amd.tracer import GpuScene
+    gs = GpuScene(sc)
+    return gs, DiffRenderer(gs, st, params["tex_first"], params["tex_count"])
+
+
+def test_diff_forward_equals_plain_linear_render(torch_mod):
+    torch = torch_mod
+    sc, p = _scene()
+    st = scenegen.inverse_settings(width=40, height=32, samples=8, seed=3)
+    gs, dr = _renderer(sc, st, p)
+    img = dr.forward(sc.camera, 3).cpu().numpy()
+    ref, _, _ = gs.render(st)
+    # the scene's own values (the texture is not a scene texture, so only without overrides)
+    np.testing.assert_array_equal(img, ref)
+    # with the parameters as overrides the image changes only through the texture
+    d = _dev(torch, p)
+    img2 = dr.forward(sc.camera, 3, **d).cpu().numpy()
+    assert np.isfinite(img2).all() and not np.array_equal(img2, img)
+
+
+def test_diff_forward_matches_oracle_estimator(torch_mod):
+    torch = torch_mod
+    sc, p = _scene()
+    st = scenegen.inverse_settings(width=32, height=32, samples=16, seed=5)
+    gs, dr = _renderer(sc, st, p)
+    img = dr.forward(sc.camera, 5, **_dev(torch, p)).cpu().numpy()
+    ref, _ = po.OracleScene(sc).render_diff(st, p)
+    # same paths: most pixels identical to float rounding, a few differ through libm ULPs
+    close = np.isclose(img, ref, rtol=1e-4, atol=1e-5).all(axis=-1)
+    assert close.mean() > 0.9, close.mean()
+    assert abs(img.mean() - ref.mean()) < 0.02 * ref.mean()
+
+
+@pytest.mark.parametrize("group", GROUPS)
+def test_adjoint_matches_oracle_tangent(torch_mod, group):
+    torch = torch_mod
+    sc, p = _scene()
+    p = dict(p)
+    p["emis"] = np.full_like(p["emis"], 0.25)  # so that emission has an effect
+    st = scenegen.inverse_settings(width=32, height=32, samples=16, seed=7)
+    gs, dr = _renderer(sc, st, p)
+    rng = np.random.default_rng(17)
+    gimg = rng.normal(size=(32, 32, 3)).astype(np.float32)
+    v = rng.normal(size=np.shape(p[group])).astype(np.float32)
+    d = _dev(torch, p)
+    dr.forward(sc.camera, 7, **d)
+    g = dr.backward(sc.camera, 7, torch.tensor(gimg, device="cuda"), **d)
+    adj = float((g[group].cpu().numpy().astype(np.float64) * v).sum())
+    _, dimg = po.OracleScene(sc).render_diff(st, p, {group: v})
+    ref = float((dimg.astype(np.float64) * gimg).sum())
+    print(f"{group}: adjoint {adj:.6g} oracle tangent {ref:.6g}")
+    assert abs(ref) > 1e-3
+    assert abs(adj - ref) <= 0.03 * abs(ref), (adj, ref)
+
+
+def test_autograd_function_and_store_regeneration(torch_mod):
+    torch = torch_mod
+    sc, p = _scene()
+    st = scenegen.inverse_settings(width=24, height=24, samples=8, seed=2)
+    gs, dr = _renderer(sc, st, p)
+    d = {k: t.clone().requires_grad_(True) for k, t in _dev(torch, p).items()}
+    cams = scenegen.inverse_poses(2)
+    target = torch.rand(24, 24, 3, device="cuda")
+    # two renders before one backward: the first one's per-path store must be recorded again
+    img_a = dr.render(cams[0], 11, **d)
+    img_b = dr.render(cams[1], 12, **d)
+    loss = ((img_a - target) ** 2).sum() + ((img_b - target) ** 2).sum()
+    loss.backward()
+    auto = {k: d[k].grad.clone() for k in GROUPS}
+    # the same gradients from direct forward/backward pairs
+    ref = {k: torch.zeros_like(d[k]) for k in GROUPS}
+    plain = {k: t.detach() for k, t in d.items()}
+    for cam, seed in ((cams[0], 11), (cams[1], 12)):
+        img = dr.forward(cam, seed, **plain)
+        g = dr.backward(cam, seed, 2.0 * (img - target), **plain)
+        for k in GROUPS:
+            ref[k] += g[k]
+    for k in GROUPS:
+        a, b = auto[k].cpu().numpy(), ref[k].cpu().numpy()
+        np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * (np.abs(b).max() + 1e-12))
+        assert np.abs(b).max() > 0 or k == "emis"
+
+
+def test_emission_gradient_is_exact(torch_mod):
+    """the image is linear in emission and no decision depends on it: the adjoint equals the
+    central difference of two renders with the same seed to float32 rounding"""
+    torch = torch_mod
+    sc, p = _scene()
+    p = dict(p)
+    p["emis"] = np.full_like(p["emis"], 0.5)
+    st = scenegen.inverse_settings(width=32, height=32, samples=16, seed=4)
+    gs, dr = _renderer(sc, st, p)
+    rng = np.random.default_rng(3)
+    w = torch.tensor(rng.normal(size=(32, 32, 3)).astype(np.float32), device="cuda")
+    d = _dev(torch, p)
+    dr.forward(sc.camera, 4, **d)
+    g = dr.backward(sc.camera, 4, w, **d)["emis"].cpu().numpy()
+    h = 0.05
+    for mi in (0, 24, 48, 49):
+        for c in range(3):
+            dp, dm = dict(d), dict(d)
+            e = d["emis"].clone()
+            e[mi * 3 + c] += h
+            dp["emis"] = e
+            e2 = d["emis"].clone()
+            e2[mi * 3 + c] -= h
+            dm["emis"] = e2
+            lp = float((dr.forward(sc.camera, 4, **dp) * w).sum())
+            lm = float((dr.forward(sc.camera, 4, **dm) * w).sum())
+            fd = (lp - lm) / (2 * h)
+            assert abs(g[mi * 3 + c] - fd) <= 2e-3 * max(abs(fd), 1e-2), (mi, c, g[mi * 3 + c], fd)
+
+
+@pytest.mark.parametrize("group", ("tex", "rough", "metal"))
+def test_adjoint_vs_central_fd_with_common_random_numbers(torch_mod, group):
+    """directional derivative of a random linear loss, 64x64 at 2048 spp"""
+    torch = torch_mod
+    sc, p = _scene(segments=64, rings=32, tex_res=16)
+    st = scenegen.inverse_settings(width=64, height=64, samples=2048, seed=21,
+                                   paths_in_flight=1 << 23)
+    gs, dr = _renderer(sc, st, p)
+    rng = np.random.default_rng(23)
+    w = torch.tensor(rng.uniform(0.0, 1.0, size=(64, 64, 3)).astype(np.float32), device="cuda")
+    d = _dev(torch, p)
+    if group == "tex":
+        v = rng.normal(size=p["tex"].shape).astype(np.float32)
+    else:
+        v = np.zeros_like(p[group])
+        v[:49] = rng.normal(size=49)  # the spheres' scalars (the ground's metallic is 0)
+        v[np.abs(p[group] - 0.05) < 2 * 1e-2] = 0.0  # no difference across the 0.05 clamp
+    vt = torch.tensor(v, device="cuda")
+    dr.forward(sc.camera, 21, **d)
+    g = dr.backward(sc.camera, 21, w, **d)[group]
+    adj = float((g * vt).sum())
+    h = 1e-2
+    dp, dm = dict(d), dict(d)
+    dp[group] = (d[group] + h * vt).contiguous()
+    dm[group] = (d[group] - h * vt).contiguous()
+    lp = float((dr.forward(sc.camera, 21, **dp).double() * w).sum())
+    lm = float((dr.forward(sc.camera, 21, **dm).double() * w).sum())
+    fd = (lp - lm) / (2 * h)
+    print(f"{group}: adjoint {adj:.6g} central difference {fd:.6g}")
+    assert abs(adj - fd) <= 0.05 * abs(fd), (adj, fd)
+
+
+def test_diff_rejects_bad_arguments(gpu):
+    from mitrace_amd.tracer import GpuScene
+    import ctypes as C
+    sc = load_scene("cornell_box")
+    gs = GpuScene(sc)
+    st = scenegen.inverse_settings(width=8, height=8, samples=1, bounces=40).to_desc()
+    h = C.c_void_p()
+    assert gpu.mt_diff_create(gs.h, C.byref(st), C.byref(h)) == abi.MT_E_UNSUPPORTED
+    assert not h.value
