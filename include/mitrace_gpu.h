@@ -305,13 +305,17 @@ int mt_brdf_eval(uint32_t n, const float* in, float* out);
  * with MT_OUTPUT_LINEAR and no firefly rejection.  mt_diff_forward writes the linear image
  * (width*height*3 floats, device) and stores the per-path data; mt_diff_backward replays every
  * path from its counter-RNG key and accumulates d(loss)/d(params) given grad_image_dev =
- * d(loss)/d(image).  seed replaces settings->seed (a backward call must use the seed, camera
+ * d(loss)/d(image).  sampling_params (NULL = params) freezes the sampling: every decision
+ * (lobe, direction, pdf, termination) is taken with sampling_params and only the values are
+ * evaluated with params -- the estimator whose derivative the adjoint computes, so central
+ * differences of it check the adjoint exactly.  seed replaces settings->seed (a backward call must use the seed, camera
  * and params of the forward call it follows).  stream: a hipStream_t to run on (0 = the
  * library's own stream, after a device synchronisation); both calls return after their work
  * on that stream has completed. */
 int mt_diff_create(mt_scene* scene, const mt_settings* settings, mt_diff** out);
 int mt_diff_forward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_params* params,
-                    uint64_t seed, float* image_dev, uint64_t stream);
+                    const mt_diff_params* sampling_params, uint64_t seed, float* image_dev,
+                    uint64_t stream);
 int mt_diff_backward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_params* params,
                      uint64_t seed, const float* grad_image_dev, const mt_diff_grads* grads,
                      uint64_t stream);

@@ -165,8 +165,8 @@ SIGNATURES = {
     "mt_brdf_eval": (C.c_int, [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mt_diff_create": (C.c_int, [C.c_void_p, C.POINTER(mt_settings), C.POINTER(C.c_void_p)]),
     "mt_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(mt_camera_desc),
-                                  C.POINTER(mt_diff_params), C.c_uint64, C.c_void_p,
-                                  C.c_uint64]),
+                                  C.POINTER(mt_diff_params), C.POINTER(mt_diff_params),
+                                  C.c_uint64, C.c_void_p, C.c_uint64]),
     "mt_diff_backward": (C.c_int, [C.c_void_p, C.POINTER(mt_camera_desc),
                                    C.POINTER(mt_diff_params), C.c_uint64, C.c_void_p,
                                    C.POINTER(mt_diff_grads), C.c_uint64]),

@@ -291,19 +291,22 @@ int mt_diff_create(mt_scene* scene, const mt_settings* settings, mt_diff** out) 
 }
 
 int mt_diff_forward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_params* params,
-                    uint64_t seed, float* image_dev, uint64_t stream) {
+                    const mt_diff_params* sampling_params, uint64_t seed, float* image_dev,
+                    uint64_t stream) {
     if (!diff || !diff->ready || !camera || !image_dev)
         return mt::fail(MT_E_INVALID, "null diff handle, camera or image");
     try {
-        mt::DiffParams dp;
+        mt::DiffParams dp, dps;
         int rc = diff_params(diff, params, dp);
         if (rc) return rc;
+        if (sampling_params && (rc = diff_params(diff, sampling_params, dps))) return rc;
         if (hipSetDevice(diff->scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
         if (!stream && hipDeviceSynchronize() != hipSuccess)
             return mt::fail(MT_E_HIP, "hipDeviceSynchronize failed");
         diff->renderer.set_camera(*camera);
         diff->renderer.set_seed(seed);
-        rc = diff->renderer.diff_forward(dp, image_dev, reinterpret_cast<hipStream_t>(stream));
+        rc = diff->renderer.diff_forward(dp, sampling_params ? &dps : nullptr, image_dev,
+                                         reinterpret_cast<hipStream_t>(stream));
         return rc;
     } catch (const std::exception& e) {
         return mt::fail(MT_E_INVALID, e.what());

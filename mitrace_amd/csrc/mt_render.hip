@@ -544,22 +544,38 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 const float metal = ms.metal, rough = ms.rough;
                 const V3 N = shading_normal(gN, ms.ntex);
                 const V3 view = neg(d);
+                // frozen sampling (differentiable forward only): every decision is taken with
+                // the sampling parameter set, the values are evaluated with the other one
+                V3 basec_s = basec;
+                float metal_s = metal, rough_s = rough;
+                const bool frozen = kDiff == 1 && c.frozen;
+                if (frozen) {
+                    MatSample mss;
+                    eval_material<1>(S, c.dps, mi, tu, tv, mss);
+                    basec_s = add(scale(mss.base, 0.96f), mk(0.04f));
+                    metal_s = mss.metal;
+                    rough_s = mss.rough;
+                }
                 // BRDF sampling (Tracer.cpp:177-200)
-                const float ps = spec_probability(view, N, basec, metal);
+                const float ps = spec_probability(view, N, basec_s, metal_s);
                 V3 nd;
                 if (draw(key, dim0 + kDimLobe) < ps) {
                     const float u1 = draw(key, dim0 + kDimDir), u2 = draw(key, dim0 + kDimDir + 1);
-                    if (!ggx_dir(u1, u2, d, N, rough, nd))
+                    if (!ggx_dir(u1, u2, d, N, rough_s, nd))
                         nd = cosine_dir(draw(key, dim0 + kDimFallback), draw(key, dim0 + kDimFallback + 1), N);
                 } else {
                     nd = cosine_dir(draw(key, dim0 + kDimDir), draw(key, dim0 + kDimDir + 1), N);
                 }
-                const float pdf = (1.0f - ps) * diffuse_pdf(nd, N) + ps * specular_pdf(nd, view, N, rough);
+                const float pdf = (1.0f - ps) * diffuse_pdf(nd, N) + ps * specular_pdf(nd, view, N, rough_s);
                 const V3 fb = eval_brdf(nd, view, N, basec, rough, metal);
                 const float nl = gmax(dot(N, nd), 0.0f);
                 V3 energy = (pdf > 0.0f) ? scale(fb, nl / pdf) : mk(0.0f);
                 // termination bookkeeping of GeneratePath (Tracer.cpp:206-208)
-                V3 tterm = mul(xyz(b.t_term[p]), energy);
+                V3 energy_t = energy;
+                if (frozen)
+                    energy_t = (pdf > 0.0f) ? scale(eval_brdf(nd, view, N, basec_s, rough_s, metal_s), nl / pdf)
+                                            : mk(0.0f);
+                V3 tterm = mul(xyz(b.t_term[p]), energy_t);
                 bool stop;
                 float rr_q = 1.0f;
                 if (c.mode == MT_MODE_FIXED) {
@@ -1123,9 +1139,12 @@ int Renderer::run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s
     return MT_OK;
 }
 
-int Renderer::diff_forward(const DiffParams& dp, float* img_dev, hipStream_t s) {
+int Renderer::diff_forward(const DiffParams& dp, const DiffParams* sampling, float* img_dev,
+                           hipStream_t s) {
     if (!store_) return fail(MT_E_INVALID, "renderer was not set up for differentiable rendering");
     params_.dp = dp;
+    params_.frozen = sampling != nullptr;
+    params_.dps = sampling ? *sampling : dp;
     params_.gr = DiffGradPtrs{};
     params_.grad_img = nullptr;
     hipStream_t st = s ? s : stream_;
@@ -1140,6 +1159,7 @@ int Renderer::diff_adjoint(const DiffParams& dp, const DiffGradPtrs& gr, const f
                            hipStream_t s) {
     if (!store_) return fail(MT_E_INVALID, "renderer was not set up for differentiable rendering");
     params_.dp = dp;
+    params_.frozen = 0;
     params_.gr = gr;
     params_.grad_img = grad_img;
     hipStream_t st = s ? s : stream_;

@@ -59,7 +59,9 @@ struct ChunkParams {
     uint32_t setup_min;   // lanes needing an instance set-up that trigger a set-up step
     // differentiable rendering: 0 plain, 1 forward with per-path store, 2 adjoint replay
     int32_t diff;
+    int32_t frozen;        // differentiable forward: decisions from dps, values from dp
     DiffParams dp;
+    DiffParams dps;
     DiffGradPtrs gr;
     float4* path_store;    // [npix_total * spp] {linear radiance, shadow visibility bits}
     const float* grad_img; // [npix_total * 3] dLoss/dImage (adjoint replay)
@@ -97,7 +99,8 @@ class Renderer {
     // differentiable rendering over the whole image on the caller's stream (0 = own stream):
     // forward writes the linear image (mean radiance per pixel) to img_dev and the per-path
     // store; adjoint replays the same paths and accumulates into the gradient buffers
-    int diff_forward(const DiffParams& dp, float* img_dev, hipStream_t s);
+    int diff_forward(const DiffParams& dp, const DiffParams* sampling, float* img_dev,
+                     hipStream_t s);
     int diff_adjoint(const DiffParams& dp, const DiffGradPtrs& gr, const float* grad_img,
                      hipStream_t s);
     int finish(float* rgb_out);

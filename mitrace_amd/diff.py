@@ -75,14 +75,19 @@ class DiffRenderer:
         p.material_count = self.material_count
         return p
 
-    def forward(self, camera, seed, tex=None, rough=None, metal=None, emis=None):
-        """linear image (H, W, 3) on the GPU; records the per-path data for backward()"""
+    def forward(self, camera, seed, tex=None, rough=None, metal=None, emis=None, sampling=None):
+        """linear image (H, W, 3) on the GPU; records the per-path data for backward().
+        sampling: optional dict {tex, rough, metal, emis} that takes every sampling decision
+        (frozen sampling) while the given parameters only set the values."""
         s = self.settings
         img = torch.empty((s.height, s.width, 3), dtype=torch.float32, device="cuda")
         p = self._params(tex, rough, metal, emis)
+        ps = None
+        if sampling is not None:
+            ps = C.byref(self._params(*(sampling.get(n) for n in PARAM_NAMES)))
         cam = camera.to_desc()
         stream = torch.cuda.current_stream().cuda_stream
-        check(self.L.mt_diff_forward(self.h, C.byref(cam), C.byref(p), seed, img.data_ptr(),
+        check(self.L.mt_diff_forward(self.h, C.byref(cam), C.byref(p), ps, seed, img.data_ptr(),
                                      stream))
         self.generation += 1
         return img

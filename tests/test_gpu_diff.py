@@ -4,9 +4,12 @@ against central finite differences with common random numbers.
 Tolerances: the adjoint and the oracle differentiate the same estimator on the same counter-RNG
 paths, so their directional derivatives agree to float32 accumulation error except on the few
 paths whose decisions differ by libm ULPs (a few percent of samples in the image parity tests);
-the comparison allows 3 % on a sum over every pixel.  Finite differences see the reparameterised
-sampling as well, which the detached-sampling adjoint leaves out, so they agree only in
-expectation: 5 % at 2048 samples per pixel.
+the comparison allows 3 % on a sum over every pixel.  Central differences of the estimator
+with its sampling frozen (decisions from the unperturbed parameters) differentiate the same
+function as the adjoint: 5 % per parameter at h = 1e-3.  Central differences with common random
+numbers alone also see the changed sampling decisions, whose effect is zero-mean but noisy:
+5 % on the mean over seeds, for the texture (for roughness and metallic that noise is larger
+than the derivative at test sizes).
 """
 import numpy as np
 import pytest
@@ -155,9 +158,59 @@ def test_emission_gradient_is_exact(torch_mod):
             assert abs(g[mi * 3 + c] - fd) <= 2e-3 * max(abs(fd), 1e-2), (mi, c, g[mi * 3 + c], fd)
 
 
-@pytest.mark.parametrize("group", ("tex", "rough", "metal"))
-def test_adjoint_vs_central_fd_with_common_random_numbers(torch_mod, group):
-    """directional derivative of a random linear loss, 64x64 at 2048 spp"""
+def test_adjoint_vs_frozen_sampling_central_differences(torch_mod):
+    """per parameter: 24 random texels, every sphere's roughness and metallic, 4 emission
+    entries; central differences (h = 1e-3) of the estimator with its sampling frozen at the
+    unperturbed parameters -- the estimator the adjoint differentiates"""
+    torch = torch_mod
+    sc, p = _scene(segments=32, rings=16, tex_res=32)
+    p = dict(p)
+    p["emis"] = np.full_like(p["emis"], 0.25)
+    st = scenegen.inverse_settings(width=64, height=64, samples=64, seed=31)
+    gs, dr = _renderer(sc, st, p)
+    rng = np.random.default_rng(29)
+    w = torch.tensor(rng.uniform(0.0, 1.0, size=(64, 64, 3)).astype(np.float32), device="cuda")
+    d = _dev(torch, p)
+    dr.forward(sc.camera, 31, **d)
+    g = {k: v.cpu().numpy() for k, v in dr.backward(sc.camera, 31, w, **d).items()}
+    h = 1e-3
+    cases = [("tex", int(i)) for i in rng.choice(p["tex"].size, 24, replace=False)]
+    cases += [("rough", i) for i in range(49) if p["rough"][i] > 0.05 + 6 * h]
+    cases += [("metal", i) for i in range(49)]
+    cases += [("emis", i) for i in (0, 7, 100, 149)]
+
+    def central(group, i, step):
+        flat = d[group].reshape(-1)
+        dp_, dm_ = dict(d), dict(d)
+        e = flat.clone()
+        e[i] += step
+        dp_[group] = e.reshape(d[group].shape)
+        e = flat.clone()
+        e[i] -= step
+        dm_[group] = e.reshape(d[group].shape)
+        ip = dr.forward(sc.camera, 31, sampling=d, **dp_).double()
+        im = dr.forward(sc.camera, 31, sampling=d, **dm_).double()
+        return float(((ip - im) * w).sum()) / (2 * step)
+
+    within, worst = 0, 0.0
+    for group, i in cases:
+        fd = central(group, i, h)
+        # the float32 image rounds each sample: the difference of two step sizes bounds that
+        # noise for samples whose values are large next to the derivative
+        noise = abs(fd - central(group, i, 3 * h))
+        adj = float(g[group].reshape(-1)[i])
+        floor = 1e-3 * np.abs(g[group]).max()
+        err = abs(adj - fd)
+        within += err <= 0.05 * abs(fd) + floor
+        worst = max(worst, err / max(abs(fd), floor))
+        assert err <= 0.05 * abs(fd) + 2.0 * noise + floor, (group, i, adj, fd, noise)
+    print(f"{len(cases)} parameters, {within} within 5 %, worst relative difference {worst:.2e}")
+    assert within >= 0.95 * len(cases)
+
+
+def test_adjoint_vs_crn_central_differences_texture(torch_mod):
+    """sampling not frozen: central differences with common random numbers also see the
+    changed sampling decisions, which are zero-mean; mean over 4 seeds, 64x64 at 2048 spp"""
     torch = torch_mod
     sc, p = _scene(segments=64, rings=32, tex_res=16)
     st = scenegen.inverse_settings(width=64, height=64, samples=2048, seed=21,
@@ -166,25 +219,21 @@ def test_adjoint_vs_central_fd_with_common_random_numbers(torch_mod, group):
     rng = np.random.default_rng(23)
     w = torch.tensor(rng.uniform(0.0, 1.0, size=(64, 64, 3)).astype(np.float32), device="cuda")
     d = _dev(torch, p)
-    if group == "tex":
-        v = rng.normal(size=p["tex"].shape).astype(np.float32)
-    else:
-        v = np.zeros_like(p[group])
-        v[:49] = rng.normal(size=49)  # the spheres' scalars (the ground's metallic is 0)
-        v[np.abs(p[group] - 0.05) < 2 * 1e-2] = 0.0  # no difference across the 0.05 clamp
-    vt = torch.tensor(v, device="cuda")
-    dr.forward(sc.camera, 21, **d)
-    g = dr.backward(sc.camera, 21, w, **d)[group]
-    adj = float((g * vt).sum())
+    vt = torch.tensor(rng.normal(size=p["tex"].shape).astype(np.float32), device="cuda")
     h = 1e-2
-    dp, dm = dict(d), dict(d)
-    dp[group] = (d[group] + h * vt).contiguous()
-    dm[group] = (d[group] - h * vt).contiguous()
-    lp = float((dr.forward(sc.camera, 21, **dp).double() * w).sum())
-    lm = float((dr.forward(sc.camera, 21, **dm).double() * w).sum())
-    fd = (lp - lm) / (2 * h)
-    print(f"{group}: adjoint {adj:.6g} central difference {fd:.6g}")
-    assert abs(adj - fd) <= 0.05 * abs(fd), (adj, fd)
+    adj, fd = [], []
+    for seed in (21, 22, 23, 24):
+        dr.forward(sc.camera, seed, **d)
+        adj.append(float((dr.backward(sc.camera, seed, w, **d)["tex"] * vt).sum()))
+        dp_, dm_ = dict(d), dict(d)
+        dp_["tex"] = (d["tex"] + h * vt).contiguous()
+        dm_["tex"] = (d["tex"] - h * vt).contiguous()
+        lp = float((dr.forward(sc.camera, seed, **dp_).double() * w).sum())
+        lm = float((dr.forward(sc.camera, seed, **dm_).double() * w).sum())
+        fd.append((lp - lm) / (2 * h))
+    a, f = np.mean(adj), np.mean(fd)
+    print(f"adjoint {adj} central differences {fd}")
+    assert abs(a - f) <= 0.05 * abs(f), (a, f)
 
 
 def test_diff_rejects_bad_arguments(gpu):
