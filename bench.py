@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate wall time of the CPU-baseline sample")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the fwd+adjoint timing of the inverse-rendering workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a device-to-device copy of 1 GiB (counter calibration)")
     return ap.parse_args()
@@ -91,6 +93,60 @@ def copy_calibration(nbytes=1 << 30):
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
+
+
+def adjoint_timing(reps=5):
+    """fwd+adjoint ms per iteration of the inverse-rendering workload: the sphere grid with the
+    shared 512x512 float texture, one 512x512 pose at 32 spp (8,388,608 paths), every
+    parameter differentiated.  Device buffers come from the HIP runtime the library uses."""
+    import ctypes as C
+    import numpy as np
+    from mitrace_amd import abi, scenegen
+    from mitrace_amd._lib import check
+    from mitrace_amd.tracer import GpuScene
+    hip = C.CDLL("libamdhip64.so.7")
+    sc, p = scenegen.inverse_scene()
+    st = scenegen.inverse_settings()
+    g = GpuScene(sc)
+    L = g.L
+    bufs = []
+
+    def dev(a):
+        a = np.ascontiguousarray(a, np.float32)
+        ptr = C.c_void_p()
+        assert hip.hipMalloc(C.byref(ptr), C.c_size_t(max(a.nbytes, 4))) == 0
+        assert hip.hipMemcpy(ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0
+        bufs.append(ptr)
+        return ptr.value
+
+    prm = abi.mt_diff_params(dev(p["tex"]), 512, 512, 0, 49, dev(p["rough"]), dev(p["metal"]),
+                             dev(p["emis"]), len(sc.materials))
+    npix = st.width * st.height
+    img = dev(np.zeros(npix * 3))
+    gimg = dev(np.random.default_rng(0).random(npix * 3))
+    grads = abi.mt_diff_grads(dev(np.zeros(p["tex"].size)), dev(np.zeros(len(sc.materials))),
+                              dev(np.zeros(len(sc.materials))), dev(np.zeros(len(sc.materials) * 3)))
+    h = C.c_void_p()
+    check(L.mt_diff_create(g.h, C.byref(st.to_desc()), C.byref(h)))
+    cam = scenegen.inverse_poses(1)[0].to_desc()
+    fw, bw = [], []
+    for i in range(reps + 1):
+        t0 = time.perf_counter()
+        check(L.mt_diff_forward(h, C.byref(cam), C.byref(prm), None, 100 + i, img, 0))
+        t1 = time.perf_counter()
+        check(L.mt_diff_backward(h, C.byref(cam), C.byref(prm), 100 + i, gimg, C.byref(grads), 0))
+        t2 = time.perf_counter()
+        if i:  # the first pair is a warm-up
+            fw.append(t1 - t0)
+            bw.append(t2 - t1)
+    L.mt_diff_destroy(h)
+    for b in bufs:
+        hip.hipFree(b)
+    g.close()
+    f, b = 1e3 * float(np.median(fw)), 1e3 * float(np.median(bw))
+    return {"workload": "inverse-rendering sphere grid (512x512 float texture, 4 point lights), "
+                        "512x512 at 32 spp, one pose", "fwd_ms": f, "adjoint_ms": b,
+            "fwd_plus_adjoint_ms": f + b, "ratio_to_fwd": (f + b) / f}
 
 
 def main():
@@ -172,6 +228,9 @@ def main():
                                  ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
         "cpu_baseline": None,
     }
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_timing()
+        print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(scene, settings, args.cpu_seconds)
         line["cpu_baseline"] = cb
