@@ -257,9 +257,45 @@ typedef struct mt_diff_grads {
     float* emission_dev;
 } mt_diff_grads;
 
+/* Batched rendering (no reference counterpart): knobs of the persistent iteration loop.
+ * Zero fields take the defaults in brackets. */
+typedef struct mt_batch_options {
+    uint32_t paths_per_iteration; /* wavefront width: camera paths in flight [4194304]   */
+    uint32_t tile_pixels;         /* pixels per work item of the device queue [1024]      */
+    uint32_t iteration_tiles;     /* queue capacity: tiles one iteration may take [4096] */
+    uint32_t trace_blocks_per_cu; /* persistent traversal blocks per CU [8]              */
+} mt_batch_options;
+
+typedef struct mt_batch_job_info {
+    int32_t done;       /* 1 once the iteration holding the job's last tile has completed */
+    int32_t status;     /* MT_OK, or the error that stopped the batch                      */
+    uint32_t tiles;
+    uint32_t pad;
+    uint64_t samples;   /* camera paths = width * height * samples                         */
+    double submit_ms;   /* host clock, ms since mt_batch_create                            */
+    double complete_ms; /* device completion of the job's last iteration, same clock       */
+} mt_batch_job_info;
+
+typedef struct mt_batch_stats {
+    uint64_t iterations;       /* graph launches                                       */
+    uint64_t graphs_captured;  /* captures (one per scene and bounce count)            */
+    uint64_t samples;          /* camera paths of all jobs                             */
+    uint32_t jobs;
+    uint32_t scenes;
+    mt_batch_options options;  /* the values in effect                                 */
+} mt_batch_stats;
+
+/* one trial of mt_batch_autotune */
+typedef struct mt_autotune_trial {
+    mt_batch_options options;
+    double seconds;
+    double msamples_per_s;
+} mt_autotune_trial;
+
 typedef struct mt_scene mt_scene;
 typedef struct mt_job mt_job;
 typedef struct mt_diff mt_diff;
+typedef struct mt_batch mt_batch;
 
 /* ---- functions -------------------------------------------------------------------------- */
 int mt_abi_version(void);
@@ -320,6 +356,30 @@ int mt_diff_backward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_
                      uint64_t seed, const float* grad_image_dev, const mt_diff_grads* grads,
                      uint64_t stream);
 int mt_diff_destroy(mt_diff* diff);
+
+/* Batched rendering (no reference counterpart; for serving many render requests).  Jobs of
+ * any scene, resolution, sample count and depth are split into tiles of tile_pixels pixels on
+ * a per-scene device queue; one host thread replays, per iteration, a captured graph of the
+ * whole wavefront step (schedule tiles from the queue, generate, extend/shade/shadow per
+ * bounce, finalize, resolve into each job's image) for the scene of the oldest unfinished
+ * job.  Every job's image equals mt_render's with the same scene, camera and settings.
+ * mt_batch_submit may be called from any thread before or while the batch runs; rgb_out
+ * (host, width*height*3 floats, may be NULL) is written by mt_batch_finish. */
+int mt_batch_create(const mt_batch_options* options, mt_batch** out);
+int mt_batch_submit(mt_batch* batch, mt_scene* scene, const mt_camera_desc* camera,
+                    const mt_settings* settings, float* rgb_out, uint32_t* job_id);
+int mt_batch_start(mt_batch* batch);
+/* no more submissions: waits for every job and copies the images out */
+int mt_batch_finish(mt_batch* batch);
+int mt_batch_get_job_info(const mt_batch* batch, uint32_t job_id, mt_batch_job_info* info);
+int mt_batch_get_stats(const mt_batch* batch, mt_batch_stats* stats);
+int mt_batch_destroy(mt_batch* batch);
+/* Picks paths_per_iteration, tile_pixels, iteration_tiles and trace_blocks_per_cu by
+ * coordinate descent over measured runs of the n calibration jobs (all submitted at once).
+ * trials (may be NULL) receives up to max_trials measurements. */
+int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
+                      const mt_settings* settings, uint32_t n, mt_batch_options* best,
+                      mt_autotune_trial* trials, uint32_t max_trials, uint32_t* n_trials);
 
 #ifdef __cplusplus
 }

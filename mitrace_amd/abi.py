@@ -141,6 +141,31 @@ class mt_diff_grads(C.Structure):
                 ("metallic_dev", C.c_void_p), ("emission_dev", C.c_void_p)]
 
 
+class mt_batch_options(C.Structure):
+    _fields_ = [("paths_per_iteration", C.c_uint32), ("tile_pixels", C.c_uint32),
+                ("iteration_tiles", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class mt_batch_job_info(C.Structure):
+    _fields_ = [("done", C.c_int32), ("status", C.c_int32), ("tiles", C.c_uint32),
+                ("pad", C.c_uint32), ("samples", C.c_uint64), ("submit_ms", C.c_double),
+                ("complete_ms", C.c_double)]
+
+
+class mt_batch_stats(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("graphs_captured", C.c_uint64),
+                ("samples", C.c_uint64), ("jobs", C.c_uint32), ("scenes", C.c_uint32),
+                ("options", mt_batch_options)]
+
+
+class mt_autotune_trial(C.Structure):
+    _fields_ = [("options", mt_batch_options), ("seconds", C.c_double),
+                ("msamples_per_s", C.c_double)]
+
+
 # every exported symbol of the library and its signature: (restype, argtypes)
 SIGNATURES = {
     "mt_abi_version": (C.c_int, []),
@@ -171,6 +196,19 @@ SIGNATURES = {
                                    C.POINTER(mt_diff_params), C.c_uint64, C.c_void_p,
                                    C.POINTER(mt_diff_grads), C.c_uint64]),
     "mt_diff_destroy": (C.c_int, [C.c_void_p]),
+    "mt_batch_create": (C.c_int, [C.POINTER(mt_batch_options), C.POINTER(C.c_void_p)]),
+    "mt_batch_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(mt_camera_desc),
+                                  C.POINTER(mt_settings), C.POINTER(C.c_float),
+                                  C.POINTER(C.c_uint32)]),
+    "mt_batch_start": (C.c_int, [C.c_void_p]),
+    "mt_batch_finish": (C.c_int, [C.c_void_p]),
+    "mt_batch_get_job_info": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(mt_batch_job_info)]),
+    "mt_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(mt_batch_stats)]),
+    "mt_batch_destroy": (C.c_int, [C.c_void_p]),
+    "mt_batch_autotune": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(mt_camera_desc),
+                                    C.POINTER(mt_settings), C.c_uint32,
+                                    C.POINTER(mt_batch_options), C.POINTER(mt_autotune_trial),
+                                    C.c_uint32, C.POINTER(C.c_uint32)]),
 }
 
 

@@ -1,0 +1,535 @@
+// mt_batch.cpp -- batched rendering: per-scene device tile queues, one captured graph per scene
+// for a whole wavefront iteration, a host scheduler thread, and the autotuner.
+//
+// The scheduler keeps an exact host mirror of the device queues: the device scheduler kernel
+// and the host apply the same greedy rule (tiles in queue order while their paths fit the
+// iteration, at least one), so the host knows which iteration finishes each job without
+// reading anything back, and only waits on events to bound the number of queued iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "mt_internal.hpp"
+#include "mt_render.hpp"
+
+namespace {
+
+constexpr uint32_t kMaxJobs = 1u << 14;
+constexpr uint32_t kQueueTiles = 1u << 20;  // per scene
+constexpr uint32_t kMaxStack = 64;          // traversal stack entries (LDS + spill)
+constexpr uint32_t kLds = 16;
+constexpr uint32_t kBlockSize = 256;
+constexpr uint32_t kInflight = 3;           // iterations queued ahead of the device
+
+double now_ms(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct HostJob {
+    mt_scene* scene = nullptr;
+    uint32_t slot = 0;
+    mt_camera_desc cam{};
+    mt_settings st{};
+    float* rgb_out = nullptr;
+    float* img_dev = nullptr;
+    uint32_t tiles = 0, tiles_left = 0;
+    uint64_t samples = 0;
+    double submit_ms = 0.0;
+    hipEvent_t done_ev = nullptr;            // set before `recorded`
+    std::atomic<bool> recorded{false};
+};
+
+struct SceneSlot {
+    mt_scene* scene = nullptr;
+    uint4* q_dev = nullptr;        // device queue
+    uint32_t* q_state = nullptr;   // device {head, tail}
+    uint4* q_host = nullptr;       // pinned, append-only copy of the queue
+    uint32_t head = 0, tail = 0;   // host mirror
+    uint32_t bounces = 0;          // largest bounce count of the scene's jobs
+    uint32_t graph_bounces = 0;    // bounce count of the captured graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    mt::ChunkParams c{};
+    mt::PathBuffers b{};
+};
+
+}  // namespace
+
+struct mt_batch {
+    int device = 0;
+    mt_batch_options opt{};
+    hipStream_t stream = nullptr, capture = nullptr;
+    std::vector<void*> owned;
+    mt::PathBuffers buf{};
+    mt::BatchJob* jobs_dev = nullptr;
+    mt::BatchJob* jobs_host = nullptr;  // pinned staging, one record per job
+    uint32_t trace_blocks = 0, shade_blocks = 0;
+    std::chrono::steady_clock::time_point created;
+    // shared with submitters
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::unique_ptr<HostJob>> jobs;
+    std::vector<std::unique_ptr<SceneSlot>> slots;
+    uint32_t admitted = 0;  // jobs [0, admitted) are on the device queues
+    std::vector<HostJob*> view;  // scheduler's copy of the job pointers
+    std::atomic<bool> finishing{false};
+    std::atomic<bool> started{false};
+    std::thread worker;
+    int status = MT_OK;
+    std::string error;
+    // scheduler state
+    hipEvent_t e0 = nullptr;
+    double t0_ms = 0.0;
+    uint64_t iterations = 0, captures = 0;
+    uint32_t first_open = 0;  // oldest job with tiles not yet scheduled
+    std::deque<hipEvent_t> inflight;
+    std::vector<hipEvent_t> spare;
+
+    ~mt_batch();
+    int init(const mt_batch_options* o);
+    int admit(uint32_t id);
+    int ensure_graph(SceneSlot& s);
+    int run();
+    int fail(int code, const std::string& m) {
+        status = code;
+        error = m;
+        mt::set_error(m);
+        return code;
+    }
+};
+
+#define BT_TRY(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) return fail(MT_E_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+mt_batch::~mt_batch() {
+    if (worker.joinable()) {
+        finishing.store(true);
+        cv.notify_all();
+        worker.join();
+    }
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& s : slots) {
+        if (s->exec) (void)hipGraphExecDestroy(s->exec);
+        if (s->graph) (void)hipGraphDestroy(s->graph);
+        if (s->q_host) (void)hipHostFree(s->q_host);
+    }
+    for (auto& j : jobs)
+        if (j->done_ev) (void)hipEventDestroy(j->done_ev);
+    for (hipEvent_t e : inflight) (void)hipEventDestroy(e);
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
+    if (e0) (void)hipEventDestroy(e0);
+    for (void* p : owned) (void)hipFree(p);
+    if (jobs_host) (void)hipHostFree(jobs_host);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (capture) (void)hipStreamDestroy(capture);
+}
+
+int mt_batch::init(const mt_batch_options* o) {
+    created = std::chrono::steady_clock::now();
+    if (o) opt = *o;
+    if (!opt.paths_per_iteration) opt.paths_per_iteration = 1u << 22;
+    if (!opt.tile_pixels) opt.tile_pixels = 1024;
+    if (!opt.iteration_tiles) opt.iteration_tiles = 4096;
+    if (!opt.trace_blocks_per_cu) opt.trace_blocks_per_cu = 8;
+    if (opt.paths_per_iteration > (1u << 26) || opt.iteration_tiles > (1u << 20) ||
+        opt.trace_blocks_per_cu > 32)
+        return fail(MT_E_UNSUPPORTED, "batch options out of range");
+    BT_TRY(hipGetDevice(&device));
+    BT_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    BT_TRY(hipStreamCreateWithFlags(&capture, hipStreamNonBlocking));
+    const int cus = mt::cu_count();
+    trace_blocks = static_cast<uint32_t>(cus) * opt.trace_blocks_per_cu;
+    const uint64_t P = opt.paths_per_iteration;
+    shade_blocks = static_cast<uint32_t>(std::min<uint64_t>((P + kBlockSize - 1) / kBlockSize, cus * 16ull));
+    const uint64_t spill = static_cast<uint64_t>(kMaxStack - kLds) * trace_blocks * kBlockSize;
+    int rc = mt::alloc_path_buffers(owned, buf, P, spill, false, true);
+    if (rc) return fail(rc, mt_last_error());
+    auto dev_alloc = [&](void** p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) return fail(MT_E_NOMEM, "hipMalloc of batch buffers failed");
+        owned.push_back(*p);
+        return MT_OK;
+    };
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&buf.it_tiles), sizeof(uint4) * opt.iteration_tiles)) ||
+        (rc = dev_alloc(reinterpret_cast<void**>(&buf.it_pix_off), sizeof(uint32_t) * (opt.iteration_tiles + 1))) ||
+        (rc = dev_alloc(reinterpret_cast<void**>(&jobs_dev), sizeof(mt::BatchJob) * kMaxJobs)))
+        return rc;
+    BT_TRY(hipHostMalloc(reinterpret_cast<void**>(&jobs_host), sizeof(mt::BatchJob) * kMaxJobs));
+    buf.jobs = jobs_dev;
+    buf.it_capacity = opt.paths_per_iteration;
+    buf.it_tile_cap = opt.iteration_tiles;
+    BT_TRY(hipEventCreate(&e0));
+    return MT_OK;
+}
+
+// puts job `id` on its scene's device queue (scheduler thread)
+int mt_batch::admit(uint32_t id) {
+    HostJob& j = *view[id];
+    SceneSlot* s = nullptr;
+    for (auto& x : slots)
+        if (x->scene == j.scene) s = x.get();
+    if (!s) {
+        auto ns = std::make_unique<SceneSlot>();
+        ns->scene = j.scene;
+        if (mt::stack_capacity(j.scene) > kMaxStack)
+            return fail(MT_E_UNSUPPORTED, "scene BVH too deep for batched rendering");
+        if (hipMalloc(reinterpret_cast<void**>(&ns->q_dev), sizeof(uint4) * kQueueTiles) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&ns->q_state), sizeof(uint32_t) * 2) != hipSuccess)
+            return fail(MT_E_NOMEM, "hipMalloc of a tile queue failed");
+        owned.push_back(ns->q_dev);
+        owned.push_back(ns->q_state);
+        BT_TRY(hipHostMalloc(reinterpret_cast<void**>(&ns->q_host), sizeof(uint4) * kQueueTiles));
+        mt::launch_set_u32(ns->q_state, 0u, stream);
+        mt::launch_set_u32(ns->q_state + 1, 0u, stream);
+        // scene-level launch constants (the per-job ones come from the job table)
+        mt::fill_params(ns->c, j.scene, j.cam, j.st);
+        ns->c.count_work = 0;
+        ns->b = buf;
+        ns->b.q_tiles = ns->q_dev;
+        ns->b.q_state = ns->q_state;
+        slots.push_back(std::move(ns));
+        s = slots.back().get();
+    }
+    j.slot = 0;
+    for (uint32_t i = 0; i < slots.size(); ++i)
+        if (slots[i].get() == s) j.slot = i;
+    // job record
+    mt::BatchJob& rec = jobs_host[id];
+    mt::fill_params(rec.p, j.scene, j.cam, j.st);
+    rec.p.count_work = 0;
+    rec.img = j.img_dev;
+    BT_TRY(hipMemcpyAsync(jobs_dev + id, &rec, sizeof(rec), hipMemcpyHostToDevice, stream));
+    // tiles: whole pixels, at most paths_per_iteration paths each
+    const uint32_t npix = j.st.width * j.st.height;
+    const uint32_t tp = std::max<uint32_t>(1u, std::min<uint32_t>(opt.tile_pixels, opt.paths_per_iteration / j.st.samples));
+    const uint32_t first = s->tail;
+    for (uint32_t p0 = 0; p0 < npix; p0 += tp) {
+        if (s->tail >= kQueueTiles) return fail(MT_E_UNSUPPORTED, "tile queue of a scene is full");
+        const uint32_t n = std::min(tp, npix - p0);
+        s->q_host[s->tail++] = make_uint4(id, p0, n, n * j.st.samples);
+    }
+    j.tiles_left = s->tail - first;
+    {
+        std::scoped_lock lk(mu);
+        j.tiles = j.tiles_left;
+    }
+    BT_TRY(hipMemcpyAsync(s->q_dev + first, s->q_host + first, sizeof(uint4) * (s->tail - first),
+                          hipMemcpyHostToDevice, stream));
+    mt::launch_set_u32(s->q_state + 1, s->tail, stream);
+    BT_TRY(hipGetLastError());
+    s->bounces = std::max(s->bounces, j.st.bounces);
+    return MT_OK;
+}
+
+int mt_batch::ensure_graph(SceneSlot& s) {
+    if (s.exec && s.graph_bounces >= s.bounces) return MT_OK;
+    if (s.exec) {
+        BT_TRY(hipStreamSynchronize(stream));
+        BT_TRY(hipGraphExecDestroy(s.exec));
+        BT_TRY(hipGraphDestroy(s.graph));
+        s.exec = nullptr;
+        s.graph = nullptr;
+    }
+    mt::BatchLaunch L{s.bounces, kLds, trace_blocks, shade_blocks};
+    BT_TRY(hipStreamBeginCapture(capture, hipStreamCaptureModeRelaxed));
+    mt::launch_batch_iteration(s.scene, s.c, s.b, L, capture);
+    const hipError_t le = hipGetLastError();
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(capture, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(MT_E_HIP, std::string("graph capture failed: ") +
+                                  hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    s.graph = g;
+    BT_TRY(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+    s.graph_bounces = s.bounces;
+    ++captures;
+    return MT_OK;
+}
+
+int mt_batch::run() {
+    if (hipSetDevice(device) != hipSuccess) return fail(MT_E_HIP, "hipSetDevice failed");
+    while (true) {
+        // admit everything submitted so far
+        {
+            std::scoped_lock lk(mu);
+            for (size_t i = view.size(); i < jobs.size(); ++i) view.push_back(jobs[i].get());
+        }
+        const uint32_t total = static_cast<uint32_t>(view.size());
+        while (admitted < total) {
+            const int rc = admit(admitted);
+            if (rc) return rc;
+            ++admitted;
+        }
+        // the oldest job that still has unscheduled tiles decides the scene
+        while (first_open < admitted && view[first_open]->tiles_left == 0) ++first_open;
+        if (first_open == admitted) {
+            std::unique_lock lk(mu);
+            if (finishing.load() && jobs.size() == admitted) break;
+            cv.wait_for(lk, std::chrono::milliseconds(1));
+            continue;
+        }
+        SceneSlot& s = *slots[view[first_open]->slot];
+        int rc = ensure_graph(s);
+        if (rc) return rc;
+        BT_TRY(hipGraphLaunch(s.exec, stream));
+        ++iterations;
+        // mirror of k_batch_schedule
+        uint32_t n = 0;
+        uint64_t paths = 0;
+        while (s.head < s.tail && n < opt.iteration_tiles) {
+            const uint4 t = s.q_host[s.head];
+            if (n > 0 && paths + t.w > opt.paths_per_iteration) break;
+            paths += t.w;
+            ++n;
+            ++s.head;
+            HostJob& j = *view[t.x];
+            if (--j.tiles_left == 0) {
+                BT_TRY(hipEventCreate(&j.done_ev));
+                BT_TRY(hipEventRecord(j.done_ev, stream));
+                j.recorded.store(true, std::memory_order_release);
+            }
+        }
+        // keep a few iterations queued, so that new jobs are admitted promptly
+        hipEvent_t ev;
+        if (spare.empty()) {
+            BT_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        } else {
+            ev = spare.back();
+            spare.pop_back();
+        }
+        BT_TRY(hipEventRecord(ev, stream));
+        inflight.push_back(ev);
+        while (inflight.size() > kInflight) {
+            BT_TRY(hipEventSynchronize(inflight.front()));
+            spare.push_back(inflight.front());
+            inflight.pop_front();
+        }
+    }
+    BT_TRY(hipStreamSynchronize(stream));
+    return MT_OK;
+}
+
+namespace {
+
+bool valid_job(mt_scene* scene, const mt_camera_desc* cam, const mt_settings* st, uint32_t cap) {
+    if (!scene || !cam || !st) {
+        mt::set_error("null scene, camera or settings");
+        return false;
+    }
+    if (st->width == 0 || st->height == 0 || st->samples == 0) {
+        mt::set_error("width, height and samples must be positive");
+        return false;
+    }
+    if (st->bounces > mt::kMaxBounces || st->samples > cap ||
+        static_cast<uint64_t>(st->width) * st->height * 3 > 0xFFFFFFFFull) {
+        mt::set_error("job exceeds the batch limits (bounces <= 64, samples <= paths per iteration)");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mt_batch_create(const mt_batch_options* options, mt_batch** out) {
+    if (!out) return mt::fail(MT_E_INVALID, "null output handle");
+    *out = nullptr;
+    try {
+        auto b = std::make_unique<mt_batch>();
+        const int rc = b->init(options);
+        if (rc) return rc;
+        *out = b.release();
+        return MT_OK;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_NOMEM, e.what());
+    }
+}
+
+int mt_batch_submit(mt_batch* batch, mt_scene* scene, const mt_camera_desc* camera,
+                    const mt_settings* settings, float* rgb_out, uint32_t* job_id) {
+    if (!batch) return mt::fail(MT_E_INVALID, "null batch");
+    if (!valid_job(scene, camera, settings, batch->opt.paths_per_iteration)) return MT_E_INVALID;
+    if (batch->finishing.load()) return mt::fail(MT_E_INVALID, "batch is finishing");
+    if (batch->status) return mt::fail(batch->status, batch->error);
+    try {
+        auto j = std::make_unique<HostJob>();
+        j->scene = scene;
+        j->cam = *camera;
+        j->st = *settings;
+        j->rgb_out = rgb_out;
+        j->samples = static_cast<uint64_t>(settings->width) * settings->height * settings->samples;
+        j->submit_ms = now_ms(batch->created);
+        if (hipSetDevice(batch->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+        const size_t bytes = sizeof(float) * 3ull * settings->width * settings->height;
+        if (hipMalloc(reinterpret_cast<void**>(&j->img_dev), bytes) != hipSuccess)
+            return mt::fail(MT_E_NOMEM, "hipMalloc of a job image failed");
+        std::scoped_lock lk(batch->mu);
+        if (batch->jobs.size() >= kMaxJobs) {
+            (void)hipFree(j->img_dev);
+            return mt::fail(MT_E_UNSUPPORTED, "too many jobs in one batch");
+        }
+        batch->owned.push_back(j->img_dev);
+        if (job_id) *job_id = static_cast<uint32_t>(batch->jobs.size());
+        batch->jobs.push_back(std::move(j));
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_NOMEM, e.what());
+    }
+    batch->cv.notify_all();
+    return MT_OK;
+}
+
+int mt_batch_start(mt_batch* batch) {
+    if (!batch) return mt::fail(MT_E_INVALID, "null batch");
+    if (batch->started.exchange(true)) return mt::fail(MT_E_INVALID, "batch already started");
+    if (hipSetDevice(batch->device) != hipSuccess ||
+        hipEventRecord(batch->e0, batch->stream) != hipSuccess ||
+        hipEventSynchronize(batch->e0) != hipSuccess)
+        return mt::fail(MT_E_HIP, "batch start event failed");
+    batch->t0_ms = now_ms(batch->created);
+    batch->worker = std::thread([batch]() {
+        const int rc = batch->run();
+        if (rc && !batch->status) {
+            batch->status = rc;
+            batch->error = mt_last_error();
+        }
+    });
+    return MT_OK;
+}
+
+int mt_batch_finish(mt_batch* batch) {
+    if (!batch) return mt::fail(MT_E_INVALID, "null batch");
+    if (!batch->started.load()) {
+        const int rc = mt_batch_start(batch);
+        if (rc) return rc;
+    }
+    batch->finishing.store(true);
+    batch->cv.notify_all();
+    if (batch->worker.joinable()) batch->worker.join();
+    if (batch->status) return mt::fail(batch->status, batch->error);
+    if (hipSetDevice(batch->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+    for (auto& j : batch->jobs) {
+        if (!j->rgb_out) continue;
+        const size_t bytes = sizeof(float) * 3ull * j->st.width * j->st.height;
+        if (hipMemcpy(j->rgb_out, j->img_dev, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            return mt::fail(MT_E_HIP, "copy of a job image failed");
+    }
+    return MT_OK;
+}
+
+int mt_batch_get_job_info(const mt_batch* batch, uint32_t job_id, mt_batch_job_info* info) {
+    if (!batch || !info) return mt::fail(MT_E_INVALID, "null batch or info");
+    auto* b = const_cast<mt_batch*>(batch);
+    std::scoped_lock lk(b->mu);
+    if (job_id >= b->jobs.size()) return mt::fail(MT_E_INVALID, "unknown job id");
+    const HostJob& j = *b->jobs[job_id];
+    std::memset(info, 0, sizeof(*info));
+    info->status = b->status;
+    info->tiles = j.tiles;
+    info->samples = j.samples;
+    info->submit_ms = j.submit_ms;
+    info->complete_ms = -1.0;
+    if (j.recorded.load(std::memory_order_acquire) && hipEventQuery(j.done_ev) == hipSuccess) {
+        float ms = 0.0f;
+        info->done = 1;
+        if (hipEventElapsedTime(&ms, b->e0, j.done_ev) == hipSuccess) info->complete_ms = b->t0_ms + ms;
+    }
+    return MT_OK;
+}
+
+int mt_batch_get_stats(const mt_batch* batch, mt_batch_stats* stats) {
+    if (!batch || !stats) return mt::fail(MT_E_INVALID, "null batch or stats");
+    auto* b = const_cast<mt_batch*>(batch);
+    std::scoped_lock lk(b->mu);
+    std::memset(stats, 0, sizeof(*stats));
+    stats->iterations = b->iterations;
+    stats->graphs_captured = b->captures;
+    stats->jobs = static_cast<uint32_t>(b->jobs.size());
+    stats->scenes = static_cast<uint32_t>(b->slots.size());
+    for (auto& j : b->jobs) stats->samples += j->samples;
+    stats->options = b->opt;
+    return MT_OK;
+}
+
+int mt_batch_destroy(mt_batch* batch) {
+    delete batch;
+    return MT_OK;
+}
+
+int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
+                      const mt_settings* settings, uint32_t n, mt_batch_options* best,
+                      mt_autotune_trial* trials, uint32_t max_trials, uint32_t* n_trials) {
+    if (!scenes || !cameras || !settings || !best || n == 0)
+        return mt::fail(MT_E_INVALID, "autotune needs calibration jobs and an output");
+    if (n_trials) *n_trials = 0;
+    uint32_t done = 0;
+    auto measure = [&](const mt_batch_options& o, double& msps) -> int {
+        mt_batch* b = nullptr;
+        int rc = mt_batch_create(&o, &b);
+        if (rc) return rc;
+        std::unique_ptr<mt_batch> guard(b);
+        uint64_t samples = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (settings[i].samples > o.paths_per_iteration) return mt::fail(MT_E_INVALID, "calibration job too large");
+            rc = mt_batch_submit(b, scenes[i], &cameras[i], &settings[i], nullptr, nullptr);
+            if (rc) return rc;
+            samples += static_cast<uint64_t>(settings[i].width) * settings[i].height * settings[i].samples;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((rc = mt_batch_start(b)) || (rc = mt_batch_finish(b))) return rc;
+        const double s = now_ms(t0) / 1e3;
+        msps = samples / s / 1e6;
+        if (trials && done < max_trials) trials[done] = {o, s, msps};
+        ++done;
+        if (n_trials) *n_trials = std::min(done, max_trials);
+        return MT_OK;
+    };
+    mt_batch_options cur{1u << 22, 1024, 4096, 8};
+    double cur_v = 0.0;
+    int rc = measure(cur, cur_v);  // warm-up (first captures, first touches)
+    if (rc) return rc;
+    if ((rc = measure(cur, cur_v))) return rc;
+    const uint32_t paths[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    const uint32_t tiles[] = {256, 1024, 4096, 16384};
+    const uint32_t blocks[] = {4, 8, 12};
+    const uint32_t itiles[] = {1024, 4096, 16384};
+    auto sweep = [&](auto field, const uint32_t* vals, size_t nv) -> int {
+        mt_batch_options b0 = cur;
+        for (size_t i = 0; i < nv; ++i) {
+            mt_batch_options o = b0;
+            o.*field = vals[i];
+            if (o.*field == b0.*field) continue;
+            double v = 0.0;
+            const int r = measure(o, v);
+            if (r) return r;
+            if (v > cur_v) {
+                cur = o;
+                cur_v = v;
+            }
+        }
+        return MT_OK;
+    };
+    if ((rc = sweep(&mt_batch_options::paths_per_iteration, paths, 4)) ||
+        (rc = sweep(&mt_batch_options::tile_pixels, tiles, 4)) ||
+        (rc = sweep(&mt_batch_options::trace_blocks_per_cu, blocks, 3)) ||
+        (rc = sweep(&mt_batch_options::iteration_tiles, itiles, 3)))
+        return rc;
+    *best = cur;
+    return MT_OK;
+}
+
+}  // extern "C"

@@ -66,33 +66,27 @@ __device__ __forceinline__ void path_coords(const ChunkParams& c, uint32_t p, ui
     pix = c.pix_begin + lp;
 }
 
-__global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers b) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
-        for (uint32_t k = 0; k < kMaxCounters; ++k) b.counters[k] = 0;
-        b.counters[kCntExtend] = c.paths;
-    }
-    if (p >= c.paths) return;
-    uint32_t pix, s;
-    path_coords(c, p, pix, s);
-    const uint32_t x = pix % c.width, y = pix / c.width;
-    const uint32_t key = path_key(c.seed_key, pix, s);
+// camera ray and path state of path p (pixel pix, sample s) of a render with parameters J
+__device__ __forceinline__ void init_path(const ChunkParams& J, const PathBuffers& b, uint32_t p,
+                                          uint32_t pix, uint32_t s) {
+    const uint32_t x = pix % J.width, y = pix / J.width;
+    const uint32_t key = path_key(J.seed_key, pix, s);
     // jitter: u = x/W + U[0, 1/W) (Tracer.cpp:567-586)
-    const float u = static_cast<float>(x) * c.px_w + (draw(key, 0) * (c.px_w - 0.0f) + 0.0f);
-    const float v = static_cast<float>(y) * c.px_h + (draw(key, 1) * (c.px_h - 0.0f) + 0.0f);
+    const float u = static_cast<float>(x) * J.px_w + (draw(key, 0) * (J.px_w - 0.0f) + 0.0f);
+    const float v = static_cast<float>(y) * J.px_h + (draw(key, 1) * (J.px_h - 0.0f) + 0.0f);
     V3 o, d;
-    if (c.cam_type == MT_CAMERA_PERSPECTIVE) {  // Tracer/Camera.cpp:17-29
-        const float px = (2.0f * u - 1.0f) * c.fov_scale * c.aspect;
-        const float py = (2.0f * v - 1.0f) * c.fov_scale;
+    if (J.cam_type == MT_CAMERA_PERSPECTIVE) {  // Tracer/Camera.cpp:17-29
+        const float px = (2.0f * u - 1.0f) * J.fov_scale * J.aspect;
+        const float py = (2.0f * v - 1.0f) * J.fov_scale;
         const V3 dc = normalize(V3{px, -py, -1.0f});
-        o = xform(c.cam, mk(0.0f), 1.0f);
-        d = normalize(xform(c.cam, dc, 0.0f));
+        o = xform(J.cam, mk(0.0f), 1.0f);
+        d = normalize(xform(J.cam, dc, 0.0f));
     } else {  // Camera.cpp:31-44
-        const float px = (2.0f * u - 1.0f) * c.ortho_x * c.aspect / c.cam_aspect;
-        const float py = (2.0f * v - 1.0f) * c.ortho_y;
+        const float px = (2.0f * u - 1.0f) * J.ortho_x * J.aspect / J.cam_aspect;
+        const float py = (2.0f * v - 1.0f) * J.ortho_y;
         const V3 dc = normalize(V3{0.0f, 0.0f, -1.0f});
-        o = xform(c.cam, V3{px, -py, 0.0f}, 1.0f);
-        d = normalize(xform(c.cam, dc, 0.0f));
+        o = xform(J.cam, V3{px, -py, 0.0f}, 1.0f);
+        d = normalize(xform(J.cam, dc, 0.0f));
     }
     b.ray_o[p] = f4(o, 0.0f);
     b.ray_d[p] = f4(d, 0.0f);
@@ -104,12 +98,102 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
     b.acc_d[p] = z;
     b.acc_i[p] = z;
     b.acc_s[p] = z;
+    b.queue[0][p] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        for (uint32_t k = 0; k < kMaxCounters; ++k) b.counters[k] = 0;
+        b.counters[kCntExtend] = c.paths;
+    }
+    if (p >= c.paths) return;
+    uint32_t pix, s;
+    path_coords(c, p, pix, s);
+    init_path(c, b, p, pix, s);
     if (c.diff) {
         b.vis[p] = 0u;
-        b.adj_s[p] = z;
+        b.adj_s[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         b.adj_prev[p] = make_float4(0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu), 0.0f);
     }
-    b.queue[0][p] = p;
+}
+
+// ------------------------------------------------------------------ batched rendering
+// One iteration of a batch: the scheduler takes tiles from the scene's device queue in order
+// while their paths fit the iteration's capacity (always at least one), lays the paths out
+// tile after tile (pixel-major inside a tile) and resets the counters.
+__global__ void k_batch_schedule(PathBuffers b) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t head = b.q_state[0];
+    const uint32_t tail = b.q_state[1];
+    uint32_t paths = 0, npix = 0, n = 0;
+    while (head < tail && n < b.it_tile_cap) {
+        const uint4 t = b.q_tiles[head];
+        if (n > 0 && paths + t.w > b.it_capacity) break;
+        b.it_tiles[n] = make_uint4(t.x, t.y, t.z, paths);
+        b.it_pix_off[n] = npix;
+        paths += t.w;
+        npix += t.z;
+        ++n;
+        ++head;
+    }
+    b.it_pix_off[n] = npix;
+    b.q_state[0] = head;
+    for (uint32_t k = 0; k < kMaxCounters; ++k) b.counters[k] = 0;
+    b.counters[kCntExtend] = paths;
+    b.counters[kCntPaths] = paths;
+    b.counters[kCntPixels] = npix;
+    b.counters[kCntTiles] = n;
+}
+
+// last tile whose offset (in paths: .w, in pixels: it_pix_off) is <= x
+__device__ __forceinline__ uint32_t tile_of_path(const PathBuffers& b, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b.it_tiles[mid].w <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t tile_of_pixel(const PathBuffers& b, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b.it_pix_off[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <bool kBatch>
+__device__ __forceinline__ const ChunkParams& job_params(const ChunkParams& c, const PathBuffers& b,
+                                                         uint32_t p) {
+    if constexpr (kBatch) return b.jobs[b.path_job[p]].p;
+    else return c;
+}
+template <bool kBatch>
+__device__ __forceinline__ void coords(const ChunkParams& c, const PathBuffers& b, uint32_t p,
+                                       uint32_t& pix, uint32_t& s) {
+    if constexpr (kBatch) {
+        pix = b.path_pix[p];
+        s = b.path_smp[p];
+    } else {
+        path_coords(c, p, pix, s);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_generate(PathBuffers b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= b.counters[kCntPaths]) return;
+    const uint4 t = b.it_tiles[tile_of_path(b, b.counters[kCntTiles], p)];
+    const ChunkParams& J = b.jobs[t.x].p;
+    const uint32_t local = p - t.w;
+    const uint32_t lp = local / J.spp;
+    const uint32_t s = local - lp * J.spp;
+    const uint32_t pix = t.y + lp;
+    b.path_job[p] = t.x;
+    b.path_pix[p] = pix;
+    b.path_smp[p] = s;
+    init_path(J, b, p, pix, s);
 }
 
 // rays of the test hook (mode 2)
@@ -454,13 +538,12 @@ __device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const Mat
 //   throughput w = f*cos/pdf        -> (L - S) * (dw/dtheta) / w with the sampling detached
 //   sky term       base'_{k-1}*T*sky -> d/d(base of the previous hit)
 // where T is the radiance throughput of the reference and S the radiance collected so far.
-template <int kDiff>
+template <int kDiff, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, PathBuffers b,
                                                   uint32_t k) {
     const uint32_t count = b.counters[kCntExtend + k];
     const uint32_t* qin = b.queue[k & 1];
     uint32_t* qout = b.queue[(k + 1) & 1];
-    const bool last = (k + 1 >= c.bounces);
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t hits = 0, misses = 0;
     const DiffParams& dp = c.dp;
@@ -478,6 +561,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
         bool gvalid = false;
         if (valid) {
             p = qin[idx];
+            // the render this path belongs to (batched: its job's settings)
+            const ChunkParams& J = job_params<kBatch>(c, b, p);
+            const bool last = k + 1 >= J.bounces;
             // adjoint replay: dLoss/dL of this path and what the forward pass recorded for it
             V3 dl = mk(0.0f), ltot = mk(0.0f);
             uint32_t visbits = 0;
@@ -496,8 +582,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
             if (inst < 0) {  // miss: sky term (Tracer.cpp:244-252)
                 misses++;
                 const V3 col = (k > 0) ? xyz(b.prev_base[p]) : mk(1.0f);
-                const float mult = (k == 0) ? c.hdri_primary : c.hdri_secondary;
-                const V3 sk = scale(sample_env(S, d, c.hdri_rot_apply, c.hdri_rot_offset), mult);
+                const float mult = (k == 0) ? J.hdri_primary : J.hdri_secondary;
+                const V3 sk = scale(sample_env(S, d, J.hdri_rot_apply, J.hdri_rot_offset), mult);
                 b.acc_s[p] = f4(mul(mul(col, trad), sk), 0.0f);
                 if (kDiff == 2 && k > 0) {
                     const float4 pv = b.adj_prev[p];
@@ -511,9 +597,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
             } else {
                 hits++;
                 uint32_t pix, s;
-                path_coords(c, p, pix, s);
-                const uint32_t key = path_key(c.seed_key, pix, s);
-                const uint32_t dim0 = 2u + k * c.light_stride;
+                coords<kBatch>(c, b, p, pix, s);
+                const uint32_t key = path_key(J.seed_key, pix, s);
+                const uint32_t dim0 = 2u + k * J.light_stride;
                 const V3 o = xyz(b.ray_o[p]);
                 const float4 hp = b.hit_pos[p];
                 const V3 wpos = xyz(hp);
@@ -578,9 +664,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 V3 tterm = mul(xyz(b.t_term[p]), energy_t);
                 bool stop;
                 float rr_q = 1.0f;
-                if (c.mode == MT_MODE_FIXED) {
+                if (J.mode == MT_MODE_FIXED) {
                     stop = false;
-                    if (k + 1 >= c.rr_depth) {
+                    if (k + 1 >= J.rr_depth) {
                         const float q = clampf(gmax(gmax(tterm.x, tterm.y), tterm.z), 0.05f, 1.0f);
                         if (draw(key, dim0 + kDimRR) >= q) {
                             stop = true;
@@ -591,10 +677,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                         }
                     }
                 } else {
-                    stop = len(tterm) < c.terminate_energy;
+                    stop = len(tterm) < J.terminate_energy;
                 }
                 // ProcessRayForward for this step (Tracer.cpp:254-339)
-                const V3 e_term = mul(scale(trad, c.emission_base), ms.emission);
+                const V3 e_term = mul(scale(trad, J.emission_base), ms.emission);
                 {
                     const float4 ae = b.acc_e[p];
                     b.acc_e[p] = make_float4(ae.x + e_term.x, ae.y + e_term.y, ae.z + e_term.z, 0.0f);
@@ -604,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     ssum = add(xyz(b.adj_s[p]), e_term);
                     gmat = mi;
                     gvalid = true;
-                    g_e = mul(mul(dl, scale(trad, c.emission_base)), ms.em_scale);
+                    g_e = mul(mul(dl, scale(trad, J.emission_base)), ms.em_scale);
                 }
                 int chosen = -1;
                 float chosen_pdf = 0.0f, total = 0.0f;
@@ -723,29 +809,29 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
     }
 }
 
+template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers b) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= c.paths) return;
+    if (p >= (kBatch ? b.counters[kCntPaths] : c.paths)) return;
+    const ChunkParams& J = job_params<kBatch>(c, b, p);
     const V3 e = xyz(b.acc_e[p]), d = xyz(b.acc_d[p]), i = xyz(b.acc_i[p]), s = xyz(b.acc_s[p]);
     const V3 L = add(add(add(e, d), i), s);  // Tracer.cpp:344-345
-    const V3 col = c.output_linear ? L : aces(L, c.exposure);
+    const V3 col = J.output_linear ? L : aces(L, J.exposure);
     b.sample_rgb[p] = f4(col, 0.0f);
-    if (c.diff) {
+    if (!kBatch && c.diff) {
         uint32_t pix, s;
         path_coords(c, p, pix, s);
         c.path_store[static_cast<size_t>(pix) * c.spp + s] = make_float4(L.x, L.y, L.z, __uint_as_float(b.vis[p]));
     }
 }
 
-// per-pixel mean of the samples in sample order, with CleanupFireflies (Tracer.cpp:490-548)
-__global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b, float* img) {
-    const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lp >= c.npix) return;
-    const uint32_t n = c.spp;
-    const float4* sp = b.sample_rgb + static_cast<size_t>(lp) * c.spp;
+// per-pixel mean of the samples sp[0..n) in sample order, with CleanupFireflies
+// (Tracer.cpp:490-548)
+__device__ __forceinline__ void resolve_pixel(const ChunkParams& J, const float4* sp, float* out) {
+    const uint32_t n = J.spp;
     uint32_t cnt = n;
     float avg = 0.0f, thr = 0.0f;
-    if (c.firefly) {
+    if (J.firefly) {
         float sum = 0.0f;
         for (uint32_t s = 0; s < n; ++s) {
             const float4 v = sp[s];
@@ -759,13 +845,13 @@ __global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b
             var = var + dd * dd;
         }
         var = var / static_cast<float>(n);
-        thr = c.firefly_threshold * __builtin_sqrtf(var);
+        thr = J.firefly_threshold * __builtin_sqrtf(var);
     }
     V3 acc = mk(0.0f);
     for (uint32_t s = 0; s < n; ++s) {
         const float4 v = sp[s];
         V3 col = xyz(v);
-        if (c.firefly && fabsf((v.x + v.y + v.z) / 3.0f - avg) > thr) {
+        if (J.firefly && fabsf((v.x + v.y + v.z) / 3.0f - avg) > thr) {
             col = mk(0.0f);
             cnt--;
         }
@@ -773,10 +859,27 @@ __global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b
     }
     if (cnt == 0) cnt = 1;
     const V3 r = divs(acc, static_cast<float>(cnt));
-    const uint32_t pix = c.pix_begin + lp;
-    img[pix * 3 + 0] = r.x;
-    img[pix * 3 + 1] = r.y;
-    img[pix * 3 + 2] = r.z;
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b, float* img) {
+    const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lp >= c.npix) return;
+    resolve_pixel(c, b.sample_rgb + static_cast<size_t>(lp) * c.spp, img + (c.pix_begin + lp) * 3ull);
+}
+
+// batched: one thread per pixel of the iteration, written into its job's image
+__global__ __launch_bounds__(kBlock) void k_batch_resolve(PathBuffers b) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= b.counters[kCntPixels]) return;
+    const uint32_t ti = tile_of_pixel(b, b.counters[kCntTiles], q);
+    const uint4 t = b.it_tiles[ti];
+    const BatchJob& job = b.jobs[t.x];
+    const uint32_t lp = q - b.it_pix_off[ti];
+    resolve_pixel(job.p, b.sample_rgb + t.w + static_cast<size_t>(lp) * job.p.spp,
+                  job.img + (t.y + lp) * 3ull);
 }
 
 __global__ void k_brdf(uint32_t n, const float* in, float* out) {
@@ -831,6 +934,8 @@ int alloc(std::vector<void*>& owned, T** p, size_t n) {
 }
 
 }  // namespace
+
+int cu_count() { return device_cu_count(); }
 
 uint32_t stack_capacity(const mt_scene* s) {
     const uint32_t blas = std::max<uint32_t>(s->max_bvh_depth + 1u, 2u);
@@ -923,10 +1028,96 @@ void Renderer::set_camera(const mt_camera_desc& cam) {
     c.cam_aspect = cam.aspect;
 }
 
+uint32_t seed_key(uint64_t seed) {
+    const uint32_t k0 = seed_mix(static_cast<uint32_t>(seed) ^ 0xA511E9B3u);
+    return seed_mix(k0 ^ static_cast<uint32_t>(seed >> 32) ^ 0x63D83595u);
+}
+
 void Renderer::set_seed(uint64_t seed) {
     st_.seed = seed;
-    const uint32_t k0 = seed_mix(static_cast<uint32_t>(seed) ^ 0xA511E9B3u);
-    params_.seed_key = seed_mix(k0 ^ static_cast<uint32_t>(seed >> 32) ^ 0x63D83595u);
+    params_.seed_key = seed_key(seed);
+}
+
+void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& cam,
+                 const mt_settings& st) {
+    std::memset(&c, 0, sizeof(c));
+    c.width = st.width;
+    c.spp = st.samples;
+    c.bounces = st.bounces;
+    c.px_w = 1.0f / static_cast<float>(st.width);
+    c.px_h = 1.0f / static_cast<float>(st.height);
+    c.aspect = static_cast<float>(st.width) / static_cast<float>(st.height);
+    c.cam_type = cam.type;
+    std::memcpy(c.cam, cam.camera_to_world, sizeof(c.cam));
+    c.fov_scale = std::tan(cam.yfov * 0.5f);
+    c.ortho_x = cam.ortho_size[0];
+    c.ortho_y = cam.ortho_size[1];
+    c.cam_aspect = cam.aspect;
+    c.seed_key = seed_key(st.seed);
+    c.light_stride = 8u + scene->light_count;
+    c.mode = st.mode;
+    c.rr_depth = st.rr_depth;
+    c.terminate_energy = st.terminate_energy;
+    c.emission_base = st.emission_base_intensity;
+    c.hdri_primary = st.hdri_primary_intensity;
+    c.hdri_secondary = st.hdri_secondary_intensity;
+    c.hdri_rot_apply = st.hdri_rotation_apply;
+    c.hdri_rot_offset = st.hdri_rotation / 360.0f;
+    c.exposure = std::pow(2.0f, -st.ev_exposure);
+    c.output_linear = st.output == MT_OUTPUT_LINEAR;
+    c.firefly = st.firefly_enable;
+    c.firefly_threshold = st.firefly_threshold;
+    c.count_work = st.count_work;
+    c.stack_cap = stack_capacity(scene);
+    c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
+    c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
+    c.inv_spp = 1.0f / static_cast<float>(st.samples);
+}
+
+void launch_batch_iteration(const mt_scene* scene, const ChunkParams& c, const PathBuffers& b,
+                            const BatchLaunch& L, hipStream_t s) {
+    const uint32_t grid = (b.it_capacity + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_batch_schedule, dim3(1), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_batch_generate, dim3(grid), dim3(kBlock), 0, s, b);
+    for (uint32_t k = 0; k < L.bounces; ++k) {
+        launch_trace<0>(L.lds_depth, false, L.trace_blocks, s, scene->dev, c, b, k);
+        hipLaunchKernelGGL((k_shade<0, true>), dim3(L.shade_blocks), dim3(kBlock), 0, s, scene->dev, c, b, k);
+        if (scene->light_count) launch_trace<1>(L.lds_depth, false, L.trace_blocks, s, scene->dev, c, b, k);
+    }
+    hipLaunchKernelGGL(k_finalize<true>, dim3(grid), dim3(kBlock), 0, s, c, b);
+    hipLaunchKernelGGL(k_batch_resolve, dim3(grid), dim3(kBlock), 0, s, b);
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, p, v);
+}
+
+int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, uint64_t spill_entries,
+                       bool diff, bool batch) {
+    int rc;
+    if ((rc = alloc(owned, &b.ray_o, P)) || (rc = alloc(owned, &b.ray_d, P)) ||
+        (rc = alloc(owned, &b.t_rad, P)) || (rc = alloc(owned, &b.t_term, P)) ||
+        (rc = alloc(owned, &b.prev_base, P)) || (rc = alloc(owned, &b.acc_e, P)) ||
+        (rc = alloc(owned, &b.acc_d, P)) || (rc = alloc(owned, &b.acc_i, P)) ||
+        (rc = alloc(owned, &b.acc_s, P)) || (rc = alloc(owned, &b.hit_inst, P)) ||
+        (rc = alloc(owned, &b.hit_tri, P)) || (rc = alloc(owned, &b.hit_uv, P)) ||
+        (rc = alloc(owned, &b.hit_pos, P)) || (rc = alloc(owned, &b.queue[0], P)) ||
+        (rc = alloc(owned, &b.queue[1], P)) || (rc = alloc(owned, &b.sh_o, P)) ||
+        (rc = alloc(owned, &b.sh_d, P)) || (rc = alloc(owned, &b.sh_c, P)) ||
+        (rc = alloc(owned, &b.sample_rgb, P)) || (rc = alloc(owned, &b.spill, spill_entries)) ||
+        (rc = alloc(owned, &b.counters, kMaxCounters)) || (rc = alloc(owned, &b.work, 16)))
+        return rc;
+    if (diff && ((rc = alloc(owned, &b.vis, P)) || (rc = alloc(owned, &b.adj_s, P)) ||
+                 (rc = alloc(owned, &b.adj_prev, P))))
+        return rc;
+    if (batch && ((rc = alloc(owned, &b.path_job, P)) || (rc = alloc(owned, &b.path_pix, P)) ||
+                  (rc = alloc(owned, &b.path_smp, P))))
+        return rc;
+    return MT_OK;
 }
 
 int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff) {
@@ -959,52 +1150,16 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     const size_t trace_threads = static_cast<size_t>(trace_blocks_) * kBlock;
     PathBuffers& b = buf_;
     int rc;
-    if ((rc = alloc(owned_, &b.ray_o, P)) || (rc = alloc(owned_, &b.ray_d, P)) ||
-        (rc = alloc(owned_, &b.t_rad, P)) || (rc = alloc(owned_, &b.t_term, P)) ||
-        (rc = alloc(owned_, &b.prev_base, P)) || (rc = alloc(owned_, &b.acc_e, P)) ||
-        (rc = alloc(owned_, &b.acc_d, P)) || (rc = alloc(owned_, &b.acc_i, P)) ||
-        (rc = alloc(owned_, &b.acc_s, P)) || (rc = alloc(owned_, &b.hit_inst, P)) ||
-        (rc = alloc(owned_, &b.hit_tri, P)) || (rc = alloc(owned_, &b.hit_uv, P)) ||
-        (rc = alloc(owned_, &b.hit_pos, P)) || (rc = alloc(owned_, &b.queue[0], P)) ||
-        (rc = alloc(owned_, &b.queue[1], P)) || (rc = alloc(owned_, &b.sh_o, P)) ||
-        (rc = alloc(owned_, &b.sh_d, P)) || (rc = alloc(owned_, &b.sh_c, P)) ||
-        (rc = alloc(owned_, &b.sample_rgb, P)) ||
-        (rc = alloc(owned_, &b.spill, spill_depth * trace_threads)) ||
-        (rc = alloc(owned_, &b.counters, kMaxCounters)) || (rc = alloc(owned_, &b.work, 16)) ||
+    if ((rc = alloc_path_buffers(owned_, b, P, spill_depth * trace_threads, diff, false)) ||
         (rc = alloc(owned_, &img_, npix_total * 3)))
         return rc;
-    if (diff && ((rc = alloc(owned_, &b.vis, P)) || (rc = alloc(owned_, &b.adj_s, P)) ||
-                 (rc = alloc(owned_, &b.adj_prev, P)) ||
-                 (rc = alloc(owned_, &store_, npix_total * st.samples))))
-        return rc;
+    if (diff && (rc = alloc(owned_, &store_, npix_total * st.samples))) return rc;
     HIP_TRY(hipMemsetAsync(b.work, 0, sizeof(unsigned long long) * 16, stream_));
     HIP_TRY(hipMemsetAsync(img_, 0, sizeof(float) * npix_total * 3, stream_));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_counters_), sizeof(uint32_t) * kMaxCounters));
 
     ChunkParams& c = params_;
-    std::memset(&c, 0, sizeof(c));
-    c.width = st.width;
-    c.spp = st.samples;
-    c.bounces = st.bounces;
-    c.px_w = 1.0f / static_cast<float>(st.width);
-    c.px_h = 1.0f / static_cast<float>(st.height);
-    c.aspect = static_cast<float>(st.width) / static_cast<float>(st.height);
-    set_camera(cam);
-    set_seed(st.seed);
-    c.light_stride = 8u + scene->light_count;
-    c.mode = st.mode;
-    c.rr_depth = st.rr_depth;
-    c.terminate_energy = st.terminate_energy;
-    c.emission_base = st.emission_base_intensity;
-    c.hdri_primary = st.hdri_primary_intensity;
-    c.hdri_secondary = st.hdri_secondary_intensity;
-    c.hdri_rot_apply = st.hdri_rotation_apply;
-    c.hdri_rot_offset = st.hdri_rotation / 360.0f;
-    c.exposure = std::pow(2.0f, -st.ev_exposure);
-    c.output_linear = st.output == MT_OUTPUT_LINEAR;
-    c.firefly = st.firefly_enable;
-    c.firefly_threshold = st.firefly_threshold;
-    c.count_work = st.count_work;
+    fill_params(c, scene, cam, st);
     c.stack_cap = cap;
     c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
     c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
@@ -1048,14 +1203,14 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
         HIP_TRY(hipEventRecord(ev[0], s));
         launch_trace<0>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[1], s));
-        hipLaunchKernelGGL(k_shade<0>, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[2], s));
         if (scene_->light_count)
             launch_trace<1>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
         HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_finalize, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
     hipLaunchKernelGGL(k_resolve, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, buf_, img_);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(events_[1], s));
@@ -1113,16 +1268,16 @@ int Renderer::run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s
     for (uint32_t k = 0; k < st_.bounces; ++k) {
         launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
         if (mode == 1) {
-            hipLaunchKernelGGL(k_shade<1>, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+            hipLaunchKernelGGL((k_shade<1, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
             if (scene_->light_count)
                 launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
         } else {
-            hipLaunchKernelGGL(k_shade<2>, dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+            hipLaunchKernelGGL((k_shade<2, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
         }
         HIP_TRY(hipGetLastError());
     }
     if (mode == 1) {
-        hipLaunchKernelGGL(k_finalize, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+        hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
         hipLaunchKernelGGL(k_resolve, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, buf_, img);
     }
     HIP_TRY(hipGetLastError());

@@ -16,7 +16,10 @@ constexpr uint32_t kCntShadow = kCntExtend + kMaxBounces + 1;   // [kMaxBounces]
 constexpr uint32_t kCntFetchExt = kCntShadow + kMaxBounces;     // [kMaxBounces] work fetch
 constexpr uint32_t kCntFetchSh = kCntFetchExt + kMaxBounces;    // [kMaxBounces] work fetch
 constexpr uint32_t kCntError = kCntFetchSh + kMaxBounces;       // traversal stack overflow flag
-constexpr uint32_t kMaxCounters = kCntError + 1;
+constexpr uint32_t kCntPaths = kCntError + 1;                   // batched: paths this iteration
+constexpr uint32_t kCntPixels = kCntPaths + 1;                  // batched: pixels this iteration
+constexpr uint32_t kCntTiles = kCntPixels + 1;                  // batched: tiles this iteration
+constexpr uint32_t kMaxCounters = kCntTiles + 1;
 
 // differentiable parameters (device pointers, null = the scene's value): a float RGB texture that
 // replaces the base colour of materials [tex_first, tex_first + tex_count), and per-material
@@ -87,6 +90,21 @@ struct PathBuffers {
     uint32_t* vis;      // bit k: the shadow ray of bounce k reached its light
     float4* adj_s;      // radiance collected so far during the replay
     float4* adj_prev;   // {u, v, material or ~0u} of the previous hit (sky term adjoint)
+    // batched rendering
+    uint32_t *path_job, *path_pix, *path_smp;  // owner job, pixel and sample of every path
+    uint4* it_tiles;        // this iteration's tiles {job, first pixel, pixels, first path}
+    uint32_t* it_pix_off;   // [tiles + 1] first iteration pixel of every tile
+    const uint4* q_tiles;   // the scene's tile queue {job, first pixel, pixels, paths}
+    uint32_t* q_state;      // {head, tail} of that queue
+    const struct BatchJob* jobs;
+    uint32_t it_capacity;   // paths per iteration
+    uint32_t it_tile_cap;   // tiles per iteration
+};
+
+// one job of a batch, resident in device memory
+struct BatchJob {
+    ChunkParams p;  // the job's settings and camera (pix_begin/npix/paths unused)
+    float* img;     // width * height * 3, device
 };
 
 class Renderer {
@@ -127,8 +145,28 @@ class Renderer {
     mt_stats stats_{};
 };
 
+// launch shape of one batched iteration
+struct BatchLaunch {
+    uint32_t bounces;       // extend/shade/shadow rounds (the largest of the scene's jobs)
+    uint32_t lds_depth;     // traversal stack entries kept in LDS
+    uint32_t trace_blocks;  // persistent traversal blocks
+    uint32_t shade_blocks;
+};
+
 uint32_t seed_mix(uint32_t x);
+uint32_t seed_key(uint64_t seed);
 uint32_t stack_capacity(const mt_scene* s);
+int cu_count();
+// the per-render parameters of a render of `scene` with this camera and these settings
+void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& cam,
+                 const mt_settings& st);
+int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t paths,
+                       uint64_t spill_entries, bool diff, bool batch);
+// enqueue one batched iteration (schedule, generate, bounces, finalize, resolve): no host
+// synchronisation and no memset, so that it can be captured into a graph
+void launch_batch_iteration(const mt_scene* scene, const ChunkParams& c, const PathBuffers& b,
+                            const BatchLaunch& launch, hipStream_t s);
+void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits);
 int brdf_eval(uint32_t n, const float* in, float* out);
 
