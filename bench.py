@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate wall time of the CPU-baseline sample")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-requests workload")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the fwd+adjoint timing of the inverse-rendering workload")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -149,6 +151,47 @@ def adjoint_timing(reps=5):
             "fwd_plus_adjoint_ms": f + b, "ratio_to_fwd": (f + b) / f}
 
 
+def batch_leg(rate_hz=16.0):
+    """The batched-requests workload: 64 seeded jobs over 8 scene instances (3 Cornell boxes,
+    3 sphere grids, 2 terrain scenes at 512x512 terrain vertices and 1000 icospheres),
+    128^2-1024^2 pixels, 16-256 spp, depth 8-16, Poisson arrivals at rate_hz, served by one
+    running batch whose options come from the autotuner (calibrated on one 256x256 16-spp
+    job per scene)."""
+    from mitrace_amd import batch as B
+    from mitrace_amd.tracer import GpuScene
+    t0 = time.perf_counter()
+    scenes = B.workload_scenes()
+    gscenes = [GpuScene(sc) for sc, _ in scenes]
+    fns = [fn for _, fn in scenes]
+    build_s = time.perf_counter() - t0
+    calib = []
+    for i, g in enumerate(gscenes):
+        job = {"width": 256, "height": 256, "spp": 16, "bounces": 8, "seed": 7 + i,
+               "pose_seed": 11 + i}
+        st, cam = B.job_settings(job, fns[i], g.scene)
+        calib.append((g, st, cam))
+    t1 = time.perf_counter()
+    best, trials = B.autotune(calib)
+    tune_s = time.perf_counter() - t1
+    print(f"# batch: scenes built in {build_s:.1f} s, autotuned in {tune_s:.1f} s: {best}",
+          file=sys.stderr, flush=True)
+    jobs = B.workload_jobs(rate_hz=rate_hz)
+    rep, _ = B.run_stream(gscenes, fns, jobs, options=best)
+    # the same jobs one after another through mt_render (no arrivals: pure device time)
+    t2 = time.perf_counter()
+    for j in jobs:
+        st, cam = B.job_settings(j, fns[j["scene"]], gscenes[j["scene"]].scene)
+        gscenes[j["scene"]].render(st, camera=cam)
+    seq_s = time.perf_counter() - t2
+    rep["sequential_mt_render"] = {"seconds": seq_s, "msamples_per_s": rep["samples"] / seq_s / 1e6}
+    for g in gscenes:
+        g.close()
+    rep.update({"workload": "64 jobs over 8 scene instances, 128^2-1024^2 px, 16-256 spp, "
+                            "depth 8-16", "arrival_rate_hz": rate_hz,
+                "autotuned": best, "autotune_trials": trials, "autotune_s": tune_s})
+    return rep
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -228,6 +271,11 @@ def main():
                                  ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
         "cpu_baseline": None,
     }
+    if not args.no_batch:
+        line["batch"] = batch_leg()
+        print(f"# batch: {line['batch']['msamples_per_s']:.1f} Msamples/s, p50 "
+              f"{line['batch']['completion_ms_p50']:.0f} ms, p99 "
+              f"{line['batch']['completion_ms_p99']:.0f} ms", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_timing()
         print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)

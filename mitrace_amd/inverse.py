@@ -76,12 +76,26 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024):
     gt = to_dev(truth)
     log = []
 
+    seen = torch.zeros(tex.shape[:2], dtype=torch.int32, device="cuda")
+
     def errors():
         with torch.no_grad():
             return (float(torch.sqrt(torch.mean((tex - gt["tex"]) ** 2))),
                     float(torch.mean(torch.abs(rough[:49] - gt["rough"][:49]))))
 
+    def image_rmse(spp_eval=128):
+        """RMSE of the 8 poses rendered with the current parameters against the targets"""
+        ev = DiffRenderer(gs, scenegen.inverse_settings(samples=spp_eval), truth["tex_first"],
+                          truth["tex_count"])
+        with torch.no_grad():
+            err = [float(torch.mean((ev.forward(c, 700_000 + i, tex.detach(), rough.detach(),
+                                                metal, emis) - tg[i]) ** 2))
+                   for i, c in enumerate(cams)]
+        ev.close()
+        return float(np.sqrt(np.mean(err)))
+
     e0 = errors()
+    img0 = image_rmse()
     it_times = []
     for it in range(iters):
         torch.cuda.synchronize()
@@ -92,7 +106,9 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024):
             img = dr.render(cam, 1 + it * len(cams) + i, tex, rough, metal, emis)
             loss = torch.mean((img - tg[i]) ** 2) / len(cams)
             loss.backward()
-            total += float(loss)
+            total += float(loss.detach())
+        with torch.no_grad():
+            seen += (tex.grad.abs().sum(-1) > 0).int()
         opt.step()
         with torch.no_grad():
             tex.clamp_(0.0, 1.0)
@@ -105,10 +121,21 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024):
             print(f"iter {it}: loss {total:.5f} texture RMSE {te:.4f} roughness MAE {re:.4f}",
                   flush=True)
     te, re = errors()
+    img1 = image_rmse()
+    with torch.no_grad():
+        # texels with a gradient in at least half of the iterations
+        m = seen >= max(1, iters // 2)
+        init_t = torch.tensor(init["tex"], device="cuda")
+        seen_rmse = lambda t: float(torch.sqrt(torch.mean(((t - gt["tex"])[m]) ** 2)))
+        seen_frac = float(m.float().mean())
+        s0, s1 = seen_rmse(init_t), seen_rmse(tex)
     return {"iters": iters, "spp": spp, "poses": len(cams), "lr": lr,
             "targets_spp": target_spp, "targets_s": t_targets,
-            "initial": {"tex_rmse": e0[0], "rough_mae": e0[1]},
-            "final": {"tex_rmse": te, "rough_mae": re},
+            "initial": {"tex_rmse": e0[0], "tex_rmse_observed": s0, "rough_mae": e0[1],
+                        "image_rmse": img0},
+            "final": {"tex_rmse": te, "tex_rmse_observed": s1, "rough_mae": re,
+                      "image_rmse": img1},
+            "observed_texel_fraction": seen_frac,
             "ms_per_iter": 1e3 * float(np.median(it_times)), "log": log}
 
 
