@@ -292,8 +292,25 @@ typedef struct mt_autotune_trial {
     double msamples_per_s;
 } mt_autotune_trial;
 
+/* Radiance cache (no reference counterpart): an MLP (32 frequency-encoded inputs from the
+ * position and the direction, 4 hidden layers of 64, 3 outputs; BF16 on MFMA, FP32
+ * accumulation) trained online on the radiance of the tracer's own training paths.
+ * Zero fields take the defaults in brackets. */
+typedef struct mt_cache_options {
+    uint32_t seed;            /* weight initialisation [1]                                 */
+    uint32_t train_batch;     /* samples per Adam step [16384]                             */
+    float learning_rate;      /* Adam step size [0.005]                                    */
+    uint32_t train_every;     /* one full-length training path per this many paths [16];
+                                 0xFFFFFFFF: no training paths                              */
+    uint32_t query_depth;     /* bounce at which the other paths stop and query [1]        */
+    uint32_t warmup_passes;   /* training-only passes before the first counted pass (0: none) */
+    uint32_t steps_per_pass;  /* at most this many Adam steps after every chunk [8]        */
+    uint32_t spp_per_pass;    /* samples per pixel of one pass [1]                          */
+} mt_cache_options;
+
 typedef struct mt_scene mt_scene;
 typedef struct mt_job mt_job;
+typedef struct mt_cache mt_cache;
 typedef struct mt_diff mt_diff;
 typedef struct mt_batch mt_batch;
 
@@ -356,6 +373,27 @@ int mt_diff_backward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_
                      uint64_t seed, const float* grad_image_dev, const mt_diff_grads* grads,
                      uint64_t stream);
 int mt_diff_destroy(mt_diff* diff);
+
+/* Radiance cache.  mt_cache_create sizes the training buffers and initialises the weights;
+ * the position encoding spans the scene's bounds.  mt_render_cached renders like mt_render
+ * (same settings, camera, output and stats; no firefly rejection) in passes of spp_per_pass
+ * samples per pixel: 1 path in train_every runs the plain estimator to its end and records
+ * training samples (position, direction, radiance of the rest of the path) at its hits from
+ * query_depth on; every other path stops at its hit of bounce query_depth and adds
+ * throughput x cache(position, direction).  The cache is trained after every chunk, and keeps
+ * its weights across calls.  The remaining calls are test hooks over host arrays (3 floats per
+ * position, direction, target or output; weights: 14787 floats, W then b per layer, W[out][in]). */
+int mt_cache_create(mt_scene* scene, const mt_cache_options* options, mt_cache** out);
+int mt_cache_destroy(mt_cache* cache);
+int mt_render_cached(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,
+                     mt_cache* cache, float* rgb_out, mt_stats* stats);
+int mt_cache_get_weights(mt_cache* cache, float* weights);
+int mt_cache_set_weights(mt_cache* cache, const float* weights);
+int mt_cache_infer(mt_cache* cache, uint32_t n, const float* pos, const float* dir, float* out);
+/* one Adam step on n samples; loss (optional) = mean relative L2 of the step's forward pass,
+ * grad (optional, 14787 floats) = the gradient it applied */
+int mt_cache_train(mt_cache* cache, uint32_t n, const float* pos, const float* dir,
+                   const float* target, float* loss, float* grad);
 
 /* Batched rendering (no reference counterpart; for serving many render requests).  Jobs of
  * any scene, resolution, sample count and depth are split into tiles of tile_pixels pixels on

@@ -141,6 +141,13 @@ class mt_diff_grads(C.Structure):
                 ("metallic_dev", C.c_void_p), ("emission_dev", C.c_void_p)]
 
 
+class mt_cache_options(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("train_batch", C.c_uint32), ("learning_rate", C.c_float),
+                ("train_every", C.c_uint32), ("query_depth", C.c_uint32),
+                ("warmup_passes", C.c_uint32), ("steps_per_pass", C.c_uint32),
+                ("spp_per_pass", C.c_uint32)]
+
+
 class mt_batch_options(C.Structure):
     _fields_ = [("paths_per_iteration", C.c_uint32), ("tile_pixels", C.c_uint32),
                 ("iteration_tiles", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32)]
@@ -196,6 +203,17 @@ SIGNATURES = {
                                    C.POINTER(mt_diff_params), C.c_uint64, C.c_void_p,
                                    C.POINTER(mt_diff_grads), C.c_uint64]),
     "mt_diff_destroy": (C.c_int, [C.c_void_p]),
+    "mt_cache_create": (C.c_int, [C.c_void_p, C.POINTER(mt_cache_options), C.POINTER(C.c_void_p)]),
+    "mt_cache_destroy": (C.c_int, [C.c_void_p]),
+    "mt_render_cached": (C.c_int, [C.c_void_p, C.POINTER(mt_camera_desc), C.POINTER(mt_settings),
+                                   C.c_void_p, C.POINTER(C.c_float), C.POINTER(mt_stats)]),
+    "mt_cache_get_weights": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "mt_cache_set_weights": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "mt_cache_infer": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mt_cache_train": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float)]),
     "mt_batch_create": (C.c_int, [C.POINTER(mt_batch_options), C.POINTER(C.c_void_p)]),
     "mt_batch_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(mt_camera_desc),
                                   C.POINTER(mt_settings), C.POINTER(C.c_float),

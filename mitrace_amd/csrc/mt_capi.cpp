@@ -11,6 +11,7 @@
 #include <thread>
 
 #include "mt_internal.hpp"
+#include "mt_mlp.hpp"
 #include "mt_render.hpp"
 
 namespace mt {
@@ -42,6 +43,13 @@ struct mt_job {
     mutable std::mutex mu;
     mt_stats snapshot{};
     std::chrono::steady_clock::time_point start;
+};
+
+// Radiance cache behind mt_cache*: the MLP and its options; it belongs to one scene.
+struct mt_cache {
+    mt_scene* scene = nullptr;
+    mt_cache_options opt{};
+    mt::Mlp mlp;
 };
 
 // Differentiable renderer behind mt_diff*: a Renderer set up once with the per-path store.
@@ -343,6 +351,139 @@ int mt_diff_backward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_
 
 int mt_diff_destroy(mt_diff* diff) {
     delete diff;
+    return MT_OK;
+}
+
+int mt_cache_create(mt_scene* scene, const mt_cache_options* options, mt_cache** out) {
+    if (!out) return mt::fail(MT_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (!scene) return mt::fail(MT_E_INVALID, "null scene");
+    try {
+        auto c = std::make_unique<mt_cache>();
+        c->scene = scene;
+        if (options) c->opt = *options;
+        if (!c->opt.seed) c->opt.seed = 1;
+        if (!c->opt.train_batch) c->opt.train_batch = 16384;
+        if (!(c->opt.learning_rate > 0.0f)) c->opt.learning_rate = 5e-3f;
+        if (!c->opt.train_every) c->opt.train_every = 16;
+        if (!c->opt.query_depth) c->opt.query_depth = 1;
+        if (!c->opt.steps_per_pass) c->opt.steps_per_pass = 8;
+        if (!c->opt.spp_per_pass) c->opt.spp_per_pass = 1;
+        if (c->opt.train_batch > (1u << 22)) return mt::fail(MT_E_UNSUPPORTED, "train_batch above 4M");
+        if (hipSetDevice(scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+        const int rc = c->mlp.init(c->opt.seed, std::max<uint32_t>(c->opt.train_batch, 1u << 16),
+                                   c->opt.learning_rate);
+        if (rc) return rc;
+        c->mlp.set_bounds(scene->world_lo, scene->world_hi);
+        *out = c.release();
+        return MT_OK;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_NOMEM, e.what());
+    }
+}
+
+int mt_cache_destroy(mt_cache* cache) {
+    delete cache;
+    return MT_OK;
+}
+
+int mt_render_cached(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,
+                     mt_cache* cache, float* rgb_out, mt_stats* stats) {
+    if (!valid_args(scene, camera, settings)) return MT_E_INVALID;
+    if (!cache || cache->scene != scene) return mt::fail(MT_E_INVALID, "the cache belongs to another scene");
+    try {
+        if (hipSetDevice(scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
+        const auto t0 = std::chrono::steady_clock::now();
+        mt::Renderer r;
+        int rc = r.setup_cached(scene, *camera, *settings, cache->opt);
+        if (rc) return rc;
+        rc = r.render_cached(cache->mlp, rgb_out);
+        if (rc) return rc;
+        if (stats) {
+            *stats = r.stats();
+            stats->progress = 1.0f;
+            stats->time_elapsed =
+                std::chrono::duration<float>(std::chrono::steady_clock::now() - t0).count();
+            stats->estimated_time_remaining = 0.0f;
+        }
+        return MT_OK;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_cache_get_weights(mt_cache* cache, float* weights) {
+    if (!cache || !weights) return mt::fail(MT_E_INVALID, "null cache or weights");
+    return cache->mlp.get_master(weights);
+}
+
+int mt_cache_set_weights(mt_cache* cache, const float* weights) {
+    if (!cache || !weights) return mt::fail(MT_E_INVALID, "null cache or weights");
+    return cache->mlp.set_master(weights);
+}
+
+namespace {
+
+// host float3 arrays -> device float4 arrays
+int upload3(std::vector<void*>& owned, uint32_t n, const float* src, float4** dst) {
+    std::vector<float4> tmp(n);
+    for (uint32_t i = 0; i < n; ++i) tmp[i] = make_float4(src[i * 3], src[i * 3 + 1], src[i * 3 + 2], 0.0f);
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(float4) * std::max<uint32_t>(n, 1)) != hipSuccess)
+        return mt::fail(MT_E_NOMEM, "hipMalloc failed");
+    owned.push_back(p);
+    *dst = static_cast<float4*>(p);
+    if (n && hipMemcpy(p, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return mt::fail(MT_E_HIP, "upload failed");
+    return MT_OK;
+}
+
+struct FreeAll {
+    std::vector<void*>& v;
+    ~FreeAll() {
+        for (void* p : v) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+int mt_cache_infer(mt_cache* cache, uint32_t n, const float* pos, const float* dir, float* out) {
+    if (!cache || (n && (!pos || !dir || !out))) return mt::fail(MT_E_INVALID, "bad arguments");
+    if (n == 0) return MT_OK;
+    std::vector<void*> owned;
+    FreeAll guard{owned};
+    float4 *dp, *dd, *dout;
+    int rc;
+    if ((rc = upload3(owned, n, pos, &dp)) || (rc = upload3(owned, n, dir, &dd)) ||
+        (rc = upload3(owned, n, pos, &dout)))
+        return rc;
+    if ((rc = cache->mlp.infer(n, dp, dd, dout, nullptr))) return rc;
+    std::vector<float4> tmp(n);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(tmp.data(), dout, sizeof(float4) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return mt::fail(MT_E_HIP, "inference failed");
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i * 3] = tmp[i].x;
+        out[i * 3 + 1] = tmp[i].y;
+        out[i * 3 + 2] = tmp[i].z;
+    }
+    return MT_OK;
+}
+
+int mt_cache_train(mt_cache* cache, uint32_t n, const float* pos, const float* dir,
+                   const float* target, float* loss, float* grad) {
+    if (!cache || (n && (!pos || !dir || !target))) return mt::fail(MT_E_INVALID, "bad arguments");
+    if (n == 0) return MT_OK;
+    std::vector<void*> owned;
+    FreeAll guard{owned};
+    float4 *dp, *dd, *dt;
+    int rc;
+    if ((rc = upload3(owned, n, pos, &dp)) || (rc = upload3(owned, n, dir, &dd)) ||
+        (rc = upload3(owned, n, target, &dt)))
+        return rc;
+    float l = 0.0f;
+    if ((rc = cache->mlp.train(n, dp, dd, dt, nullptr, &l, grad))) return rc;
+    if (loss) *loss = l;
     return MT_OK;
 }
 

@@ -154,6 +154,7 @@ struct mt_scene {
     uint32_t instance_count = 0;
     uint32_t light_count = 0;
     uint32_t material_count = 0;
+    float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};  // union of the instance boxes
     uint64_t prim_slots = 0;
     uint64_t node_count = 0;
     uint64_t triangle_count = 0;

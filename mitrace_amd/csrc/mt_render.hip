@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "mt_device.hpp"
+#include "mt_mlp.hpp"
 #include "mt_render.hpp"
 
 namespace mt {
@@ -62,7 +63,7 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
 __device__ __forceinline__ void path_coords(const ChunkParams& c, uint32_t p, uint32_t& pix,
                                             uint32_t& s) {
     const uint32_t lp = p / c.spp;  // pixel-major: a wave holds the samples of one pixel
-    s = p - lp * c.spp;
+    s = p - lp * c.spp + c.sample_base;
     pix = c.pix_begin + lp;
 }
 
@@ -111,6 +112,17 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
     uint32_t pix, s;
     path_coords(c, p, pix, s);
     init_path(c, b, p, pix, s);
+    if (c.cache) {  // one path in train_every is a training path (the set moves every pass)
+        const uint32_t q = p + c.train_offset;
+        uint32_t slot = kNoSlot;
+        if (c.train_every != kNoSlot && q % c.train_every == 0) slot = q / c.train_every;
+        b.train_slot[p] = slot;
+        if (slot < c.train_cap) {
+            b.tr_path[slot] = p;
+            for (uint32_t j = 0; j < c.train_nrec; ++j)
+                b.tr_w[static_cast<size_t>(slot) * c.train_nrec + j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
     if (c.diff) {
         b.vis[p] = 0u;
         b.adj_s[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -554,6 +566,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
         bool cont = false, shadow = false;
         float4 sho{}, shd{}, shc{};
         uint32_t p = 0;
+        // cache query of a path that stops here
+        bool query = false;
+        float4 qpos{}, qdir{};
         // adjoint addends for the per-material wave reductions
         uint32_t gmat = 0;
         float g_r = 0.0f, g_m = 0.0f;
@@ -594,8 +609,28 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                         add_texel_grad(gr, pms, mul(dl, mul(trad, sk)));
                     }
                 }
+            } else if (kDiff == 0 && !kBatch && c.cache && k == c.cache_depth &&
+                       b.train_slot[p] == kNoSlot) {
+                // the cache supplies the radiance leaving this vertex toward the previous one
+                hits++;
+                query = true;
+                qpos = b.hit_pos[p];
+                qdir = f4(neg(d), 0.0f);
             } else {
                 hits++;
+                if (kDiff == 0 && !kBatch && c.cache && k >= c.cache_depth) {
+                    const uint32_t slot = b.train_slot[p];
+                    if (slot < c.train_cap && k - c.cache_depth < c.train_nrec) {
+                        // training vertex: radiance collected so far, throughput at the vertex
+                        const size_t r = static_cast<size_t>(slot) * c.train_nrec + (k - c.cache_depth);
+                        const V3 sofar = add(add(add(xyz(b.acc_e[p]), xyz(b.acc_d[p])), xyz(b.acc_i[p])),
+                                             xyz(b.acc_s[p]));
+                        b.tr_pos[r] = b.hit_pos[p];
+                        b.tr_dir[r] = f4(neg(d), 0.0f);
+                        b.tr_w[r] = f4(trad, 1.0f);
+                        b.tr_s[r] = f4(sofar, 0.0f);
+                    }
+                }
                 uint32_t pix, s;
                 coords<kBatch>(c, b, p, pix, s);
                 const uint32_t key = path_key(J.seed_key, pix, s);
@@ -799,6 +834,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 b.sh_d[ss] = shd;
                 b.sh_c[ss] = shc;
             }
+            if (kDiff == 0 && !kBatch && c.cache) {
+                const uint32_t qs = wave_push(b.counters + kCntQuery, query);
+                if (query) {
+                    b.cq_path[qs] = p;
+                    b.cq_pos[qs] = qpos;
+                    b.cq_dir[qs] = qdir;
+                }
+            }
         }
         const uint32_t cs = wave_push(b.counters + kCntExtend + k + 1, cont);
         if (cont) qout[cs] = p;
@@ -880,6 +923,81 @@ __global__ __launch_bounds__(kBlock) void k_batch_resolve(PathBuffers b) {
     const uint32_t lp = q - b.it_pix_off[ti];
     resolve_pixel(job.p, b.sample_rgb + t.w + static_cast<size_t>(lp) * job.p.spp,
                   job.img + (t.y + lp) * 3ull);
+}
+
+// ------------------------------------------------------------------ radiance cache passes
+// adds throughput x prediction to every path that stopped at a cache query
+__global__ __launch_bounds__(kBlock) void k_cache_apply(PathBuffers b, uint32_t n) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t p = b.cq_path[q];
+    const float4 o = b.cq_out[q];
+    const V3 pred = {gmax(o.x, 0.0f), gmax(o.y, 0.0f), gmax(o.z, 0.0f)};
+    const V3 add_ = mul(xyz(b.t_rad[p]), pred);
+    const float4 a = b.acc_i[p];
+    b.acc_i[p] = make_float4(a.x + add_.x, a.y + add_.y, a.z + add_.z, 0.0f);
+}
+
+// per chunk pixel: adds its samples of this pass, in sample order, to the pixel's sum
+__global__ __launch_bounds__(kBlock) void k_cache_accum(ChunkParams c, PathBuffers b) {
+    const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lp >= c.npix) return;
+    float* dst = b.img_sum + (c.pix_begin + lp) * 3ull;
+    V3 acc = {dst[0], dst[1], dst[2]};
+    const float4* sp = b.sample_rgb + static_cast<size_t>(lp) * c.spp;
+    for (uint32_t s = 0; s < c.spp; ++s) acc = add(acc, xyz(sp[s]));
+    dst[0] = acc.x;
+    dst[1] = acc.y;
+    dst[2] = acc.z;
+}
+
+// training samples from the recorded vertices of the training paths: target = radiance the
+// path collected from the vertex on, divided by the throughput at the vertex
+__global__ __launch_bounds__(kBlock) void k_cache_targets(ChunkParams c, PathBuffers b, uint32_t slot0,
+                                                          uint32_t slots) {
+    const uint32_t slot = slot0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok_slot = slot < slot0 + slots && slot < c.train_cap;
+    V3 L = mk(0.0f);
+    uint32_t p = 0;
+    if (ok_slot) {
+        p = b.tr_path[slot];
+        L = add(add(add(xyz(b.acc_e[p]), xyz(b.acc_d[p])), xyz(b.acc_i[p])), xyz(b.acc_s[p]));
+    }
+    for (uint32_t j = 0; j < c.train_nrec; ++j) {
+        bool ok = false;
+        float4 pos{}, dir{}, tgt{};
+        if (ok_slot) {
+            const size_t r = static_cast<size_t>(slot) * c.train_nrec + j;
+            const float4 w = b.tr_w[r];
+            if (w.w > 0.0f && w.x > 0.0f && w.y > 0.0f && w.z > 0.0f) {
+                const V3 s = xyz(b.tr_s[r]);
+                const V3 t = {(L.x - s.x) / w.x, (L.y - s.y) / w.y, (L.z - s.z) / w.z};
+                if (finite3(t)) {
+                    ok = true;
+                    pos = b.tr_pos[r];
+                    dir = b.tr_dir[r];
+                    tgt = make_float4(gmax(t.x, 0.0f), gmax(t.y, 0.0f), gmax(t.z, 0.0f), 0.0f);
+                }
+            }
+        }
+        const uint32_t ts = wave_push(b.counters + kCntTrainRec, ok);
+        if (ok) {
+            b.tp_pos[ts] = pos;
+            b.tp_dir[ts] = dir;
+            b.tp_tgt[ts] = tgt;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cache_resolve(const float* sum, uint32_t n, float spp,
+                                                          float* img) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) img[i] = sum[i] / spp;
+}
+
+__global__ void k_fill(float* p, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
 }
 
 __global__ void k_brdf(uint32_t n, const float* in, float* out) {
@@ -1322,6 +1440,139 @@ int Renderer::diff_adjoint(const DiffParams& dp, const DiffGradPtrs& gr, const f
         const int rc = run_diff_chunk(ch, 2, nullptr, st);
         if (rc) return rc;
     }
+    return MT_OK;
+}
+
+int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st,
+                           const mt_cache_options& opt) {
+    copt_ = opt;
+    if (copt_.spp_per_pass == 0) copt_.spp_per_pass = 1;
+    if (copt_.train_every == 0) copt_.train_every = 16;
+    if (copt_.query_depth == 0) copt_.query_depth = 1;
+    if (copt_.train_batch == 0) copt_.train_batch = 16384;
+    if (copt_.steps_per_pass == 0) copt_.steps_per_pass = 8;
+    if (st.samples % copt_.spp_per_pass)
+        return fail(MT_E_INVALID, "samples must be a multiple of spp_per_pass");
+    if (copt_.query_depth >= st.bounces)
+        return fail(MT_E_INVALID, "query_depth must be below the bounce count");
+    total_spp_ = st.samples;
+    mt_settings pass = st;
+    pass.samples = copt_.spp_per_pass;
+    pass.firefly_enable = 0;
+    int rc = setup(scene, cam, pass, false);
+    if (rc) return rc;
+    const uint64_t P = capacity_;
+    const uint32_t nrec = st.bounces - copt_.query_depth;
+    const uint64_t slots = P + 1;  // warm-up passes may train on every path
+    PathBuffers& b = buf_;
+    const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
+    if ((rc = alloc(owned_, &b.train_slot, P)) || (rc = alloc(owned_, &b.tr_path, slots)) ||
+        (rc = alloc(owned_, &b.tr_pos, slots * nrec)) || (rc = alloc(owned_, &b.tr_dir, slots * nrec)) ||
+        (rc = alloc(owned_, &b.tr_w, slots * nrec)) || (rc = alloc(owned_, &b.tr_s, slots * nrec)) ||
+        (rc = alloc(owned_, &b.cq_path, P)) || (rc = alloc(owned_, &b.cq_pos, P)) ||
+        (rc = alloc(owned_, &b.cq_dir, P)) || (rc = alloc(owned_, &b.cq_out, P)) ||
+        (rc = alloc(owned_, &b.tp_pos, slots * nrec)) || (rc = alloc(owned_, &b.tp_dir, slots * nrec)) ||
+        (rc = alloc(owned_, &b.tp_tgt, slots * nrec)) || (rc = alloc(owned_, &b.img_sum, npix_total * 3)))
+        return rc;
+    params_.cache = 1;
+    params_.cache_depth = copt_.query_depth;
+    params_.train_cap = static_cast<uint32_t>(slots);
+    params_.train_nrec = nrec;
+    return MT_OK;
+}
+
+int Renderer::run_cache_chunk(Mlp& mlp, uint32_t chunk, uint32_t sample_base, uint32_t train_every,
+                              uint32_t train_offset, bool accumulate, uint64_t& trained) {
+    const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
+    ChunkParams c = params_;
+    c.pix_begin = chunk * npix_chunk_;
+    c.npix = static_cast<uint32_t>(std::min<uint64_t>(npix_chunk_, npix_total - c.pix_begin));
+    c.paths = c.npix * st_.samples;
+    c.sample_base = sample_base;
+    c.train_every = train_every;
+    c.train_offset = train_offset % train_every;
+    const uint32_t gen_blocks = (c.paths + kBlock - 1) / kBlock;
+    hipStream_t s = stream_;
+    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    for (uint32_t k = 0; k < st_.bounces; ++k) {
+        launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
+        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        if (scene_->light_count) launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host_counters_, buf_.counters, sizeof(uint32_t) * kMaxCounters,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (host_counters_[kCntError]) return fail(MT_E_INVALID, "traversal stack overflow");
+    const uint32_t nq = host_counters_[kCntQuery];
+    uint64_t rays = 0;
+    for (uint32_t k = 0; k < st_.bounces; ++k)
+        rays += static_cast<uint64_t>(host_counters_[kCntExtend + k]) + host_counters_[kCntShadow + k];
+    int rc = mlp.infer(nq, buf_.cq_pos, buf_.cq_dir, buf_.cq_out, s);
+    if (rc) return rc;
+    if (nq) hipLaunchKernelGGL(k_cache_apply, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, s, buf_, nq);
+    hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    if (accumulate)
+        hipLaunchKernelGGL(k_cache_accum, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, buf_);
+    // training slots of this chunk: paths p with (p + offset) % train_every == 0
+    const uint32_t off = c.train_offset;
+    const uint32_t first = (train_every - off) % train_every;
+    const uint32_t nslots = (train_every != kNoSlot && c.paths > first) ? (c.paths - 1 - first) / train_every + 1 : 0;
+    const uint32_t slot0 = (first + off) / train_every;
+    if (nslots) {
+        // slots are numbered (p + offset) / train_every: [slot0, slot0 + nslots)
+        hipLaunchKernelGGL(k_cache_targets, dim3((nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                           c, buf_, slot0, nslots);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host_counters_, buf_.counters, sizeof(uint32_t) * kMaxCounters,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const uint32_t nrec = host_counters_[kCntTrainRec];
+    const uint32_t batch = copt_.train_batch;
+    for (uint32_t st = 0, o = 0; st < copt_.steps_per_pass && o < nrec; ++st, o += batch) {
+        const uint32_t n = std::min(batch, nrec - o);
+        if ((rc = mlp.train(n, buf_.tp_pos + o, buf_.tp_dir + o, buf_.tp_tgt + o, s, nullptr, nullptr)))
+            return rc;
+        trained += n;
+    }
+    stats_.rays += rays;
+    if (accumulate) stats_.samples += c.paths;
+    return MT_OK;
+}
+
+int Renderer::render_cached(Mlp& mlp, float* rgb_out) {
+    const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
+    hipStream_t s = stream_;
+    mlp.set_bounds(scene_->world_lo, scene_->world_hi);
+    hipLaunchKernelGGL(k_fill, dim3(static_cast<uint32_t>((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       buf_.img_sum, npix_total * 3);
+    HIP_TRY(hipEventRecord(events_[0], s));
+    uint64_t trained = 0;
+    const uint32_t te = copt_.train_every;
+    // warm-up: training-only passes with more training paths, on samples the image never uses
+    for (uint32_t w = 0; w < copt_.warmup_passes; ++w)
+        for (uint32_t ch = 0; ch < chunks_; ++ch) {
+            const int rc = run_cache_chunk(mlp, ch, 0x40000000u + w * copt_.spp_per_pass,
+                                           te == kNoSlot ? te : std::max<uint32_t>(1u, te / 4), w,
+                                           false, trained);
+            if (rc) return rc;
+        }
+    const uint32_t passes = total_spp_ / copt_.spp_per_pass;
+    for (uint32_t pass = 0; pass < passes; ++pass)
+        for (uint32_t ch = 0; ch < chunks_; ++ch) {
+            const int rc = run_cache_chunk(mlp, ch, pass * copt_.spp_per_pass, te, pass * 7u, true, trained);
+            if (rc) return rc;
+        }
+    hipLaunchKernelGGL(k_cache_resolve, dim3(static_cast<uint32_t>((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, s, buf_.img_sum, static_cast<uint32_t>(npix_total * 3), static_cast<float>(total_spp_), img_);
+    HIP_TRY(hipEventRecord(events_[1], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float total = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&total, events_[0], events_[1]));
+    stats_.kernel_ms_total += total;
+    stats_.diag[8] = trained;
+    if (rgb_out) HIP_TRY(hipMemcpy(rgb_out, img_, sizeof(float) * npix_total * 3, hipMemcpyDeviceToHost));
     return MT_OK;
 }
 

@@ -19,7 +19,10 @@ constexpr uint32_t kCntError = kCntFetchSh + kMaxBounces;       // traversal sta
 constexpr uint32_t kCntPaths = kCntError + 1;                   // batched: paths this iteration
 constexpr uint32_t kCntPixels = kCntPaths + 1;                  // batched: pixels this iteration
 constexpr uint32_t kCntTiles = kCntPixels + 1;                  // batched: tiles this iteration
-constexpr uint32_t kMaxCounters = kCntTiles + 1;
+constexpr uint32_t kCntQuery = kCntTiles + 1;                   // cached: cache queries
+constexpr uint32_t kCntTrainRec = kCntQuery + 1;                // cached: training samples
+constexpr uint32_t kMaxCounters = kCntTrainRec + 1;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 
 // differentiable parameters (device pointers, null = the scene's value): a float RGB texture that
 // replaces the base colour of materials [tex_first, tex_first + tex_count), and per-material
@@ -69,6 +72,11 @@ struct ChunkParams {
     float4* path_store;    // [npix_total * spp] {linear radiance, shadow visibility bits}
     const float* grad_img; // [npix_total * 3] dLoss/dImage (adjoint replay)
     float inv_spp;
+    uint32_t sample_base;  // sample index of the chunk's first sample of every pixel
+    // radiance cache: paths that are not training paths stop at their hit of bounce
+    // cache_depth and take the rest of their radiance from the cache
+    int32_t cache;
+    uint32_t cache_depth, train_every, train_offset, train_cap, train_nrec;
 };
 
 // SoA path state, queues and scratch of one render (capacity = paths per chunk)
@@ -99,6 +107,16 @@ struct PathBuffers {
     const struct BatchJob* jobs;
     uint32_t it_capacity;   // paths per iteration
     uint32_t it_tile_cap;   // tiles per iteration
+    // radiance cache
+    uint32_t* train_slot;   // per path: training slot or kNoSlot
+    uint32_t* tr_path;      // per slot: its path
+    float4 *tr_pos, *tr_dir, *tr_w, *tr_s;  // per slot and recorded bounce: vertex, direction
+                                            // to the previous vertex, throughput (w = valid),
+                                            // radiance collected before the vertex
+    uint32_t* cq_path;      // cache queries: path, position, direction, prediction
+    float4 *cq_pos, *cq_dir, *cq_out;
+    float4 *tp_pos, *tp_dir, *tp_tgt;       // compact training samples
+    float* img_sum;         // per pixel: sum of the accumulated passes' samples
 };
 
 // one job of a batch, resident in device memory
@@ -121,6 +139,10 @@ class Renderer {
                      hipStream_t s);
     int diff_adjoint(const DiffParams& dp, const DiffGradPtrs& gr, const float* grad_img,
                      hipStream_t s);
+    // render with the radiance cache: passes of spp_pass samples per pixel; rgb_out (host)
+    int setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st,
+                     const mt_cache_options& opt);
+    int render_cached(class Mlp& mlp, float* rgb_out);
     int finish(float* rgb_out);
     uint32_t chunks() const { return chunks_; }
     const mt_stats& stats() const { return stats_; }
@@ -129,6 +151,10 @@ class Renderer {
    private:
     void release();
     int run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s);
+    int run_cache_chunk(class Mlp& mlp, uint32_t chunk, uint32_t sample_base, uint32_t train_every,
+                        uint32_t train_offset, bool accumulate, uint64_t& trained);
+    mt_cache_options copt_{};
+    uint32_t total_spp_ = 0;
     mt_scene* scene_ = nullptr;
     mt_camera_desc cam_{};
     mt_settings st_{};

@@ -574,6 +574,16 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         }
     }
 
+    // bounds of the whole scene (the radiance cache's position encoding)
+    for (int k = 0; k < 3; ++k) {
+        s->world_lo[k] = FLT_MAX;
+        s->world_hi[k] = -FLT_MAX;
+    }
+    for (uint32_t i = 0; i < d->instance_count; ++i)
+        for (int k = 0; k < 3; ++k) {
+            s->world_lo[k] = fmin_g(s->world_lo[k], inst[i].wmin[k]);
+            s->world_hi[k] = fmax_g(s->world_hi[k], inst[i].wmax[k]);
+        }
     // top-level BVH for scenes with more instances than one wave-wide mask covers
     std::vector<float4> tlas_pairs;
     std::vector<uint32_t> tlas_inst;

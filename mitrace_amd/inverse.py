@@ -60,7 +60,11 @@ def targets(gs, truth, cams, spp=1024, width=512, height=512):
     return out
 
 
-def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024):
+def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024, decorrelated=True):
+    """decorrelated: the residual (image - target) comes from an independent render (another
+    seed) than the one whose adjoint is taken.  With one render for both, the gradient of the
+    squared error carries 2 Cov(I, dI/dtheta), which for a base colour (I roughly linear in it)
+    is positive and steadily darkens the texture."""
     sc, truth, cams, gs = setup()
     t0 = time.perf_counter()
     tg = targets(gs, truth, cams, spp=target_spp)
@@ -103,10 +107,20 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024):
         opt.zero_grad()
         total = 0.0
         for i, cam in enumerate(cams):
-            img = dr.render(cam, 1 + it * len(cams) + i, tex, rough, metal, emis)
-            loss = torch.mean((img - tg[i]) ** 2) / len(cams)
-            loss.backward()
-            total += float(loss.detach())
+            seed = 1 + 2 * (it * len(cams) + i)
+            if decorrelated:
+                with torch.no_grad():
+                    resid = dr.forward(cam, seed + 1, tex.detach(), rough.detach(), metal, emis) - tg[i]
+                img = dr.render(cam, seed, tex, rough, metal, emis)
+                # surrogate whose gradient is 2 * resid * dI/dtheta / N
+                loss = torch.sum(2.0 * resid * img) / resid.numel() / len(cams)
+                loss.backward()
+                total += float(torch.mean(resid ** 2)) / len(cams)
+            else:
+                img = dr.render(cam, seed, tex, rough, metal, emis)
+                loss = torch.mean((img - tg[i]) ** 2) / len(cams)
+                loss.backward()
+                total += float(loss.detach())
         with torch.no_grad():
             seen += (tex.grad.abs().sum(-1) > 0).int()
         opt.step()
@@ -129,7 +143,7 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024):
         seen_rmse = lambda t: float(torch.sqrt(torch.mean(((t - gt["tex"])[m]) ** 2)))
         seen_frac = float(m.float().mean())
         s0, s1 = seen_rmse(init_t), seen_rmse(tex)
-    return {"iters": iters, "spp": spp, "poses": len(cams), "lr": lr,
+    return {"iters": iters, "spp": spp, "poses": len(cams), "lr": lr, "decorrelated": decorrelated,
             "targets_spp": target_spp, "targets_s": t_targets,
             "initial": {"tex_rmse": e0[0], "tex_rmse_observed": s0, "rough_mae": e0[1],
                         "image_rmse": img0},
@@ -222,9 +236,11 @@ def main(argv=None):
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--crn-spp", type=int, default=4096)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--correlated", action="store_true",
+                    help="one render per pose for both the residual and the adjoint")
     a = ap.parse_args(argv)
     if a.mode == "optimize":
-        out = optimize(iters=a.iters, spp=a.spp)
+        out = optimize(iters=a.iters, spp=a.spp, decorrelated=not a.correlated)
     else:
         out = gradcheck(crn_spp=a.crn_spp)
     summary = {k: v for k, v in out.items() if k not in ("log", "rows")}

@@ -37,7 +37,7 @@ def test_struct_layouts_match_the_header():
                "mt_instance_desc", "mt_light_desc", "mt_camera_desc", "mt_env_desc",
                "mt_bvh_options", "mt_scene_desc", "mt_settings", "mt_stats", "mt_hit",
                "mt_diff_params", "mt_diff_grads", "mt_batch_options", "mt_batch_job_info",
-               "mt_batch_stats", "mt_autotune_trial"]
+               "mt_batch_stats", "mt_autotune_trial", "mt_cache_options"]
     src = '#include <stdio.h>\n#include <stddef.h>\n#include "%s"\nint main(void){\n' % HEADER
     for s in structs:
         src += f'printf("{s} %zu\\n", sizeof({s}));\n'
