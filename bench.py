@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate wall time of the CPU-baseline sample")
+    ap.add_argument("--no-cache", action="store_true",
+                    help="skip the plain vs radiance-cached comparison on the large scene")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-requests workload")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -192,6 +194,26 @@ def batch_leg(rate_hz=16.0):
     return rep
 
 
+def cache_leg():
+    """The large terrain scene (2048x2048-vertex fBm terrain, 5000 icospheres, 16 point
+    lights, four 2048^2 textures, sky) at 1920x1080 and 64 spp: plain and radiance-cached
+    throughput, RMSE ladders against a 4096-spp plain render at 320x180, and the speed-up at the
+    plain render's RMSE."""
+    from mitrace_amd import cache_eval, scenegen
+    from mitrace_amd.tracer import GpuScene
+    t0 = time.perf_counter()
+    sc = scenegen.large_scene()
+    g = GpuScene(sc)
+    build_s = time.perf_counter() - t0
+    print(f"# cache: large scene built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    out = cache_eval.compare(g, scenegen.large_scene_settings,
+                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    g.close()
+    out["workload"] = ("large terrain scene (8.4M-triangle terrain, 5000 icospheres, 16 lights), "
+                       "1920x1080, 64 spp")
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -271,6 +293,8 @@ def main():
                                  ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
         "cpu_baseline": None,
     }
+    if not args.no_cache:
+        line["cache"] = cache_leg()
     if not args.no_batch:
         line["batch"] = batch_leg()
         print(f"# batch: {line['batch']['msamples_per_s']:.1f} Msamples/s, p50 "

@@ -70,15 +70,45 @@ int run_job(mt_job* j, float* sample_rgb_out) {
         std::scoped_lock lk(j->mu);
         j->chunks_total = j->renderer.chunks();
     }
-    for (uint32_t c = 0; c < j->renderer.chunks(); ++c) {
-        if (j->killed.load()) return mt::fail(MT_E_KILLED, "render killed");
-        rc = j->renderer.run_chunk(c, sample_rgb_out);
-        if (rc) return rc;
-        j->chunks_done.store(c + 1);
+    // chunks go round-robin to the renderer's stream slots; a slot's previous chunk is
+    // completed (waited for, counted) before it takes the next, so completion is in order
+    mt::Renderer& R = j->renderer;
+    const uint32_t n = R.chunks(), slots = R.slots();
+    uint32_t done = 0;
+    auto complete = [&](uint32_t slot) -> int {
+        if (!R.slot_busy(slot)) return MT_OK;
+        const int r = R.complete_chunk(slot, sample_rgb_out);
+        if (r) return r;
+        j->chunks_done.store(++done);
         std::scoped_lock lk(j->mu);
-        j->snapshot = j->renderer.stats();
+        j->snapshot = R.stats();
+        return MT_OK;
+    };
+    auto drain = [&]() {  // error path: wait for whatever is in flight
+        for (uint32_t k = 0; k < slots; ++k) (void)complete(k);
+    };
+    for (uint32_t c = 0; c < n; ++c) {
+        const uint32_t slot = c % slots;
+        if ((rc = complete(slot))) {
+            drain();
+            return rc;
+        }
+        if (j->killed.load()) {
+            drain();
+            return mt::fail(MT_E_KILLED, "render killed");
+        }
+        if ((rc = R.enqueue_chunk(slot, c))) {
+            drain();
+            return rc;
+        }
     }
-    rc = j->renderer.finish(j->rgb_out);
+    const uint32_t oldest = done;  // the remaining chunks, oldest first
+    for (uint32_t k = 0; k < slots; ++k)
+        if ((rc = complete((oldest + k) % slots))) {
+            drain();
+            return rc;
+        }
+    rc = R.finish(j->rgb_out);
     std::scoped_lock lk(j->mu);
     j->snapshot = j->renderer.stats();
     return rc;

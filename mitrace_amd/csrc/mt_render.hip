@@ -1125,6 +1125,15 @@ int brdf_eval(uint32_t n, const float* in, float* out) {
 Renderer::~Renderer() { release(); }
 
 void Renderer::release() {
+    for (size_t i = 1; i < slots_.size(); ++i) {
+        StreamSlot& S = slots_[i];
+        if (S.stream) (void)hipStreamSynchronize(S.stream);
+        for (hipEvent_t e : S.events) (void)hipEventDestroy(e);
+        if (S.host_counters) (void)hipHostFree(S.host_counters);
+        if (S.stream) (void)hipStreamDestroy(S.stream);
+    }
+    slots_.clear();
+    if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : owned_) (void)hipFree(p);
     owned_.clear();
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
@@ -1238,7 +1247,8 @@ int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, ui
     return MT_OK;
 }
 
-int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff) {
+int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff,
+                    uint32_t max_slots) {
     scene_ = scene;
     cam_ = cam;
     st_ = st;
@@ -1292,6 +1302,25 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         HIP_TRY(hipEventCreate(&e));
         events_.push_back(e);
     }
+    slots_.clear();
+    slots_.push_back(StreamSlot{buf_, stream_, host_counters_, events_, -1, {}});
+    // more slots for plain renders with several chunks (MT_STREAMS, default 3)
+    const uint32_t cap_slots = max_slots ? max_slots : std::min<uint32_t>(env_u32("MT_STREAMS", 3), 8);
+    const uint32_t want = diff ? 1u : std::min<uint32_t>(cap_slots, chunks_);
+    for (uint32_t i = 1; i < want; ++i) {
+        StreamSlot L;
+        HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        slots_.push_back(L);  // registered first, so that release() frees what follows
+        StreamSlot& S = slots_.back();
+        if ((rc = alloc_path_buffers(owned_, S.b, P, spill_depth * trace_threads, false, false))) return rc;
+        HIP_TRY(hipMemsetAsync(S.b.work, 0, sizeof(unsigned long long) * 16, S.stream));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&S.host_counters), sizeof(uint32_t) * kMaxCounters));
+        for (uint32_t k = 0; k < 2 + 4 * st.bounces; ++k) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            S.events.push_back(e);
+        }
+    }
     return MT_OK;
 }
 
@@ -1305,40 +1334,60 @@ uint32_t seed_mix(uint32_t x) {
 }
 
 int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
+    const int rc = enqueue_chunk(0, chunk);
+    return rc ? rc : complete_chunk(0, sample_rgb_out);
+}
+
+int Renderer::enqueue_chunk(uint32_t slot, uint32_t chunk) {
+    StreamSlot& L = slots_[slot];
     const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
     ChunkParams c = params_;
     c.pix_begin = chunk * npix_chunk_;
     c.npix = static_cast<uint32_t>(std::min<uint64_t>(npix_chunk_, npix_total - c.pix_begin));
     c.paths = c.npix * st_.samples;
     const uint32_t gen_blocks = (c.paths + kBlock - 1) / kBlock;
-    hipStream_t s = stream_;
+    hipStream_t s = L.stream;
+    const PathBuffers& b = L.b;
+    std::vector<hipEvent_t>& events = L.events;
     // events: [0] chunk start, [1] chunk end, then per bounce 4: before extend, after extend,
-    // after shade, after shadow (all on this stream, read after the chunk's sync)
-    HIP_TRY(hipEventRecord(events_[0], s));
-    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    // after shade, after shadow (all on this slot's stream, read after the chunk's sync)
+    HIP_TRY(hipEventRecord(events[0], s));
+    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, b);
     for (uint32_t k = 0; k < st_.bounces; ++k) {
-        hipEvent_t* ev = &events_[2 + 4 * k];
+        hipEvent_t* ev = &events[2 + 4 * k];
         HIP_TRY(hipEventRecord(ev[0], s));
-        launch_trace<0>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
+        launch_trace<0>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, b, k);
         HIP_TRY(hipEventRecord(ev[1], s));
-        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
         HIP_TRY(hipEventRecord(ev[2], s));
         if (scene_->light_count)
-            launch_trace<1>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, buf_, k);
+            launch_trace<1>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, b, k);
         HIP_TRY(hipEventRecord(ev[3], s));
         HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
-    hipLaunchKernelGGL(k_resolve, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, buf_, img_);
+    hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, s, c, b);
+    hipLaunchKernelGGL(k_resolve, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, b, img_);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(events_[1], s));
-    HIP_TRY(hipMemcpyAsync(host_counters_, buf_.counters, sizeof(uint32_t) * kMaxCounters,
+    HIP_TRY(hipEventRecord(events[1], s));
+    HIP_TRY(hipMemcpyAsync(L.host_counters, b.counters, sizeof(uint32_t) * kMaxCounters,
                            hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    L.chunk = chunk;
+    L.c = c;
+    return MT_OK;
+}
+
+int Renderer::complete_chunk(uint32_t slot, float* sample_rgb_out) {
+    StreamSlot& L = slots_[slot];
+    if (L.chunk < 0) return MT_OK;
+    const ChunkParams& c = L.c;
+    L.chunk = -1;
+    HIP_TRY(hipStreamSynchronize(L.stream));
+    const std::vector<hipEvent_t>& events = L.events;
+    const uint32_t* hc = L.host_counters;
     float total = 0, ms_ext = 0, ms_shade = 0, ms_sh = 0;
-    HIP_TRY(hipEventElapsedTime(&total, events_[0], events_[1]));
+    HIP_TRY(hipEventElapsedTime(&total, events[0], events[1]));
     for (uint32_t k = 0; k < st_.bounces; ++k) {
-        hipEvent_t* ev = &events_[2 + 4 * k];
+        const hipEvent_t* ev = &events[2 + 4 * k];
         float a = 0, b2 = 0, d = 0;
         HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
         HIP_TRY(hipEventElapsedTime(&b2, ev[1], ev[2]));
@@ -1349,10 +1398,10 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
     }
     stats_.extend_launches += st_.bounces;
     if (scene_->light_count) stats_.shadow_launches += st_.bounces;
-    if (host_counters_[kCntError]) return fail(MT_E_INVALID, "traversal stack overflow");
+    if (hc[kCntError]) return fail(MT_E_INVALID, "traversal stack overflow");
     uint64_t rays = 0;
     for (uint32_t k = 0; k < st_.bounces; ++k)
-        rays += static_cast<uint64_t>(host_counters_[kCntExtend + k]) + host_counters_[kCntShadow + k];
+        rays += static_cast<uint64_t>(hc[kCntExtend + k]) + hc[kCntShadow + k];
     stats_.rays += rays;
     stats_.samples += c.paths;
     stats_.kernel_ms_total += total;
@@ -1361,7 +1410,7 @@ int Renderer::run_chunk(uint32_t chunk, float* sample_rgb_out) {
     stats_.kernel_ms_shadow += ms_sh;
     if (sample_rgb_out) {
         std::vector<float4> tmp(c.paths);
-        HIP_TRY(hipMemcpy(tmp.data(), buf_.sample_rgb, sizeof(float4) * c.paths, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(tmp.data(), L.b.sample_rgb, sizeof(float4) * c.paths, hipMemcpyDeviceToHost));
         for (uint32_t p = 0; p < c.paths; ++p) {
             const uint32_t lp = p / c.spp, smp = p % c.spp;
             float* q = sample_rgb_out + (static_cast<size_t>(c.pix_begin + lp) * st_.samples + smp) * 3;
@@ -1459,7 +1508,7 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
     mt_settings pass = st;
     pass.samples = copt_.spp_per_pass;
     pass.firefly_enable = 0;
-    int rc = setup(scene, cam, pass, false);
+    int rc = setup(scene, cam, pass, false, 1);
     if (rc) return rc;
     const uint64_t P = capacity_;
     const uint32_t nrec = st.bounces - copt_.query_depth;
@@ -1579,8 +1628,12 @@ int Renderer::render_cached(Mlp& mlp, float* rgb_out) {
 int Renderer::finish(float* rgb_out) {
     const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
     if (st_.count_work) {
-        unsigned long long w[16];
-        HIP_TRY(hipMemcpy(w, buf_.work, sizeof(w), hipMemcpyDeviceToHost));
+        unsigned long long w[16] = {};
+        for (const StreamSlot& S : slots_) {
+            unsigned long long ws[16];
+            HIP_TRY(hipMemcpy(ws, S.b.work, sizeof(ws), hipMemcpyDeviceToHost));
+            for (int i = 0; i < 16; ++i) w[i] += ws[i];
+        }
         for (int i = 0; i < 9; ++i) stats_.diag[i] = w[7 + i];
         stats_.inst_aabb_tests = w[0];
         stats_.inst_entered = w[1];

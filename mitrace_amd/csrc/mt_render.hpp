@@ -125,13 +125,31 @@ struct BatchJob {
     float* img;     // width * height * 3, device
 };
 
+// one stream with its own path state: chunks on different slots run concurrently, so that
+// one chunk's last, sparsely populated bounces overlap another chunk's full ones
+struct StreamSlot {
+    PathBuffers b{};
+    hipStream_t stream = nullptr;
+    uint32_t* host_counters = nullptr;
+    std::vector<hipEvent_t> events;
+    int64_t chunk = -1;  // chunk in flight, -1 = idle
+    ChunkParams c{};
+};
+
 class Renderer {
    public:
     ~Renderer();
-    int setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff = false);
+    // max_slots: concurrent chunk streams (0 = MT_STREAMS or 3, at most the chunk count)
+    int setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff = false,
+              uint32_t max_slots = 0);
     void set_camera(const mt_camera_desc& cam);
     void set_seed(uint64_t seed);
     int run_chunk(uint32_t chunk, float* sample_rgb_out);
+    // pipelined chunks: enqueue on an idle slot, complete (wait + statistics) in FIFO order
+    uint32_t slots() const { return static_cast<uint32_t>(slots_.size()); }
+    bool slot_busy(uint32_t i) const { return slots_[i].chunk >= 0; }
+    int enqueue_chunk(uint32_t slot, uint32_t chunk);
+    int complete_chunk(uint32_t slot, float* sample_rgb_out);
     // differentiable rendering over the whole image on the caller's stream (0 = own stream):
     // forward writes the linear image (mean radiance per pixel) to img_dev and the per-path
     // store; adjoint replays the same paths and accumulates into the gradient buffers
@@ -169,6 +187,7 @@ class Renderer {
     uint32_t capacity_ = 0, npix_chunk_ = 0, chunks_ = 0;
     uint32_t trace_blocks_ = 0, shade_blocks_ = 0, lds_depth_ = 16;
     mt_stats stats_{};
+    std::vector<StreamSlot> slots_;  // slots_[0] uses buf_, stream_, host_counters_, events_
 };
 
 // launch shape of one batched iteration
