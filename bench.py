@@ -289,6 +289,9 @@ def main():
                      "kernel": "k_trace<false,0> (extend: closest hit)",
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3,
+                     # chunks run on concurrent streams, so launches overlap: the whole frame's
+                     # extend bytes over the frame time is a lower bound of the aggregate rate
+                     "achieved_aggregate_lower_bound": bytes_frame / (total_t / args.steps) / 1e9,
                      "per_ray": {k: cnt[k] / cnt["rays"] for k in
                                  ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
         "cpu_baseline": None,

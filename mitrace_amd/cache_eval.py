@@ -14,8 +14,8 @@ import numpy as np
 
 from .cache import RadianceCache
 
-DEFAULT_CACHE = {"train_every": 16, "query_depth": 1, "warmup_passes": 4, "steps_per_pass": 8,
-                 "train_batch": 16384, "learning_rate": 5e-3, "spp_per_pass": 1, "seed": 1}
+DEFAULT_CACHE = {"train_every": 16, "query_depth": 2, "warmup_passes": 4, "steps_per_pass": 8,
+                 "train_batch": 16384, "learning_rate": 5e-3, "spp_per_pass": 4, "seed": 1}
 
 
 def rmse(a, b):
@@ -82,7 +82,8 @@ def compare(gscene, settings_fn, spp=64, small=(320, 180), ref_spp=4096, ladder=
     if need:
         # the cached render at full resolution with the samples it needs for that RMSE
         # (warm-up and training included), against the plain render at `spp`
-        k = max(1, int(np.ceil(need)))
+        per = opts["spp_per_pass"]
+        k = max(per, int(np.ceil(need / per)) * per)
         cache = RadianceCache(gscene, opts)
         t0 = time.perf_counter()
         cache.render(settings_fn(samples=k, seed=5))

@@ -29,7 +29,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kWave = 64;
 constexpr float kPiF = 3.14159265358979323846f;
-constexpr float kLossEps = 0.01f;  // relative L2: (y - t)^2 / (y^2 + eps)
+constexpr float kLossEps = 0.01f;  // relative error r = (y - t) / sqrt(y^2 + eps), y held
+constexpr float kHuber = 10.0f;    // loss r^2 for |r| <= 10, linear beyond (rare huge targets)
 
 __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
@@ -175,8 +176,16 @@ __device__ __forceinline__ void forward(const Net& N, const bf16x8 enc[2], f32x1
     out5 = o[0];
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(Net N, EncCtx E, uint32_t n, const float4* pos,
-                                                   const float4* dir, float4* out) {
+// a step's sample count: n_host, or (when n_dev is set) min(n_host, *n_dev - offset) clamped at 0
+__device__ __forceinline__ uint32_t step_count(const uint32_t* n_dev, uint32_t offset, uint32_t n_host) {
+    if (!n_dev) return n_host;
+    const uint32_t avail = *n_dev;
+    return avail > offset ? min(n_host, avail - offset) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(Net N, EncCtx E, uint32_t n_host, const uint32_t* n_dev,
+                                                   const float4* pos, const float4* dir, float4* out) {
+    const uint32_t n = step_count(n_dev, 0, n_host);
     const int lane = threadIdx.x & 63, g = lane >> 5, s = lane & 31;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
@@ -200,9 +209,16 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Net N, EncCtx E, uint32_t n, 
 
 // training pass for 32 samples per wave: forward keeping H0..H4, relative-L2 loss gradient,
 // backward through the layers keeping dZ1..dZ5 (bf16, natural feature order)
-__global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n, const float4* pos,
+__global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n_host, const uint32_t* n_dev,
+                                                   uint32_t offset, const float4* pos,
                                                    const float4* dir, const float4* target,
-                                                   CacheTrainBuffers T, float inv_count) {
+                                                   CacheTrainBuffers T) {
+    const uint32_t n = step_count(n_dev, offset, n_host);
+    if (n == 0) return;
+    pos += offset;
+    dir += offset;
+    target += offset;
+    const float inv_count = 1.0f / static_cast<float>(n);
     const int lane = threadIdx.x & 63, g = lane >> 5, s = lane & 31;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
@@ -232,11 +248,13 @@ __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n, 
             const float tv[3] = {t.x, t.y, t.z};
             for (int c = 0; c < 3; ++c) {
                 const float y = o[c] + b[c];
-                const float den = y * y + kLossEps;
-                const float e = y - tv[c];
-                loss += e * e / den;
-                // d/dy of (y - t)^2 / sg(y^2 + eps), averaged over samples and channels
-                dz5[c] = 2.0f * e / den * inv_count / 3.0f;
+                const float sd = sqrtf(y * y + kLossEps);
+                const float r = (y - tv[c]) / sd;
+                const float ar = fabsf(r);
+                loss += ar <= kHuber ? r * r : kHuber * (2.0f * ar - kHuber);
+                // d/dy of the Huber loss of r with the denominator held, averaged over samples
+                // and channels
+                dz5[c] = 2.0f * fminf(fmaxf(r, -kHuber), kHuber) / sd * inv_count / 3.0f;
             }
         }
         // dZ5 chunk 0 of the output layer: k-slots t<4 of group 0 are outputs 0..3
@@ -279,7 +297,9 @@ __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n, 
 // weight gradients: every block sums dW = dZ^T H over its sample range with the samples as K,
 // from 32-sample slabs transposed into LDS ([feature][sample]); writes its partial sums
 constexpr int kSlab = 32;
-__global__ __launch_bounds__(256) void k_mlp_wgrad(CacheTrainBuffers T, uint32_t n) {
+__global__ __launch_bounds__(256) void k_mlp_wgrad(CacheTrainBuffers T, uint32_t n_host, const uint32_t* n_dev,
+                                                   uint32_t offset) {
+    const uint32_t n = step_count(n_dev, offset, n_host);
     __shared__ __attribute__((aligned(16))) uint16_t sH[kHWidth][kSlab];
     __shared__ __attribute__((aligned(16))) uint16_t sD[kDZWidth][kSlab];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 5, r = lane & 31;
@@ -363,9 +383,12 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(CacheTrainBuffers T, uint32_t
 // Adam on the fp32 master weights from the sum of the blocks' partial gradients
 __global__ void k_mlp_adam(float* master, float* m, float* v, const float* partial, uint32_t blocks,
                            float lr, float b1, float b2, float eps, float c1, float c2,
-                           float* grad_out) {
+                           float* grad_out, uint32_t n_host, const uint32_t* n_dev, uint32_t offset,
+                           unsigned long long* trained) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= static_cast<uint32_t>(kParams)) return;
+    const uint32_t n = step_count(n_dev, offset, n_host);
+    if (n == 0 || p >= static_cast<uint32_t>(kParams)) return;  // an empty step changes nothing
+    if (p == 0 && trained) atomicAdd(trained, static_cast<unsigned long long>(n));
     float gsum = 0.0f;
     for (uint32_t b = 0; b < blocks; ++b) gsum += partial[static_cast<size_t>(b) * kParams + p];
     if (grad_out) grad_out[p] = gsum;
@@ -462,6 +485,12 @@ int Mlp::set_master(const float* in) {
     return hipDeviceSynchronize() == hipSuccess ? MT_OK : fail(MT_E_HIP, "cache pack failed");
 }
 
+uint64_t Mlp::trained_samples() {
+    unsigned long long v = 0;
+    if (trained_) (void)hipMemcpy(&v, trained_, sizeof(v), hipMemcpyDeviceToHost);
+    return v;
+}
+
 int Mlp::get_master(float* out) {
     if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(out, master_, sizeof(float) * kParams, hipMemcpyDeviceToHost) != hipSuccess)
@@ -482,29 +511,31 @@ int Mlp::pack(hipStream_t s) {
     return hipGetLastError() == hipSuccess ? MT_OK : fail(MT_E_HIP, "k_mlp_pack launch failed");
 }
 
-int Mlp::infer(uint32_t n, const float4* pos, const float4* dir, float4* out, hipStream_t s) {
+int Mlp::infer(uint32_t n, const float4* pos, const float4* dir, float4* out, hipStream_t s,
+                const uint32_t* n_dev) {
     if (n == 0) return MT_OK;
     Net N{reinterpret_cast<const bf16x8*>(fwd_), reinterpret_cast<const bf16x8*>(bwd_), master_};
     EncCtx E{{lo_[0], lo_[1], lo_[2]}, {inv_ext_[0], inv_ext_[1], inv_ext_[2]}};
     const uint32_t groups = (n + 31) / 32;
     const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 4096);
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, N, E, n, pos, dir, out);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, N, E, n, n_dev, pos, dir, out);
     return hipGetLastError() == hipSuccess ? MT_OK : fail(MT_E_HIP, "k_mlp_infer launch failed");
 }
 
 int Mlp::train(uint32_t n, const float4* pos, const float4* dir, const float4* target, hipStream_t s,
-               float* loss_host, float* grad_host) {
+               float* loss_host, float* grad_host, const uint32_t* n_dev, uint32_t offset) {
     if (n == 0) return MT_OK;
     if (n > tb_.capacity) return fail(MT_E_INVALID, "training batch exceeds the cache's capacity");
+    if (!trained_ && dalloc(owned_, &trained_, 1)) return MT_E_NOMEM;
     Net N{reinterpret_cast<const bf16x8*>(fwd_), reinterpret_cast<const bf16x8*>(bwd_), master_};
     EncCtx E{{lo_[0], lo_[1], lo_[2]}, {inv_ext_[0], inv_ext_[1], inv_ext_[2]}};
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, tb_.loss, 1u);
     const uint32_t groups = (n + 31) / 32;
     const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 2048);
-    hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, s, N, E, n, pos, dir, target, tb_,
-                       1.0f / static_cast<float>(n));
+    hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, s, N, E, n, n_dev, offset, pos, dir,
+                       target, tb_);
     const uint32_t wb = std::min<uint32_t>(tb_.blocks, (n + kSlab - 1) / kSlab);
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(wb), dim3(256), 0, s, tb_, n);
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3(wb), dim3(256), 0, s, tb_, n, n_dev, offset);
     ++step_;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, static_cast<float>(step_));
@@ -512,7 +543,8 @@ int Mlp::train(uint32_t n, const float4* pos, const float4* dir, const float4* t
     float* gdev = nullptr;
     if (grad_host && dalloc(owned_, &gdev, kParams)) return MT_E_NOMEM;
     hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, master_, adam_m_,
-                       adam_v_, tb_.partial, wb, lr_, b1, b2, 1e-8f, c1, c2, gdev);
+                       adam_v_, tb_.partial, wb, lr_, b1, b2, 1e-8f, c1, c2, gdev, n, n_dev, offset,
+                       trained_);
     int rc = pack(s);
     if (rc) return rc;
     if (hipGetLastError() != hipSuccess) return fail(MT_E_HIP, "cache training launch failed");

@@ -56,12 +56,17 @@ class Mlp {
    public:
     ~Mlp();
     int init(uint32_t seed, uint32_t train_capacity, float lr);
-    // device pointers: pos/dir float4 (xyz used), out float4; bounds = scene box for the encoding
-    int infer(uint32_t n, const float4* pos, const float4* dir, float4* out, hipStream_t s);
+    // device pointers: pos/dir float4 (xyz used), out float4; bounds = scene box for the encoding.
+    // n_dev (optional): the count is min(n, *n_dev), read by the kernels (no host sync)
+    int infer(uint32_t n, const float4* pos, const float4* dir, float4* out, hipStream_t s,
+              const uint32_t* n_dev = nullptr);
     // one Adam step on n samples with targets float4 (xyz); *loss_host (optional) receives the
-    // mean loss (synchronises the stream)
+    // mean loss (synchronises the stream).  n_dev (optional): the step takes samples
+    // [offset, offset + min(n, *n_dev - offset)) and is skipped when that is empty
     int train(uint32_t n, const float4* pos, const float4* dir, const float4* target, hipStream_t s,
-              float* loss_host, float* grad_host);
+              float* loss_host, float* grad_host, const uint32_t* n_dev = nullptr,
+              uint32_t offset = 0);
+    uint64_t trained_samples();
     int get_master(float* out);
     int set_master(const float* in);
     void set_bounds(const float lo[3], const float hi[3]);
@@ -80,6 +85,7 @@ class Mlp {
     float lo_[3] = {0, 0, 0}, inv_ext_[3] = {1, 1, 1};
     float lr_ = 1e-2f;
     uint64_t step_ = 0;
+    unsigned long long* trained_ = nullptr;  // samples of all non-empty steps
 };
 
 }  // namespace mt

@@ -105,7 +105,9 @@ __device__ __forceinline__ void init_path(const ChunkParams& J, const PathBuffer
 __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers b) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) {
-        for (uint32_t k = 0; k < kMaxCounters; ++k) b.counters[k] = 0;
+        // a cached render reads the overflow flag once at the end: keep it across chunks
+        for (uint32_t k = 0; k < kMaxCounters; ++k)
+            if (k != kCntError || !c.cache) b.counters[k] = 0;
         b.counters[kCntExtend] = c.paths;
     }
     if (p >= c.paths) return;
@@ -927,9 +929,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_resolve(PathBuffers b) {
 
 // ------------------------------------------------------------------ radiance cache passes
 // adds throughput x prediction to every path that stopped at a cache query
-__global__ __launch_bounds__(kBlock) void k_cache_apply(PathBuffers b, uint32_t n) {
+__global__ __launch_bounds__(kBlock) void k_cache_apply(PathBuffers b) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+    if (q >= b.counters[kCntQuery]) return;
     const uint32_t p = b.cq_path[q];
     const float4 o = b.cq_out[q];
     const V3 pred = {gmax(o.x, 0.0f), gmax(o.y, 0.0f), gmax(o.z, 0.0f)};
@@ -993,6 +995,14 @@ __global__ __launch_bounds__(kBlock) void k_cache_resolve(const float* sum, uint
                                                           float* img) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) img[i] = sum[i] / spp;
+}
+
+// adds the chunk's rays (extension + shadow, every bounce) to a running total
+__global__ void k_count_rays(PathBuffers b, uint32_t bounces, unsigned long long* total) {
+    if (threadIdx.x != 0) return;
+    unsigned long long r = 0;
+    for (uint32_t k = 0; k < bounces; ++k) r += b.counters[kCntExtend + k] + b.counters[kCntShadow + k];
+    atomicAdd(total, r);
 }
 
 __global__ void k_fill(float* p, uint64_t n) {
@@ -1125,14 +1135,20 @@ int brdf_eval(uint32_t n, const float* in, float* out) {
 Renderer::~Renderer() { release(); }
 
 void Renderer::release() {
-    for (size_t i = 1; i < slots_.size(); ++i) {
+    if (tail_stream_) (void)hipStreamSynchronize(tail_stream_);
+    for (size_t i = 0; i < slots_.size(); ++i) {
         StreamSlot& S = slots_[i];
+        if (S.ev_bounce) (void)hipEventDestroy(S.ev_bounce);
+        if (S.ev_tail) (void)hipEventDestroy(S.ev_tail);
+        if (i == 0) continue;  // slot 0's stream, events and counters are the renderer's own
         if (S.stream) (void)hipStreamSynchronize(S.stream);
         for (hipEvent_t e : S.events) (void)hipEventDestroy(e);
         if (S.host_counters) (void)hipHostFree(S.host_counters);
         if (S.stream) (void)hipStreamDestroy(S.stream);
     }
     slots_.clear();
+    if (tail_stream_) (void)hipStreamDestroy(tail_stream_);
+    tail_stream_ = nullptr;
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : owned_) (void)hipFree(p);
     owned_.clear();
@@ -1305,8 +1321,10 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     slots_.clear();
     slots_.push_back(StreamSlot{buf_, stream_, host_counters_, events_, -1, {}});
     // more slots for plain renders with several chunks (MT_STREAMS, default 3)
-    const uint32_t cap_slots = max_slots ? max_slots : std::min<uint32_t>(env_u32("MT_STREAMS", 3), 8);
-    const uint32_t want = diff ? 1u : std::min<uint32_t>(cap_slots, chunks_);
+    // an explicit max_slots is taken as is (cached renders pipeline across passes too)
+    const uint32_t want = diff ? 1u
+                          : max_slots ? std::min<uint32_t>(max_slots, 8)
+                                      : std::min<uint32_t>(std::min<uint32_t>(env_u32("MT_STREAMS", 3), 8), chunks_);
     for (uint32_t i = 1; i < want; ++i) {
         StreamSlot L;
         HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
@@ -1508,30 +1526,46 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
     mt_settings pass = st;
     pass.samples = copt_.spp_per_pass;
     pass.firefly_enable = 0;
-    int rc = setup(scene, cam, pass, false, 1);
+    const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
+    int rc = setup(scene, cam, pass, false, std::min<uint32_t>(env_u32("MT_STREAMS", 3), 8));
     if (rc) return rc;
     const uint64_t P = capacity_;
     const uint32_t nrec = st.bounces - copt_.query_depth;
-    const uint64_t slots = P + 1;  // warm-up passes may train on every path
-    PathBuffers& b = buf_;
-    const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
-    if ((rc = alloc(owned_, &b.train_slot, P)) || (rc = alloc(owned_, &b.tr_path, slots)) ||
-        (rc = alloc(owned_, &b.tr_pos, slots * nrec)) || (rc = alloc(owned_, &b.tr_dir, slots * nrec)) ||
-        (rc = alloc(owned_, &b.tr_w, slots * nrec)) || (rc = alloc(owned_, &b.tr_s, slots * nrec)) ||
-        (rc = alloc(owned_, &b.cq_path, P)) || (rc = alloc(owned_, &b.cq_pos, P)) ||
-        (rc = alloc(owned_, &b.cq_dir, P)) || (rc = alloc(owned_, &b.cq_out, P)) ||
-        (rc = alloc(owned_, &b.tp_pos, slots * nrec)) || (rc = alloc(owned_, &b.tp_dir, slots * nrec)) ||
-        (rc = alloc(owned_, &b.tp_tgt, slots * nrec)) || (rc = alloc(owned_, &b.img_sum, npix_total * 3)))
+    const uint64_t tslots = P + 1;  // warm-up passes may train on every path
+    float* img_sum = nullptr;
+    if ((rc = alloc(owned_, &img_sum, npix_total * 3)) ||
+        (rc = alloc(owned_, &rays_dev_, 1)))
         return rc;
+    for (StreamSlot& S : slots_) {
+        PathBuffers& b = S.b;
+        if ((rc = alloc(owned_, &b.train_slot, P)) || (rc = alloc(owned_, &b.tr_path, tslots)) ||
+            (rc = alloc(owned_, &b.tr_pos, tslots * nrec)) || (rc = alloc(owned_, &b.tr_dir, tslots * nrec)) ||
+            (rc = alloc(owned_, &b.tr_w, tslots * nrec)) || (rc = alloc(owned_, &b.tr_s, tslots * nrec)) ||
+            (rc = alloc(owned_, &b.cq_path, P)) || (rc = alloc(owned_, &b.cq_pos, P)) ||
+            (rc = alloc(owned_, &b.cq_dir, P)) || (rc = alloc(owned_, &b.cq_out, P)) ||
+            (rc = alloc(owned_, &b.tp_pos, tslots * nrec)) || (rc = alloc(owned_, &b.tp_dir, tslots * nrec)) ||
+            (rc = alloc(owned_, &b.tp_tgt, tslots * nrec)))
+            return rc;
+        b.img_sum = img_sum;
+        HIP_TRY(hipEventCreateWithFlags(&S.ev_bounce, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&S.ev_tail, hipEventDisableTiming));
+    }
+    buf_ = slots_[0].b;
+    HIP_TRY(hipStreamCreateWithFlags(&tail_stream_, hipStreamNonBlocking));
     params_.cache = 1;
     params_.cache_depth = copt_.query_depth;
-    params_.train_cap = static_cast<uint32_t>(slots);
+    params_.train_cap = static_cast<uint32_t>(tslots);
     params_.train_nrec = nrec;
     return MT_OK;
 }
 
-int Renderer::run_cache_chunk(Mlp& mlp, uint32_t chunk, uint32_t sample_base, uint32_t train_every,
-                              uint32_t train_offset, bool accumulate, uint64_t& trained) {
+// One chunk of a cached pass.  The bounce loop runs on the slot's stream; everything that
+// touches the cache (inference, its use, finalize/accumulate, training samples, Adam steps)
+// runs in chunk order on one tail stream, so the cache is trained on chunk c before chunk c+1
+// queries it and no two kernels use its buffers at once.  Nothing waits on the host.
+int Renderer::run_cache_chunk(Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t sample_base,
+                              uint32_t train_every, uint32_t train_offset, bool accumulate) {
+    StreamSlot& S = slots_[slot];
     const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
     ChunkParams c = params_;
     c.pix_begin = chunk * npix_chunk_;
@@ -1541,86 +1575,93 @@ int Renderer::run_cache_chunk(Mlp& mlp, uint32_t chunk, uint32_t sample_base, ui
     c.train_every = train_every;
     c.train_offset = train_offset % train_every;
     const uint32_t gen_blocks = (c.paths + kBlock - 1) / kBlock;
-    hipStream_t s = stream_;
-    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    hipStream_t s = S.stream, m = tail_stream_;
+    const PathBuffers& b = S.b;
+    HIP_TRY(hipStreamWaitEvent(s, S.ev_tail, 0));  // the slot's previous chunk is finished
+    hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, b);
     for (uint32_t k = 0; k < st_.bounces; ++k) {
-        launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
-        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
-        if (scene_->light_count) launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
+        launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, b, k);
+        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
+        if (scene_->light_count) launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, b, k);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(host_counters_, buf_.counters, sizeof(uint32_t) * kMaxCounters,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if (host_counters_[kCntError]) return fail(MT_E_INVALID, "traversal stack overflow");
-    const uint32_t nq = host_counters_[kCntQuery];
-    uint64_t rays = 0;
-    for (uint32_t k = 0; k < st_.bounces; ++k)
-        rays += static_cast<uint64_t>(host_counters_[kCntExtend + k]) + host_counters_[kCntShadow + k];
-    int rc = mlp.infer(nq, buf_.cq_pos, buf_.cq_dir, buf_.cq_out, s);
+    HIP_TRY(hipEventRecord(S.ev_bounce, s));
+    HIP_TRY(hipStreamWaitEvent(m, S.ev_bounce, 0));
+    hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, m, b, st_.bounces, rays_dev_);
+    int rc = mlp.infer(c.paths, b.cq_pos, b.cq_dir, b.cq_out, m, b.counters + kCntQuery);
     if (rc) return rc;
-    if (nq) hipLaunchKernelGGL(k_cache_apply, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, s, buf_, nq);
-    hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, s, c, buf_);
+    hipLaunchKernelGGL(k_cache_apply, dim3(gen_blocks), dim3(kBlock), 0, m, b);
+    hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, m, c, b);
     if (accumulate)
-        hipLaunchKernelGGL(k_cache_accum, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, c, buf_);
-    // training slots of this chunk: paths p with (p + offset) % train_every == 0
+        hipLaunchKernelGGL(k_cache_accum, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, m, c, b);
+    // training slots of this chunk: paths p with (p + offset) % train_every == 0, numbered
+    // (p + offset) / train_every: [slot0, slot0 + nslots)
     const uint32_t off = c.train_offset;
     const uint32_t first = (train_every - off) % train_every;
     const uint32_t nslots = (train_every != kNoSlot && c.paths > first) ? (c.paths - 1 - first) / train_every + 1 : 0;
     const uint32_t slot0 = (first + off) / train_every;
     if (nslots) {
-        // slots are numbered (p + offset) / train_every: [slot0, slot0 + nslots)
-        hipLaunchKernelGGL(k_cache_targets, dim3((nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                           c, buf_, slot0, nslots);
+        hipLaunchKernelGGL(k_cache_targets, dim3((nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, m, c, b,
+                           slot0, nslots);
+        const uint32_t batch = copt_.train_batch;
+        for (uint32_t st = 0; st < copt_.steps_per_pass; ++st)
+            if ((rc = mlp.train(batch, b.tp_pos, b.tp_dir, b.tp_tgt, m, nullptr, nullptr,
+                                b.counters + kCntTrainRec, st * batch)))
+                return rc;
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(host_counters_, buf_.counters, sizeof(uint32_t) * kMaxCounters,
-                           hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const uint32_t nrec = host_counters_[kCntTrainRec];
-    const uint32_t batch = copt_.train_batch;
-    for (uint32_t st = 0, o = 0; st < copt_.steps_per_pass && o < nrec; ++st, o += batch) {
-        const uint32_t n = std::min(batch, nrec - o);
-        if ((rc = mlp.train(n, buf_.tp_pos + o, buf_.tp_dir + o, buf_.tp_tgt + o, s, nullptr, nullptr)))
-            return rc;
-        trained += n;
-    }
-    stats_.rays += rays;
+    HIP_TRY(hipEventRecord(S.ev_tail, m));
     if (accumulate) stats_.samples += c.paths;
     return MT_OK;
 }
 
 int Renderer::render_cached(Mlp& mlp, float* rgb_out) {
     const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
-    hipStream_t s = stream_;
+    hipStream_t m = tail_stream_;
     mlp.set_bounds(scene_->world_lo, scene_->world_hi);
-    hipLaunchKernelGGL(k_fill, dim3(static_cast<uint32_t>((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+    const uint64_t trained0 = mlp.trained_samples();
+    HIP_TRY(hipDeviceSynchronize());
+    hipLaunchKernelGGL(k_fill, dim3(static_cast<uint32_t>((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, m,
                        buf_.img_sum, npix_total * 3);
-    HIP_TRY(hipEventRecord(events_[0], s));
-    uint64_t trained = 0;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, m, reinterpret_cast<float*>(rays_dev_), 2ull);
+    HIP_TRY(hipEventRecord(events_[0], m));
+    for (StreamSlot& S : slots_) {  // counters start at zero (the overflow flag is sticky here)
+        hipLaunchKernelGGL(k_fill, dim3((kMaxCounters + kBlock - 1) / kBlock), dim3(kBlock), 0, m,
+                           reinterpret_cast<float*>(S.b.counters), static_cast<uint64_t>(kMaxCounters));
+        HIP_TRY(hipEventRecord(S.ev_tail, m));  // slots start after the fills
+    }
     const uint32_t te = copt_.train_every;
+    const uint32_t nslot = static_cast<uint32_t>(slots_.size());
+    uint32_t seq = 0;
     // warm-up: training-only passes with more training paths, on samples the image never uses
     for (uint32_t w = 0; w < copt_.warmup_passes; ++w)
-        for (uint32_t ch = 0; ch < chunks_; ++ch) {
-            const int rc = run_cache_chunk(mlp, ch, 0x40000000u + w * copt_.spp_per_pass,
-                                           te == kNoSlot ? te : std::max<uint32_t>(1u, te / 4), w,
-                                           false, trained);
+        for (uint32_t ch = 0; ch < chunks_; ++ch, ++seq) {
+            const int rc = run_cache_chunk(mlp, seq % nslot, ch, 0x40000000u + w * copt_.spp_per_pass,
+                                           te == kNoSlot ? te : std::max<uint32_t>(1u, te / 4), w, false);
             if (rc) return rc;
         }
     const uint32_t passes = total_spp_ / copt_.spp_per_pass;
     for (uint32_t pass = 0; pass < passes; ++pass)
-        for (uint32_t ch = 0; ch < chunks_; ++ch) {
-            const int rc = run_cache_chunk(mlp, ch, pass * copt_.spp_per_pass, te, pass * 7u, true, trained);
+        for (uint32_t ch = 0; ch < chunks_; ++ch, ++seq) {
+            const int rc = run_cache_chunk(mlp, seq % nslot, ch, pass * copt_.spp_per_pass, te, pass * 7u, true);
             if (rc) return rc;
         }
     hipLaunchKernelGGL(k_cache_resolve, dim3(static_cast<uint32_t>((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, buf_.img_sum, static_cast<uint32_t>(npix_total * 3), static_cast<float>(total_spp_), img_);
-    HIP_TRY(hipEventRecord(events_[1], s));
-    HIP_TRY(hipStreamSynchronize(s));
+                       0, m, buf_.img_sum, static_cast<uint32_t>(npix_total * 3), static_cast<float>(total_spp_), img_);
+    HIP_TRY(hipEventRecord(events_[1], m));
+    HIP_TRY(hipDeviceSynchronize());
     float total = 0.0f;
     HIP_TRY(hipEventElapsedTime(&total, events_[0], events_[1]));
     stats_.kernel_ms_total += total;
-    stats_.diag[8] = trained;
+    unsigned long long rays = 0;
+    HIP_TRY(hipMemcpy(&rays, rays_dev_, sizeof(rays), hipMemcpyDeviceToHost));
+    stats_.rays += rays;
+    stats_.diag[8] = mlp.trained_samples() - trained0;
+    for (const StreamSlot& S : slots_) {
+        uint32_t err = 0;
+        HIP_TRY(hipMemcpy(&err, S.b.counters + kCntError, sizeof(err), hipMemcpyDeviceToHost));
+        if (err) return fail(MT_E_INVALID, "traversal stack overflow");
+    }
     if (rgb_out) HIP_TRY(hipMemcpy(rgb_out, img_, sizeof(float) * npix_total * 3, hipMemcpyDeviceToHost));
     return MT_OK;
 }

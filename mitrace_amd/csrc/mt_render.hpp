@@ -134,6 +134,7 @@ struct StreamSlot {
     std::vector<hipEvent_t> events;
     int64_t chunk = -1;  // chunk in flight, -1 = idle
     ChunkParams c{};
+    hipEvent_t ev_bounce = nullptr, ev_tail = nullptr;  // cached renders
 };
 
 class Renderer {
@@ -169,8 +170,10 @@ class Renderer {
    private:
     void release();
     int run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s);
-    int run_cache_chunk(class Mlp& mlp, uint32_t chunk, uint32_t sample_base, uint32_t train_every,
-                        uint32_t train_offset, bool accumulate, uint64_t& trained);
+    int run_cache_chunk(class Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t sample_base,
+                        uint32_t train_every, uint32_t train_offset, bool accumulate);
+    hipStream_t tail_stream_ = nullptr;
+    unsigned long long* rays_dev_ = nullptr;
     mt_cache_options copt_{};
     uint32_t total_spp_ = 0;
     mt_scene* scene_ = nullptr;

@@ -1,12 +1,14 @@
 """float32 numpy reference of the radiance-cache MLP with the GPU kernels' BF16 roundings:
 BF16 weights, BF16 inputs to every product (the encoding and each hidden activation, the
 backward gradients), FP32 accumulation and biases.  Layer l: z = W_l h + b_l, ReLU on the
-four hidden layers, linear output; loss = mean over samples and channels of
-(y - t)^2 / (y^2 + 0.01) with the denominator held constant."""
+four hidden layers, linear output; loss = mean over samples and channels of the
+Huber function (r^2 up to |r| = 10, linear beyond) of r = (y - t) / sqrt(y^2 + 0.01) with the
+denominator held constant."""
 import numpy as np
 
 LAYERS = [(32, 64), (64, 64), (64, 64), (64, 64), (64, 3)]
 EPS = 0.01
+HUBER = 10.0
 DTYPE = np.float32  # arithmetic type of forward/backward (float64 when pinning the math)
 
 
@@ -66,10 +68,11 @@ def forward(w, enc):
 def loss_and_grad(w, enc, target):
     y, hs, zs = forward(w, enc)
     n = len(enc)
-    den = y * y + EPS
-    e = y - target
-    loss = float((e * e / den).sum() / (3 * n))
-    dz = bf16((2 * e / den / n / 3).astype(DTYPE))
+    sd = np.sqrt(y * y + EPS)
+    r = (y - target) / sd
+    ar = np.abs(r)
+    loss = float(np.where(ar <= HUBER, r * r, HUBER * (2 * ar - HUBER)).sum() / (3 * n))
+    dz = bf16((2 * np.clip(r, -HUBER, HUBER) / sd / n / 3).astype(DTYPE))
     grads = []
     layers = unpack(w)
     for l in range(4, -1, -1):

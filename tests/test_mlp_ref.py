@@ -14,15 +14,20 @@ def test_reference_gradient_matches_central_differences(monkeypatch):
     w = rng.normal(0, 0.3, 14787)
     enc = rng.normal(size=(40, 32))
     t = rng.uniform(0, 1, size=(40, 3))
+    t[:6] *= 1000.0  # beyond the Huber threshold: the linear branch is exercised too
     loss, g = mlp_ref.loss_and_grad(w, enc, t)
     y0, _, _ = mlp_ref.forward(w, enc)
-    den = y0 * y0 + mlp_ref.EPS
+    sd = np.sqrt(y0 * y0 + mlp_ref.EPS)
+    H = mlp_ref.HUBER
 
     def f(wv):
         y, _, _ = mlp_ref.forward(wv, enc)
-        return ((y - t) ** 2 / den).sum() / (3 * len(enc))
+        r = np.abs((y - t) / sd)
+        return np.where(r <= H, r * r, H * (2 * r - H)).sum() / (3 * len(enc))
 
     assert abs(f(w) - loss) < 1e-12
+    r = np.abs((y0 - t) / sd)
+    assert (r > H).any() and (r <= H).any()
     idx = np.concatenate([rng.choice(14787, 60, replace=False), [14784, 14785, 14786, 2048, 2049]])
     h = 1e-6
     for i in idx:
