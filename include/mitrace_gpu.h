@@ -264,6 +264,7 @@ typedef struct mt_batch_options {
     uint32_t tile_pixels;         /* pixels per work item of the device queue [1024]      */
     uint32_t iteration_tiles;     /* queue capacity: tiles one iteration may take [4096] */
     uint32_t trace_blocks_per_cu; /* persistent traversal blocks per CU [8]              */
+    uint32_t streams;             /* concurrent lanes, each a stream with its path state [3] */
 } mt_batch_options;
 
 typedef struct mt_batch_job_info {
@@ -397,8 +398,8 @@ int mt_cache_train(mt_cache* cache, uint32_t n, const float* pos, const float* d
 
 /* Batched rendering (no reference counterpart; for serving many render requests).  Jobs of
  * any scene, resolution, sample count and depth are split into tiles of tile_pixels pixels on
- * a per-scene device queue; one host thread replays, per iteration, a captured graph of the
- * whole wavefront step (schedule tiles from the queue, generate, extend/shade/shadow per
+ * per-scene device queues (one per lane); one host thread replays, per iteration, a captured
+ * graph of the whole wavefront step (schedule tiles from the queue, generate, extend/shade/shadow per
  * bounce, finalize, resolve into each job's image) for the scene of the oldest unfinished
  * job.  Every job's image equals mt_render's with the same scene, camera and settings.
  * mt_batch_submit may be called from any thread before or while the batch runs; rgb_out

@@ -150,7 +150,8 @@ class mt_cache_options(C.Structure):
 
 class mt_batch_options(C.Structure):
     _fields_ = [("paths_per_iteration", C.c_uint32), ("tile_pixels", C.c_uint32),
-                ("iteration_tiles", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32)]
+                ("iteration_tiles", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("streams", C.c_uint32)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -16,7 +16,8 @@ import numpy as np
 from . import abi, scenegen
 from ._lib import check, load
 
-OPTION_NAMES = ("paths_per_iteration", "tile_pixels", "iteration_tiles", "trace_blocks_per_cu")
+OPTION_NAMES = ("paths_per_iteration", "tile_pixels", "iteration_tiles", "trace_blocks_per_cu",
+                "streams")
 
 
 def _options(d):

@@ -1,6 +1,9 @@
 // mt_batch.cpp -- batched rendering: per-scene device tile queues, one captured graph per scene
 // for a whole wavefront iteration, a host scheduler thread, and the autotuner.
 //
+// Each scene has one queue and one captured graph per lane (a stream with its own path state);
+// a job's tiles are dealt round-robin over the lanes, so iterations of one scene run
+// concurrently and one lane's sparse last bounces overlap another's full ones.
 // The scheduler keeps an exact host mirror of the device queues: the device scheduler kernel
 // and the host apply the same greedy rule (tiles in queue order while their paths fit the
 // iteration, at least one), so the host knows which iteration finishes each job without
@@ -35,6 +38,8 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+constexpr uint32_t kMaxLanes = 8;
+
 struct HostJob {
     mt_scene* scene = nullptr;
     uint32_t slot = 0;
@@ -43,24 +48,36 @@ struct HostJob {
     float* rgb_out = nullptr;
     float* img_dev = nullptr;
     uint32_t tiles = 0, tiles_left = 0;
+    uint32_t lane_left[kMaxLanes] = {};  // unscheduled tiles per lane
     uint64_t samples = 0;
     double submit_ms = 0.0;
-    hipEvent_t done_ev = nullptr;            // set before `recorded`
+    std::vector<hipEvent_t> done_evs;         // one per lane that had tiles; complete before `recorded`
     std::atomic<bool> recorded{false};
 };
 
-struct SceneSlot {
-    mt_scene* scene = nullptr;
+// one stream with its own path state; every scene has a queue and a graph per lane
+struct Lane {
+    hipStream_t stream = nullptr;
+    mt::PathBuffers buf{};
+    std::deque<hipEvent_t> inflight;
+};
+
+struct SceneQueue {
     uint4* q_dev = nullptr;        // device queue
     uint32_t* q_state = nullptr;   // device {head, tail}
     uint4* q_host = nullptr;       // pinned, append-only copy of the queue
     uint32_t head = 0, tail = 0;   // host mirror
-    uint32_t bounces = 0;          // largest bounce count of the scene's jobs
     uint32_t graph_bounces = 0;    // bounce count of the captured graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    mt::ChunkParams c{};
     mt::PathBuffers b{};
+};
+
+struct SceneSlot {
+    mt_scene* scene = nullptr;
+    uint32_t bounces = 0;          // largest bounce count of the scene's jobs
+    mt::ChunkParams c{};
+    SceneQueue q[kMaxLanes];
 };
 
 }  // namespace
@@ -68,9 +85,10 @@ struct SceneSlot {
 struct mt_batch {
     int device = 0;
     mt_batch_options opt{};
-    hipStream_t stream = nullptr, capture = nullptr;
+    hipStream_t stream = nullptr, capture = nullptr;  // uploads; graph capture
     std::vector<void*> owned;
-    mt::PathBuffers buf{};
+    std::vector<Lane> lanes;
+    uint32_t next_lane = 0;
     mt::BatchJob* jobs_dev = nullptr;
     mt::BatchJob* jobs_host = nullptr;  // pinned staging, one record per job
     uint32_t trace_blocks = 0, shade_blocks = 0;
@@ -92,13 +110,12 @@ struct mt_batch {
     double t0_ms = 0.0;
     uint64_t iterations = 0, captures = 0;
     uint32_t first_open = 0;  // oldest job with tiles not yet scheduled
-    std::deque<hipEvent_t> inflight;
     std::vector<hipEvent_t> spare;
 
     ~mt_batch();
     int init(const mt_batch_options* o);
     int admit(uint32_t id);
-    int ensure_graph(SceneSlot& s);
+    int ensure_graph(SceneSlot& s, uint32_t lane);
     int run();
     int fail(int code, const std::string& m) {
         status = code;
@@ -121,14 +138,20 @@ mt_batch::~mt_batch() {
         worker.join();
     }
     if (stream) (void)hipStreamSynchronize(stream);
-    for (auto& s : slots) {
-        if (s->exec) (void)hipGraphExecDestroy(s->exec);
-        if (s->graph) (void)hipGraphDestroy(s->graph);
-        if (s->q_host) (void)hipHostFree(s->q_host);
-    }
+    for (Lane& L : lanes)
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+    for (auto& s : slots)
+        for (SceneQueue& q : s->q) {
+            if (q.exec) (void)hipGraphExecDestroy(q.exec);
+            if (q.graph) (void)hipGraphDestroy(q.graph);
+            if (q.q_host) (void)hipHostFree(q.q_host);
+        }
     for (auto& j : jobs)
-        if (j->done_ev) (void)hipEventDestroy(j->done_ev);
-    for (hipEvent_t e : inflight) (void)hipEventDestroy(e);
+        for (hipEvent_t e : j->done_evs) (void)hipEventDestroy(e);
+    for (Lane& L : lanes) {
+        for (hipEvent_t e : L.inflight) (void)hipEventDestroy(e);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
     for (hipEvent_t e : spare) (void)hipEventDestroy(e);
     if (e0) (void)hipEventDestroy(e0);
     for (void* p : owned) (void)hipFree(p);
@@ -144,8 +167,9 @@ int mt_batch::init(const mt_batch_options* o) {
     if (!opt.tile_pixels) opt.tile_pixels = 1024;
     if (!opt.iteration_tiles) opt.iteration_tiles = 4096;
     if (!opt.trace_blocks_per_cu) opt.trace_blocks_per_cu = 8;
+    if (!opt.streams) opt.streams = 3;
     if (opt.paths_per_iteration > (1u << 26) || opt.iteration_tiles > (1u << 20) ||
-        opt.trace_blocks_per_cu > 32)
+        opt.trace_blocks_per_cu > 32 || opt.streams > kMaxLanes)
         return fail(MT_E_UNSUPPORTED, "batch options out of range");
     BT_TRY(hipGetDevice(&device));
     BT_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -155,28 +179,35 @@ int mt_batch::init(const mt_batch_options* o) {
     const uint64_t P = opt.paths_per_iteration;
     shade_blocks = static_cast<uint32_t>(std::min<uint64_t>((P + kBlockSize - 1) / kBlockSize, cus * 16ull));
     const uint64_t spill = static_cast<uint64_t>(kMaxStack - kLds) * trace_blocks * kBlockSize;
-    int rc = mt::alloc_path_buffers(owned, buf, P, spill, false, true);
-    if (rc) return fail(rc, mt_last_error());
     auto dev_alloc = [&](void** p, size_t bytes) -> int {
         if (hipMalloc(p, bytes) != hipSuccess) return fail(MT_E_NOMEM, "hipMalloc of batch buffers failed");
         owned.push_back(*p);
         return MT_OK;
     };
-    if ((rc = dev_alloc(reinterpret_cast<void**>(&buf.it_tiles), sizeof(uint4) * opt.iteration_tiles)) ||
-        (rc = dev_alloc(reinterpret_cast<void**>(&buf.it_pix_off), sizeof(uint32_t) * (opt.iteration_tiles + 1))) ||
-        (rc = dev_alloc(reinterpret_cast<void**>(&jobs_dev), sizeof(mt::BatchJob) * kMaxJobs)))
-        return rc;
+    int rc;
+    if ((rc = dev_alloc(reinterpret_cast<void**>(&jobs_dev), sizeof(mt::BatchJob) * kMaxJobs))) return rc;
+    lanes.resize(opt.streams);
+    for (Lane& L : lanes) {
+        BT_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        mt::PathBuffers& b = L.buf;
+        if ((rc = mt::alloc_path_buffers(owned, b, P, spill, false, true))) return fail(rc, mt_last_error());
+        if ((rc = dev_alloc(reinterpret_cast<void**>(&b.it_tiles), sizeof(uint4) * opt.iteration_tiles)) ||
+            (rc = dev_alloc(reinterpret_cast<void**>(&b.it_pix_off), sizeof(uint32_t) * (opt.iteration_tiles + 1))))
+            return rc;
+        b.jobs = jobs_dev;
+        b.it_capacity = opt.paths_per_iteration;
+        b.it_tile_cap = opt.iteration_tiles;
+    }
     BT_TRY(hipHostMalloc(reinterpret_cast<void**>(&jobs_host), sizeof(mt::BatchJob) * kMaxJobs));
-    buf.jobs = jobs_dev;
-    buf.it_capacity = opt.paths_per_iteration;
-    buf.it_tile_cap = opt.iteration_tiles;
     BT_TRY(hipEventCreate(&e0));
     return MT_OK;
 }
 
-// puts job `id` on its scene's device queue (scheduler thread)
+// puts job `id` on its scene's device queues, its tiles dealt round-robin over the lanes
+// (scheduler thread; uploads on the upload stream, which every lane waits for)
 int mt_batch::admit(uint32_t id) {
     HostJob& j = *view[id];
+    const uint32_t nl = static_cast<uint32_t>(lanes.size());
     SceneSlot* s = nullptr;
     for (auto& x : slots)
         if (x->scene == j.scene) s = x.get();
@@ -185,20 +216,23 @@ int mt_batch::admit(uint32_t id) {
         ns->scene = j.scene;
         if (mt::stack_capacity(j.scene) > kMaxStack)
             return fail(MT_E_UNSUPPORTED, "scene BVH too deep for batched rendering");
-        if (hipMalloc(reinterpret_cast<void**>(&ns->q_dev), sizeof(uint4) * kQueueTiles) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void**>(&ns->q_state), sizeof(uint32_t) * 2) != hipSuccess)
-            return fail(MT_E_NOMEM, "hipMalloc of a tile queue failed");
-        owned.push_back(ns->q_dev);
-        owned.push_back(ns->q_state);
-        BT_TRY(hipHostMalloc(reinterpret_cast<void**>(&ns->q_host), sizeof(uint4) * kQueueTiles));
-        mt::launch_set_u32(ns->q_state, 0u, stream);
-        mt::launch_set_u32(ns->q_state + 1, 0u, stream);
         // scene-level launch constants (the per-job ones come from the job table)
         mt::fill_params(ns->c, j.scene, j.cam, j.st);
         ns->c.count_work = 0;
-        ns->b = buf;
-        ns->b.q_tiles = ns->q_dev;
-        ns->b.q_state = ns->q_state;
+        for (uint32_t l = 0; l < nl; ++l) {
+            SceneQueue& q = ns->q[l];
+            if (hipMalloc(reinterpret_cast<void**>(&q.q_dev), sizeof(uint4) * kQueueTiles) != hipSuccess ||
+                hipMalloc(reinterpret_cast<void**>(&q.q_state), sizeof(uint32_t) * 2) != hipSuccess)
+                return fail(MT_E_NOMEM, "hipMalloc of a tile queue failed");
+            owned.push_back(q.q_dev);
+            owned.push_back(q.q_state);
+            BT_TRY(hipHostMalloc(reinterpret_cast<void**>(&q.q_host), sizeof(uint4) * kQueueTiles));
+            mt::launch_set_u32(q.q_state, 0u, stream);
+            mt::launch_set_u32(q.q_state + 1, 0u, stream);
+            q.b = lanes[l].buf;
+            q.b.q_tiles = q.q_dev;
+            q.b.q_state = q.q_state;
+        }
         slots.push_back(std::move(ns));
         s = slots.back().get();
     }
@@ -214,37 +248,58 @@ int mt_batch::admit(uint32_t id) {
     // tiles: whole pixels, at most paths_per_iteration paths each
     const uint32_t npix = j.st.width * j.st.height;
     const uint32_t tp = std::max<uint32_t>(1u, std::min<uint32_t>(opt.tile_pixels, opt.paths_per_iteration / j.st.samples));
-    const uint32_t first = s->tail;
-    for (uint32_t p0 = 0; p0 < npix; p0 += tp) {
-        if (s->tail >= kQueueTiles) return fail(MT_E_UNSUPPORTED, "tile queue of a scene is full");
+    uint32_t first[kMaxLanes];
+    for (uint32_t l = 0; l < nl; ++l) first[l] = s->q[l].tail;
+    uint32_t count = 0;
+    for (uint32_t p0 = 0; p0 < npix; p0 += tp, ++count) {
+        const uint32_t l = (count + id) % nl;
+        SceneQueue& q = s->q[l];
+        if (q.tail >= kQueueTiles) return fail(MT_E_UNSUPPORTED, "tile queue of a scene is full");
         const uint32_t n = std::min(tp, npix - p0);
-        s->q_host[s->tail++] = make_uint4(id, p0, n, n * j.st.samples);
+        q.q_host[q.tail++] = make_uint4(id, p0, n, n * j.st.samples);
+        ++j.lane_left[l];
     }
-    j.tiles_left = s->tail - first;
+    j.tiles_left = count;
     {
         std::scoped_lock lk(mu);
-        j.tiles = j.tiles_left;
+        j.tiles = count;
     }
-    BT_TRY(hipMemcpyAsync(s->q_dev + first, s->q_host + first, sizeof(uint4) * (s->tail - first),
-                          hipMemcpyHostToDevice, stream));
-    mt::launch_set_u32(s->q_state + 1, s->tail, stream);
+    for (uint32_t l = 0; l < nl; ++l) {
+        SceneQueue& q = s->q[l];
+        if (q.tail == first[l]) continue;
+        BT_TRY(hipMemcpyAsync(q.q_dev + first[l], q.q_host + first[l], sizeof(uint4) * (q.tail - first[l]),
+                              hipMemcpyHostToDevice, stream));
+        mt::launch_set_u32(q.q_state + 1, q.tail, stream);
+    }
     BT_TRY(hipGetLastError());
+    // every lane sees the uploads before its next iteration
+    hipEvent_t up;
+    if (spare.empty()) {
+        BT_TRY(hipEventCreateWithFlags(&up, hipEventDisableTiming));
+    } else {
+        up = spare.back();
+        spare.pop_back();
+    }
+    BT_TRY(hipEventRecord(up, stream));
+    for (Lane& L : lanes) BT_TRY(hipStreamWaitEvent(L.stream, up, 0));
+    spare.push_back(up);  // reusable: a wait refers to the record at the time of the call
     s->bounces = std::max(s->bounces, j.st.bounces);
     return MT_OK;
 }
 
-int mt_batch::ensure_graph(SceneSlot& s) {
-    if (s.exec && s.graph_bounces >= s.bounces) return MT_OK;
-    if (s.exec) {
-        BT_TRY(hipStreamSynchronize(stream));
-        BT_TRY(hipGraphExecDestroy(s.exec));
-        BT_TRY(hipGraphDestroy(s.graph));
-        s.exec = nullptr;
-        s.graph = nullptr;
+int mt_batch::ensure_graph(SceneSlot& s, uint32_t lane) {
+    SceneQueue& q = s.q[lane];
+    if (q.exec && q.graph_bounces >= s.bounces) return MT_OK;
+    if (q.exec) {
+        BT_TRY(hipStreamSynchronize(lanes[lane].stream));
+        BT_TRY(hipGraphExecDestroy(q.exec));
+        BT_TRY(hipGraphDestroy(q.graph));
+        q.exec = nullptr;
+        q.graph = nullptr;
     }
     mt::BatchLaunch L{s.bounces, kLds, trace_blocks, shade_blocks};
     BT_TRY(hipStreamBeginCapture(capture, hipStreamCaptureModeRelaxed));
-    mt::launch_batch_iteration(s.scene, s.c, s.b, L, capture);
+    mt::launch_batch_iteration(s.scene, s.c, q.b, L, capture);
     const hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(capture, &g);
@@ -253,15 +308,16 @@ int mt_batch::ensure_graph(SceneSlot& s) {
         return fail(MT_E_HIP, std::string("graph capture failed: ") +
                                   hipGetErrorString(le != hipSuccess ? le : ce));
     }
-    s.graph = g;
-    BT_TRY(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
-    s.graph_bounces = s.bounces;
+    q.graph = g;
+    BT_TRY(hipGraphInstantiate(&q.exec, q.graph, nullptr, nullptr, 0));
+    q.graph_bounces = s.bounces;
     ++captures;
     return MT_OK;
 }
 
 int mt_batch::run() {
     if (hipSetDevice(device) != hipSuccess) return fail(MT_E_HIP, "hipSetDevice failed");
+    const uint32_t nl = static_cast<uint32_t>(lanes.size());
     while (true) {
         // admit everything submitted so far
         {
@@ -274,7 +330,8 @@ int mt_batch::run() {
             if (rc) return rc;
             ++admitted;
         }
-        // the oldest job that still has unscheduled tiles decides the scene
+        // the oldest job that still has unscheduled tiles decides the scene; its lanes with
+        // pending tiles take turns
         while (first_open < admitted && view[first_open]->tiles_left == 0) ++first_open;
         if (first_open == admitted) {
             std::unique_lock lk(mu);
@@ -283,27 +340,41 @@ int mt_batch::run() {
             continue;
         }
         SceneSlot& s = *slots[view[first_open]->slot];
-        int rc = ensure_graph(s);
+        uint32_t lane = nl;
+        for (uint32_t k = 0; k < nl; ++k) {
+            const uint32_t l = (next_lane + k) % nl;
+            if (s.q[l].head < s.q[l].tail) {
+                lane = l;
+                break;
+            }
+        }
+        next_lane = (lane + 1) % nl;
+        SceneQueue& q = s.q[lane];
+        Lane& L = lanes[lane];
+        int rc = ensure_graph(s, lane);
         if (rc) return rc;
-        BT_TRY(hipGraphLaunch(s.exec, stream));
+        BT_TRY(hipGraphLaunch(q.exec, L.stream));
         ++iterations;
-        // mirror of k_batch_schedule
+        // mirror of k_batch_schedule on this lane's queue
         uint32_t n = 0;
         uint64_t paths = 0;
-        while (s.head < s.tail && n < opt.iteration_tiles) {
-            const uint4 t = s.q_host[s.head];
+        while (q.head < q.tail && n < opt.iteration_tiles) {
+            const uint4 t = q.q_host[q.head];
             if (n > 0 && paths + t.w > opt.paths_per_iteration) break;
             paths += t.w;
             ++n;
-            ++s.head;
+            ++q.head;
             HostJob& j = *view[t.x];
-            if (--j.tiles_left == 0) {
-                BT_TRY(hipEventCreate(&j.done_ev));
-                BT_TRY(hipEventRecord(j.done_ev, stream));
-                j.recorded.store(true, std::memory_order_release);
+            --j.tiles_left;
+            if (--j.lane_left[lane] == 0) {
+                hipEvent_t ev;
+                BT_TRY(hipEventCreate(&ev));
+                BT_TRY(hipEventRecord(ev, L.stream));
+                j.done_evs.push_back(ev);
+                if (j.tiles_left == 0) j.recorded.store(true, std::memory_order_release);
             }
         }
-        // keep a few iterations queued, so that new jobs are admitted promptly
+        // keep a few iterations queued per lane, so that new jobs are admitted promptly
         hipEvent_t ev;
         if (spare.empty()) {
             BT_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -311,14 +382,15 @@ int mt_batch::run() {
             ev = spare.back();
             spare.pop_back();
         }
-        BT_TRY(hipEventRecord(ev, stream));
-        inflight.push_back(ev);
-        while (inflight.size() > kInflight) {
-            BT_TRY(hipEventSynchronize(inflight.front()));
-            spare.push_back(inflight.front());
-            inflight.pop_front();
+        BT_TRY(hipEventRecord(ev, L.stream));
+        L.inflight.push_back(ev);
+        while (L.inflight.size() > kInflight) {
+            BT_TRY(hipEventSynchronize(L.inflight.front()));
+            spare.push_back(L.inflight.front());
+            L.inflight.pop_front();
         }
     }
+    for (Lane& L : lanes) BT_TRY(hipStreamSynchronize(L.stream));
     BT_TRY(hipStreamSynchronize(stream));
     return MT_OK;
 }
@@ -443,10 +515,21 @@ int mt_batch_get_job_info(const mt_batch* batch, uint32_t job_id, mt_batch_job_i
     info->samples = j.samples;
     info->submit_ms = j.submit_ms;
     info->complete_ms = -1.0;
-    if (j.recorded.load(std::memory_order_acquire) && hipEventQuery(j.done_ev) == hipSuccess) {
-        float ms = 0.0f;
-        info->done = 1;
-        if (hipEventElapsedTime(&ms, b->e0, j.done_ev) == hipSuccess) info->complete_ms = b->t0_ms + ms;
+    if (j.recorded.load(std::memory_order_acquire)) {
+        bool all = true;
+        float latest = 0.0f;
+        for (hipEvent_t e : j.done_evs) {
+            float ms = 0.0f;
+            if (hipEventQuery(e) != hipSuccess || hipEventElapsedTime(&ms, b->e0, e) != hipSuccess) {
+                all = false;
+                break;
+            }
+            latest = std::max(latest, ms);
+        }
+        if (all) {
+            info->done = 1;
+            info->complete_ms = b->t0_ms + latest;
+        }
     }
     return MT_OK;
 }
@@ -498,7 +581,7 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
         if (n_trials) *n_trials = std::min(done, max_trials);
         return MT_OK;
     };
-    mt_batch_options cur{1u << 22, 1024, 4096, 8};
+    mt_batch_options cur{1u << 22, 1024, 4096, 8, 3};
     double cur_v = 0.0;
     int rc = measure(cur, cur_v);  // warm-up (first captures, first touches)
     if (rc) return rc;
@@ -507,6 +590,7 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
     const uint32_t tiles[] = {256, 1024, 4096, 16384};
     const uint32_t blocks[] = {4, 8, 12};
     const uint32_t itiles[] = {1024, 4096, 16384};
+    const uint32_t nstreams[] = {1, 2, 3, 4};
     auto sweep = [&](auto field, const uint32_t* vals, size_t nv) -> int {
         mt_batch_options b0 = cur;
         for (size_t i = 0; i < nv; ++i) {
@@ -526,7 +610,8 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
     if ((rc = sweep(&mt_batch_options::paths_per_iteration, paths, 4)) ||
         (rc = sweep(&mt_batch_options::tile_pixels, tiles, 4)) ||
         (rc = sweep(&mt_batch_options::trace_blocks_per_cu, blocks, 3)) ||
-        (rc = sweep(&mt_batch_options::iteration_tiles, itiles, 3)))
+        (rc = sweep(&mt_batch_options::iteration_tiles, itiles, 3)) ||
+        (rc = sweep(&mt_batch_options::streams, nstreams, 4)))
         return rc;
     *best = cur;
     return MT_OK;
