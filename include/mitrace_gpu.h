@@ -68,6 +68,32 @@ extern "C" {
 /* per-sample output */
 #define MT_OUTPUT_TONEMAPPED 0 /* ACES per sample before averaging (Tracer.cpp:212-227,346) */
 #define MT_OUTPUT_LINEAR 1     /* mean of linear radiance                                   */
+/* debug views: Config::DebugMode (src/CLI/Config.hpp:16-41) in the same order, rendered as
+ * ProcessDebugRay (src/Tracer/Tracer.cpp:366-488) and post-processed as
+ * ApplyDebugPostProcessing (Tracer.cpp:642-716) */
+#define MT_DEBUG_NONE 0
+#define MT_DEBUG_PRIMARY_BVH_TESTS 1
+#define MT_DEBUG_TOTAL_BVH_TESTS 2
+#define MT_DEBUG_PRIMARY_TRIANGLE_TESTS 3
+#define MT_DEBUG_TOTAL_TRIANGLE_TESTS 4
+#define MT_DEBUG_ALBEDO 5
+#define MT_DEBUG_METALLIC_ROUGHNESS 6
+#define MT_DEBUG_GEOMETRIC_NORMAL 7
+#define MT_DEBUG_SHADING_NORMAL 8
+#define MT_DEBUG_DIRECT_ONLY 9
+#define MT_DEBUG_INDIRECT_ONLY 10
+#define MT_DEBUG_EMISSION 11
+#define MT_DEBUG_COLOR_PER_MESH 12
+#define MT_DEBUG_COLOR_PER_TRIANGLE 13
+#define MT_DEBUG_COLOR_PER_BOUNDING_VOLUME 14
+#define MT_DEBUG_BOUNCES 15
+#define MT_DEBUG_DEPTH 16
+#define MT_DEBUG_FIRST_HIT_FRESNEL 17
+#define MT_DEBUG_FIRST_HIT_BRDF 18
+#define MT_DEBUG_FIRST_HIT_PDF 19
+#define MT_DEBUG_REFLECTED_DIRECTION 20
+#define MT_DEBUG_PIXEL_STANDARD_DEVIATION 21
+#define MT_DEBUG_FIREFLY_ELIMINATION 22
 
 /* ---- scene description (mirrors Scene::Mesh / MeshInstance / Material / Light / Camera) - */
 
@@ -192,6 +218,7 @@ typedef struct mt_settings {
     int32_t hdri_rotation_apply;  /* 0 = ignore rotation like the reference (Q21)     */
     uint32_t paths_in_flight;     /* wavefront size; 0 = automatic                    */
     int32_t count_work;           /* 1 = gather traversal work counters (slower)      */
+    int32_t debug_mode;           /* debugMode (Config.hpp:74), MT_DEBUG_*            */
 } mt_settings;
 
 /* Tracer::Stats (src/Tracer/Tracer.hpp:78-85) plus work counters. */

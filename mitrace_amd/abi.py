@@ -6,6 +6,14 @@ check the sizes against the compiled library.
 import ctypes as C
 
 MT_OK = 0
+(MT_DEBUG_NONE, MT_DEBUG_PRIMARY_BVH_TESTS, MT_DEBUG_TOTAL_BVH_TESTS,
+ MT_DEBUG_PRIMARY_TRIANGLE_TESTS, MT_DEBUG_TOTAL_TRIANGLE_TESTS, MT_DEBUG_ALBEDO,
+ MT_DEBUG_METALLIC_ROUGHNESS, MT_DEBUG_GEOMETRIC_NORMAL, MT_DEBUG_SHADING_NORMAL,
+ MT_DEBUG_DIRECT_ONLY, MT_DEBUG_INDIRECT_ONLY, MT_DEBUG_EMISSION, MT_DEBUG_COLOR_PER_MESH,
+ MT_DEBUG_COLOR_PER_TRIANGLE, MT_DEBUG_COLOR_PER_BOUNDING_VOLUME, MT_DEBUG_BOUNCES,
+ MT_DEBUG_DEPTH, MT_DEBUG_FIRST_HIT_FRESNEL, MT_DEBUG_FIRST_HIT_BRDF, MT_DEBUG_FIRST_HIT_PDF,
+ MT_DEBUG_REFLECTED_DIRECTION, MT_DEBUG_PIXEL_STANDARD_DEVIATION,
+ MT_DEBUG_FIREFLY_ELIMINATION) = range(23)
 MT_E_INVALID = -1
 MT_E_HIP = -2
 MT_E_NOMEM = -3
@@ -104,7 +112,7 @@ class mt_settings(C.Structure):
                 ("mode", C.c_int32), ("rr_depth", C.c_uint32), ("output", C.c_int32),
                 ("hdri_rotation_apply", C.c_int32),
                 ("paths_in_flight", C.c_uint32),
-                ("count_work", C.c_int32)]
+                ("count_work", C.c_int32), ("debug_mode", C.c_int32)]
 
 
 class mt_stats(C.Structure):

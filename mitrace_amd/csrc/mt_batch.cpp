@@ -406,6 +406,10 @@ bool valid_job(mt_scene* scene, const mt_camera_desc* cam, const mt_settings* st
         mt::set_error("width, height and samples must be positive");
         return false;
     }
+    if (st->debug_mode) {
+        mt::set_error("debug views are rendered by mt_render / mt_render_start");
+        return false;
+    }
     if (st->bounces > mt::kMaxBounces || st->samples > cap ||
         static_cast<uint64_t>(st->width) * st->height * 3 > 0xFFFFFFFFull) {
         mt::set_error("job exceeds the batch limits (bounces <= 64, samples <= paths per iteration)");

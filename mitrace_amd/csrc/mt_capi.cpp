@@ -308,6 +308,8 @@ int mt_diff_create(mt_scene* scene, const mt_settings* settings, mt_diff** out) 
     if (!scene || !settings) return mt::fail(MT_E_INVALID, "null scene or settings");
     if (settings->bounces > 32)
         return mt::fail(MT_E_UNSUPPORTED, "differentiable rendering supports at most 32 bounces");
+    if (settings->debug_mode)
+        return mt::fail(MT_E_INVALID, "debug views are rendered by mt_render / mt_render_start");
     try {
         auto d = std::make_unique<mt_diff>();
         d->scene = scene;

@@ -114,6 +114,10 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
     uint32_t pix, s;
     path_coords(c, p, pix, s);
     init_path(c, b, p, pix, s);
+    if (c.debug) {
+        b.dbg_i[p] = make_uint4(0, 0, 0, 0);
+        b.dbg_j[p] = make_uint4(0, 0, 0, 0);
+    }
     if (c.cache) {  // one path in train_every is a training path (the set moves every pass)
         const uint32_t q = p + c.train_offset;
         uint32_t slot = kNoSlot;
@@ -358,6 +362,19 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                     b.hit_tri[p] = h.tri;
                     b.hit_uv[p] = make_float2(h.u, h.v);
                     b.hit_pos[p] = f4(h.wpos, h.dist);
+                    if (c.debug && h.inst >= 0) {  // the winning instance's tests (Tracer.cpp:170-175)
+                        uint4 di = b.dbg_i[p];
+                        if (bounce == 0) {
+                            di.x = h.bvh_tests;
+                            di.y = h.tri_tests;
+                            uint4 dj = b.dbg_j[p];
+                            dj.z = h.bv & 0xffffu;
+                            b.dbg_j[p] = dj;
+                        }
+                        di.z += h.bvh_tests;
+                        di.w += h.tri_tests;
+                        b.dbg_i[p] = di;
+                    }
                 } else if (kMode == 1) {
                     const float4 so = b.sh_o[item];
                     if (h.inst < 0 || h.dist > so.w) {  // Tracer.cpp:322
@@ -596,6 +613,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
             const V3 d = xyz(b.ray_d[p]);
             const int inst = b.hit_inst[p];
             const V3 trad = xyz(b.t_rad[p]);
+            if (kDiff == 0 && !kBatch && c.debug) {  // one more path step; the second one's ray
+                uint4 dj = b.dbg_j[p];
+                dj.w += 4u;
+                if (k == 1) {
+                    dj.w |= 2u;
+                    b.dbg_e[p] = f4(d, b.dbg_e[p].w);
+                }
+                b.dbg_j[p] = dj;
+            }
             if (inst < 0) {  // miss: sky term (Tracer.cpp:244-252)
                 misses++;
                 const V3 col = (k > 0) ? xyz(b.prev_base[p]) : mk(1.0f);
@@ -693,6 +719,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 const V3 fb = eval_brdf(nd, view, N, basec, rough, metal);
                 const float nl = gmax(dot(N, nd), 0.0f);
                 V3 energy = (pdf > 0.0f) ? scale(fb, nl / pdf) : mk(0.0f);
+                if (kDiff == 0 && !kBatch && c.debug && k == 0) {  // first-hit debug record
+                    const V3 fr = fresnel(gmax(dot(N, view), 0.0f), base_refl(basec, metal));
+                    b.dbg_a[p] = f4(basec, metal);
+                    b.dbg_b[p] = make_float4(rough, gN.x, gN.y, gN.z);
+                    b.dbg_c[p] = f4(N, len(sub(wpos, o)));
+                    b.dbg_d[p] = f4(fr, (fb.x + fb.y) + fb.z);
+                    b.dbg_e[p] = make_float4(0.0f, 0.0f, 0.0f, pdf);
+                    uint4 dj = b.dbg_j[p];
+                    dj.x = static_cast<uint32_t>(inst) & 0xffffu;
+                    dj.y = tri & 0xffffu;
+                    dj.w |= 1u;
+                    b.dbg_j[p] = dj;
+                }
                 // termination bookkeeping of GeneratePath (Tracer.cpp:206-208)
                 V3 energy_t = energy;
                 if (frozen)
@@ -854,6 +893,55 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
     }
 }
 
+// RandomColorFromU16 (Tracer.cpp:350-364)
+__device__ __forceinline__ V3 color_from_u16(uint32_t value) {
+    uint32_t x = value & 0xffffu;
+    x ^= x >> 7;
+    x *= 0x2C1B3C6Du;
+    x ^= x >> 12;
+    x *= 0x297A2D39u;
+    x ^= x >> 15;
+    const uint32_t v = x & 0xffffu;
+    return {static_cast<float>((v & 0x01C0u) >> 6) / 7.0f, static_cast<float>((v & 0x0038u) >> 3) / 7.0f,
+            static_cast<float>(v & 0x0007u) / 7.0f};
+}
+
+// ProcessDebugRay (Tracer.cpp:366-488) from the path's debug records; `total` is the sample's
+// regular colour, kept by the modes the function does not handle
+__device__ V3 debug_color(const ChunkParams& c, const PathBuffers& b, uint32_t p, V3 e, V3 d, V3 i,
+                          V3 s, V3 total) {
+    const uint4 di = b.dbg_i[p], dj = b.dbg_j[p];
+    const bool hit = dj.w & 1u;
+    const V3 black = mk(0.0f);
+    switch (c.debug) {
+        case MT_DEBUG_PRIMARY_BVH_TESTS: return hit ? mk(static_cast<float>(di.x)) : black;
+        case MT_DEBUG_TOTAL_BVH_TESTS: return hit ? mk(static_cast<float>(di.z)) : black;
+        case MT_DEBUG_PRIMARY_TRIANGLE_TESTS: return hit ? mk(static_cast<float>(di.y)) : black;
+        case MT_DEBUG_TOTAL_TRIANGLE_TESTS: return hit ? mk(static_cast<float>(di.w)) : black;
+        case MT_DEBUG_BOUNCES: return hit ? mk(static_cast<float>(dj.w >> 2)) : black;
+        case MT_DEBUG_ALBEDO: return hit ? xyz(b.dbg_a[p]) : black;
+        case MT_DEBUG_METALLIC_ROUGHNESS: return hit ? V3{0.0f, b.dbg_a[p].w, b.dbg_b[p].x} : black;
+        case MT_DEBUG_GEOMETRIC_NORMAL: {
+            const float4 q = b.dbg_b[p];
+            return hit ? add(scale(V3{q.y, q.z, q.w}, 0.5f), mk(0.5f)) : black;
+        }
+        case MT_DEBUG_SHADING_NORMAL: return hit ? add(scale(xyz(b.dbg_c[p]), 0.5f), mk(0.5f)) : black;
+        case MT_DEBUG_DIRECT_ONLY: return aces(d, c.exposure);
+        case MT_DEBUG_INDIRECT_ONLY: return aces(add(i, s), c.exposure);
+        case MT_DEBUG_EMISSION: return aces(e, c.exposure);
+        case MT_DEBUG_COLOR_PER_MESH: return hit ? color_from_u16(dj.x) : black;
+        case MT_DEBUG_COLOR_PER_TRIANGLE: return hit ? color_from_u16(dj.y) : black;
+        case MT_DEBUG_COLOR_PER_BOUNDING_VOLUME: return hit ? color_from_u16(dj.z) : black;
+        case MT_DEBUG_DEPTH: return hit ? mk(b.dbg_c[p].w) : black;
+        case MT_DEBUG_FIRST_HIT_FRESNEL: return hit ? xyz(b.dbg_d[p]) : black;
+        case MT_DEBUG_FIRST_HIT_BRDF: return hit ? mk(b.dbg_d[p].w) : black;
+        case MT_DEBUG_FIRST_HIT_PDF: return hit ? mk(b.dbg_e[p].w) : black;
+        case MT_DEBUG_REFLECTED_DIRECTION:
+            return (dj.w & 2u) ? add(scale(xyz(b.dbg_e[p]), 0.5f), mk(0.5f)) : black;
+        default: return total;
+    }
+}
+
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers b) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -861,7 +949,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers 
     const ChunkParams& J = job_params<kBatch>(c, b, p);
     const V3 e = xyz(b.acc_e[p]), d = xyz(b.acc_d[p]), i = xyz(b.acc_i[p]), s = xyz(b.acc_s[p]);
     const V3 L = add(add(add(e, d), i), s);  // Tracer.cpp:344-345
-    const V3 col = J.output_linear ? L : aces(L, J.exposure);
+    V3 col = J.output_linear ? L : aces(L, J.exposure);
+    if (!kBatch && c.debug) col = debug_color(c, b, p, e, d, i, s, col);
     b.sample_rgb[p] = f4(col, 0.0f);
     if (!kBatch && c.diff) {
         uint32_t pix, s;
@@ -890,7 +979,12 @@ __device__ __forceinline__ void resolve_pixel(const ChunkParams& J, const float4
             var = var + dd * dd;
         }
         var = var / static_cast<float>(n);
-        thr = J.firefly_threshold * __builtin_sqrtf(var);
+        const float sd = __builtin_sqrtf(var);
+        if (J.debug == MT_DEBUG_PIXEL_STANDARD_DEVIATION) {  // Tracer.cpp:515-523
+            out[0] = out[1] = out[2] = sd;
+            return;
+        }
+        thr = J.firefly_threshold * sd;
     }
     V3 acc = mk(0.0f);
     for (uint32_t s = 0; s < n; ++s) {
@@ -901,6 +995,10 @@ __device__ __forceinline__ void resolve_pixel(const ChunkParams& J, const float4
             cnt--;
         }
         acc = add(acc, col);
+    }
+    if (J.firefly && J.debug == MT_DEBUG_FIREFLY_ELIMINATION) {  // Tracer.cpp:536-544
+        out[0] = out[1] = out[2] = static_cast<float>(n - cnt);
+        return;
     }
     if (cnt == 0) cnt = 1;
     const V3 r = divs(acc, static_cast<float>(cnt));
@@ -1211,6 +1309,7 @@ void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& ca
     c.firefly = st.firefly_enable;
     c.firefly_threshold = st.firefly_threshold;
     c.count_work = st.count_work;
+    c.debug = st.debug_mode;
     c.stack_cap = stack_capacity(scene);
     c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
     c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
@@ -1240,7 +1339,7 @@ void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
 }
 
 int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, uint64_t spill_entries,
-                       bool diff, bool batch) {
+                       bool diff, bool batch, bool debug) {
     int rc;
     if ((rc = alloc(owned, &b.ray_o, P)) || (rc = alloc(owned, &b.ray_d, P)) ||
         (rc = alloc(owned, &b.t_rad, P)) || (rc = alloc(owned, &b.t_term, P)) ||
@@ -1259,6 +1358,11 @@ int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, ui
         return rc;
     if (batch && ((rc = alloc(owned, &b.path_job, P)) || (rc = alloc(owned, &b.path_pix, P)) ||
                   (rc = alloc(owned, &b.path_smp, P))))
+        return rc;
+    if (debug && ((rc = alloc(owned, &b.dbg_a, P)) || (rc = alloc(owned, &b.dbg_b, P)) ||
+                  (rc = alloc(owned, &b.dbg_c, P)) || (rc = alloc(owned, &b.dbg_d, P)) ||
+                  (rc = alloc(owned, &b.dbg_e, P)) || (rc = alloc(owned, &b.dbg_i, P)) ||
+                  (rc = alloc(owned, &b.dbg_j, P))))
         return rc;
     return MT_OK;
 }
@@ -1294,7 +1398,7 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     const size_t trace_threads = static_cast<size_t>(trace_blocks_) * kBlock;
     PathBuffers& b = buf_;
     int rc;
-    if ((rc = alloc_path_buffers(owned_, b, P, spill_depth * trace_threads, diff, false)) ||
+    if ((rc = alloc_path_buffers(owned_, b, P, spill_depth * trace_threads, diff, false, st.debug_mode != 0)) ||
         (rc = alloc(owned_, &img_, npix_total * 3)))
         return rc;
     if (diff && (rc = alloc(owned_, &store_, npix_total * st.samples))) return rc;
@@ -1330,7 +1434,9 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
         slots_.push_back(L);  // registered first, so that release() frees what follows
         StreamSlot& S = slots_.back();
-        if ((rc = alloc_path_buffers(owned_, S.b, P, spill_depth * trace_threads, false, false))) return rc;
+        if ((rc = alloc_path_buffers(owned_, S.b, P, spill_depth * trace_threads, false, false,
+                                     st.debug_mode != 0)))
+            return rc;
         HIP_TRY(hipMemsetAsync(S.b.work, 0, sizeof(unsigned long long) * 16, S.stream));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&S.host_counters), sizeof(uint32_t) * kMaxCounters));
         for (uint32_t k = 0; k < 2 + 4 * st.bounces; ++k) {
@@ -1522,6 +1628,7 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
         return fail(MT_E_INVALID, "samples must be a multiple of spp_per_pass");
     if (copt_.query_depth >= st.bounces)
         return fail(MT_E_INVALID, "query_depth must be below the bounce count");
+    if (st.debug_mode) return fail(MT_E_INVALID, "debug views are rendered by mt_render");
     total_spp_ = st.samples;
     mt_settings pass = st;
     pass.samples = copt_.spp_per_pass;
@@ -1683,8 +1790,45 @@ int Renderer::finish(float* rgb_out) {
         stats_.hits_shaded = w[4];
         stats_.misses = w[5];
     }
-    if (rgb_out) HIP_TRY(hipMemcpy(rgb_out, img_, sizeof(float) * npix_total * 3, hipMemcpyDeviceToHost));
+    if (rgb_out) {
+        HIP_TRY(hipMemcpy(rgb_out, img_, sizeof(float) * npix_total * 3, hipMemcpyDeviceToHost));
+        if (st_.debug_mode) debug_post(st_.debug_mode, rgb_out, npix_total);
+    }
     return MT_OK;
+}
+
+// ApplyDebugPostProcessing (Tracer.cpp:642-716): a heat map (colourful or grey) of
+// (r+g+b)/3 over the image's maximum, for the counting and measuring views
+void debug_post(int mode, float* rgb, uint64_t npix) {
+    bool colorful;
+    switch (mode) {
+        case MT_DEBUG_PRIMARY_BVH_TESTS: case MT_DEBUG_TOTAL_BVH_TESTS:
+        case MT_DEBUG_PRIMARY_TRIANGLE_TESTS: case MT_DEBUG_TOTAL_TRIANGLE_TESTS:
+        case MT_DEBUG_BOUNCES: case MT_DEBUG_FIREFLY_ELIMINATION:
+            colorful = true;
+            break;
+        case MT_DEBUG_DEPTH: case MT_DEBUG_FIRST_HIT_BRDF: case MT_DEBUG_PIXEL_STANDARD_DEVIATION:
+            colorful = false;
+            break;
+        default:
+            return;
+    }
+    static const float ramp[5][3] = {{0, 0, 1}, {0, 1, 1}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}};
+    float mx = 0.0f;
+    for (uint64_t i = 0; i < npix; ++i) mx = std::max(mx, (rgb[i * 3] + rgb[i * 3 + 1] + rgb[i * 3 + 2]) / 3.0f);
+    for (uint64_t i = 0; i < npix; ++i) {
+        const float in = (rgb[i * 3] + rgb[i * 3 + 1] + rgb[i * 3 + 2]) / 3.0f;
+        float t = mx > 0.0f ? in / mx : 0.0f;
+        if (!colorful) {
+            rgb[i * 3] = rgb[i * 3 + 1] = rgb[i * 3 + 2] = t;
+            continue;
+        }
+        t = std::min(std::max(t, 0.0f), 1.0f);
+        const float x = t * 4.0f;
+        const int k = std::min(static_cast<int>(x), 3);
+        const float f = x - static_cast<float>(k);
+        for (int ch = 0; ch < 3; ++ch) rgb[i * 3 + ch] = ramp[k][ch] * (1.0f - f) + ramp[k + 1][ch] * f;
+    }
 }
 
 }  // namespace mt

@@ -77,6 +77,7 @@ struct ChunkParams {
     // cache_depth and take the rest of their radiance from the cache
     int32_t cache;
     uint32_t cache_depth, train_every, train_offset, train_cap, train_nrec;
+    int32_t debug;  // MT_DEBUG_* view (plain renders)
 };
 
 // SoA path state, queues and scratch of one render (capacity = paths per chunk)
@@ -117,6 +118,12 @@ struct PathBuffers {
     float4 *cq_pos, *cq_dir, *cq_out;
     float4 *tp_pos, *tp_dir, *tp_tgt;       // compact training samples
     float* img_sum;         // per pixel: sum of the accumulated passes' samples
+    // debug views (first-hit records of ProcessDebugRay)
+    float4 *dbg_a, *dbg_b, *dbg_c, *dbg_d, *dbg_e;  // {base', metal} {rough, geom normal}
+                                                     // {normal, depth} {Fresnel, brdf sum}
+                                                     // {second ray direction, pdf}
+    uint4 *dbg_i, *dbg_j;   // {primary bvh, primary tri, total bvh, total tri tests}
+                            // {instance, triangle, leaf (all & 0xffff), bounces<<2 | dir<<1 | hit}
 };
 
 // one job of a batch, resident in device memory
@@ -209,12 +216,14 @@ int cu_count();
 void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& cam,
                  const mt_settings& st);
 int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t paths,
-                       uint64_t spill_entries, bool diff, bool batch);
+                       uint64_t spill_entries, bool diff, bool batch, bool debug = false);
 // enqueue one batched iteration (schedule, generate, bounces, finalize, resolve): no host
 // synchronisation and no memset, so that it can be captured into a graph
 void launch_batch_iteration(const mt_scene* scene, const ChunkParams& c, const PathBuffers& b,
                             const BatchLaunch& launch, hipStream_t s);
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+// the heat-map post-process of the counting and measuring debug views (host, in place)
+void debug_post(int mode, float* rgb, uint64_t npix);
 int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits);
 int brdf_eval(uint32_t n, const float* in, float* out);
 

@@ -233,6 +233,7 @@ class Settings:
     hdri_rotation_apply: bool = False
     paths_in_flight: int = 0
     count_work: bool = False
+    debug_mode: int = 0  # MT_DEBUG_* (Config::DebugMode)
 
     def to_desc(self):
         s = abi.mt_settings()

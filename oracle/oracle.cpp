@@ -710,17 +710,25 @@ struct Settings {
     float expo;
 };
 
-struct PathStep {  // Tracer.hpp:34-56 (the fields the estimator reads)
+struct PathStep {  // Tracer.hpp:34-56 (the fields the estimator and the debug views read)
     Ray ray;
     V3 hitPos, base, geomNormal, normal, energy, emission;
     float metallic, roughness;
+    float brdfIntensity = 0.0f, pdf = 0.0f;
+    uint32_t bvhTests = 0, triangleTests = 0;
+    uint32_t triangleIndex = 0, bvIndex = 0, meshIndex = 0;  // & 0xffff
     bool didHit;
+};
+
+struct LightOut {  // Tracer.hpp LightOutput: the four sums before their total
+    V3 emitted, direct, indirect, sky;
 };
 
 struct ShadeOut {
     V3 dir, energy;
     float pdf;
     bool spec;
+    float brdfIntensity;
 };
 
 // the BRDF-sampling half of one GeneratePath iteration (Tracer.cpp:177-200)
@@ -746,6 +754,7 @@ ShadeOut sampleBounce(V3 view, V3 n, V3 base, float rough, float metal, Rng& rng
     const V3 f = evalBrdf(o.dir, view, n, base, rough, metal);
     const float nl = gmax(dot(n, o.dir), 0.0f);
     o.energy = (o.pdf > 0.0f) ? f * (nl / o.pdf) : v3(0.0f);
+    o.brdfIntensity = (f.x + f.y) + f.z;  // brdf.r + brdf.g + brdf.b (Tracer.cpp:203)
     return o;
 }
 
@@ -788,7 +797,7 @@ struct Tracer {
 
     // GeneratePath (Tracer.cpp:134-210) + ProcessRayForward (229-348) for one camera ray.
     V3 forward(const Ray& camRay, Rng& rng, std::vector<PathStep>& path, Counters* c,
-               uint64_t& rays, uint64_t& hits, uint64_t& misses) const {
+               uint64_t& rays, uint64_t& hits, uint64_t& misses, LightOut* parts = nullptr) const {
         const mt_settings& st = cfg.s;
         path.clear();
         Ray cur = camRay;
@@ -834,8 +843,15 @@ struct Tracer {
             step.normal = computeNormal(gN, ntex);
             step.emission = emission;
             step.didHit = true;
+            step.bvhTests = h.bvhTests;  // of the winning instance (Tracer.cpp:170-175)
+            step.triangleTests = h.triTests;
+            step.bvIndex = h.bv & 0xffffu;
+            step.triangleIndex = h.tri & 0xffffu;
+            step.meshIndex = static_cast<uint32_t>(h.inst) & 0xffffu;  // instance, not a pointer
             const V3 view = -cur.d;
             const ShadeOut so = sampleBounce(view, step.normal, base, rough, metal, rng);
+            step.brdfIntensity = so.brdfIntensity;
+            step.pdf = so.pdf;
             Ray nr;
             nr.o = h.o + h.d * h.dist + 0.0001f * step.normal;  // ReflectSpecular/Diffuse
             nr.d = so.dir;
@@ -923,9 +939,102 @@ struct Tracer {
             }
             cE = cE * step.energy;
         }
+        if (parts) *parts = LightOut{emitted, direct, indirect, sky};
         return ((emitted + direct) + indirect) + sky;
     }
 };
+
+// RandomColorFromU16 (Tracer.cpp:350-364)
+V3 colorFromU16(uint32_t value) {
+    uint32_t x = value & 0xffffu;
+    x ^= x >> 7;
+    x *= 0x2C1B3C6Du;
+    x ^= x >> 12;
+    x *= 0x297A2D39u;
+    x ^= x >> 15;
+    const uint32_t v = x & 0xffffu;
+    return {static_cast<float>((v & 0x01C0u) >> 6) / 7.0f, static_cast<float>((v & 0x0038u) >> 3) / 7.0f,
+            static_cast<float>(v & 0x0007u) / 7.0f};
+}
+
+// ProcessDebugRay (Tracer.cpp:366-488) for a path already traced; `total` is the sample's
+// regular colour (used by the modes the function does not handle)
+V3 debugColor(int mode, const std::vector<PathStep>& path, const LightOut& L, const Ray& ray,
+              V3 total, float expo) {
+    const bool hit = !path.empty() && path[0].didHit;
+    const V3 black = v3(0.0f);
+    switch (mode) {
+        case MT_DEBUG_PRIMARY_BVH_TESTS: return hit ? v3(static_cast<float>(path[0].bvhTests)) : black;
+        case MT_DEBUG_TOTAL_BVH_TESTS: {
+            float sum = 0.0f;
+            for (const PathStep& st : path) sum += static_cast<float>(st.didHit ? st.bvhTests : 0u);
+            return hit ? v3(sum) : black;
+        }
+        case MT_DEBUG_PRIMARY_TRIANGLE_TESTS: return hit ? v3(static_cast<float>(path[0].triangleTests)) : black;
+        case MT_DEBUG_TOTAL_TRIANGLE_TESTS: {
+            float sum = 0.0f;
+            for (const PathStep& st : path) sum += static_cast<float>(st.didHit ? st.triangleTests : 0u);
+            return hit ? v3(sum) : black;
+        }
+        case MT_DEBUG_BOUNCES: return hit ? v3(static_cast<float>(path.size())) : black;
+        case MT_DEBUG_ALBEDO: return hit ? path[0].base : black;
+        case MT_DEBUG_METALLIC_ROUGHNESS: return hit ? V3{0.0f, path[0].metallic, path[0].roughness} : black;
+        case MT_DEBUG_GEOMETRIC_NORMAL: return hit ? path[0].geomNormal * 0.5f + v3(0.5f) : black;
+        case MT_DEBUG_SHADING_NORMAL: return hit ? path[0].normal * 0.5f + v3(0.5f) : black;
+        case MT_DEBUG_DIRECT_ONLY: return aces(L.direct, expo);
+        case MT_DEBUG_INDIRECT_ONLY: return aces(L.indirect + L.sky, expo);
+        case MT_DEBUG_EMISSION: return aces(L.emitted, expo);
+        case MT_DEBUG_COLOR_PER_MESH: return hit ? colorFromU16(path[0].meshIndex) : black;
+        case MT_DEBUG_COLOR_PER_TRIANGLE: return hit ? colorFromU16(path[0].triangleIndex) : black;
+        case MT_DEBUG_COLOR_PER_BOUNDING_VOLUME: return hit ? colorFromU16(path[0].bvIndex) : black;
+        case MT_DEBUG_DEPTH: return hit ? v3(length(path[0].hitPos - ray.o)) : black;
+        case MT_DEBUG_FIRST_HIT_FRESNEL: {
+            if (!hit) return black;
+            const V3 f0 = baseRefl(path[0].base, path[0].metallic);
+            return fresnel(gmax(dot(path[0].normal, -ray.d), 0.0f), f0);
+        }
+        case MT_DEBUG_FIRST_HIT_BRDF: return hit ? v3(path[0].brdfIntensity) : black;
+        case MT_DEBUG_FIRST_HIT_PDF: return hit ? v3(path[0].pdf) : black;
+        case MT_DEBUG_REFLECTED_DIRECTION: return path.size() > 1 ? path[1].ray.d * 0.5f + v3(0.5f) : black;
+        default: return total;
+    }
+}
+
+// ApplyDebugPostProcessing (Tracer.cpp:642-716): heat map of (r+g+b)/3 over the image maximum
+V3 heatmap(float t) {
+    t = clampf(t, 0.0f, 1.0f);
+    static const V3 colors[5] = {{0, 0, 1}, {0, 1, 1}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}};
+    const float x = t * 4.0f;
+    const int i = std::min(static_cast<int>(x), 3);
+    const float f = x - static_cast<float>(i);
+    return {mixf(colors[i].x, colors[i + 1].x, f), mixf(colors[i].y, colors[i + 1].y, f),
+            mixf(colors[i].z, colors[i + 1].z, f)};
+}
+void debugPost(int mode, float* rgb, size_t npix) {
+    bool colorful;
+    switch (mode) {
+        case MT_DEBUG_PRIMARY_BVH_TESTS: case MT_DEBUG_TOTAL_BVH_TESTS:
+        case MT_DEBUG_PRIMARY_TRIANGLE_TESTS: case MT_DEBUG_TOTAL_TRIANGLE_TESTS:
+        case MT_DEBUG_BOUNCES: case MT_DEBUG_FIREFLY_ELIMINATION:
+            colorful = true;
+            break;
+        case MT_DEBUG_DEPTH: case MT_DEBUG_FIRST_HIT_BRDF: case MT_DEBUG_PIXEL_STANDARD_DEVIATION:
+            colorful = false;
+            break;
+        default:
+            return;
+    }
+    float mx = 0.0f;
+    for (size_t i = 0; i < npix; ++i) mx = std::max(mx, (rgb[i * 3] + rgb[i * 3 + 1] + rgb[i * 3 + 2]) / 3.0f);
+    for (size_t i = 0; i < npix; ++i) {
+        const float in = (rgb[i * 3] + rgb[i * 3 + 1] + rgb[i * 3 + 2]) / 3.0f;
+        const float t = mx > 0.0f ? in / mx : 0.0f;
+        const V3 c = colorful ? heatmap(t) : v3(t);
+        rgb[i * 3] = c.x;
+        rgb[i * 3 + 1] = c.y;
+        rgb[i * 3 + 2] = c.z;
+    }
+}
 
 // ---------------------------------------------------------------- differentiable estimator
 // Forward-mode (tangent) derivative of the per-sample radiance along one direction in parameter
@@ -1228,7 +1337,7 @@ Ray cameraRay(const mt_camera_desc& cam, float u, float v, float aspect) {
     return r;
 }
 
-uint32_t cleanupFireflies(std::vector<V3>& buf, float thr) {  // Tracer.cpp:490-548
+uint32_t cleanupFireflies(std::vector<V3>& buf, float thr, int debugMode = 0) {  // Tracer.cpp:490-548
     auto inten = [](V3 c) { return (c.x + c.y + c.z) / 3.0f; };
     size_t n = buf.size();
     float sum = 0.0f;
@@ -1241,12 +1350,20 @@ uint32_t cleanupFireflies(std::vector<V3>& buf, float thr) {  // Tracer.cpp:490-
     }
     var = var / static_cast<float>(n);
     const float sd = std::sqrt(var);
+    if (debugMode == MT_DEBUG_PIXEL_STANDARD_DEVIATION) {  // Tracer.cpp:515-523
+        for (V3& c : buf) c = v3(sd);
+        return static_cast<uint32_t>(buf.size());
+    }
     const float t = thr * sd;
     for (V3& c : buf) {
         if (std::fabs(inten(c) - avg) > t) {
             c = v3(0.0f);
             n--;
         }
+    }
+    if (debugMode == MT_DEBUG_FIREFLY_ELIMINATION) {  // Tracer.cpp:536-544
+        for (V3& c : buf) c = v3(static_cast<float>(buf.size() - n));
+        return static_cast<uint32_t>(buf.size());
     }
     return n == 0 ? 1u : static_cast<uint32_t>(n);
 }
@@ -1685,9 +1802,11 @@ int orc_render(const void* sp, const mt_camera_desc* cam, const mt_settings* st,
                         const float ju = bu + rng.uRange(0, 0.0f, pxW);
                         const float jv = bv + rng.uRange(1, 0.0f, pxH);
                         const Ray r = cameraRay(*cam, ju, jv, aspect);
-                        V3 L = tr.forward(r, rng, path, wantCounters ? &c : nullptr, lr, lh, lm);
+                        LightOut parts;
+                        V3 L = tr.forward(r, rng, path, wantCounters ? &c : nullptr, lr, lh, lm, &parts);
                         ls++;
-                        const V3 col = (st->output == MT_OUTPUT_LINEAR) ? L : aces(L, tr.cfg.expo);
+                        V3 col = (st->output == MT_OUTPUT_LINEAR) ? L : aces(L, tr.cfg.expo);
+                        if (st->debug_mode) col = debugColor(st->debug_mode, path, parts, r, col, tr.cfg.expo);
                         sbuf.push_back(col);
                         if (sample_rgb_out) {
                             float* q = sample_rgb_out + (static_cast<size_t>(pix) * spp + s) * 3;
@@ -1697,7 +1816,7 @@ int orc_render(const void* sp, const mt_camera_desc* cam, const mt_settings* st,
                         }
                     }
                     uint32_t cnt = spp;
-                    if (st->firefly_enable) cnt = cleanupFireflies(sbuf, st->firefly_threshold);
+                    if (st->firefly_enable) cnt = cleanupFireflies(sbuf, st->firefly_threshold, st->debug_mode);
                     V3 acc = v3(0.0f);
                     for (const V3& c2 : sbuf) acc = acc + c2;
                     const V3 col = acc / static_cast<float>(cnt);
@@ -1723,6 +1842,7 @@ int orc_render(const void* sp, const mt_camera_desc* cam, const mt_settings* st,
     for (int i = 0; i < nt; ++i) th.emplace_back(worker);
     for (auto& t : th) t.join();
     const auto t1 = std::chrono::steady_clock::now();
+    if (st->debug_mode) debugPost(st->debug_mode, rgb_out, static_cast<size_t>(W) * H);
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     if (work) {
         work->rays = rays;
