@@ -60,7 +60,8 @@ def targets(gs, truth, cams, spp=1024, width=512, height=512):
     return out
 
 
-def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024, decorrelated=True):
+def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024, decorrelated=True,
+             tex_lr=None):
     """decorrelated: the residual (image - target) comes from an independent render (another
     seed) than the one whose adjoint is taken.  With one render for both, the gradient of the
     squared error carries 2 Cov(I, dI/dtheta), which for a base colour (I roughly linear in it)
@@ -76,7 +77,9 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024, decorrel
     tex = dev["tex"].clone().requires_grad_(True)
     rough = dev["rough"].clone().requires_grad_(True)
     metal, emis = dev["metal"], dev["emis"]
-    opt = torch.optim.Adam([tex, rough], lr=lr)
+    # Adam normalises every texel's step to about lr whatever its gradient's signal-to-noise
+    # ratio, so texels seen by few noisy pixels random-walk; tex_lr sets their own step size
+    opt = torch.optim.Adam([{"params": [tex], "lr": tex_lr or lr}, {"params": [rough], "lr": lr}])
     gt = to_dev(truth)
     log = []
 
@@ -143,7 +146,8 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024, decorrel
         seen_rmse = lambda t: float(torch.sqrt(torch.mean(((t - gt["tex"])[m]) ** 2)))
         seen_frac = float(m.float().mean())
         s0, s1 = seen_rmse(init_t), seen_rmse(tex)
-    return {"iters": iters, "spp": spp, "poses": len(cams), "lr": lr, "decorrelated": decorrelated,
+    return {"iters": iters, "spp": spp, "poses": len(cams), "lr": lr, "tex_lr": tex_lr or lr,
+            "decorrelated": decorrelated,
             "targets_spp": target_spp, "targets_s": t_targets,
             "initial": {"tex_rmse": e0[0], "tex_rmse_observed": s0, "rough_mae": e0[1],
                         "image_rmse": img0},
@@ -236,11 +240,12 @@ def main(argv=None):
     ap.add_argument("--spp", type=int, default=32)
     ap.add_argument("--crn-spp", type=int, default=4096)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--tex-lr", type=float, default=None, help="Adam step for the texture")
     ap.add_argument("--correlated", action="store_true",
                     help="one render per pose for both the residual and the adjoint")
     a = ap.parse_args(argv)
     if a.mode == "optimize":
-        out = optimize(iters=a.iters, spp=a.spp, decorrelated=not a.correlated)
+        out = optimize(iters=a.iters, spp=a.spp, decorrelated=not a.correlated, tex_lr=a.tex_lr)
     else:
         out = gradcheck(crn_spp=a.crn_spp)
     summary = {k: v for k, v in out.items() if k not in ("log", "rows")}
