@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate wall time of the CPU-baseline sample")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the large-scene leg (plain traversals, CPU oracle, cache)")
+    ap.add_argument("--cache-fast", action="store_true",
+                    help="run the radiance-cache comparison with the fast traversal")
     ap.add_argument("--no-cache", action="store_true",
                     help="skip the plain vs radiance-cached comparison on the large scene")
     ap.add_argument("--no-batch", action="store_true",
@@ -194,24 +198,87 @@ def batch_leg(rate_hz=16.0):
     return rep
 
 
-def cache_leg():
+def large_cpu(sc, settings_fn, seconds, threads, tlas):
+    """The oracle (release flags, mt19937 per 32x32 tile, faithful per-ray inverse) on a band of
+    rows of the 1080p frame at 1 spp, grown until about `seconds` of CPU time; with `tlas` the
+    instances are culled through a tree over their world boxes (same results)."""
+    from oracle import pyoracle as po
+    o = po.OracleScene(sc, fast=True)
+    rows, r0, total_s, total_n, total_r = 16, 532, 0.0, 0, 0
+    while True:
+        st = settings_fn(samples=1, seed=2000 + r0)
+        _, _, w, secs = o.render(st, threads=threads, rng=po.ORC_RNG_MT19937,
+                                 faithful_inverse=True, rows=(r0, r0 + rows), tlas=tlas)
+        total_s += secs
+        total_n += w["samples"]
+        total_r += w["rays"]
+        r0 += rows
+        if total_s >= seconds * 0.5 or r0 + rows > 1080:
+            break
+        rows = max(1, min(1080 - r0, int(rows * (seconds - total_s) / max(secs, 1e-3))))
+    o.close()
+    return {"value": total_n / total_s / 1e6, "unit": "Msamples/s", "cores": threads,
+            "kind": "port", "tlas": bool(tlas), "mrays_per_s": total_r / total_s / 1e6,
+            "sample": f"rows 532-{r0} of the 1920x1080 frame at 1 spp ({total_n} camera paths, "
+                      f"{total_r} rays) in {total_s:.1f} s; oracle with -Ofast -flto "
+                      f"-march=x86-64-v3, mt19937 per 32x32 tile, {threads} threads"
+                      + ("; instances culled by a tree over their world boxes" if tlas else
+                         "; the reference's linear loop over all 5,001 instances")}
+
+
+def large_leg(args):
     """The large terrain scene (2048x2048-vertex fBm terrain, 5000 icospheres, 16 point
-    lights, four 2048^2 textures, sky) at 1920x1080 and 64 spp: plain and radiance-cached
-    throughput, RMSE ladders against a 4096-spp plain render at 320x180, and the speed-up at the
-    plain render's RMSE."""
-    from mitrace_amd import cache_eval, scenegen
+    lights, four 2048^2 textures, sky) at 1920x1080: the plain tracer with the reference
+    traversal and with the fast traversal, the CPU oracle beside them (linear instance loop and
+    instance tree), and the radiance-cache comparison (64 spp; RMSE ladders against a 4096-spp
+    plain render at 320x180; the speed-up at the plain render's RMSE)."""
+    from mitrace_amd import abi, cache_eval, scenegen
     from mitrace_amd.tracer import GpuScene
     t0 = time.perf_counter()
     sc = scenegen.large_scene()
+    sc.bvh.fast = True
     g = GpuScene(sc)
     build_s = time.perf_counter() - t0
-    print(f"# cache: large scene built in {build_s:.1f} s", file=sys.stderr, flush=True)
-    out = cache_eval.compare(g, scenegen.large_scene_settings,
-                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(f"# large: scene generated and built (both BVHs) in {build_s:.1f} s", file=sys.stderr, flush=True)
+    out = {"workload": "large terrain scene (8.4M-triangle terrain, 5000 icospheres, 16 lights), "
+                       "1920x1080", "scene_build_s": build_s}
+    for name, trav, spp in (("gpu_reference_traversal", abi.MT_TRAVERSAL_REFERENCE, 8),
+                            ("gpu_fast_traversal", abi.MT_TRAVERSAL_FAST, 64)):
+        g.render(scenegen.large_scene_settings(samples=1, seed=4, traversal=trav))
+        _, _, cnt = g.render(scenegen.large_scene_settings(samples=1, seed=5, traversal=trav,
+                                                           count_work=True))
+        t = time.perf_counter()
+        _, _, s = g.render(scenegen.large_scene_settings(samples=spp, seed=6, traversal=trav))
+        dt = time.perf_counter() - t
+        out[name] = {"spp": spp, "seconds": dt, "msamples_per_s": s["samples"] / dt / 1e6,
+                     "mrays_per_s": s["rays"] / dt / 1e6,
+                     "per_ray": {k: cnt[k] / cnt["rays"] for k in
+                                 ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}}
+        print(f"# large: {name} {out[name]['msamples_per_s']:.1f} Msamples/s", file=sys.stderr, flush=True)
+    if not args.no_cpu_baseline:
+        threads = cpu_threads()
+        for tlas in (False, True):
+            key = "cpu_oracle_instance_tree" if tlas else "cpu_oracle_linear_loop"
+            out[key] = large_cpu(sc, scenegen.large_scene_settings, args.cpu_seconds, threads, tlas)
+            print(f"# large: {key} {out[key]['value']:.3f} Msamples/s", file=sys.stderr, flush=True)
+        ref = out["gpu_reference_traversal"]["msamples_per_s"]
+        fast = out["gpu_fast_traversal"]["msamples_per_s"]
+        out["gpu_over_cpu"] = {
+            "reference_traversal_vs_linear_loop": ref / out["cpu_oracle_linear_loop"]["value"],
+            "reference_traversal_vs_instance_tree": ref / out["cpu_oracle_instance_tree"]["value"],
+            "fast_traversal_vs_instance_tree": fast / out["cpu_oracle_instance_tree"]["value"]}
+    cache = None
+    if not args.no_cache:
+        trav = abi.MT_TRAVERSAL_FAST if args.cache_fast else abi.MT_TRAVERSAL_REFERENCE
+
+        def fn(**kw):
+            kw.setdefault("traversal", trav)
+            return scenegen.large_scene_settings(**kw)
+        cache = cache_eval.compare(g, fn, log=lambda m: print(m, file=sys.stderr, flush=True))
+        cache["workload"] = out["workload"] + ", 64 spp"
+        cache["traversal"] = "fast" if args.cache_fast else "reference"
     g.close()
-    out["workload"] = ("large terrain scene (8.4M-triangle terrain, 5000 icospheres, 16 lights), "
-                       "1920x1080, 64 spp")
-    return out
+    return out, cache
 
 
 def main():
@@ -296,8 +363,10 @@ def main():
                                  ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
         "cpu_baseline": None,
     }
-    if not args.no_cache:
-        line["cache"] = cache_leg()
+    if not args.no_large:
+        line["large"], cache = large_leg(args)
+        if cache is not None:
+            line["cache"] = cache
     if not args.no_batch:
         line["batch"] = batch_leg()
         print(f"# batch: {line['batch']['msamples_per_s']:.1f} Msamples/s, p50 "

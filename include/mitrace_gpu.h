@@ -66,6 +66,17 @@ extern "C" {
 #define MT_MODE_PARITY 0 /* the reference estimator, quirks included (Tracer.cpp:134-348) */
 #define MT_MODE_FIXED 1  /* Russian roulette instead of the energy cut-off                  */
 /* per-sample output */
+/* Traversal.  REFERENCE visits exactly the reference's nodes and triangles (Q4-Q7: the
+ * reference's BVHs, exit-distance culling with its 0.1 slack, every instance whose world box
+ * the ray line crosses, closest-hit shadow rays), so hits are the reference's, including its
+ * occasional non-closest ones.  FAST uses per-mesh SAH BVHs with tight boxes (built when the
+ * scene sets mt_bvh_options.fast), culls nodes and instances by entry distance against the
+ * closest hit so far, and ends a shadow ray at its first occluder: hits are the true closest
+ * hits, and images differ from REFERENCE ones only where the reference returns a non-closest
+ * hit.  No reference counterpart (a GPU speed mode). */
+#define MT_TRAVERSAL_REFERENCE 0
+#define MT_TRAVERSAL_FAST 1
+
 #define MT_OUTPUT_TONEMAPPED 0 /* ACES per sample before averaging (Tracer.cpp:212-227,346) */
 #define MT_OUTPUT_LINEAR 1     /* mean of linear radiance                                   */
 /* debug views: Config::DebugMode (src/CLI/Config.hpp:16-41) in the same order, rendered as
@@ -178,6 +189,8 @@ typedef struct mt_bvh_options {
     int32_t disable;        /* --disable-bvh        (default 0)  */
     uint32_t max_triangles; /* --bvh-max-triangles  (default 32) */
     uint32_t max_depth;     /* --bvh-max-depth      (default 32) */
+    int32_t fast;           /* 1 = also build the SAH BVHs of MT_TRAVERSAL_FAST (no reference
+                               counterpart; the reference builds only the trees above)       */
 } mt_bvh_options;
 
 typedef struct mt_scene_desc {
@@ -219,6 +232,7 @@ typedef struct mt_settings {
     uint32_t paths_in_flight;     /* wavefront size; 0 = automatic                    */
     int32_t count_work;           /* 1 = gather traversal work counters (slower)      */
     int32_t debug_mode;           /* debugMode (Config.hpp:74), MT_DEBUG_*            */
+    int32_t traversal;            /* MT_TRAVERSAL_*                                   */
 } mt_settings;
 
 /* Tracer::Stats (src/Tracer/Tracer.hpp:78-85) plus work counters. */
@@ -374,6 +388,17 @@ int mt_job_destroy(mt_job* job);
 
 /* IntersectScene over n rays; rays holds 6 floats per ray {origin.xyz, direction.xyz}. */
 int mt_trace_closest(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hits);
+
+/* mt_trace_closest with a choice of traversal (MT_TRAVERSAL_*).  No reference counterpart:
+ * the test hook of the fast traversal. */
+int mt_trace_closest_ex(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hits,
+                        int32_t traversal);
+
+/* Shadow-ray test hook (Tracer.cpp:317-323): rays = n x {ox, oy, oz, dx, dy, dz, dist} host
+ * floats; occluded[i] = 1 when the closest hit is at distance <= dist (REFERENCE: the full
+ * closest-hit query, FAST: the first occluder found), else 0. */
+int mt_trace_occluded(mt_scene* scene, uint32_t n, const float* rays, uint32_t* occluded,
+                      int32_t traversal);
 
 /* BRDF::EvaluateBRDF, DiffusePdf, SpecularPdf on the GPU.  in: 14 floats per case
  * {L.xyz, V.xyz, N.xyz, base.rgb, roughness, metallic}; out: 5 floats per case

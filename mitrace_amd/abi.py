@@ -26,6 +26,7 @@ MT_FILTER_NEAREST, MT_FILTER_LINEAR = 0, 1
 MT_ENV_NONE, MT_ENV_RGB8, MT_ENV_RGB32F = 0, 1, 2
 MT_MODE_PARITY, MT_MODE_FIXED = 0, 1
 MT_OUTPUT_TONEMAPPED, MT_OUTPUT_LINEAR = 0, 1
+MT_TRAVERSAL_REFERENCE, MT_TRAVERSAL_FAST = 0, 1
 
 
 class mt_mesh_desc(C.Structure):
@@ -86,7 +87,8 @@ class mt_env_desc(C.Structure):
 
 
 class mt_bvh_options(C.Structure):
-    _fields_ = [("disable", C.c_int32), ("max_triangles", C.c_uint32), ("max_depth", C.c_uint32)]
+    _fields_ = [("disable", C.c_int32), ("max_triangles", C.c_uint32), ("max_depth", C.c_uint32),
+                ("fast", C.c_int32)]
 
 
 class mt_scene_desc(C.Structure):
@@ -112,7 +114,8 @@ class mt_settings(C.Structure):
                 ("mode", C.c_int32), ("rr_depth", C.c_uint32), ("output", C.c_int32),
                 ("hdri_rotation_apply", C.c_int32),
                 ("paths_in_flight", C.c_uint32),
-                ("count_work", C.c_int32), ("debug_mode", C.c_int32)]
+                ("count_work", C.c_int32), ("debug_mode", C.c_int32),
+                ("traversal", C.c_int32)]
 
 
 class mt_stats(C.Structure):
@@ -203,6 +206,10 @@ SIGNATURES = {
     "mt_job_destroy": (C.c_int, [C.c_void_p]),
     "mt_trace_closest": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
                                    C.POINTER(mt_hit)]),
+    "mt_trace_closest_ex": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                      C.POINTER(mt_hit), C.c_int32]),
+    "mt_trace_occluded": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_uint32), C.c_int32]),
     "mt_brdf_eval": (C.c_int, [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mt_diff_create": (C.c_int, [C.c_void_p, C.POINTER(mt_settings), C.POINTER(C.c_void_p)]),
     "mt_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(mt_camera_desc),

@@ -209,8 +209,10 @@ int mt_batch::admit(uint32_t id) {
     HostJob& j = *view[id];
     const uint32_t nl = static_cast<uint32_t>(lanes.size());
     SceneSlot* s = nullptr;
+    // one slot per scene and traversal: the slot's launch constants pick the traversal kernel
+    const int32_t accel = j.st.traversal == MT_TRAVERSAL_FAST ? 1 : 0;
     for (auto& x : slots)
-        if (x->scene == j.scene) s = x.get();
+        if (x->scene == j.scene && x->c.accel == accel) s = x.get();
     if (!s) {
         auto ns = std::make_unique<SceneSlot>();
         ns->scene = j.scene;
@@ -408,6 +410,14 @@ bool valid_job(mt_scene* scene, const mt_camera_desc* cam, const mt_settings* st
     }
     if (st->debug_mode) {
         mt::set_error("debug views are rendered by mt_render / mt_render_start");
+        return false;
+    }
+    if (st->traversal != MT_TRAVERSAL_REFERENCE && st->traversal != MT_TRAVERSAL_FAST) {
+        mt::set_error("unknown traversal");
+        return false;
+    }
+    if (st->traversal == MT_TRAVERSAL_FAST && !scene->dev.fast_meshes && scene->mesh_count) {
+        mt::set_error("the scene was created without fast BVHs (mt_bvh_options.fast)");
         return false;
     }
     if (st->bounces > mt::kMaxBounces || st->samples > cap ||

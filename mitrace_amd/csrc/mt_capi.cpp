@@ -285,9 +285,24 @@ int mt_job_destroy(mt_job* job) {
 }
 
 int mt_trace_closest(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hits) {
+    return mt_trace_closest_ex(scene, n, rays, hits, MT_TRAVERSAL_REFERENCE);
+}
+
+int mt_trace_closest_ex(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hits,
+                        int32_t traversal) {
     if (!scene || (n && (!rays || !hits))) return mt::fail(MT_E_INVALID, "bad arguments");
     try {
-        return mt::trace_rays(scene, n, rays, hits);
+        return mt::trace_rays(scene, n, rays, hits, traversal, nullptr);
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
+int mt_trace_occluded(mt_scene* scene, uint32_t n, const float* rays, uint32_t* occluded,
+                      int32_t traversal) {
+    if (!scene || (n && (!rays || !occluded))) return mt::fail(MT_E_INVALID, "bad arguments");
+    try {
+        return mt::trace_rays(scene, n, rays, nullptr, traversal, occluded);
     } catch (const std::exception& e) {
         return mt::fail(MT_E_INVALID, e.what());
     }

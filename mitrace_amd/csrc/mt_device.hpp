@@ -673,5 +673,215 @@ __device__ __forceinline__ void trav_tri(const DevScene& S, Trav& t) {
     t.leaf = (__float_as_uint(p2.w) & kPrimLast) ? kNoLeaf : t.leaf + 1;
 }
 
+// ---- fast traversal (MT_TRAVERSAL_FAST) ----------------------------------------------------
+// Per-mesh SAH BVHs with tight boxes (DevScene::fast_meshes).  The local ray keeps the length
+// of the world direction (no re-normalisation), so one parameter t measures the hit in every
+// instance and in world space; nodes, top-level entries and instances are culled when their
+// entry distance is not below the closest hit so far (t.R.dist while entering, t.ldist inside
+// an instance).  Shadow rays (kAny) start with the light distance as the bound and stop at the
+// first occluder.
+
+// entry distance of the ray into the box, clamped at 0; +inf when the box is missed or lies
+// beyond tmax
+__device__ __forceinline__ float slab_enter(V3 o, V3 inv, V3 mn, V3 mx, float tmax) {
+    const V3 t1 = mul(sub(mn, o), inv);
+    const V3 t2 = mul(sub(mx, o), inv);
+    const float tn = fmaxf(fmaxf(fminf(t1.x, t2.x), fminf(t1.y, t2.y)), fmaxf(fminf(t1.z, t2.z), 0.0f));
+    const float tf = fminf(fminf(fmaxf(t1.x, t2.x), fmaxf(t1.y, t2.y)), fminf(fmaxf(t1.z, t2.z), tmax));
+    return tn <= tf ? tn : __builtin_huge_valf();
+}
+
+__device__ __forceinline__ void ftrav_tlas_begin(const DevScene& S, Trav& t, Stack& st) {
+    t.tl = true;
+    t.mask = 0;
+    t.cur = -1;
+    t.leaf = kNoLeaf;
+    t.has_next = false;
+    st.sp = 0;
+    const V3 mn = {S.tlas_root_min[0], S.tlas_root_min[1], S.tlas_root_min[2]};
+    const V3 mx = {S.tlas_root_max[0], S.tlas_root_max[1], S.tlas_root_max[2]};
+    const float te = slab_enter(t.o, t.invd, mn, mx, t.R.dist);
+    if (te < t.R.dist) st.push(S.tlas_root_word, te);
+    t.base = st.sp;
+}
+
+// Set-up step of the fast traversal: finish the current instance, then enter the next one
+// whose world box the ray enters before the closest hit so far.  Returns true when the ray is
+// finished (for kAny also as soon as an occluder is found).
+template <bool kCount, bool kAny>
+__device__ bool ftrav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool& need_mask) {
+    if (t.cur >= 0) {
+        if (kCount) {
+            w.pops += t.btests;
+            w.tris += t.ttests;
+        }
+        if (t.ldist < t.R.dist) {
+            const uint32_t ii = trav_instance(S, t, t.cur);
+            const V3 lp = add(t.lo, scale(t.ld, t.ldist));
+            t.R.dist = t.ldist;
+            t.R.inst = static_cast<int>(ii);
+            t.R.tri = t.btri;
+            t.R.bv = t.bbv;
+            t.R.u = t.bu;
+            t.R.v = t.bvv;
+            t.R.wpos = xform(S.instances[ii].M, lp, 1.0f);
+            t.R.bvh_tests = t.btests;
+            t.R.tri_tests = t.ttests;
+            if (kAny) return true;
+        }
+        t.cur = -1;
+    }
+    const DevInstance* I;
+    while (true) {
+        if (t.mask == 0) {
+            if (!t.tl) {
+                if (t.cb + 64u < S.instance_count) {
+                    t.cb += 64u;
+                    need_mask = true;
+                    return false;
+                }
+                return true;
+            }
+            // top level, nearest entry first
+            while (true) {
+                if (st.sp == 0) return true;
+                const uint2 e = st.pop();
+                if (__uint_as_float(e.y) >= t.R.dist) continue;
+                const uint32_t word = e.x;
+                if (word & kLeafBit) {
+                    const uint32_t first = word & 0x00FFFFFFu, cnt = (word >> 24) & 0x7Fu;
+                    uint64_t m = 0;
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        const DevInstance* J = S.instances + S.tlas_inst[first + j];
+                        const V3 mn = {J->wmin[0], J->wmin[1], J->wmin[2]};
+                        const V3 mx = {J->wmax[0], J->wmax[1], J->wmax[2]};
+                        if (slab_enter(t.o, t.invd, mn, mx, t.R.dist) < t.R.dist) m |= (1ull << j);
+                    }
+                    if (kCount) w.aabb += cnt;
+                    if (m) {
+                        t.cb = first;
+                        t.mask = m;
+                        break;
+                    }
+                } else {
+                    const float4* q = S.tlas_pairs + 4ull * word;
+                    const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
+                    const float da = slab_enter(t.o, t.invd, xyz(a0), xyz(a1), t.R.dist);
+                    const float db = slab_enter(t.o, t.invd, xyz(b0), xyz(b1), t.R.dist);
+                    const uint32_t wa = __float_as_uint(a0.w), wb = __float_as_uint(a1.w);
+                    const bool pa = da < t.R.dist, pb = db < t.R.dist;
+                    if (pa && pb) {  // the nearer child is popped first
+                        const bool a_first = da <= db;
+                        st.push(a_first ? wb : wa, a_first ? db : da);
+                        st.push(a_first ? wa : wb, a_first ? da : db);
+                    } else if (pa) {
+                        st.push(wa, da);
+                    } else if (pb) {
+                        st.push(wb, db);
+                    }
+                }
+            }
+        }
+        t.cur = __builtin_ctzll(t.mask);
+        t.mask &= t.mask - 1;
+        I = S.instances + trav_instance(S, t, t.cur);
+        // the closest hit may have moved nearer since the box was accepted
+        const V3 mn = {I->wmin[0], I->wmin[1], I->wmin[2]};
+        const V3 mx = {I->wmax[0], I->wmax[1], I->wmax[2]};
+        if (slab_enter(t.o, t.invd, mn, mx, t.R.dist) < t.R.dist) break;
+        t.cur = -1;
+    }
+    t.base = st.sp;
+    float inv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) inv[k] = I->inv[k];
+    t.lo = xform(inv, t.o, 1.0f);
+    t.ld = xform(inv, t.d, 0.0f);
+    t.linv = {1.0f / t.ld.x, 1.0f / t.ld.y, 1.0f / t.ld.z};
+    t.ldist = t.R.dist;
+    t.bu = t.bvv = 0.0f;
+    t.btri = t.bbv = t.btests = t.ttests = 0;
+    const DevMesh& m = S.fast_meshes[I->mesh];
+    t.pair_base = m.pair_base;
+    t.prim_base = m.prim_base;
+    if (kCount) w.enter++;
+    if (m.flags & kMeshHasBvh) {
+        const V3 rmn = {m.root_min[0], m.root_min[1], m.root_min[2]};
+        const V3 rmx = {m.root_max[0], m.root_max[1], m.root_max[2]};
+        const float te = slab_enter(t.lo, t.linv, rmn, rmx, t.ldist);
+        if (te < t.ldist) {
+            t.nxt_word = m.root_word;
+            t.nxt_dist = te;
+            t.has_next = true;
+        }
+    } else if (m.prim_count) {
+        t.leaf = 0;
+    }
+    return false;
+}
+
+// One node pop of the fast traversal; returns false when the pop was culled.
+__device__ __forceinline__ bool ftrav_node(const DevScene& S, Trav& t, Stack& st) {
+    uint32_t word;
+    float nd;
+    if (t.has_next) {
+        word = t.nxt_word;
+        nd = t.nxt_dist;
+        t.has_next = false;
+    } else {
+        const uint2 e = st.pop();
+        word = e.x;
+        nd = __uint_as_float(e.y);
+    }
+    t.btests++;
+    if (nd >= t.ldist) return false;
+    if (word & kLeafBit) {
+        t.leaf = word & ~kLeafBit;
+        return true;
+    }
+    const float4* q = S.pairs + 4ull * (t.pair_base + word);
+    const float4 a0 = q[0], a1 = q[1], b0 = q[2], b1 = q[3];
+    const float da = slab_enter(t.lo, t.linv, xyz(a0), xyz(a1), t.ldist);
+    const float db = slab_enter(t.lo, t.linv, xyz(b0), xyz(b1), t.ldist);
+    const uint32_t wa = __float_as_uint(a0.w), wb = __float_as_uint(a1.w);
+    const bool pa = da < t.ldist, pb = db < t.ldist;
+    if (pa && pb) {
+        const bool a_first = da <= db;
+        st.push(a_first ? wb : wa, a_first ? db : da);
+        t.nxt_word = a_first ? wa : wb;
+        t.nxt_dist = a_first ? da : db;
+        t.has_next = true;
+    } else if (pa || pb) {
+        t.nxt_word = pa ? wa : wb;
+        t.nxt_dist = pa ? da : db;
+        t.has_next = true;
+    }
+    return true;
+}
+
+// One triangle test of the fast traversal; with kAny the first hit closes the instance.
+template <bool kAny>
+__device__ __forceinline__ void ftrav_tri(const DevScene& S, Trav& t, Stack& st) {
+    const float4* P = S.prims + 3ull * (t.prim_base + t.leaf);
+    const float4 p0 = P[0], p1 = P[1], p2 = P[2];
+    float u, v, tt;
+    t.ttests++;
+    t.leaf = (__float_as_uint(p2.w) & kPrimLast) ? kNoLeaf : t.leaf + 1;
+    if (tri_test(t.lo, t.ld, xyz(p0), xyz(p1), xyz(p2), u, v, tt)) {
+        if (tt > 0.0f && tt < t.ldist) {
+            t.ldist = tt;
+            t.bu = u;
+            t.bvv = v;
+            t.bbv = __float_as_uint(p1.w);
+            t.btri = __float_as_uint(p0.w);
+            if (kAny) {
+                t.leaf = kNoLeaf;
+                t.has_next = false;
+                st.sp = t.base;
+            }
+        }
+    }
+}
+
 }  // namespace dev
 }  // namespace mt

@@ -18,6 +18,9 @@
 //   texels     uint32[]         RGBA8 texels of all images; images DevImage[] {offset, w, h}
 //   env        float4[W*H]      equirect texels decoded to float (u8/255 or float input)
 //   lights     DevLight[]
+//   fast_meshes DevMesh[M]      (scenes built with mt_bvh_options.fast) the same records for
+//                                     per-mesh SAH BVHs, whose pair records and prim slots are
+//                                     appended to pairs/prims: tight boxes, no duplicates
 #pragma once
 
 #include <cstdint>
@@ -109,6 +112,7 @@ struct DevScene {
     uint32_t light_count;
     // top-level BVH over instance world boxes (used when instance_count > 64): pair records
     // like the meshes', leaf word = kLeafBit | count << 24 | first into tlas_inst
+    const DevMesh* fast_meshes;  // per mesh: the SAH BVH of the fast traversal (or null)
     const float4* tlas_pairs;
     const uint32_t* tlas_inst;
     float tlas_root_min[3], tlas_root_max[3];
@@ -150,6 +154,7 @@ struct mt_scene {
     std::vector<uint32_t> mesh_has_bvh;
     uint32_t max_bvh_depth = 0;
     uint32_t tlas_depth = 0;
+    uint32_t max_fast_depth = 0;  // levels of the deepest fast-traversal BVH (0 = none built)
     uint32_t mesh_count = 0;
     uint32_t instance_count = 0;
     uint32_t light_count = 0;
@@ -168,6 +173,11 @@ int fail(int code, const std::string& msg);
 int upload_scene(const mt_scene_desc* desc, mt_scene* scene);
 // top-level BVH over boxes (6 floats each): SAH over 12 bins per axis, up to 4 boxes a leaf.
 // pairs/indices use the device format of DevScene::tlas_*; returns the number of levels.
+// SAH BVH of one mesh for the fast traversal: tight boxes, every triangle in exactly one leaf
+// of at most 4; appends its pair records and prim slots (formats of DevScene::pairs/prims) and
+// fills dm's root, pair_base, prim_base, prim_count and flags.  Returns the number of levels.
+uint32_t build_fast_bvh(const float* positions, uint32_t tri_count, std::vector<float4>& pairs,
+                        std::vector<float4>& prims, DevMesh& dm);
 uint32_t build_tlas(const std::vector<float>& boxes, std::vector<float4>& pairs,
                     std::vector<uint32_t>& indices, float root_min[3], float root_max[3],
                     uint32_t& root_word);

@@ -214,23 +214,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_generate(PathBuffers b) {
     init_path(J, b, p, pix, s);
 }
 
-// rays of the test hook (mode 2)
+// rays of the test hooks (mode 2: closest hits, mode 3: occlusion up to a distance)
 struct TestIO {
-    const float* rays;
+    const float* rays;  // mode 2: 6 floats per ray, mode 3: 7 (origin, direction, distance)
     mt_hit* out;
     uint32_t n;
     uint32_t* fetch;
+    uint32_t* occ;
 };
 
 // Persistent closest-hit kernel.  mode 0: extend (queue of path ids), 1: shadow records,
-// 2: test rays.  A wave keeps 64 lanes busy: whenever at least refill_min lanes are idle it
+// 2/3: test rays.  kAccel 0 runs the reference traversal, 1 the fast one (for shadow rays an
+// any-hit query bounded by the light distance).  A wave keeps 64 lanes busy: whenever at least refill_min lanes are idle it
 // takes that many new work items with one atomic (ballot + prefix for the lane slots), runs
 // the instance culling for them together, and every lane then advances its own ray by one
 // unit of work per loop iteration.  Waves leave the loop when the queue is drained.
 #ifndef MT_TRACE_WAVES_PER_SIMD
 #define MT_TRACE_WAVES_PER_SIMD 6
 #endif
-template <bool kCount, int kMode, uint32_t kDepth>
+template <bool kCount, int kMode, uint32_t kDepth, int kAccel>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(DevScene S, ChunkParams c, PathBuffers b,
                                                   uint32_t bounce, TestIO io) {
@@ -250,6 +252,7 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
         count = io.n;
         fetch = io.fetch;
     }
+    constexpr bool kAny = kAccel == 1 && (kMode == 1 || kMode == 3);
     const uint32_t* q = b.queue[bounce & 1];
     const uint32_t refill_min = c.refill_min;
     const uint64_t lanes_below = (1ull << lane_id()) - 1ull;
@@ -270,21 +273,34 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                 if (my < count) {
                     item = my;
                     V3 o, d;
+                    float bound = kFltMax;
                     if (kMode == 0) {
                         const uint32_t p = q[my];
                         o = xyz(b.ray_o[p]);
                         d = xyz(b.ray_d[p]);
                     } else if (kMode == 1) {
-                        o = xyz(b.sh_o[my]);
+                        const float4 so = b.sh_o[my];
+                        o = xyz(so);
                         d = xyz(b.sh_d[my]);
-                    } else {
+                        bound = so.w;
+                    } else if (kMode == 2) {
                         o = V3{io.rays[my * 6 + 0], io.rays[my * 6 + 1], io.rays[my * 6 + 2]};
                         d = V3{io.rays[my * 6 + 3], io.rays[my * 6 + 4], io.rays[my * 6 + 5]};
+                    } else {
+                        o = V3{io.rays[my * 7 + 0], io.rays[my * 7 + 1], io.rays[my * 7 + 2]};
+                        d = V3{io.rays[my * 7 + 3], io.rays[my * 7 + 4], io.rays[my * 7 + 5]};
+                        bound = io.rays[my * 7 + 6];
                     }
                     trav_begin(t, o, d);
+                    // an occluder counts up to and including the light distance (Tracer.cpp:322)
+                    if (kAny) t.R.dist = bound >= 0.0f ? __uint_as_float(__float_as_uint(bound) + 1u) : bound;
                     active = true;
                     if (S.use_tlas && finite3(t.invd) && finite3(o)) {
-                        trav_tlas_begin(S, t, st);
+                        if (kAccel) {
+                            ftrav_tlas_begin(S, t, st);
+                        } else {
+                            trav_tlas_begin(S, t, st);
+                        }
                         need = false;
                     } else {
                         need = true;
@@ -331,11 +347,21 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
             if (wants_setup) {
                 if (run_setup) {
                     kind = kStepSetup;
-                    finished = trav_setup<kCount>(S, t, st, w, nm);
+                    if (kAccel) {
+                        finished = ftrav_setup<kCount, kAny>(S, t, st, w, nm);
+                    } else {
+                        finished = trav_setup<kCount>(S, t, st, w, nm);
+                    }
                 }
             } else if (t.leaf != kNoLeaf) {
                 kind = kStepLeaf;
-                trav_tri(S, t);
+                if (kAccel) {
+                    ftrav_tri<kAny>(S, t, st);
+                } else {
+                    trav_tri(S, t);
+                }
+            } else if (kAccel) {
+                kind = ftrav_node(S, t, st) ? kStepInner : kStepCull;
             } else if (t.fast) {
                 kind = trav_node<true>(S, t, st) ? kStepInner : kStepCull;
             } else {
@@ -386,6 +412,8 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                         // one shadow ray per path and bounce: no other lane writes this word
                         if (c.diff) b.vis[p] |= 1u << bounce;
                     }
+                } else if (kMode == 3) {
+                    io.occ[item] = (h.inst >= 0 && !(h.dist > io.rays[item * 7 + 6])) ? 1u : 0u;
                 } else {
                     mt_hit r;
                     r.instance = h.inst;
@@ -424,24 +452,39 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
     }
 }
 
-template <int kMode>
-void launch_trace(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, const DevScene& S,
-                  const ChunkParams& c, const PathBuffers& b, uint32_t k, TestIO io = {}) {
+template <int kMode, int kAccel>
+void launch_trace_as(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, const DevScene& S,
+                     const ChunkParams& c, const PathBuffers& b, uint32_t k, TestIO io) {
+    if constexpr (kMode >= 2) {  // test hooks: no counting build, one stack depth
+        hipLaunchKernelGGL((k_trace<false, kMode, 16, kAccel>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io);
+        return;
+    } else {
 #define MT_TRACE_CASE(D)                                                                          \
     case D:                                                                                       \
         if (count)                                                                                \
-            hipLaunchKernelGGL((k_trace<true, kMode, D>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io); \
+            hipLaunchKernelGGL((k_trace<true, kMode, D, kAccel>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io); \
         else                                                                                      \
-            hipLaunchKernelGGL((k_trace<false, kMode, D>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io); \
+            hipLaunchKernelGGL((k_trace<false, kMode, D, kAccel>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io); \
         break;
-    switch (depth) {
-        MT_TRACE_CASE(4)
-        MT_TRACE_CASE(8)
-        MT_TRACE_CASE(12)
-        default:
-        MT_TRACE_CASE(16)
-    }
+        switch (depth) {
+            MT_TRACE_CASE(4)
+            MT_TRACE_CASE(8)
+            MT_TRACE_CASE(12)
+            default:
+            MT_TRACE_CASE(16)
+        }
 #undef MT_TRACE_CASE
+    }
+}
+
+// the traversal is chosen per chunk (ChunkParams::accel)
+template <int kMode>
+void launch_trace(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, const DevScene& S,
+                  const ChunkParams& c, const PathBuffers& b, uint32_t k, TestIO io = {}) {
+    if (c.accel)
+        launch_trace_as<kMode, 1>(depth, count, blocks, s, S, c, b, k, io);
+    else
+        launch_trace_as<kMode, 0>(depth, count, blocks, s, S, c, b, k, io);
 }
 
 // Material::SampleMaterial (Material.hpp:82-135) plus the differentiable overrides.
@@ -1164,11 +1207,16 @@ int alloc(std::vector<void*>& owned, T** p, size_t n) {
 int cu_count() { return device_cu_count(); }
 
 uint32_t stack_capacity(const mt_scene* s) {
-    const uint32_t blas = std::max<uint32_t>(s->max_bvh_depth + 1u, 2u);
+    const uint32_t blas = std::max<uint32_t>(std::max(s->max_bvh_depth, s->max_fast_depth) + 1u, 2u);
     return s->dev.use_tlas ? blas + s->tlas_depth + 1u : blas;
 }
 
-int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits) {
+int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t traversal,
+               uint32_t* occluded) {
+    if (traversal != MT_TRAVERSAL_REFERENCE && traversal != MT_TRAVERSAL_FAST)
+        return fail(MT_E_INVALID, "unknown traversal");
+    if (traversal == MT_TRAVERSAL_FAST && !s->dev.fast_meshes && s->mesh_count)
+        return fail(MT_E_INVALID, "the scene was created without fast BVHs (mt_bvh_options.fast)");
     if (n == 0) return MT_OK;
     std::vector<void*> owned;
     struct Guard {
@@ -1177,33 +1225,45 @@ int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits) {
             for (void* p : v) (void)hipFree(p);
         }
     } guard{owned};
+    const uint32_t stride = occluded ? 7u : 6u;
     float* drays;
-    mt_hit* dhits;
+    mt_hit* dhits = nullptr;
+    uint32_t* docc = nullptr;
     PathBuffers b{};
-    const uint32_t depth = kLdsDepth;
     const uint32_t cap = stack_capacity(s);
-    const uint32_t spill_depth = cap > depth ? cap - depth : 1;
+    const uint32_t spill_depth = cap > 16u ? cap - 16u : 1;  // the test kernels keep 16 in LDS
     const uint32_t blocks = std::max<uint32_t>(
         1u, std::min<uint32_t>((n + kBlock - 1) / kBlock, static_cast<uint32_t>(device_cu_count() * 8)));
     int rc;
-    if ((rc = alloc(owned, &drays, static_cast<size_t>(n) * 6))) return rc;
-    if ((rc = alloc(owned, &dhits, n))) return rc;
+    if ((rc = alloc(owned, &drays, static_cast<size_t>(n) * stride))) return rc;
+    if (occluded) {
+        if ((rc = alloc(owned, &docc, n))) return rc;
+    } else {
+        if ((rc = alloc(owned, &dhits, n))) return rc;
+    }
     if ((rc = alloc(owned, &b.spill, static_cast<size_t>(spill_depth) * blocks * kBlock))) return rc;
     if ((rc = alloc(owned, &b.counters, kMaxCounters + 1))) return rc;
-    HIP_TRY(hipMemcpy(drays, rays, sizeof(float) * 6 * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(drays, rays, sizeof(float) * stride * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(b.counters, 0, sizeof(uint32_t) * (kMaxCounters + 1)));
     ChunkParams c{};
     c.stack_cap = cap;
     c.refill_min = kRefillMin;
     c.setup_min = kSetupMin;
-    TestIO io{drays, dhits, n, b.counters + kMaxCounters};
-    launch_trace<2>(depth, false, blocks, nullptr, s->dev, c, b, 0, io);
+    c.accel = traversal == MT_TRAVERSAL_FAST ? 1 : 0;
+    TestIO io{drays, dhits, n, b.counters + kMaxCounters, docc};
+    if (occluded)
+        launch_trace<3>(16, false, blocks, nullptr, s->dev, c, b, 0, io);
+    else
+        launch_trace<2>(16, false, blocks, nullptr, s->dev, c, b, 0, io);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     uint32_t herr = 0;
     HIP_TRY(hipMemcpy(&herr, b.counters + kCntError, sizeof(herr), hipMemcpyDeviceToHost));
     if (herr) return fail(MT_E_INVALID, "traversal stack overflow");
-    HIP_TRY(hipMemcpy(hits, dhits, sizeof(mt_hit) * n, hipMemcpyDeviceToHost));
+    if (occluded)
+        HIP_TRY(hipMemcpy(occluded, docc, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    else
+        HIP_TRY(hipMemcpy(hits, dhits, sizeof(mt_hit) * n, hipMemcpyDeviceToHost));
     return MT_OK;
 }
 
@@ -1310,6 +1370,7 @@ void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& ca
     c.firefly_threshold = st.firefly_threshold;
     c.count_work = st.count_work;
     c.debug = st.debug_mode;
+    c.accel = st.traversal == MT_TRAVERSAL_FAST ? 1 : 0;
     c.stack_cap = stack_capacity(scene);
     c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
     c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
@@ -1375,6 +1436,10 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     if (st.width == 0 || st.height == 0 || st.samples == 0)
         return fail(MT_E_INVALID, "width, height and samples must be positive");
     if (st.bounces > kMaxBounces) return fail(MT_E_UNSUPPORTED, "bounces above 64 are not supported");
+    if (st.traversal != MT_TRAVERSAL_REFERENCE && st.traversal != MT_TRAVERSAL_FAST)
+        return fail(MT_E_INVALID, "unknown traversal");
+    if (st.traversal == MT_TRAVERSAL_FAST && !scene->dev.fast_meshes && scene->mesh_count)
+        return fail(MT_E_INVALID, "the scene was created without fast BVHs (mt_bvh_options.fast)");
     const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
     if (npix_total * 3 > 0xFFFFFFFFull) return fail(MT_E_UNSUPPORTED, "image too large");
     uint64_t target = st.paths_in_flight ? st.paths_in_flight : (1ull << 22);

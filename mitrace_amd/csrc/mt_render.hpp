@@ -78,6 +78,7 @@ struct ChunkParams {
     int32_t cache;
     uint32_t cache_depth, train_every, train_offset, train_cap, train_nrec;
     int32_t debug;  // MT_DEBUG_* view (plain renders)
+    int32_t accel;  // 1 = fast traversal (MT_TRAVERSAL_FAST)
 };
 
 // SoA path state, queues and scratch of one render (capacity = paths per chunk)
@@ -224,7 +225,10 @@ void launch_batch_iteration(const mt_scene* scene, const ChunkParams& c, const P
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 // the heat-map post-process of the counting and measuring debug views (host, in place)
 void debug_post(int mode, float* rgb, uint64_t npix);
-int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits);
+// test hooks: closest hits (occluded == null, 6 floats per ray) or occlusion up to a distance
+// (7 floats per ray)
+int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t traversal,
+               uint32_t* occluded);
 int brdf_eval(uint32_t n, const float* in, float* out);
 
 }  // namespace mt

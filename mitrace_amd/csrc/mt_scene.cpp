@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <thread>
 
 #include "mt_internal.hpp"
 
@@ -240,125 +241,179 @@ void invert4(const float* m, float* out) {
 
 namespace {
 
-struct TBuild {
-    const std::vector<float>& boxes;
-    std::vector<uint32_t> ids;
+// Binned SAH tree over axis-aligned boxes (6 floats each: min xyz, max xyz).  All three axes
+// are binned in one pass over the node's items (16 bins on the box centres); a node with at
+// most leaf_max items is a leaf.  The first levels hand one child to a new thread, so large
+// meshes build on several cores; every thread owns a disjoint range of ids.
+struct SahTree {
     struct Node {
         float mn[3], mx[3];
         int32_t a = -1, b = -1;  // children, -1 = leaf
         uint32_t first = 0, count = 0;
     };
+    static constexpr int kBins = 16;
+    const float* boxes = nullptr;
+    std::vector<float> cent;      // box centres
+    std::vector<uint32_t> ids;    // item order: a leaf holds ids[first, first + count)
     std::vector<Node> nodes;
-    explicit TBuild(const std::vector<float>& b) : boxes(b) {}
+    uint32_t leaf_max = 4;
+    uint32_t depth = 0;           // levels of the tree
 
-    void bounds(uint32_t lo, uint32_t hi, float* mn, float* mx) const {
+    static float area(const float* mn, const float* mx) {
+        const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+
+    // best split of ids[lo, hi): returns the split position (lo < mid < hi)
+    uint32_t split(uint32_t lo, uint32_t hi, float* mn, float* mx) {
+        float cmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, cmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
         for (int k = 0; k < 3; ++k) {
             mn[k] = FLT_MAX;
             mx[k] = -FLT_MAX;
         }
         for (uint32_t i = lo; i < hi; ++i) {
-            const float* bx = &boxes[6 * ids[i]];
+            const float* bx = boxes + 6ull * ids[i];
+            const float* c = &cent[3ull * ids[i]];
             for (int k = 0; k < 3; ++k) {
                 mn[k] = std::min(mn[k], bx[k]);
                 mx[k] = std::max(mx[k], bx[3 + k]);
+                cmn[k] = std::min(cmn[k], c[k]);
+                cmx[k] = std::max(cmx[k], c[k]);
             }
         }
-    }
-    static float area(const float* mn, const float* mx) {
-        const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
-        return dx * dy + dy * dz + dz * dx;
-    }
-    int32_t build(uint32_t lo, uint32_t hi, uint32_t level, uint32_t& depth) {
-        depth = std::max(depth, level);
-        const int32_t id = static_cast<int32_t>(nodes.size());
-        nodes.emplace_back();
-        float mn[3], mx[3];
-        bounds(lo, hi, mn, mx);
-        std::memcpy(nodes[id].mn, mn, sizeof(mn));
-        std::memcpy(nodes[id].mx, mx, sizeof(mx));
         const uint32_t n = hi - lo;
-        if (n <= 4) {
-            nodes[id].first = lo;
-            nodes[id].count = n;
-            return id;
-        }
-        // binned SAH on box centres
-        constexpr int kBins = 12;
-        float best = FLT_MAX;
-        int best_ax = -1, best_bin = 0;
-        float cmin[3], cmax[3];
+        if (n <= leaf_max) return lo;
+        float scale[3];
         for (int k = 0; k < 3; ++k) {
-            cmin[k] = FLT_MAX;
-            cmax[k] = -FLT_MAX;
+            const float ext = cmx[k] - cmn[k];
+            scale[k] = ext > 0.0f ? kBins / ext : 0.0f;
         }
-        for (uint32_t i = lo; i < hi; ++i)
-            for (int k = 0; k < 3; ++k) {
-                const float c = 0.5f * (boxes[6 * ids[i] + k] + boxes[6 * ids[i] + 3 + k]);
-                cmin[k] = std::min(cmin[k], c);
-                cmax[k] = std::max(cmax[k], c);
-            }
-        for (int ax = 0; ax < 3; ++ax) {
-            const float ext = cmax[ax] - cmin[ax];
-            if (!(ext > 0.0f)) continue;
-            float bmn[kBins][3], bmx[kBins][3];
-            uint32_t cnt[kBins] = {};
+        float bmn[3][kBins][3], bmx[3][kBins][3];
+        uint32_t cnt[3][kBins] = {};
+        for (int ax = 0; ax < 3; ++ax)
             for (int b = 0; b < kBins; ++b)
                 for (int k = 0; k < 3; ++k) {
-                    bmn[b][k] = FLT_MAX;
-                    bmx[b][k] = -FLT_MAX;
+                    bmn[ax][b][k] = FLT_MAX;
+                    bmx[ax][b][k] = -FLT_MAX;
                 }
-            for (uint32_t i = lo; i < hi; ++i) {
-                const float* bx = &boxes[6 * ids[i]];
-                const float c = 0.5f * (bx[ax] + bx[3 + ax]);
-                int b = static_cast<int>((c - cmin[ax]) / ext * kBins);
-                b = std::min(std::max(b, 0), kBins - 1);
-                cnt[b]++;
+        for (uint32_t i = lo; i < hi; ++i) {
+            const float* bx = boxes + 6ull * ids[i];
+            const float* c = &cent[3ull * ids[i]];
+            for (int ax = 0; ax < 3; ++ax) {
+                const int b = std::min(kBins - 1, static_cast<int>((c[ax] - cmn[ax]) * scale[ax]));
+                cnt[ax][b]++;
                 for (int k = 0; k < 3; ++k) {
-                    bmn[b][k] = std::min(bmn[b][k], bx[k]);
-                    bmx[b][k] = std::max(bmx[b][k], bx[3 + k]);
+                    bmn[ax][b][k] = std::min(bmn[ax][b][k], bx[k]);
+                    bmx[ax][b][k] = std::max(bmx[ax][b][k], bx[3 + k]);
                 }
             }
-            for (int s = 1; s < kBins; ++s) {
-                float lmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-                float rmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, rmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-                uint32_t nl = 0, nr = 0;
-                for (int b = 0; b < kBins; ++b) {
-                    float* tmn = b < s ? lmn : rmn;
-                    float* tmx = b < s ? lmx : rmx;
-                    (b < s ? nl : nr) += cnt[b];
-                    for (int k = 0; k < 3; ++k) {
-                        tmn[k] = std::min(tmn[k], bmn[b][k]);
-                        tmx[k] = std::max(tmx[k], bmx[b][k]);
-                    }
+        }
+        float best = FLT_MAX;
+        int best_ax = -1, best_bin = 0;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (scale[ax] == 0.0f) continue;
+            // suffix areas and counts, then a prefix sweep
+            float ra[kBins];
+            uint32_t rc[kBins];
+            float smn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, smx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            uint32_t sc = 0;
+            for (int b = kBins - 1; b > 0; --b) {
+                for (int k = 0; k < 3; ++k) {
+                    smn[k] = std::min(smn[k], bmn[ax][b][k]);
+                    smx[k] = std::max(smx[k], bmx[ax][b][k]);
                 }
-                if (!nl || !nr) continue;
-                const float cost = nl * area(lmn, lmx) + nr * area(rmn, rmx);
+                sc += cnt[ax][b];
+                ra[b] = sc ? area(smn, smx) : 0.0f;
+                rc[b] = sc;
+            }
+            float pmn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, pmx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            uint32_t pc = 0;
+            for (int b = 1; b < kBins; ++b) {
+                for (int k = 0; k < 3; ++k) {
+                    pmn[k] = std::min(pmn[k], bmn[ax][b - 1][k]);
+                    pmx[k] = std::max(pmx[k], bmx[ax][b - 1][k]);
+                }
+                pc += cnt[ax][b - 1];
+                if (!pc || !rc[b]) continue;
+                const float cost = pc * area(pmn, pmx) + rc[b] * ra[b];
                 if (cost < best) {
                     best = cost;
                     best_ax = ax;
-                    best_bin = s;
+                    best_bin = b;
                 }
             }
         }
-        uint32_t mid;
-        if (best_ax < 0) {
-            mid = lo + n / 2;  // coincident centres: split the list
-        } else {
-            const float ext = cmax[best_ax] - cmin[best_ax];
-            auto it = std::partition(ids.begin() + lo, ids.begin() + hi, [&](uint32_t id2) {
-                const float c = 0.5f * (boxes[6 * id2 + best_ax] + boxes[6 * id2 + 3 + best_ax]);
-                int b = static_cast<int>((c - cmin[best_ax]) / ext * kBins);
-                b = std::min(std::max(b, 0), kBins - 1);
+        uint32_t mid = lo + n / 2;  // coincident centres: split the list
+        if (best_ax >= 0) {
+            const int ax = best_ax;
+            auto it = std::partition(ids.begin() + lo, ids.begin() + hi, [&](uint32_t id) {
+                const int b = std::min(kBins - 1, static_cast<int>((cent[3ull * id + ax] - cmn[ax]) * scale[ax]));
                 return b < best_bin;
             });
-            mid = static_cast<uint32_t>(it - ids.begin());
-            if (mid == lo || mid == hi) mid = lo + n / 2;
+            const uint32_t m = static_cast<uint32_t>(it - ids.begin());
+            if (m > lo && m < hi) mid = m;
         }
-        const int32_t a = build(lo, mid, level + 1, depth);
-        const int32_t b = build(mid, hi, level + 1, depth);
-        nodes[id].a = a;
-        nodes[id].b = b;
+        return mid;
+    }
+
+    // builds the subtree of ids[lo, hi) into out; returns its root index in out
+    int32_t grow(std::vector<Node>& out, uint32_t lo, uint32_t hi, uint32_t level, uint32_t& dep,
+                 int par_levels) {
+        dep = std::max(dep, level);
+        const int32_t id = static_cast<int32_t>(out.size());
+        out.emplace_back();
+        float mn[3], mx[3];
+        const uint32_t mid = split(lo, hi, mn, mx);
+        std::memcpy(out[id].mn, mn, sizeof(mn));
+        std::memcpy(out[id].mx, mx, sizeof(mx));
+        if (mid == lo) {
+            out[id].first = lo;
+            out[id].count = hi - lo;
+            return id;
+        }
+        int32_t a, b;
+        if (par_levels > 0 && hi - lo > 65536) {
+            std::vector<Node> sub;
+            uint32_t dep_a = 0;
+            int32_t ra = 0;
+            std::thread th([&] { ra = grow(sub, lo, mid, level + 1, dep_a, par_levels - 1); });
+            b = grow(out, mid, hi, level + 1, dep, par_levels - 1);
+            th.join();
+            dep = std::max(dep, dep_a);
+            const int32_t off = static_cast<int32_t>(out.size());
+            for (Node& nd : sub) {
+                if (nd.a >= 0) {
+                    nd.a += off;
+                    nd.b += off;
+                }
+                out.push_back(nd);
+            }
+            a = ra + off;
+        } else {
+            a = grow(out, lo, mid, level + 1, dep, 0);
+            b = grow(out, mid, hi, level + 1, dep, 0);
+        }
+        out[id].a = a;
+        out[id].b = b;
         return id;
+    }
+
+    int32_t build(const float* bx, uint32_t n, uint32_t lmax) {
+        boxes = bx;
+        leaf_max = lmax;
+        cent.resize(3ull * n);
+        ids.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            ids[i] = i;
+            for (int k = 0; k < 3; ++k) cent[3ull * i + k] = 0.5f * (bx[6ull * i + k] + bx[6ull * i + 3 + k]);
+        }
+        nodes.clear();
+        nodes.reserve(n ? 2ull * n / std::max(1u, lmax) + 2 : 1);
+        depth = 1;
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        int par = 0;
+        while ((1u << par) < hw) ++par;
+        return grow(nodes, 0, n, 1, depth, n > 65536 ? par : 0);
     }
 };
 
@@ -367,23 +422,20 @@ struct TBuild {
 uint32_t build_tlas(const std::vector<float>& boxes, std::vector<float4>& pairs,
                     std::vector<uint32_t>& indices, float root_min[3], float root_max[3],
                     uint32_t& root_word) {
-    TBuild tb(boxes);
+    SahTree tb;
     const uint32_t n = static_cast<uint32_t>(boxes.size() / 6);
-    tb.ids.resize(n);
-    for (uint32_t i = 0; i < n; ++i) tb.ids[i] = i;
-    uint32_t depth = 1;
-    const int32_t root = tb.build(0, n, 1, depth);
+    const int32_t root = tb.build(boxes.data(), n, 4);
     indices = tb.ids;
     // flatten: every inner node's children become one pair record, pair index = word
     pairs.clear();
     auto word_of = [&](int32_t id, auto& self) -> uint32_t {
-        const TBuild::Node& nd = tb.nodes[id];
+        const SahTree::Node& nd = tb.nodes[id];
         if (nd.a < 0) return kLeafBit | (nd.count << 24) | nd.first;
         const uint32_t q = static_cast<uint32_t>(pairs.size() / 4);
         pairs.resize(pairs.size() + 4);
         const uint32_t wa = self(nd.a, self), wb = self(nd.b, self);
-        const TBuild::Node& A = tb.nodes[nd.a];
-        const TBuild::Node& B = tb.nodes[nd.b];
+        const SahTree::Node& A = tb.nodes[nd.a];
+        const SahTree::Node& B = tb.nodes[nd.b];
         float4* r = &pairs[4 * q];
         r[0] = make_float4(A.mn[0], A.mn[1], A.mn[2], 0.0f);
         r[1] = make_float4(A.mx[0], A.mx[1], A.mx[2], 0.0f);
@@ -396,7 +448,74 @@ uint32_t build_tlas(const std::vector<float>& boxes, std::vector<float4>& pairs,
     root_word = word_of(root, word_of);
     std::memcpy(root_min, tb.nodes[root].mn, sizeof(float) * 3);
     std::memcpy(root_max, tb.nodes[root].mx, sizeof(float) * 3);
-    return depth;
+    return tb.depth;
+}
+
+uint32_t build_fast_bvh(const float* P, uint32_t tri_count, std::vector<float4>& pairs,
+                        std::vector<float4>& prims, DevMesh& dm) {
+    dm.pair_base = static_cast<uint32_t>(pairs.size() / 4);
+    dm.prim_base = static_cast<uint32_t>(prims.size() / 3);
+    dm.prim_count = tri_count;
+    auto emit = [&](uint32_t t, uint32_t leaf, bool last) {
+        const float* v = P + static_cast<uint64_t>(t) * 9;
+        const uint32_t lf = last ? kPrimLast : 0u;
+        float4 a = make_float4(v[0], v[1], v[2], 0.0f), b = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], 0.0f),
+               c = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.0f);
+        std::memcpy(&a.w, &t, 4);
+        std::memcpy(&b.w, &leaf, 4);
+        std::memcpy(&c.w, &lf, 4);
+        prims.push_back(a);
+        prims.push_back(b);
+        prims.push_back(c);
+    };
+    if (tri_count <= 4) {  // one leaf: a linear scan
+        dm.root_word = 0;
+        for (uint32_t t = 0; t < tri_count; ++t) emit(t, 0, t + 1 == tri_count);
+        return 1;
+    }
+    std::vector<float> boxes(6ull * tri_count);
+    for (uint32_t t = 0; t < tri_count; ++t) {
+        const float* v = P + static_cast<uint64_t>(t) * 9;
+        for (int k = 0; k < 3; ++k) {
+            boxes[6ull * t + k] = std::min(v[k], std::min(v[3 + k], v[6 + k]));
+            boxes[6ull * t + 3 + k] = std::max(v[k], std::max(v[3 + k], v[6 + k]));
+        }
+    }
+    SahTree tb;
+    const int32_t root = tb.build(boxes.data(), tri_count, 4);
+    // DFS flatten: inner node -> its children's pair record (mesh-local index), leaf -> first
+    // prim slot (mesh-local) | kLeafBit, its triangles contiguous with the last one flagged
+    uint32_t slot = 0;
+    auto word_of = [&](int32_t id, auto& self) -> uint32_t {
+        const SahTree::Node& nd = tb.nodes[id];
+        if (nd.a < 0) {
+            const uint32_t w = kLeafBit | slot;
+            for (uint32_t k = 0; k < nd.count; ++k)
+                emit(tb.ids[nd.first + k], static_cast<uint32_t>(id), k + 1 == nd.count);
+            slot += nd.count;
+            return w;
+        }
+        const size_t q = pairs.size() / 4;
+        pairs.resize(pairs.size() + 4);
+        const uint32_t wa = self(nd.a, self), wb = self(nd.b, self);
+        const SahTree::Node& A = tb.nodes[nd.a];
+        const SahTree::Node& B = tb.nodes[nd.b];
+        float4* r = &pairs[4 * q];
+        r[0] = make_float4(A.mn[0], A.mn[1], A.mn[2], 0.0f);
+        r[1] = make_float4(A.mx[0], A.mx[1], A.mx[2], 0.0f);
+        r[2] = make_float4(B.mn[0], B.mn[1], B.mn[2], 0.0f);
+        r[3] = make_float4(B.mx[0], B.mx[1], B.mx[2], 0.0f);
+        std::memcpy(&r[0].w, &wa, 4);
+        std::memcpy(&r[1].w, &wb, 4);
+        return static_cast<uint32_t>(q - dm.pair_base);
+    };
+    dm.root_word = word_of(root, word_of);
+    dm.flags |= kMeshHasBvh;
+    for (int k = 0; k < 3; ++k) {
+        dm.root_min[k] = tb.nodes[root].mn[k];
+        dm.root_max[k] = tb.nodes[root].mx[k];
+    }
+    return tb.depth;
 }
 
 namespace {
@@ -430,6 +549,8 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     s->light_count = d->light_count;
 
     std::vector<DevMesh> meshes(d->mesh_count);
+    std::vector<DevMesh> fast_meshes(d->bvh.fast ? d->mesh_count : 0u);
+    uint64_t fast_pairs = 0, fast_slots = 0;
     std::vector<float4> pairs, prims;
     std::vector<uint8_t> last_flags;
     std::vector<float> normals, uvs;
@@ -536,9 +657,20 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             dm.flags |= kMeshHasUv;
             uvs.insert(uvs.end(), md.texcoord0, md.texcoord0 + static_cast<uint64_t>(md.tri_count) * 6);
         }
+        if (d->bvh.fast) {
+            DevMesh& fm = fast_meshes[i];
+            fm = dm;
+            fm.flags &= ~kMeshHasBvh;
+            const size_t p0 = pairs.size(), s0 = prims.size();
+            s->max_fast_depth = std::max(s->max_fast_depth, build_fast_bvh(P, md.tri_count, pairs, prims, fm));
+            fast_pairs += (pairs.size() - p0) / 4;
+            fast_slots += (prims.size() - s0) / 3;
+        }
     }
-    s->prim_slots = prims.size() / 3;
-    s->node_count = pairs.size() / 2;
+    if (pairs.size() / 4 >= kLeafBit || prims.size() / 3 >= kLeafBit)
+        return fail(MT_E_UNSUPPORTED, "more than 2^31 BVH node pairs or leaf slots");
+    s->prim_slots = prims.size() / 3 - fast_slots;
+    s->node_count = (pairs.size() / 4 - fast_pairs) * 2;
     s->triangle_count = tri_total;
 
     std::vector<DevInstance> inst(d->instance_count);
@@ -688,6 +820,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     if ((rc = dev_upload(s, images, &s->dev.images))) return rc;
     if ((rc = dev_upload(s, env, &s->dev.env))) return rc;
     if ((rc = dev_upload(s, lights, &s->dev.lights))) return rc;
+    if ((rc = dev_upload(s, fast_meshes, &s->dev.fast_meshes))) return rc;
     if ((rc = dev_upload(s, tlas_pairs, &s->dev.tlas_pairs))) return rc;
     if ((rc = dev_upload(s, tlas_inst, &s->dev.tlas_inst))) return rc;
     s->dev.instance_count = d->instance_count;

@@ -107,6 +107,7 @@ class BvhOptions:
     disable: bool = False
     max_triangles: int = 32
     max_depth: int = 32
+    fast: bool = False  # also build the SAH BVHs of MT_TRAVERSAL_FAST
 
 
 @dataclass
@@ -204,6 +205,7 @@ class Scene:
         desc.bvh.disable = int(self.bvh.disable)
         desc.bvh.max_triangles = self.bvh.max_triangles
         desc.bvh.max_depth = self.bvh.max_depth
+        desc.bvh.fast = int(self.bvh.fast)
         keep.extend([meshes, images, mats, insts, lights])
         return desc, keep
 
@@ -234,6 +236,7 @@ class Settings:
     paths_in_flight: int = 0
     count_work: bool = False
     debug_mode: int = 0  # MT_DEBUG_* (Config::DebugMode)
+    traversal: int = abi.MT_TRAVERSAL_REFERENCE
 
     def to_desc(self):
         s = abi.mt_settings()

@@ -48,13 +48,26 @@ class GpuScene:
                                        idx.ctypes.data_as(u32), C.byref(ni)))
         return nodes, idx
 
-    def trace(self, rays):
-        """IntersectScene on the GPU for rays (N, 6) {o, d}; returns a HIT_DTYPE array."""
+    def trace(self, rays, traversal=None):
+        """IntersectScene on the GPU for rays (N, 6) {o, d}; returns a HIT_DTYPE array.
+        ``traversal`` (MT_TRAVERSAL_*) selects mt_trace_closest_ex; None = mt_trace_closest."""
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
         hits = np.zeros(len(rays), HIT_DTYPE)
-        check(self.L.mt_trace_closest(self.h, len(rays), abi.fptr(rays),
-                                      hits.ctypes.data_as(C.POINTER(abi.mt_hit))))
+        hp = hits.ctypes.data_as(C.POINTER(abi.mt_hit))
+        if traversal is None:
+            check(self.L.mt_trace_closest(self.h, len(rays), abi.fptr(rays), hp))
+        else:
+            check(self.L.mt_trace_closest_ex(self.h, len(rays), abi.fptr(rays), hp, traversal))
         return hits
+
+    def occluded(self, rays, traversal=abi.MT_TRAVERSAL_REFERENCE):
+        """Shadow-ray query for rays (N, 7) {o, d, distance}: 1 where the closest hit is at
+        most that far (Tracer.cpp:317-323)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 7)
+        occ = np.zeros(len(rays), np.uint32)
+        check(self.L.mt_trace_occluded(self.h, len(rays), abi.fptr(rays),
+                                       occ.ctypes.data_as(C.POINTER(C.c_uint32)), traversal))
+        return occ
 
     def render(self, settings, camera=None, want_samples=False):
         """Synchronous render: returns (rgb (H, W, 3), samples or None, stats dict)."""

@@ -38,6 +38,8 @@ constexpr float kPi = 3.14159265358979323846f;
 // ---------------------------------------------------------------- vector math (GLM contract)
 struct V3 {
     float x, y, z;
+    float& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
 };
 struct V4 {
     float x, y, z, w;
@@ -215,6 +217,15 @@ struct Light {
     float size, inner, outer;
     float intensity;  // PrepareLights simplified intensity
 };
+// Median-split tree over the instances' world boxes (leaves of at most 4).  It only skips
+// instances whose box the ray misses: the entered set, its order and every result are those of
+// the reference's linear loop (Intersect.cpp:201-219).  Used when orc_render_opts.tlas is set,
+// to time the CPU tracer without its 5,000-instance loop.
+struct TlasNode {
+    V3 mn, mx;
+    int a = -1, b = -1;
+    uint32_t first = 0, count = 0;
+};
 struct Scene {
     std::vector<Mesh> meshes;
     std::vector<Image> images;
@@ -224,6 +235,8 @@ struct Scene {
     Image env;
     bool hasEnv = false;
     int envFilter = 1;
+    std::vector<TlasNode> tlas;
+    std::vector<uint32_t> tlasIds;
 };
 
 // ---------------------------------------------------------------- BVH build (BVH.cpp)
@@ -526,11 +539,57 @@ bool intersectInstance(const Scene& S, int ii, const Ray& r, bool faithfulInvers
     return false;
 }
 
-Hit intersectScene(const Scene& S, const Ray& r, bool faithfulInverse, Counters* c) {
+int buildTlasNode(Scene& S, uint32_t lo, uint32_t hi) {
+    const int id = static_cast<int>(S.tlas.size());
+    S.tlas.emplace_back();
+    V3 mn = v3(FLT_MAX), mx = v3(-FLT_MAX), cmn = v3(FLT_MAX), cmx = v3(-FLT_MAX);
+    for (uint32_t k = lo; k < hi; ++k) {
+        const Instance& I = S.instances[S.tlasIds[k]];
+        const V3 c = 0.5f * (I.wmin + I.wmax);
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], I.wmin[a]);
+            mx[a] = std::max(mx[a], I.wmax[a]);
+            cmn[a] = std::min(cmn[a], c[a]);
+            cmx[a] = std::max(cmx[a], c[a]);
+        }
+    }
+    S.tlas[id].mn = mn;
+    S.tlas[id].mx = mx;
+    if (hi - lo <= 4) {
+        S.tlas[id].first = lo;
+        S.tlas[id].count = hi - lo;
+        return id;
+    }
+    int ax = 0;
+    for (int a = 1; a < 3; ++a)
+        if (cmx[a] - cmn[a] > cmx[ax] - cmn[ax]) ax = a;
+    const uint32_t mid = lo + (hi - lo) / 2;
+    std::nth_element(S.tlasIds.begin() + lo, S.tlasIds.begin() + mid, S.tlasIds.begin() + hi,
+                     [&](uint32_t x, uint32_t y) {
+                         const float cx = S.instances[x].wmin[ax] + S.instances[x].wmax[ax];
+                         const float cy = S.instances[y].wmin[ax] + S.instances[y].wmax[ax];
+                         return cx < cy;
+                     });
+    const int a = buildTlasNode(S, lo, mid);
+    const int b = buildTlasNode(S, mid, hi);
+    S.tlas[id].a = a;
+    S.tlas[id].b = b;
+    return id;
+}
+
+void buildTlas(Scene& S) {
+    S.tlas.clear();
+    S.tlasIds.resize(S.instances.size());
+    for (uint32_t i = 0; i < S.tlasIds.size(); ++i) S.tlasIds[i] = i;
+    if (!S.instances.empty()) buildTlasNode(S, 0, static_cast<uint32_t>(S.instances.size()));
+}
+
+Hit intersectScene(const Scene& S, const Ray& r, bool faithfulInverse, Counters* c,
+                   bool useTlas = false) {
     // Intersect.cpp:201-219
     float lowest = FLT_MAX;
     Hit best;
-    for (int i = 0; i < static_cast<int>(S.instances.size()); ++i) {
+    auto visit = [&](int i) {
         Hit h;
         if (intersectInstance(S, i, r, faithfulInverse, h, c)) {
             if (h.dist < lowest) {
@@ -538,7 +597,28 @@ Hit intersectScene(const Scene& S, const Ray& r, bool faithfulInverse, Counters*
                 best = h;
             }
         }
+    };
+    if (!useTlas || S.tlas.empty()) {
+        for (int i = 0; i < static_cast<int>(S.instances.size()); ++i) visit(i);
+        return best;
     }
+    // the instances whose box the tree does not rule out, visited in index order
+    thread_local std::vector<uint32_t> cand, stack;
+    cand.clear();
+    stack.assign(1, 0u);
+    while (!stack.empty()) {
+        const TlasNode& n = S.tlas[stack.back()];
+        stack.pop_back();
+        if (slab(r, n.mn, n.mx) < 0.0f) continue;
+        if (n.a < 0) {
+            for (uint32_t k = 0; k < n.count; ++k) cand.push_back(S.tlasIds[n.first + k]);
+        } else {
+            stack.push_back(static_cast<uint32_t>(n.a));
+            stack.push_back(static_cast<uint32_t>(n.b));
+        }
+    }
+    std::sort(cand.begin(), cand.end());
+    for (uint32_t i : cand) visit(static_cast<int>(i));
     return best;
 }
 
@@ -763,6 +843,7 @@ struct Tracer {
     Settings cfg;
     uint32_t lightStride;
     bool faithfulInverse;
+    bool useTlas = false;
 
     V3 materialBase(const Material& m, float u, float v, float& metal, float& rough, V3& nrmTex,
                     V3& emission) const {  // Material.hpp:82-135
@@ -807,7 +888,7 @@ struct Tracer {
             const uint32_t k = static_cast<uint32_t>(path.size());
             if (!rng.mt) rng.base = dimBase + k * lightStride;
             PathStep step;
-            const Hit h = intersectScene(S, cur, faithfulInverse, c);
+            const Hit h = intersectScene(S, cur, faithfulInverse, c, useTlas);
             rays++;
             if (!h.valid) {
                 step.ray = cur;
@@ -922,7 +1003,7 @@ struct Tracer {
             Ray sr;
             sr.o = step.hitPos + lvec * 0.0001f;
             sr.d = ldir;
-            const Hit sh = intersectScene(S, sr, faithfulInverse, c);
+            const Hit sh = intersectScene(S, sr, faithfulInverse, c, useTlas);
             rays++;
             const float llen = length(lvec);
             if (!sh.valid || sh.dist > llen) {
@@ -1514,6 +1595,7 @@ void* orc_scene_create(const mt_scene_desc* desc) {
             delete S;
             return nullptr;
         }
+        buildTlas(*S);
         return S;
     } catch (const std::exception& e) {
         g_err = e.what();
@@ -1755,7 +1837,7 @@ int orc_render(const void* sp, const mt_camera_desc* cam, const mt_settings* st,
     orc_render_opts o{};
     if (opts) o = *opts;
     Tracer tr{S, {*st, std::pow(2.0f, -st->ev_exposure)},
-              8u + static_cast<uint32_t>(S.lights.size()), o.faithful_inverse != 0};
+              8u + static_cast<uint32_t>(S.lights.size()), o.faithful_inverse != 0, o.tlas != 0};
     const uint32_t W = st->width, H = st->height, spp = st->samples;
     const uint32_t rb = (o.row_end > o.row_begin) ? o.row_begin : 0;
     const uint32_t re = (o.row_end > o.row_begin) ? std::min(o.row_end, H) : H;

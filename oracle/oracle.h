@@ -26,6 +26,8 @@ typedef struct orc_render_opts {
     int32_t faithful_inverse; /* 1 = glm::inverse per ray and instance, as Intersect.cpp:152 */
     uint32_t row_begin;       /* render only rows [row_begin, row_end) (0,0 = all)  */
     uint32_t row_end;
+    int32_t tlas;             /* 1 = skip instances through a tree over their world boxes
+                                 (same entered instances, order and results)          */
 } orc_render_opts;
 
 typedef struct orc_work {

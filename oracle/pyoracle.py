@@ -16,7 +16,7 @@ ORC_RNG_COUNTER, ORC_RNG_MT19937 = 0, 1
 
 class orc_render_opts(C.Structure):
     _fields_ = [("threads", C.c_int32), ("rng", C.c_int32), ("faithful_inverse", C.c_int32),
-                ("row_begin", C.c_uint32), ("row_end", C.c_uint32)]
+                ("row_begin", C.c_uint32), ("row_end", C.c_uint32), ("tlas", C.c_int32)]
 
 
 class orc_work(C.Structure):
@@ -167,14 +167,15 @@ class OracleScene:
         return out
 
     def render(self, settings, camera=None, threads=0, rng=ORC_RNG_COUNTER, faithful_inverse=True,
-               rows=None, want_samples=False):
+               rows=None, want_samples=False, tlas=False):
         cam = (camera or self.scene.camera).to_desc()
         st = settings.to_desc()
         W, H, S = settings.width, settings.height, settings.samples
         rgb = np.zeros((H, W, 3), np.float32)
         samples = np.zeros((H * W, S, 3), np.float32) if want_samples else None
         o = orc_render_opts(threads=threads, rng=rng, faithful_inverse=int(faithful_inverse),
-                            row_begin=rows[0] if rows else 0, row_end=rows[1] if rows else 0)
+                            row_begin=rows[0] if rows else 0, row_end=rows[1] if rows else 0,
+                            tlas=int(tlas))
         w = orc_work()
         secs = C.c_double()
         rc = self.L.orc_render(self.h, C.byref(cam), C.byref(st), C.byref(o), abi.fptr(rgb),

@@ -137,3 +137,20 @@ def test_russian_roulette_is_unbiased():
     ma, mb = sa.mean(axis=(0, 1)), sb.mean(axis=(0, 1))
     se = np.sqrt(sa.reshape(-1, 3).var(0) / sa.size * 3 + sb.reshape(-1, 3).var(0) / sb.size * 3)
     assert np.all(np.abs(ma - mb) < 4 * se), (ma, mb, se)
+
+
+def test_instance_tree_gives_the_linear_loop_results():
+    """orc_render_opts.tlas skips instances whose world box the ray misses and visits the rest
+    in index order: images, ray counts and per-instance work equal the reference's linear loop
+    (Intersect.cpp:201-219) bit for bit; only the box tests drop."""
+    sc = scenegen.large_scene(terrain_res=48, n_spheres=150, tex_res=32, env_width=32, env_height=16)
+    o = po.OracleScene(sc)
+    st = Settings(width=40, height=24, samples=2, bounces=4, seed=3, hdri_rotation=30.0,
+                  hdri_rotation_apply=True, count_work=True)
+    a, _, wa, _ = o.render(st, threads=4)
+    b, _, wb, _ = o.render(st, threads=4, tlas=True)
+    o.close()
+    np.testing.assert_array_equal(a, b)
+    for k in ("rays", "samples", "inst_entered", "node_pops", "tri_tests", "hits_shaded", "misses"):
+        assert wa[k] == wb[k], k
+    assert wb["inst_aabb_tests"] < wa["inst_aabb_tests"] / 4
