@@ -292,6 +292,7 @@ def main():
     from mitrace_amd.tracer import GpuScene
 
     scene = scenegen.sphere_grid(seed=1)
+    scene.bvh.fast = True  # also the fast traversal's BVHs (the timed frames use the reference ones)
 
     def settings(**kw):
         kw.setdefault("samples", args.spp)
@@ -363,6 +364,22 @@ def main():
                                  ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
         "cpu_baseline": None,
     }
+    # the same frame with the fast traversal (true closest hits; not the headline value)
+    from mitrace_amd import abi
+    g.render(settings(traversal=abi.MT_TRAVERSAL_FAST))
+    _, _, fc = g.render(settings(traversal=abi.MT_TRAVERSAL_FAST, count_work=True))
+    ft = []
+    for i in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        _, _, fs = g.render(settings(traversal=abi.MT_TRAVERSAL_FAST))
+        ft.append(time.perf_counter() - t0)
+    line["fast_traversal"] = {
+        "msamples_per_s": fs["samples"] * len(ft) / sum(ft) / 1e6,
+        "mrays_per_s": fs["rays"] * len(ft) / sum(ft) / 1e6,
+        "per_ray": {k: fc[k] / fc["rays"] for k in
+                    ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}}
+    print(f"# fast traversal: {line['fast_traversal']['msamples_per_s']:.1f} Msamples/s",
+          file=sys.stderr, flush=True)
     if not args.no_large:
         line["large"], cache = large_leg(args)
         if cache is not None:

@@ -66,7 +66,7 @@ extern "C" {
 #define MT_MODE_PARITY 0 /* the reference estimator, quirks included (Tracer.cpp:134-348) */
 #define MT_MODE_FIXED 1  /* Russian roulette instead of the energy cut-off                  */
 /* per-sample output */
-/* Traversal.  REFERENCE visits exactly the reference's nodes and triangles (Q4-Q7: the
+/* Traversal.  REFERENCE visits exactly the reference's nodes and triangles (Intersect.cpp:9-219: the
  * reference's BVHs, exit-distance culling with its 0.1 slack, every instance whose world box
  * the ray line crosses, closest-hit shadow rays), so hits are the reference's, including its
  * occasional non-closest ones.  FAST uses per-mesh SAH BVHs with tight boxes (built when the
@@ -108,7 +108,7 @@ extern "C" {
 
 /* ---- scene description (mirrors Scene::Mesh / MeshInstance / Material / Light / Camera) - */
 
-/* A de-indexed triangle soup (Scene::Mesh, src/Scene/Mesh.hpp:13-42; Q24): corner k of
+/* A de-indexed triangle soup (Scene::Mesh, src/Scene/Mesh.hpp:13-42): corner k of
  * triangle t is element 3*t+k of every per-corner array. */
 typedef struct mt_mesh_desc {
     uint32_t tri_count;
@@ -219,7 +219,7 @@ typedef struct mt_settings {
     float hdri_secondary_intensity;
     float hdri_primary_intensity;
     float emission_base_intensity;
-    int32_t firefly_enable;       /* useStatisticFireflyElimination (Q19: CLI default off) */
+    int32_t firefly_enable;       /* useStatisticFireflyElimination (CLI default off, Config.cpp:166-170) */
     float firefly_threshold;
     float terminate_energy;
     uint32_t image_block_size;    /* tile edge for progress/kill granularity          */
@@ -228,7 +228,7 @@ typedef struct mt_settings {
     int32_t mode;                 /* MT_MODE_*                                        */
     uint32_t rr_depth;            /* MT_MODE_FIXED: roulette starts after this depth  */
     int32_t output;               /* MT_OUTPUT_*                                      */
-    int32_t hdri_rotation_apply;  /* 0 = ignore rotation like the reference (Q21)     */
+    int32_t hdri_rotation_apply;  /* 0 = ignore rotation like the reference (Texture.cpp:35-61) */
     uint32_t paths_in_flight;     /* wavefront size; 0 = automatic                    */
     int32_t count_work;           /* 1 = gather traversal work counters (slower)      */
     int32_t debug_mode;           /* debugMode (Config.hpp:74), MT_DEBUG_*            */
@@ -240,7 +240,7 @@ typedef struct mt_stats {
     float progress;
     float time_elapsed;
     float estimated_time_remaining;
-    uint64_t rays;    /* camera + extension + shadow rays (Q26) */
+    uint64_t rays;    /* camera + extension + shadow rays (Tracer.cpp:146, 319) */
     uint64_t samples; /* camera paths                           */
     /* traversal work, filled when mt_settings.count_work != 0 */
     uint64_t inst_aabb_tests;
@@ -348,6 +348,12 @@ typedef struct mt_cache_options {
     uint32_t warmup_passes;   /* training-only passes before the first counted pass (0: none) */
     uint32_t steps_per_pass;  /* at most this many Adam steps after every chunk [8]        */
     uint32_t spp_per_pass;    /* samples per pixel of one pass [1]                          */
+    uint32_t query_after_light; /* 1: a querying path takes the emission and the light sample
+                                   of its query vertex and stops instead of tracing the next
+                                   bounce; the cache then stands for the light arriving
+                                   through that bounce; training paths record only their
+                                   query_depth vertex, and query_depth 0 (the camera ray's
+                                   hit) is allowed [0]                                      */
 } mt_cache_options;
 
 typedef struct mt_scene mt_scene;

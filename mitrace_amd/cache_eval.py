@@ -12,6 +12,7 @@ import time
 
 import numpy as np
 
+from . import abi
 from .cache import RadianceCache
 
 DEFAULT_CACHE = {"train_every": 16, "query_depth": 2, "warmup_passes": 4, "steps_per_pass": 8,
@@ -40,7 +41,14 @@ def _spp_for(rmse_target, ladder):
 def compare(gscene, settings_fn, spp=64, small=(320, 180), ref_spp=4096, ladder=(4, 8, 16, 32, 64),
             cache_opts=None, log=print):
     opts = dict(DEFAULT_CACHE, **(cache_opts or {}))
-    out = {"cache_options": opts, "spp": spp}
+    out = {"cache_options": opts, "spp": spp, "output": "linear"}
+    # linear radiance on both sides: the reference tone-maps every sample before averaging
+    # (Tracer.cpp:212-227), so any estimator with less variance would change the expected tone-mapped image
+    user_fn = settings_fn
+
+    def settings_fn(**kw):
+        kw.setdefault("output", abi.MT_OUTPUT_LINEAR)
+        return user_fn(**kw)
     # throughput at full resolution
     st = settings_fn(samples=spp, seed=5)
     gscene.render(settings_fn(samples=1, seed=4))  # first-touch warm-up

@@ -413,7 +413,7 @@ int mt_cache_create(mt_scene* scene, const mt_cache_options* options, mt_cache**
         if (!c->opt.train_batch) c->opt.train_batch = 16384;
         if (!(c->opt.learning_rate > 0.0f)) c->opt.learning_rate = 5e-3f;
         if (!c->opt.train_every) c->opt.train_every = 16;
-        if (!c->opt.query_depth) c->opt.query_depth = 1;
+        if (!c->opt.query_depth && !c->opt.query_after_light) c->opt.query_depth = 1;
         if (!c->opt.steps_per_pass) c->opt.steps_per_pass = 8;
         if (!c->opt.spp_per_pass) c->opt.spp_per_pass = 1;
         if (c->opt.train_batch > (1u << 22)) return mt::fail(MT_E_UNSUPPORTED, "train_batch above 4M");

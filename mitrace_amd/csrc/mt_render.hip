@@ -656,6 +656,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
             const V3 d = xyz(b.ray_d[p]);
             const int inst = b.hit_inst[p];
             const V3 trad = xyz(b.t_rad[p]);
+            if (kDiff == 0 && !kBatch && c.cache && c.cache_after && k > c.cache_depth) {
+                // the training record of the previous vertex: everything collected up to and
+                // including its light sample (the rest is what the bounce ray brought)
+                const uint32_t slot = b.train_slot[p];
+                const uint32_t j = k - 1 - c.cache_depth;
+                if (slot < c.train_cap && j < c.train_nrec) {
+                    const size_t r = static_cast<size_t>(slot) * c.train_nrec + j;
+                    b.tr_s[r] = f4(add(add(add(xyz(b.acc_e[p]), xyz(b.acc_d[p])), xyz(b.acc_i[p])),
+                                       xyz(b.acc_s[p])), 0.0f);
+                    const float4 tw = b.tr_w[r];
+                    b.tr_w[r] = make_float4(tw.x, tw.y, tw.z, 1.0f);
+                }
+            }
             if (kDiff == 0 && !kBatch && c.debug) {  // one more path step; the second one's ray
                 uint4 dj = b.dbg_j[p];
                 dj.w += 4u;
@@ -680,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                         add_texel_grad(gr, pms, mul(dl, mul(trad, sk)));
                     }
                 }
-            } else if (kDiff == 0 && !kBatch && c.cache && k == c.cache_depth &&
+            } else if (kDiff == 0 && !kBatch && c.cache && !c.cache_after && k == c.cache_depth &&
                        b.train_slot[p] == kNoSlot) {
                 // the cache supplies the radiance leaving this vertex toward the previous one
                 hits++;
@@ -698,7 +711,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                                              xyz(b.acc_s[p]));
                         b.tr_pos[r] = b.hit_pos[p];
                         b.tr_dir[r] = f4(neg(d), 0.0f);
-                        b.tr_w[r] = f4(trad, 1.0f);
+                        // after-light records become valid when the path goes on (next bounce)
+                        b.tr_w[r] = f4(trad, c.cache_after ? 0.0f : 1.0f);
                         b.tr_s[r] = f4(sofar, 0.0f);
                     }
                 }
@@ -892,7 +906,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     b.adj_s[p] = f4(ssum, 0.0f);
                     b.adj_prev[p] = make_float4(tu, tv, __uint_as_float(ms.tex ? mi : 0xFFFFFFFFu), 0.0f);
                 }
-                if (!stop && !last) {
+                if (!stop && !last && kDiff == 0 && !kBatch && c.cache && c.cache_after &&
+                    k == c.cache_depth && b.train_slot[p] == kNoSlot) {
+                    // a path that would go on stops here; the cache supplies what the bounce
+                    // ray would have brought (scaled by the throughput at this vertex)
+                    query = true;
+                    qpos = hp;
+                    qdir = f4(view, 0.0f);
+                } else if (!stop && !last) {
                     cont = true;
                     const V3 no = add(add(o, scale(d, dist)), scale(N, kPullout));
                     b.ray_o[p] = f4(no, 0.0f);
@@ -1686,7 +1707,7 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
     copt_ = opt;
     if (copt_.spp_per_pass == 0) copt_.spp_per_pass = 1;
     if (copt_.train_every == 0) copt_.train_every = 16;
-    if (copt_.query_depth == 0) copt_.query_depth = 1;
+    if (copt_.query_depth == 0 && !copt_.query_after_light) copt_.query_depth = 1;
     if (copt_.train_batch == 0) copt_.train_batch = 16384;
     if (copt_.steps_per_pass == 0) copt_.steps_per_pass = 8;
     if (st.samples % copt_.spp_per_pass)
@@ -1702,7 +1723,9 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
     int rc = setup(scene, cam, pass, false, std::min<uint32_t>(env_u32("MT_STREAMS", 3), 8));
     if (rc) return rc;
     const uint64_t P = capacity_;
-    const uint32_t nrec = st.bounces - copt_.query_depth;
+    // after-light: the querying paths ask only at query_depth, so the cache learns only there
+    // (deeper vertices have fewer bounces left and would pull its estimate down)
+    const uint32_t nrec = copt_.query_after_light ? 1u : st.bounces - copt_.query_depth;
     const uint64_t tslots = P + 1;  // warm-up passes may train on every path
     float* img_sum = nullptr;
     if ((rc = alloc(owned_, &img_sum, npix_total * 3)) ||
@@ -1726,6 +1749,7 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
     HIP_TRY(hipStreamCreateWithFlags(&tail_stream_, hipStreamNonBlocking));
     params_.cache = 1;
     params_.cache_depth = copt_.query_depth;
+    params_.cache_after = copt_.query_after_light ? 1 : 0;
     params_.train_cap = static_cast<uint32_t>(tslots);
     params_.train_nrec = nrec;
     return MT_OK;

@@ -77,6 +77,7 @@ struct ChunkParams {
     // cache_depth and take the rest of their radiance from the cache
     int32_t cache;
     uint32_t cache_depth, train_every, train_offset, train_cap, train_nrec;
+    int32_t cache_after;  // the query follows the vertex's emission and light sample
     int32_t debug;  // MT_DEBUG_* view (plain renders)
     int32_t accel;  // 1 = fast traversal (MT_TRAVERSAL_FAST)
 };

@@ -1,7 +1,7 @@
 // mt_scene.cpp -- host side of mt_scene_create: the reference's per-mesh BVH build and the
 // flattening of the Scene into the HBM layout described in mt_internal.hpp.
 //
-// The BVH rules follow src/Scene/BVH.cpp (Q1-Q3): split plane at the vertex-mean centroid,
+// The BVH rules follow src/Scene/BVH.cpp:29-272: split plane at the vertex-mean centroid,
 // all three axes tried, score = (|L|+|R|)/|N| (FLT_MAX when one side keeps everything),
 // straddling triangles go to both children, child boxes are the parent box clipped at the
 // plane, children of a node are an adjacent pair in DFS order, node 1 is an unused slot.

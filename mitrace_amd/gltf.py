@@ -1,14 +1,14 @@
 """glTF 2.0 reader with the reference loader's semantics (src/Loader/GLTF.cpp).
 
 It reads the subset the reference accepts -- external ``.bin`` buffers, triangle primitives
-(mode 4) with indices, tightly packed FLOAT/UINT32/UINT16 accessors (byteStride ignored, Q24),
-pbrMetallicRoughness materials with the reference's defaults (Q25), KHR_texture_transform
+(mode 4) with indices, tightly packed FLOAT/UINT32/UINT16 accessors (byteStride ignored),
+pbrMetallicRoughness materials with the reference's defaults (Types.hpp:65-82), KHR_texture_transform
 offset/scale, KHR_lights_punctual and EXT_light_attributes area lights, perspective and
 orthographic cameras -- and produces a :class:`mitrace_amd.scene.Scene`, i.e. what
 ``Scene::Scene(loader.LoadScene(i))`` plus ``SetCamera`` build in main.cpp:100-107.
 
 The EXT_sky environment is parsed by the reference loader but dropped by ``Scene::Scene``
-(Q21), so it is not read here; an environment comes only from ``hdri_override``.
+(Scene.cpp:53-59), so it is not read here; an environment comes only from ``hdri_override``.
 """
 import json
 import os
@@ -36,7 +36,7 @@ def _quat_to_mat3(q):
 
 
 def load_image_rgb8(path):
-    """stbi_load(path, ..., 3) equivalent: 8-bit RGB, alpha dropped (Image.cpp:77-104, Q20)."""
+    """stbi_load(path, ..., 3) equivalent: 8-bit RGB, alpha dropped (Image.cpp:77-104)."""
     if path.endswith(".hdr") or path.endswith(".exr"):
         raise RuntimeError("Cannot load float image")  # Image.cpp:83-88
     from PIL import Image as PILImage
@@ -178,7 +178,7 @@ class GLTF:
             mat.metallic_roughness_tex = self._texture(pbr["metallicRoughnessTexture"])
         if "normalTexture" in md:
             mat.normal_tex = self._texture(md["normalTexture"])
-        mat.normal_scale = float(md.get("normalScale", 1.0))  # material-level key (Q9)
+        mat.normal_scale = float(md.get("normalScale", 1.0))  # material-level key (GLTF.cpp:480)
         if "occlusionTexture" in md:
             mat.occlusion_tex = self._texture(md["occlusionTexture"])
         mat.occlusion_strength = float(md.get("occlusionStrength", 1.0))

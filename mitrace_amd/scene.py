@@ -1,7 +1,7 @@
 """Host-side scene model mirroring the reference's ``Scene`` layer.
 
 ``Scene`` holds what ``Scene::Scene`` (src/Scene/Scene.cpp:53-59) builds from a loaded glTF:
-de-indexed triangle-soup meshes (src/Scene/Mesh.cpp:24-41, Q24), per-instance materials and
+de-indexed triangle-soup meshes (src/Scene/Mesh.cpp:24-41), per-instance materials and
 model matrices (src/Scene/Mesh.hpp:44-72), lights (src/Scene/Light.hpp:36-53), the camera
 (src/Scene/Camera.cpp:5-27), decoded 8-bit RGB images (src/Scene/Image.cpp:77-104) and an
 optional environment map.  ``Scene.to_desc()`` packs it into the C-ABI ``mt_scene_desc``.
@@ -223,7 +223,7 @@ class Settings:
     hdri_secondary_intensity: float = 350.0
     hdri_primary_intensity: float = 350.0
     emission_base_intensity: float = 1.0
-    # the CLI flag is inverted (Config.cpp:166-170, 239): the effective default is OFF (Q19)
+    # the CLI flag is inverted (Config.cpp:166-170, 239): the effective default is OFF
     firefly_enable: bool = False
     firefly_threshold: float = 6.0
     terminate_energy: float = 0.01

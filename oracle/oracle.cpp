@@ -645,7 +645,7 @@ RGBA sampleImage(const Image& im, float uvx, float uvy, int filter) {  // Image.
         iy = std::max(iy, 0);
         return texel(im, ix, iy);
     }
-    // "linear" with weights in uv units (Image.cpp:26-44, Q20); x0/y0 are clamped to the last
+    // "linear" with weights in uv units (Image.cpp:26-44); x0/y0 are clamped to the last
     // texel where the reference would read past the row (x*W rounding up to W).
     int x0 = static_cast<int>(std::floor(x * static_cast<float>(im.w)));
     int y0 = static_cast<int>(std::floor(y * static_cast<float>(im.h)));
@@ -669,7 +669,7 @@ V3 sampleEnv(const Scene& S, V3 d, bool applyRot, float rotDeg) {  // Image.cpp:
     const float phi = std::acos(d.y / length(d));
     float u = (theta + kPi) / (2.0f * kPi);
     const float v = phi / kPi;
-    if (applyRot) u = u * 1.0f + rotDeg / 360.0f;  // TransformUv offset that Q21 bypasses
+    if (applyRot) u = u * 1.0f + rotDeg / 360.0f;  // TransformUv offset that SampleEquirectangular bypasses
     const RGBA c = sampleImage(S.env, u, v, S.envFilter);
     return {c.r, c.g, c.b};
 }
@@ -914,7 +914,7 @@ struct Tracer {
             float metal, rough;
             V3 ntex, emission;
             V3 base = materialBase(mat, tu, tv, metal, rough, ntex, emission);
-            base = base * 0.96f + v3(0.04f);  // Tracer.cpp:159 (Q10)
+            base = base * 0.96f + v3(0.04f);  // Tracer.cpp:159
             step.ray = cur;
             step.hitPos = h.wpos;
             step.base = base;
@@ -970,7 +970,7 @@ struct Tracer {
             emitted = emitted + cE * st.emission_base_intensity * step.emission;
             int chosen = -1;
             float chosenPdf = 0.0f, totalScore = 0.0f;
-            for (size_t i = 0; i < S.lights.size(); ++i) {  // Tracer.cpp:263-275 (Q15)
+            for (size_t i = 0; i < S.lights.size(); ++i) {  // Tracer.cpp:263-275
                 const V3 lp = S.lights[i].pos - step.hitPos;
                 const float d2 = dot(lp, lp);
                 const float score = S.lights[i].intensity / (d2 + 1.0f);
@@ -981,7 +981,7 @@ struct Tracer {
                 }
             }
             chosenPdf /= totalScore;
-            if (chosen == -1) continue;  // Q14: skips the throughput update
+            if (chosen == -1) continue;  // Tracer.cpp:278-279: skips the throughput update
             const Light& L = S.lights[chosen];
             V3 lpos;
             if (L.type == MT_LIGHT_POINT || L.type == MT_LIGHT_SPOT) {  // Tracer.cpp:287-298

@@ -18,7 +18,7 @@ extern "C" {
 #endif
 
 #define ORC_RNG_COUNTER 0 /* the keyed counter RNG the GPU uses (per (pixel, sample, dim)) */
-#define ORC_RNG_MT19937 1 /* std::mt19937 per tile, the reference's consumption order (Q23) */
+#define ORC_RNG_MT19937 1 /* std::mt19937 per tile, the reference's consumption order (Tracer.cpp:21, 563-586) */
 
 typedef struct orc_render_opts {
     int32_t threads;          /* worker threads (0 = hardware_concurrency)          */

@@ -70,7 +70,7 @@ def test_cornell_fixture_loads_like_the_reference():
     # 3 glTF meshes; the walls mesh has 4 primitives -> 6 instances
     assert len(sc.instances) == 6 and len(sc.meshes) == 6
     assert sum(m.tri_count for m in sc.meshes) == 42
-    # EXT_light_attributes area light -> a point light of zero jitter at the node (Q16)
+    # EXT_light_attributes area light -> a point light of zero jitter at the node (GLTF.cpp:558-571)
     (l,) = sc.lights
     assert l.type == abi.MT_LIGHT_AREA
     np.testing.assert_allclose(l.position, (0, 1.99, 1.0550744533538818), rtol=1e-6)
@@ -78,7 +78,7 @@ def test_cornell_fixture_loads_like_the_reference():
     cam = sc.camera
     assert cam.type == abi.MT_CAMERA_PERSPECTIVE and cam.yfov == pytest.approx(0.6911112070083618)
     np.testing.assert_allclose(cam.camera_to_world[:3, 3], (0, 1, -3))
-    # material defaults (Q25): present without textures
+    # material defaults (Types.hpp:65-82): present without textures
     red = [m for m in sc.materials if m.name == "WallRed"][0]
     assert red.metallic_factor == 0 and red.roughness_factor == 0.5
     assert render_params("cornell_box") == {"width": 512, "height": 512, "samples": 512, "ev-exposure": 6}
@@ -94,7 +94,7 @@ def test_material_spheres_fixture():
     tex = sc.materials[0].base_color_tex
     assert tex.image == 0 and tex.filter == abi.MT_FILTER_LINEAR  # minFilter 9987
     assert sc.lights[0].color == pytest.approx((2717.5706532941126, 2717.5706532941126 * 0.85, 2717.5706532941126 * 0.7), rel=1e-6)
-    assert sc.lights[0].point_size == 0.1  # KHR_lights_punctual "size" default (Q16)
+    assert sc.lights[0].point_size == 0.1  # KHR_lights_punctual "size" default (GLTF.cpp:558-571)
 
 
 def test_light_creep_orthographic_camera():

@@ -112,3 +112,52 @@ def test_zero_cache_equals_plain_render_cut_at_the_query_bounce(cornell):
     c.set_weights(w)
     img1, _ = c.render(st)
     assert np.all(img1 >= img) and (img1 > img).mean() > 0.5
+
+
+@pytest.mark.parametrize("depth", [0, 1])
+def test_after_light_zero_cache_equals_plain_render_ending_after_the_query_vertex(cornell, depth):
+    """query_after_light: a querying path takes its query vertex's emission and light sample
+    and stops; with an all-zero cache that is the plain render with bounces = depth + 1."""
+    from mitrace_amd.cache import RadianceCache
+    c = RadianceCache(cornell, {"train_every": 0xFFFFFFFF, "query_depth": depth,
+                                "query_after_light": 1})
+    c.set_weights(np.zeros(14787, np.float32))
+    st = small_settings(width=24, height=20, samples=4, bounces=8, seed=12,
+                        output=abi.MT_OUTPUT_LINEAR)
+    img, _ = c.render(st)
+    cut = small_settings(width=24, height=20, samples=4, bounces=depth + 1, seed=12,
+                         output=abi.MT_OUTPUT_LINEAR)
+    ref, _, _ = cornell.render(cut)
+    np.testing.assert_array_equal(img, ref)
+
+
+def test_after_light_with_all_training_paths_equals_plain(cornell):
+    from mitrace_amd.cache import RadianceCache
+    st = small_settings(width=24, height=20, samples=6, bounces=6, seed=10)
+    ref, _, _ = cornell.render(st)
+    c = RadianceCache(cornell, {"train_every": 1, "warmup_passes": 1, "train_batch": 4096,
+                                "query_depth": 0, "query_after_light": 1})
+    img, _ = c.render(st)
+    np.testing.assert_array_equal(img, ref)
+
+
+def test_trained_cache_beats_plain_at_equal_samples(cornell):
+    """What the trained network gains: in the Cornell box with Russian roulette (long paths,
+    indirect light on every wall), the after-light cache at the camera ray's hit gives a
+    lower RMSE against a 2048-spp plain render than the plain render with the same samples,
+    with a small mean bias."""
+    from mitrace_amd.cache import RadianceCache
+    kw = dict(width=128, height=128, bounces=12, mode=abi.MT_MODE_FIXED, rr_depth=3,
+              output=abi.MT_OUTPUT_LINEAR)
+    ref, _, _ = cornell.render(small_settings(samples=2048, seed=100, **kw))
+    plain, _, _ = cornell.render(small_settings(samples=16, seed=101, **kw))
+    c = RadianceCache(cornell, {"train_every": 8, "query_depth": 0, "query_after_light": 1,
+                                "warmup_passes": 8, "spp_per_pass": 4, "steps_per_pass": 8,
+                                "train_batch": 8192, "learning_rate": 5e-3})
+    img, _ = c.render(small_settings(samples=16, seed=101, **kw))
+    e_plain = np.sqrt(np.mean((plain - ref) ** 2))
+    e_cache = np.sqrt(np.mean((img - ref) ** 2))
+    bias = abs(img.mean() - ref.mean()) / ref.mean()
+    print(f"rmse plain {e_plain:.5f} cached {e_cache:.5f} relative bias {bias:.4f}")
+    assert e_cache < 0.8 * e_plain
+    assert bias < 0.05

@@ -60,7 +60,7 @@ def test_fast_closest_hit_is_the_exact_closest_hit(scenes, name):
     np.testing.assert_allclose(fh["distance"][other], bh["distance"][other], rtol=1e-4, atol=1e-6)
     # float32 vs float64 on the same triangle; grazing hits are ill-conditioned, hence 1e-3
     np.testing.assert_allclose(fh["distance"][same], bh["distance"][same], rtol=1e-3, atol=1e-5)
-    # never farther than the reference traversal's hit (which may be non-closest, Q5)
+    # never farther than the reference traversal's hit (which may be non-closest)
     rb = both & (rh["instance"] >= 0)
     assert np.all(fh["distance"][rb] <= rh["distance"][rb] * (1 + 1e-4) + 1e-6)
     # the world hit point lies on the ray at the reported distance

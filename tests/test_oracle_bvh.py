@@ -1,4 +1,4 @@
-"""The oracle's BVH build (BVH.cpp, Q1-Q3) and traversal (Intersect.cpp, Q4-Q7) pinned by
+"""The oracle's BVH build (BVH.cpp) and traversal (Intersect.cpp) pinned by
 structural invariants, a third independent restatement of the split rules in numpy float32,
 and brute-force closest hits (no GPU)."""
 import numpy as np
@@ -155,7 +155,7 @@ def test_bvh_invariants(name):
                 continue
             assert b == a + 1 and a % 2 == 0  # siblings are an adjacent pair
             ca, cb = f[a], f[b]
-            # child boxes = parent box clipped at one plane (Q1): all faces equal but one
+            # child boxes = parent box clipped at one plane (BVH.cpp:65-115): all faces equal but one
             diff_a = np.where(ca != f[n])[0]
             diff_b = np.where(cb != f[n])[0]
             assert len(diff_a) <= 1 and len(diff_b) <= 1
@@ -169,7 +169,7 @@ def test_bvh_invariants(name):
 
 @pytest.mark.parametrize("name", ["cornell_box", "material_spheres", "sphere_grid_coarse"])
 def test_traversal_against_brute_force(name):
-    """Exit-distance culling with 0.1 slack (Q5) may return a non-closest hit; the rate is
+    """Exit-distance culling with 0.1 slack (Intersect.cpp:51-62) may return a non-closest hit; the rate is
     reported and bounded, and every reported hit is a real intersection of that triangle."""
     sc = load_scene(name)
     o = po.OracleScene(sc)
@@ -190,7 +190,7 @@ def test_traversal_against_brute_force(name):
 
 
 def test_instance_transform_and_world_distance():
-    """Q7: the hit distance is |P - O| in world space for a scaled, rotated, translated mesh."""
+    """Intersect.cpp:143-199: the hit distance is |P - O| in world space for a scaled, rotated, translated mesh."""
     mesh = scenegen.unit_cube()
     M = np.array(scenegen.trs((1.0, 2.0, 3.0), 30.0, (2.0, 0.5, 1.0)), np.float32)
     cam = Camera(type=0, camera_to_world=np.eye(4, dtype=np.float32), yfov=1.0)

@@ -31,7 +31,7 @@ def settings(**kw):
 
 
 def test_sky_term_uses_previous_base_colour_and_no_throughput():
-    """Q10 + Q14: with no lights the radiance throughput stays 1, so a path floor -> sky is
+    """Tracer.cpp:159, 278-279: with no lights the radiance throughput stays 1, so a path floor -> sky is
     worth exactly base' * sky * secondary intensity with base' = 0.96 base + 0.04."""
     base = np.array([0.5, 0.3, 0.2], np.float32)
     o = po.OracleScene(floor_scene(base=tuple(base)))
@@ -56,7 +56,7 @@ def test_primary_miss_uses_primary_intensity():
 def test_energy_cutoff_and_bounce_limit():
     o = po.OracleScene(floor_scene())
     _, s, w, _ = o.render(settings(terminate_energy=1e9), want_samples=True)
-    assert np.all(s == 0) and w["rays"] == w["samples"]  # Q13: stops after the first hit
+    assert np.all(s == 0) and w["rays"] == w["samples"]  # Tracer.cpp:206-208: stops after the first hit
     _, s, w, _ = o.render(settings(bounces=1), want_samples=True)
     assert np.all(s == 0) and w["rays"] == w["samples"]
     _, s, w, _ = o.render(settings(bounces=0), want_samples=True)
@@ -64,7 +64,7 @@ def test_energy_cutoff_and_bounce_limit():
 
 
 def test_direct_light_closed_form():
-    """Q15-Q17 with one point light of radius 0: chosen with pdf 1, contribution
+    """Tracer.cpp:257-336 with one point light of radius 0: chosen with pdf 1, contribution
     f(L, V, N) * NL * lightColour / (d^2 + 1) at the (single) hit point."""
     light = Light(abi.MT_LIGHT_POINT, (1.0, 2.0, 0.5), (30.0, 20.0, 10.0), point_size=0.0)
     sc = floor_scene(base=(0.6, 0.6, 0.6), rough=0.4, metal=0.2, lights=[light], env_value=0.0, ortho=True)
@@ -83,7 +83,7 @@ def test_direct_light_closed_form():
 
 
 def test_occluded_light_contributes_nothing():
-    """Q17: a shadow ray blocked by geometry closer than the light adds nothing."""
+    """Tracer.cpp:317-336: a shadow ray blocked by geometry closer than the light adds nothing."""
     light = Light(abi.MT_LIGHT_POINT, (0.0, 2.0, 0.0), (30.0, 30.0, 30.0), point_size=0.0)
     sc = floor_scene(lights=[light], env_value=0.0, ortho=True)
     blocker = scenegen.unit_cube("blocker")
