@@ -42,8 +42,8 @@ def parse():
                     help="approximate wall time of the CPU-baseline sample")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the large-scene leg (plain traversals, CPU oracle, cache)")
-    ap.add_argument("--cache-fast", action="store_true",
-                    help="run the radiance-cache comparison with the fast traversal")
+    ap.add_argument("--cache-reference", action="store_true",
+                    help="run the radiance-cache comparison with the reference traversal")
     ap.add_argument("--no-cache", action="store_true",
                     help="skip the plain vs radiance-cached comparison on the large scene")
     ap.add_argument("--no-batch", action="store_true",
@@ -269,14 +269,19 @@ def large_leg(args):
             "fast_traversal_vs_instance_tree": fast / out["cpu_oracle_instance_tree"]["value"]}
     cache = None
     if not args.no_cache:
-        trav = abi.MT_TRAVERSAL_FAST if args.cache_fast else abi.MT_TRAVERSAL_REFERENCE
+        # against the fastest plain form (fast traversal) unless asked otherwise; two cache
+        # configurations: after-light queries at the first bounce's hit and at the camera hit
+        trav = abi.MT_TRAVERSAL_REFERENCE if args.cache_reference else abi.MT_TRAVERSAL_FAST
 
         def fn(**kw):
             kw.setdefault("traversal", trav)
             return scenegen.large_scene_settings(**kw)
-        cache = cache_eval.compare(g, fn, log=lambda m: print(m, file=sys.stderr, flush=True))
-        cache["workload"] = out["workload"] + ", 64 spp"
-        cache["traversal"] = "fast" if args.cache_fast else "reference"
+        cache = {"workload": out["workload"] + ", 64 spp, linear radiance",
+                 "traversal": "reference" if args.cache_reference else "fast", "configs": []}
+        for opts in ({"query_depth": 1, "query_after_light": 1},
+                     {"query_depth": 0, "query_after_light": 1}):
+            cache["configs"].append(cache_eval.compare(
+                g, fn, cache_opts=opts, log=lambda m: print(m, file=sys.stderr, flush=True)))
     g.close()
     return out, cache
 

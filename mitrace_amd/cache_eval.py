@@ -28,9 +28,8 @@ def _spp_for(rmse_target, ladder):
     s = np.log([x[0] for x in ladder])
     r = np.log([x[1] for x in ladder])
     t = np.log(rmse_target)
-    if t >= r[0]:  # already below the target at the smallest count: extrapolate on the slope
-        k = (r[1] - r[0]) / (s[1] - s[0])
-        return float(np.exp(s[0] + (t - r[0]) / k)) if k < 0 else float(np.exp(s[0]))
+    if t >= r[0]:  # already below the target at the smallest count measured: at most that
+        return float(ladder[0][0])
     for i in range(len(s) - 1):
         if r[i + 1] <= t <= r[i]:
             f = (t - r[i]) / (r[i + 1] - r[i])
@@ -83,10 +82,14 @@ def compare(gscene, settings_fn, spp=64, small=(320, 180), ref_spp=4096, ladder=
             out["plain_bias"] = float(np.mean(img.astype(np.float64) - ref))
     out["rmse_plain"] = lad_p
     out["rmse_cached"] = lad_c
-    out["reference"] = {"width": W, "height": H, "spp": ref_spp}
+    ref_mean = float(np.mean(ref.astype(np.float64)))
+    out["reference"] = {"width": W, "height": H, "spp": ref_spp, "mean": ref_mean}
+    out["cached_bias_relative"] = out["cached_bias"] / ref_mean if ref_mean else None
     target = lad_p[-1][1]
     need = _spp_for(target, lad_c)
-    out["equal_rmse"] = {"rmse": target, "plain_spp": ladder[-1], "cached_spp": need}
+    out["equal_rmse"] = {"rmse": target, "plain_spp": ladder[-1], "cached_spp": need,
+                         "cached_spp_is_upper_bound": bool(need is not None and need <= ladder[0]
+                                                           and lad_c[0][1] <= target)}
     if need:
         # the cached render at full resolution with the samples it needs for that RMSE
         # (warm-up and training included), against the plain render at `spp`

@@ -906,13 +906,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     b.adj_s[p] = f4(ssum, 0.0f);
                     b.adj_prev[p] = make_float4(tu, tv, __uint_as_float(ms.tex ? mi : 0xFFFFFFFFu), 0.0f);
                 }
-                if (!stop && !last && kDiff == 0 && !kBatch && c.cache && c.cache_after &&
-                    k == c.cache_depth && b.train_slot[p] == kNoSlot) {
-                    // a path that would go on stops here; the cache supplies what the bounce
-                    // ray would have brought (scaled by the throughput at this vertex)
+                const bool after_q = !stop && !last && kDiff == 0 && !kBatch && c.cache &&
+                                     c.cache_after && k == c.cache_depth;
+                if (after_q) {
+                    // a path that would go on asks the cache for what the bounce ray would
+                    // bring (scaled by the throughput at this vertex); a training path asks too
+                    // (its image takes the same estimate) and goes on to record its target
                     query = true;
                     qpos = hp;
                     qdir = f4(view, 0.0f);
+                }
+                if (after_q && b.train_slot[p] == kNoSlot) {
+                    // stops here
                 } else if (!stop && !last) {
                     cont = true;
                     const V3 no = add(add(o, scale(d, dist)), scale(N, kPullout));
@@ -1013,7 +1018,17 @@ __global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers 
     const ChunkParams& J = job_params<kBatch>(c, b, p);
     const V3 e = xyz(b.acc_e[p]), d = xyz(b.acc_d[p]), i = xyz(b.acc_i[p]), s = xyz(b.acc_s[p]);
     const V3 L = add(add(add(e, d), i), s);  // Tracer.cpp:344-345
-    V3 col = J.output_linear ? L : aces(L, J.exposure);
+    V3 Limg = L;
+    if (!kBatch && c.cache && c.cache_after) {
+        // a training path that went on past its query vertex: its image takes the cached
+        // estimate (radiance up to and including the vertex's light sample + its query)
+        const uint32_t slot = b.train_slot[p];
+        if (slot < c.train_cap) {
+            const size_t r = static_cast<size_t>(slot) * c.train_nrec;
+            if (b.tr_w[r].w > 0.0f) Limg = add(xyz(b.tr_s[r]), xyz(b.tr_img[p]));
+        }
+    }
+    V3 col = J.output_linear ? Limg : aces(Limg, J.exposure);
     if (!kBatch && c.debug) col = debug_color(c, b, p, e, d, i, s, col);
     b.sample_rgb[p] = f4(col, 0.0f);
     if (!kBatch && c.diff) {
@@ -1091,13 +1106,19 @@ __global__ __launch_bounds__(kBlock) void k_batch_resolve(PathBuffers b) {
 
 // ------------------------------------------------------------------ radiance cache passes
 // adds throughput x prediction to every path that stopped at a cache query
-__global__ __launch_bounds__(kBlock) void k_cache_apply(PathBuffers b) {
+__global__ __launch_bounds__(kBlock) void k_cache_apply(ChunkParams c, PathBuffers b) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= b.counters[kCntQuery]) return;
     const uint32_t p = b.cq_path[q];
     const float4 o = b.cq_out[q];
     const V3 pred = {gmax(o.x, 0.0f), gmax(o.y, 0.0f), gmax(o.z, 0.0f)};
     const V3 add_ = mul(xyz(b.t_rad[p]), pred);
+    if (c.cache_after && b.train_slot[p] < c.train_cap) {
+        // a training path keeps its full radiance for its target; its image value is formed
+        // in k_finalize from the radiance up to the query vertex and this estimate
+        b.tr_img[p] = f4(add_, 0.0f);
+        return;
+    }
     const float4 a = b.acc_i[p];
     b.acc_i[p] = make_float4(a.x + add_.x, a.y + add_.y, a.z + add_.z, 0.0f);
 }
@@ -1738,6 +1759,7 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
             (rc = alloc(owned_, &b.tr_w, tslots * nrec)) || (rc = alloc(owned_, &b.tr_s, tslots * nrec)) ||
             (rc = alloc(owned_, &b.cq_path, P)) || (rc = alloc(owned_, &b.cq_pos, P)) ||
             (rc = alloc(owned_, &b.cq_dir, P)) || (rc = alloc(owned_, &b.cq_out, P)) ||
+            (rc = alloc(owned_, &b.tr_img, P)) ||
             (rc = alloc(owned_, &b.tp_pos, tslots * nrec)) || (rc = alloc(owned_, &b.tp_dir, tslots * nrec)) ||
             (rc = alloc(owned_, &b.tp_tgt, tslots * nrec)))
             return rc;
@@ -1786,7 +1808,7 @@ int Renderer::run_cache_chunk(Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t 
     hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, m, b, st_.bounces, rays_dev_);
     int rc = mlp.infer(c.paths, b.cq_pos, b.cq_dir, b.cq_out, m, b.counters + kCntQuery);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_cache_apply, dim3(gen_blocks), dim3(kBlock), 0, m, b);
+    hipLaunchKernelGGL(k_cache_apply, dim3(gen_blocks), dim3(kBlock), 0, m, c, b);
     hipLaunchKernelGGL(k_finalize<false>, dim3(gen_blocks), dim3(kBlock), 0, m, c, b);
     if (accumulate)
         hipLaunchKernelGGL(k_cache_accum, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, m, c, b);

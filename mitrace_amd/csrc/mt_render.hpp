@@ -118,6 +118,8 @@ struct PathBuffers {
                                             // radiance collected before the vertex
     uint32_t* cq_path;      // cache queries: path, position, direction, prediction
     float4 *cq_pos, *cq_dir, *cq_out;
+    float4* tr_img;         // per path (after-light mode): throughput x prediction of a
+                            // training path's own query, the cached estimate its image uses
     float4 *tp_pos, *tp_dir, *tp_tgt;       // compact training samples
     float* img_sum;         // per pixel: sum of the accumulated passes' samples
     // debug views (first-hit records of ProcessDebugRay)
