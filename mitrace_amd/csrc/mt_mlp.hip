@@ -451,7 +451,7 @@ int Mlp::init(uint32_t seed, uint32_t train_capacity, float lr) {
     lr_ = lr;
     train_capacity = (train_capacity + 31) & ~31u;
     tb_.capacity = train_capacity;
-    tb_.blocks = 64;
+    tb_.blocks = 256;  // weight-gradient blocks: one per CU, each summing its slabs into a partial
     int rc;
     if ((rc = dalloc(owned_, &master_, kParams)) || (rc = dalloc(owned_, &adam_m_, kParams)) ||
         (rc = dalloc(owned_, &adam_v_, kParams)) || (rc = dalloc(owned_, &fwd_, kFwdSize)) ||

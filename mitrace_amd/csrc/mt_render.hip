@@ -612,9 +612,14 @@ __device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const Mat
 //   throughput w = f*cos/pdf        -> (L - S) * (dw/dtheta) / w with the sampling detached
 //   sky term       base'_{k-1}*T*sky -> d/d(base of the previous hit)
 // where T is the radiance throughput of the reference and S the radiance collected so far.
-template <int kDiff, bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, PathBuffers b,
-                                                  uint32_t k) {
+// kExt compiles in the debug-view records and the radiance-cache bookkeeping (plain renders
+// that use neither run the leaner kExt = false build: fewer registers, more waves in flight).
+#ifndef MT_SHADE_WAVES_PER_SIMD
+#define MT_SHADE_WAVES_PER_SIMD 4
+#endif
+template <int kDiff, bool kBatch, bool kExt>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT_SHADE_WAVES_PER_SIMD, 8)))
+void k_shade(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
     const uint32_t count = b.counters[kCntExtend + k];
     const uint32_t* qin = b.queue[k & 1];
     uint32_t* qout = b.queue[(k + 1) & 1];
@@ -656,7 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
             const V3 d = xyz(b.ray_d[p]);
             const int inst = b.hit_inst[p];
             const V3 trad = xyz(b.t_rad[p]);
-            if (kDiff == 0 && !kBatch && c.cache && c.cache_after && k > c.cache_depth) {
+            if (kDiff == 0 && !kBatch && kExt && c.cache && c.cache_after && k > c.cache_depth) {
                 // the training record of the previous vertex: everything collected up to and
                 // including its light sample (the rest is what the bounce ray brought)
                 const uint32_t slot = b.train_slot[p];
@@ -669,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     b.tr_w[r] = make_float4(tw.x, tw.y, tw.z, 1.0f);
                 }
             }
-            if (kDiff == 0 && !kBatch && c.debug) {  // one more path step; the second one's ray
+            if (kDiff == 0 && !kBatch && kExt && c.debug) {  // one more path step; the second one's ray
                 uint4 dj = b.dbg_j[p];
                 dj.w += 4u;
                 if (k == 1) {
@@ -693,7 +698,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                         add_texel_grad(gr, pms, mul(dl, mul(trad, sk)));
                     }
                 }
-            } else if (kDiff == 0 && !kBatch && c.cache && !c.cache_after && k == c.cache_depth &&
+            } else if (kDiff == 0 && !kBatch && kExt && c.cache && !c.cache_after && k == c.cache_depth &&
                        b.train_slot[p] == kNoSlot) {
                 // the cache supplies the radiance leaving this vertex toward the previous one
                 hits++;
@@ -702,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 qdir = f4(neg(d), 0.0f);
             } else {
                 hits++;
-                if (kDiff == 0 && !kBatch && c.cache && k >= c.cache_depth) {
+                if (kDiff == 0 && !kBatch && kExt && c.cache && k >= c.cache_depth) {
                     const uint32_t slot = b.train_slot[p];
                     if (slot < c.train_cap && k - c.cache_depth < c.train_nrec) {
                         // training vertex: radiance collected so far, throughput at the vertex
@@ -776,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 const V3 fb = eval_brdf(nd, view, N, basec, rough, metal);
                 const float nl = gmax(dot(N, nd), 0.0f);
                 V3 energy = (pdf > 0.0f) ? scale(fb, nl / pdf) : mk(0.0f);
-                if (kDiff == 0 && !kBatch && c.debug && k == 0) {  // first-hit debug record
+                if (kDiff == 0 && !kBatch && kExt && c.debug && k == 0) {  // first-hit debug record
                     const V3 fr = fresnel(gmax(dot(N, view), 0.0f), base_refl(basec, metal));
                     b.dbg_a[p] = f4(basec, metal);
                     b.dbg_b[p] = make_float4(rough, gN.x, gN.y, gN.z);
@@ -906,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                     b.adj_s[p] = f4(ssum, 0.0f);
                     b.adj_prev[p] = make_float4(tu, tv, __uint_as_float(ms.tex ? mi : 0xFFFFFFFFu), 0.0f);
                 }
-                const bool after_q = !stop && !last && kDiff == 0 && !kBatch && c.cache &&
+                const bool after_q = !stop && !last && kDiff == 0 && !kBatch && kExt && c.cache &&
                                      c.cache_after && k == c.cache_depth;
                 if (after_q) {
                     // a path that would go on asks the cache for what the bounce ray would
@@ -944,7 +949,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene S, ChunkParams c, Pat
                 b.sh_d[ss] = shd;
                 b.sh_c[ss] = shc;
             }
-            if (kDiff == 0 && !kBatch && c.cache) {
+            if (kDiff == 0 && !kBatch && kExt && c.cache) {
                 const uint32_t qs = wave_push(b.counters + kCntQuery, query);
                 if (query) {
                     b.cq_path[qs] = p;
@@ -1426,7 +1431,7 @@ void launch_batch_iteration(const mt_scene* scene, const ChunkParams& c, const P
     hipLaunchKernelGGL(k_batch_generate, dim3(grid), dim3(kBlock), 0, s, b);
     for (uint32_t k = 0; k < L.bounces; ++k) {
         launch_trace<0>(L.lds_depth, false, L.trace_blocks, s, scene->dev, c, b, k);
-        hipLaunchKernelGGL((k_shade<0, true>), dim3(L.shade_blocks), dim3(kBlock), 0, s, scene->dev, c, b, k);
+        hipLaunchKernelGGL((k_shade<0, true, false>), dim3(L.shade_blocks), dim3(kBlock), 0, s, scene->dev, c, b, k);
         if (scene->light_count) launch_trace<1>(L.lds_depth, false, L.trace_blocks, s, scene->dev, c, b, k);
     }
     hipLaunchKernelGGL(k_finalize<true>, dim3(grid), dim3(kBlock), 0, s, c, b);
@@ -1589,7 +1594,10 @@ int Renderer::enqueue_chunk(uint32_t slot, uint32_t chunk) {
         HIP_TRY(hipEventRecord(ev[0], s));
         launch_trace<0>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, b, k);
         HIP_TRY(hipEventRecord(ev[1], s));
-        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
+        if (c.debug || c.cache)
+            hipLaunchKernelGGL((k_shade<0, false, true>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
+        else
+            hipLaunchKernelGGL((k_shade<0, false, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
         HIP_TRY(hipEventRecord(ev[2], s));
         if (scene_->light_count)
             launch_trace<1>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, b, k);
@@ -1666,11 +1674,11 @@ int Renderer::run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s
     for (uint32_t k = 0; k < st_.bounces; ++k) {
         launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
         if (mode == 1) {
-            hipLaunchKernelGGL((k_shade<1, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+            hipLaunchKernelGGL((k_shade<1, false, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
             if (scene_->light_count)
                 launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, buf_, k);
         } else {
-            hipLaunchKernelGGL((k_shade<2, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
+            hipLaunchKernelGGL((k_shade<2, false, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, buf_, k);
         }
         HIP_TRY(hipGetLastError());
     }
@@ -1799,7 +1807,7 @@ int Renderer::run_cache_chunk(Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t 
     hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, b);
     for (uint32_t k = 0; k < st_.bounces; ++k) {
         launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, b, k);
-        hipLaunchKernelGGL((k_shade<0, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
+        hipLaunchKernelGGL((k_shade<0, false, true>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
         if (scene_->light_count) launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, b, k);
     }
     HIP_TRY(hipGetLastError());
