@@ -36,7 +36,11 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kLdsDepth = 16;
-constexpr uint32_t kRefillMin = 16;  // idle lanes that trigger a refill
+// idle lanes of a wave that trigger a refill of the traversal (measured best on the sphere grid:
+// the reference traversal's instance culling runs per refill, so it prefers mid-size refills;
+// the fast traversal's rays are short, so refilling less often wins)
+constexpr uint32_t kRefillMin = 24;
+constexpr uint32_t kRefillMinFast = 48;
 constexpr uint32_t kSetupMin = 8;    // lanes needing an instance set-up that trigger one
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -1419,7 +1423,8 @@ void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& ca
     c.debug = st.debug_mode;
     c.accel = st.traversal == MT_TRAVERSAL_FAST ? 1 : 0;
     c.stack_cap = stack_capacity(scene);
-    c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
+    c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, c.accel ? env_u32("MT_REFILL_MIN_FAST", kRefillMinFast)
+                                                                          : env_u32("MT_REFILL_MIN", kRefillMin)));
     c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
     c.inv_spp = 1.0f / static_cast<float>(st.samples);
 }
@@ -1489,7 +1494,9 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         return fail(MT_E_INVALID, "the scene was created without fast BVHs (mt_bvh_options.fast)");
     const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
     if (npix_total * 3 > 0xFFFFFFFFull) return fail(MT_E_UNSUPPORTED, "image too large");
-    uint64_t target = st.paths_in_flight ? st.paths_in_flight : (1ull << 22);
+    // 16 Mi paths per chunk by default (measured on the sphere grid: 4 Mi 375, 8 Mi 416,
+    // 16 Mi 446 Msamples/s; about 4 GB of path state per stream slot)
+    uint64_t target = st.paths_in_flight ? st.paths_in_flight : (1ull << 24);
     target = std::max<uint64_t>(target, st.samples);
     target = std::min<uint64_t>(target, 1ull << 26);
     npix_chunk_ = static_cast<uint32_t>(std::max<uint64_t>(1, target / st.samples));
@@ -1503,7 +1510,7 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     const int cus = device_cu_count();
     lds_depth_ = env_u32("MT_LDS_DEPTH", kLdsDepth);
     if (lds_depth_ != 4 && lds_depth_ != 8 && lds_depth_ != 12) lds_depth_ = 16;
-    trace_blocks_ = static_cast<uint32_t>(cus * env_u32("MT_TRACE_BLOCKS_PER_CU", 8));
+    trace_blocks_ = static_cast<uint32_t>(cus * env_u32("MT_TRACE_BLOCKS_PER_CU", 6));
     shade_blocks_ = static_cast<uint32_t>(std::min<uint64_t>((P + kBlock - 1) / kBlock, cus * 16ull));
     const uint32_t cap = stack_capacity(scene);
     const uint32_t spill_depth = cap > lds_depth_ ? cap - lds_depth_ : 1;
@@ -1521,8 +1528,6 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     ChunkParams& c = params_;
     fill_params(c, scene, cam, st);
     c.stack_cap = cap;
-    c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMin)));
-    c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
     c.inv_spp = 1.0f / static_cast<float>(st.samples);
     if (diff) {  // the differentiable image is the linear radiance, no firefly rejection
         c.output_linear = 1;
