@@ -378,6 +378,17 @@ int mt_scene_destroy(mt_scene* scene);
 int mt_scene_mesh_bvh(const mt_scene* scene, uint32_t mesh, uint32_t* nodes_out,
                       uint32_t* node_count, uint32_t* indices_out, uint32_t* index_count);
 
+/* The SAH BVH that mt_bvh_options.fast builds for one mesh, computed on the host only (no
+ * device is needed): positions = tri_count*9 floats (the mesh soup).  pairs_out receives 16
+ * floats per sibling pair {A.min, A.word} {A.max, B.word} {B.min, 0} {B.max, 0} (words as raw
+ * bits: pair index of an inner node, or first slot | 0x80000000 of a leaf), slot_tris_out the
+ * triangle of every leaf slot and slot_last_out 1 on a leaf's last slot.  Pass NULL buffers to
+ * query the counts.  No reference counterpart: the host-side test hook of the fast traversal's
+ * builder. */
+int mt_fast_bvh_build(const float* positions, uint32_t tri_count, float* pairs_out,
+                      uint32_t* pair_count, uint32_t* slot_tris_out, uint32_t* slot_last_out,
+                      uint32_t* slot_count, uint32_t* root_word, float* root_box);
+
 /* Synchronous render (main.cpp:121-142): writes width*height*3 floats to rgb_out.
  * sample_rgb_out, when not NULL, receives every per-sample colour ([pixel][sample][3]). */
 int mt_render(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,

@@ -211,6 +211,10 @@ SIGNATURES = {
     "mt_trace_occluded": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_uint32), C.c_int32]),
     "mt_brdf_eval": (C.c_int, [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mt_fast_bvh_build": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float),
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "mt_diff_create": (C.c_int, [C.c_void_p, C.POINTER(mt_settings), C.POINTER(C.c_void_p)]),
     "mt_diff_forward": (C.c_int, [C.c_void_p, C.POINTER(mt_camera_desc),
                                   C.POINTER(mt_diff_params), C.POINTER(mt_diff_params),

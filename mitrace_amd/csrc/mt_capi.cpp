@@ -2,7 +2,9 @@
 //
 // Errors never cross the ABI as exceptions: every entry point catches, records a message in a
 // thread-local slot (mt_last_error) and returns a negative MT_E* code.
+#include <algorithm>
 #include <atomic>
+#include <cfloat>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -206,6 +208,51 @@ int mt_scene_mesh_bvh(const mt_scene* scene, uint32_t mesh, uint32_t* nodes_out,
     if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(mt::HostNode));
     if (indices_out) std::memcpy(indices_out, b.indices.data(), b.indices.size() * sizeof(uint32_t));
     return MT_OK;
+}
+
+int mt_fast_bvh_build(const float* positions, uint32_t tri_count, float* pairs_out,
+                      uint32_t* pair_count, uint32_t* slot_tris_out, uint32_t* slot_last_out,
+                      uint32_t* slot_count, uint32_t* root_word, float* root_box) {
+    if (tri_count && !positions) return mt::fail(MT_E_INVALID, "positions is NULL");
+    try {
+        std::vector<float4> pairs, prims;
+        mt::DevMesh dm{};
+        mt::build_fast_bvh(positions, tri_count, pairs, prims, dm);
+        const uint32_t np = static_cast<uint32_t>(pairs.size() / 4);
+        const uint32_t ns = static_cast<uint32_t>(prims.size() / 3);
+        if (pair_count) *pair_count = np;
+        if (slot_count) *slot_count = ns;
+        if (root_word) *root_word = (dm.flags & mt::kMeshHasBvh) ? dm.root_word : (mt::kLeafBit | 0u);
+        if (root_box) {
+            if (dm.flags & mt::kMeshHasBvh) {
+                for (int k = 0; k < 3; ++k) {
+                    root_box[k] = dm.root_min[k];
+                    root_box[3 + k] = dm.root_max[k];
+                }
+            } else {  // one leaf: the bounds of its triangles
+                for (int k = 0; k < 3; ++k) {
+                    root_box[k] = FLT_MAX;
+                    root_box[3 + k] = -FLT_MAX;
+                }
+                for (uint64_t c = 0; c < 3ull * tri_count; ++c)
+                    for (int k = 0; k < 3; ++k) {
+                        root_box[k] = std::min(root_box[k], positions[3 * c + k]);
+                        root_box[3 + k] = std::max(root_box[3 + k], positions[3 * c + k]);
+                    }
+            }
+        }
+        if (pairs_out) std::memcpy(pairs_out, pairs.data(), sizeof(float4) * pairs.size());
+        for (uint32_t i = 0; i < ns; ++i) {
+            uint32_t t, last;
+            std::memcpy(&t, &prims[3 * i].w, 4);
+            std::memcpy(&last, &prims[3 * i + 2].w, 4);
+            if (slot_tris_out) slot_tris_out[i] = t;
+            if (slot_last_out) slot_last_out[i] = last & mt::kPrimLast;
+        }
+        return MT_OK;
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_NOMEM, e.what());
+    }
 }
 
 int mt_render(mt_scene* scene, const mt_camera_desc* camera, const mt_settings* settings,
