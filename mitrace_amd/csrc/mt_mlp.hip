@@ -59,8 +59,10 @@ __device__ __forceinline__ float encode(int e, const float p[3], float th, float
         x = (e < 28) ? th : ph;
         r = (e - 24) & 3;
     }
+    // |a| <= 8 pi: the hardware sine and cosine are accurate far below the BF16 rounding the
+    // encoding then takes
     const float a = kPiF * x * static_cast<float>(1 << (r >> 1));
-    return (r & 1) ? cosf(a) : sinf(a);
+    return (r & 1) ? __cosf(a) : __sinf(a);
 }
 
 struct EncCtx {
@@ -158,17 +160,24 @@ struct Net {
     const bf16x8* fwd;  // packed forward operands
     const bf16x8* bwd;  // packed backward operands
     const float* master;
+    const float* bias[kLayers];  // per layer (global master weights, or an LDS copy)
 };
+
+__host__ __device__ inline Net make_net(const uint16_t* fwd, const uint16_t* bwd, const float* master) {
+    Net N{reinterpret_cast<const bf16x8*>(fwd), reinterpret_cast<const bf16x8*>(bwd), master, {}};
+    for (int l = 0; l < kLayers; ++l) N.bias[l] = master + kBOff[l];
+    return N;
+}
 
 // full forward of one group of 32 samples; returns the 3 outputs in lanes g == 0 (acc5[0..2])
 __device__ __forceinline__ void forward(const Net& N, const bf16x8 enc[2], f32x16& out5,
                                         uint32_t masks[4], bf16x8 h[4][4], int lane) {
     f32x16 acc[2];
-    dense<2, 2>(N.fwd + kFwdOff[0] / 8, N.master + kBOff[0], enc, acc, lane);
+    dense<2, 2>(N.fwd + kFwdOff[0] / 8, N.bias[0], enc, acc, lane);
     relu_chunks(acc, h[0], masks[0]);
 #pragma unroll
     for (int l = 1; l < 4; ++l) {
-        dense<2, 4>(N.fwd + kFwdOff[l] / 8, N.master + kBOff[l], h[l - 1], acc, lane);
+        dense<2, 4>(N.fwd + kFwdOff[l] / 8, N.bias[l], h[l - 1], acc, lane);
         relu_chunks(acc, h[l], masks[l]);
     }
     f32x16 o[1];
@@ -185,6 +194,19 @@ __device__ __forceinline__ uint32_t step_count(const uint32_t* n_dev, uint32_t o
 
 __global__ __launch_bounds__(256) void k_mlp_infer(Net N, EncCtx E, uint32_t n_host, const uint32_t* n_dev,
                                                    const float4* pos, const float4* dir, float4* out) {
+    // the packed forward weights (32 KB) and the biases, staged once per block in LDS: every
+    // group of 32 queries reads its 32 operand fragments from there, not from the caches
+    __shared__ bf16x8 sW[kFwdSize / 8];
+    __shared__ float sB[4 * 64 + 3];
+    for (int i = threadIdx.x; i < kFwdSize / 8; i += blockDim.x) sW[i] = N.fwd[i];
+    for (int i = threadIdx.x; i < 4 * 64 + 3; i += blockDim.x) {
+        const int l = i / 64 < 4 ? i / 64 : 4;
+        sB[i] = N.bias[l][i - 64 * l];
+    }
+    __syncthreads();
+    Net L = N;
+    L.fwd = sW;
+    for (int l = 0; l < kLayers; ++l) L.bias[l] = sB + 64 * l;
     const uint32_t n = step_count(n_dev, 0, n_host);
     const int lane = threadIdx.x & 63, g = lane >> 5, s = lane & 31;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -199,9 +221,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Net N, EncCtx E, uint32_t n_h
         f32x16 o;
         uint32_t masks[4];
         bf16x8 h[4][4];
-        forward(N, enc, o, masks, h, lane);
+        forward(L, enc, o, masks, h, lane);
         if (ok && g == 0) {
-            const float* b = N.master + kBOff[4];
+            const float* b = L.bias[4];
             out[i] = make_float4(o[0] + b[0], o[1] + b[1], o[2] + b[2], 0.0f);
         }
     }
@@ -514,7 +536,7 @@ int Mlp::pack(hipStream_t s) {
 int Mlp::infer(uint32_t n, const float4* pos, const float4* dir, float4* out, hipStream_t s,
                 const uint32_t* n_dev) {
     if (n == 0) return MT_OK;
-    Net N{reinterpret_cast<const bf16x8*>(fwd_), reinterpret_cast<const bf16x8*>(bwd_), master_};
+    const Net N = make_net(fwd_, bwd_, master_);
     EncCtx E{{lo_[0], lo_[1], lo_[2]}, {inv_ext_[0], inv_ext_[1], inv_ext_[2]}};
     const uint32_t groups = (n + 31) / 32;
     const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 4096);
@@ -527,7 +549,7 @@ int Mlp::train(uint32_t n, const float4* pos, const float4* dir, const float4* t
     if (n == 0) return MT_OK;
     if (n > tb_.capacity) return fail(MT_E_INVALID, "training batch exceeds the cache's capacity");
     if (!trained_ && dalloc(owned_, &trained_, 1)) return MT_E_NOMEM;
-    Net N{reinterpret_cast<const bf16x8*>(fwd_), reinterpret_cast<const bf16x8*>(bwd_), master_};
+    const Net N = make_net(fwd_, bwd_, master_);
     EncCtx E{{lo_[0], lo_[1], lo_[2]}, {inv_ext_[0], inv_ext_[1], inv_ext_[2]}};
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, tb_.loss, 1u);
     const uint32_t groups = (n + 31) / 32;
