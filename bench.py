@@ -290,8 +290,9 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
-              "the multi-GPU sharding of this path is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): "
+              "a render would shard by pixel tiles, one GPU per tile set, and this measures "
+              "one GPU's share", file=sys.stderr)
         return 2
     from mitrace_amd import scenegen
     from mitrace_amd.tracer import GpuScene
