@@ -617,13 +617,14 @@ __device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const Mat
 //   sky term       base'_{k-1}*T*sky -> d/d(base of the previous hit)
 // where T is the radiance throughput of the reference and S the radiance collected so far.
 // kExt compiles in the debug-view records and the radiance-cache bookkeeping (plain renders
-// that use neither run the leaner kExt = false build: fewer registers, more waves in flight).
+// that use neither run the leaner kExt = false build: fewer registers, more waves in flight);
+// kLights = false is the build for scenes without point lights (no light choice, no shadow ray).
 #ifndef MT_SHADE_WAVES_PER_SIMD
 #define MT_SHADE_WAVES_PER_SIMD 4
 #endif
-template <int kDiff, bool kBatch, bool kExt>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT_SHADE_WAVES_PER_SIMD, 8)))
-void k_shade(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
+template <int kDiff, bool kBatch, bool kExt, bool kLights>
+__device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams& c, const PathBuffers& b,
+                                           uint32_t k) {
     const uint32_t count = b.counters[kCntExtend + k];
     const uint32_t* qin = b.queue[k & 1];
     uint32_t* qout = b.queue[(k + 1) & 1];
@@ -836,7 +837,8 @@ void k_shade(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
                 }
                 int chosen = -1;
                 float chosen_pdf = 0.0f, total = 0.0f;
-                for (uint32_t li = 0; li < S.light_count; ++li) {
+                const uint32_t n_lights = kLights ? S.light_count : 0u;
+                for (uint32_t li = 0; li < n_lights; ++li) {
                     const DevLight* L = S.lights + li;
                     const V3 lp = sub(V3{L->pos[0], L->pos[1], L->pos[2]}, wpos);
                     const float score = L->intensity / (dot(lp, lp) + 1.0f);
@@ -969,6 +971,19 @@ void k_shade(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
         wave_add_u64(b.work + 4, hits);
         wave_add_u64(b.work + 5, misses);
     }
+}
+
+template <int kDiff, bool kBatch, bool kExt, bool kLights = true>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT_SHADE_WAVES_PER_SIMD, 8)))
+void k_shade(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
+    shade_body<kDiff, kBatch, kExt, kLights>(S, c, b, k);
+}
+
+// plain shading of a scene without point lights: small enough to run five waves per SIMD
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void k_shade_unlit(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
+    shade_body<0, kBatch, false, false>(S, c, b, k);
 }
 
 // RandomColorFromU16 (Tracer.cpp:350-364)
@@ -1436,7 +1451,10 @@ void launch_batch_iteration(const mt_scene* scene, const ChunkParams& c, const P
     hipLaunchKernelGGL(k_batch_generate, dim3(grid), dim3(kBlock), 0, s, b);
     for (uint32_t k = 0; k < L.bounces; ++k) {
         launch_trace<0>(L.lds_depth, false, L.trace_blocks, s, scene->dev, c, b, k);
-        hipLaunchKernelGGL((k_shade<0, true, false>), dim3(L.shade_blocks), dim3(kBlock), 0, s, scene->dev, c, b, k);
+        if (scene->light_count)
+            hipLaunchKernelGGL((k_shade<0, true, false>), dim3(L.shade_blocks), dim3(kBlock), 0, s, scene->dev, c, b, k);
+        else
+            hipLaunchKernelGGL((k_shade_unlit<true>), dim3(L.shade_blocks), dim3(kBlock), 0, s, scene->dev, c, b, k);
         if (scene->light_count) launch_trace<1>(L.lds_depth, false, L.trace_blocks, s, scene->dev, c, b, k);
     }
     hipLaunchKernelGGL(k_finalize<true>, dim3(grid), dim3(kBlock), 0, s, c, b);
@@ -1511,7 +1529,8 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     lds_depth_ = env_u32("MT_LDS_DEPTH", kLdsDepth);
     if (lds_depth_ != 4 && lds_depth_ != 8 && lds_depth_ != 12) lds_depth_ = 16;
     trace_blocks_ = static_cast<uint32_t>(cus * env_u32("MT_TRACE_BLOCKS_PER_CU", 6));
-    shade_blocks_ = static_cast<uint32_t>(std::min<uint64_t>((P + kBlock - 1) / kBlock, cus * 16ull));
+    shade_blocks_ = static_cast<uint32_t>(
+        std::min<uint64_t>((P + kBlock - 1) / kBlock, cus * static_cast<uint64_t>(env_u32("MT_SHADE_BLOCKS_PER_CU", 16))));
     const uint32_t cap = stack_capacity(scene);
     const uint32_t spill_depth = cap > lds_depth_ ? cap - lds_depth_ : 1;
     const size_t trace_threads = static_cast<size_t>(trace_blocks_) * kBlock;
@@ -1601,8 +1620,10 @@ int Renderer::enqueue_chunk(uint32_t slot, uint32_t chunk) {
         HIP_TRY(hipEventRecord(ev[1], s));
         if (c.debug || c.cache)
             hipLaunchKernelGGL((k_shade<0, false, true>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
-        else
+        else if (scene_->light_count)
             hipLaunchKernelGGL((k_shade<0, false, false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
+        else
+            hipLaunchKernelGGL((k_shade_unlit<false>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
         HIP_TRY(hipEventRecord(ev[2], s));
         if (scene_->light_count)
             launch_trace<1>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, b, k);
