@@ -483,7 +483,9 @@ int mt_batch_get_job_info(const mt_batch* batch, uint32_t job_id, mt_batch_job_i
 int mt_batch_get_stats(const mt_batch* batch, mt_batch_stats* stats);
 int mt_batch_destroy(mt_batch* batch);
 /* Picks paths_per_iteration, tile_pixels, iteration_tiles and trace_blocks_per_cu by
- * coordinate descent over measured runs of the n calibration jobs (all submitted at once).
+ * coordinate descent over measured runs of the n calibration jobs.  Each trial creates a
+ * batch, serves the jobs once at 1 sample per pixel (graph captures, first touches; untimed),
+ * then submits them as given, all at once, and times that round to its finish.
  * trials (may be NULL) receives up to max_trials measurements. */
 int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
                       const mt_settings* settings, uint32_t n, mt_batch_options* best,

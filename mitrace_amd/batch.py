@@ -83,8 +83,10 @@ class Batch:
 
 
 def autotune(jobs, max_trials=32):
-    """jobs: [(gpu_scene, settings, camera or None)] calibration set, all submitted at once.
-    Returns (best options dict, trials [{options, seconds, msamples_per_s}])."""
+    """jobs: [(gpu_scene, settings, camera or None)] calibration set.  Each trial serves the
+    jobs once at 1 spp (untimed: graph captures, first touches), then times them as given,
+    all submitted at once.  Returns (best options dict, trials [{options, seconds,
+    msamples_per_s}])."""
     L = load()
     n = len(jobs)
     scenes = (C.c_void_p * n)(*[j[0].h.value for j in jobs])

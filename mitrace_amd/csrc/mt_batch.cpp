@@ -30,7 +30,6 @@ namespace {
 constexpr uint32_t kMaxJobs = 1u << 14;
 constexpr uint32_t kQueueTiles = 1u << 20;  // per scene
 constexpr uint32_t kMaxStack = 64;          // traversal stack entries (LDS + spill)
-constexpr uint32_t kLds = 16;
 constexpr uint32_t kBlockSize = 256;
 constexpr uint32_t kInflight = 3;           // iterations queued ahead of the device
 
@@ -92,6 +91,7 @@ struct mt_batch {
     mt::BatchJob* jobs_dev = nullptr;
     mt::BatchJob* jobs_host = nullptr;  // pinned staging, one record per job
     uint32_t trace_blocks = 0, shade_blocks = 0;
+    uint32_t lds_ref = 16, lds_fast = 16;  // traversal stack entries in LDS per traversal
     std::chrono::steady_clock::time_point created;
     // shared with submitters
     std::mutex mu;
@@ -178,7 +178,9 @@ int mt_batch::init(const mt_batch_options* o) {
     trace_blocks = static_cast<uint32_t>(cus) * opt.trace_blocks_per_cu;
     const uint64_t P = opt.paths_per_iteration;
     shade_blocks = static_cast<uint32_t>(std::min<uint64_t>((P + kBlockSize - 1) / kBlockSize, cus * 16ull));
-    const uint64_t spill = static_cast<uint64_t>(kMaxStack - kLds) * trace_blocks * kBlockSize;
+    lds_ref = mt::lds_stack_depth(false, opt.trace_blocks_per_cu);
+    lds_fast = mt::lds_stack_depth(true, opt.trace_blocks_per_cu);
+    const uint64_t spill = static_cast<uint64_t>(kMaxStack - std::min(lds_ref, lds_fast)) * trace_blocks * kBlockSize;
     auto dev_alloc = [&](void** p, size_t bytes) -> int {
         if (hipMalloc(p, bytes) != hipSuccess) return fail(MT_E_NOMEM, "hipMalloc of batch buffers failed");
         owned.push_back(*p);
@@ -299,7 +301,7 @@ int mt_batch::ensure_graph(SceneSlot& s, uint32_t lane) {
         q.exec = nullptr;
         q.graph = nullptr;
     }
-    mt::BatchLaunch L{s.bounces, kLds, trace_blocks, shade_blocks};
+    mt::BatchLaunch L{s.bounces, s.c.accel ? lds_fast : lds_ref, trace_blocks, shade_blocks};
     BT_TRY(hipStreamBeginCapture(capture, hipStreamCaptureModeRelaxed));
     mt::launch_batch_iteration(s.scene, s.c, q.b, L, capture);
     const hipError_t le = hipGetLastError();
@@ -579,15 +581,34 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
         int rc = mt_batch_create(&o, &b);
         if (rc) return rc;
         std::unique_ptr<mt_batch> guard(b);
-        uint64_t samples = 0;
+        // untimed round: every calibration job at 1 sample per pixel, so that each scene's
+        // graphs are captured and the buffers touched before the timed round
         for (uint32_t i = 0; i < n; ++i) {
             if (settings[i].samples > o.paths_per_iteration) return mt::fail(MT_E_INVALID, "calibration job too large");
-            rc = mt_batch_submit(b, scenes[i], &cameras[i], &settings[i], nullptr, nullptr);
-            if (rc) return rc;
+            mt_settings st = settings[i];
+            st.samples = 1;
+            if ((rc = mt_batch_submit(b, scenes[i], &cameras[i], &st, nullptr, nullptr))) return rc;
+        }
+        if ((rc = mt_batch_start(b))) return rc;
+        const auto tw = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < n; ++i) {
+            mt_batch_job_info info{};
+            for (;;) {
+                if ((rc = mt_batch_get_job_info(b, i, &info))) return rc;
+                if (info.status) return mt_batch_finish(b);  // reports the batch's error
+                if (info.done) break;
+                if (now_ms(tw) > 600e3) return mt::fail(MT_E_HIP, "autotune warm-up round did not complete");
+                std::this_thread::sleep_for(std::chrono::microseconds(200));
+            }
+        }
+        // timed round: the calibration jobs as given, submitted at once to the running batch
+        uint64_t samples = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < n; ++i) {
+            if ((rc = mt_batch_submit(b, scenes[i], &cameras[i], &settings[i], nullptr, nullptr))) return rc;
             samples += static_cast<uint64_t>(settings[i].width) * settings[i].height * settings[i].samples;
         }
-        const auto t0 = std::chrono::steady_clock::now();
-        if ((rc = mt_batch_start(b)) || (rc = mt_batch_finish(b))) return rc;
+        if ((rc = mt_batch_finish(b))) return rc;
         const double s = now_ms(t0) / 1e3;
         msps = samples / s / 1e6;
         if (trials && done < max_trials) trials[done] = {o, s, msps};
@@ -597,10 +618,9 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
     };
     mt_batch_options cur{1u << 22, 1024, 4096, 8, 3};
     double cur_v = 0.0;
-    int rc = measure(cur, cur_v);  // warm-up (first captures, first touches)
+    int rc = measure(cur, cur_v);
     if (rc) return rc;
-    if ((rc = measure(cur, cur_v))) return rc;
-    const uint32_t paths[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    const uint32_t paths[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24};
     const uint32_t tiles[] = {256, 1024, 4096, 16384};
     const uint32_t blocks[] = {4, 8, 12};
     const uint32_t itiles[] = {1024, 4096, 16384};
@@ -621,7 +641,7 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
         }
         return MT_OK;
     };
-    if ((rc = sweep(&mt_batch_options::paths_per_iteration, paths, 4)) ||
+    if ((rc = sweep(&mt_batch_options::paths_per_iteration, paths, 5)) ||
         (rc = sweep(&mt_batch_options::tile_pixels, tiles, 4)) ||
         (rc = sweep(&mt_batch_options::trace_blocks_per_cu, blocks, 3)) ||
         (rc = sweep(&mt_batch_options::iteration_tiles, itiles, 3)) ||

@@ -507,8 +507,8 @@ __device__ __forceinline__ bool trav_needs_setup(const Trav& t, const Stack& st)
 // Set-up step: finish the current instance (combine its hit), then start the next entered
 // instance.  Returns true when the ray is finished; sets need_mask when the lane moved on to
 // the next group of 64 instances.
-template <bool kCount>
-__device__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool& need_mask) {
+template <bool kCount, bool kDiag>
+__device__ __forceinline__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool& need_mask) {
     if (t.cur >= 0) {
         if (kCount) {
             w.pops += t.btests;
@@ -526,12 +526,14 @@ __device__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool&
                 t.R.dist = wd;
                 t.R.inst = static_cast<int>(ii);
                 t.R.tri = t.btri;
-                t.R.bv = t.bbv;
                 t.R.u = t.bu;
                 t.R.v = t.bvv;
                 t.R.wpos = wp;
-                t.R.bvh_tests = t.btests;
-                t.R.tri_tests = t.ttests;
+                if (kDiag) {
+                    t.R.bv = t.bbv;
+                    t.R.bvh_tests = t.btests;
+                    t.R.tri_tests = t.ttests;
+                }
             }
         }
         t.cur = -1;
@@ -606,8 +608,8 @@ __device__ bool trav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool&
 // One node pop (Intersect.cpp:67-78, 109-139).  The nearer accepted child is kept in
 // registers as the next node instead of being pushed and popped again, which is the same
 // visiting order.  Returns false when the pop was culled.
-template <bool kFast>
-__device__ bool trav_node(const DevScene& S, Trav& t, Stack& st) {
+template <bool kFast, bool kDiag>
+__device__ __forceinline__ bool trav_node(const DevScene& S, Trav& t, Stack& st) {
     uint32_t word;
     float nd;
     if (t.has_next) {
@@ -619,7 +621,7 @@ __device__ bool trav_node(const DevScene& S, Trav& t, Stack& st) {
         word = e.x;
         nd = __uint_as_float(e.y);
     }
-    t.btests++;
+    if (kDiag) t.btests++;
     if (nd < 0.0f) return false;
     if (nd > t.ldist + kMinDelta) return false;
     if (word & kLeafBit) {
@@ -655,18 +657,20 @@ __device__ bool trav_node(const DevScene& S, Trav& t, Stack& st) {
     return true;
 }
 
-// One triangle test of the open leaf (Intersect.cpp:84-105 / 24-44).
+// One triangle test of the open leaf (Intersect.cpp:84-105 / 24-44).  kDiag keeps the per-hit
+// diagnostics (tests of the winning instance, leaf index) of the counting and test builds.
+template <bool kDiag>
 __device__ __forceinline__ void trav_tri(const DevScene& S, Trav& t) {
     const float4* P = S.prims + 3ull * (t.prim_base + t.leaf);
     const float4 p0 = P[0], p1 = P[1], p2 = P[2];
     float u, v, tt;
-    t.ttests++;
+    if (kDiag) t.ttests++;
     if (tri_test(t.lo, t.ld, xyz(p0), xyz(p1), xyz(p2), u, v, tt)) {
         if (tt > 0.0f && tt < t.ldist) {
             t.ldist = tt;
             t.bu = u;
             t.bvv = v;
-            t.bbv = __float_as_uint(p1.w);
+            if (kDiag) t.bbv = __float_as_uint(p1.w);
             t.btri = __float_as_uint(p0.w);
         }
     }
@@ -708,8 +712,8 @@ __device__ __forceinline__ void ftrav_tlas_begin(const DevScene& S, Trav& t, Sta
 // Set-up step of the fast traversal: finish the current instance, then enter the next one
 // whose world box the ray enters before the closest hit so far.  Returns true when the ray is
 // finished (for kAny also as soon as an occluder is found).
-template <bool kCount, bool kAny>
-__device__ bool ftrav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool& need_mask) {
+template <bool kCount, bool kAny, bool kDiag>
+__device__ __forceinline__ bool ftrav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool& need_mask) {
     if (t.cur >= 0) {
         if (kCount) {
             w.pops += t.btests;
@@ -721,12 +725,14 @@ __device__ bool ftrav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool
             t.R.dist = t.ldist;
             t.R.inst = static_cast<int>(ii);
             t.R.tri = t.btri;
-            t.R.bv = t.bbv;
             t.R.u = t.bu;
             t.R.v = t.bvv;
             t.R.wpos = xform(S.instances[ii].M, lp, 1.0f);
-            t.R.bvh_tests = t.btests;
-            t.R.tri_tests = t.ttests;
+            if (kDiag) {
+                t.R.bv = t.bbv;
+                t.R.bvh_tests = t.btests;
+                t.R.tri_tests = t.ttests;
+            }
             if (kAny) return true;
         }
         t.cur = -1;
@@ -821,6 +827,7 @@ __device__ bool ftrav_setup(const DevScene& S, Trav& t, Stack& st, Work& w, bool
 }
 
 // One node pop of the fast traversal; returns false when the pop was culled.
+template <bool kDiag>
 __device__ __forceinline__ bool ftrav_node(const DevScene& S, Trav& t, Stack& st) {
     uint32_t word;
     float nd;
@@ -833,7 +840,7 @@ __device__ __forceinline__ bool ftrav_node(const DevScene& S, Trav& t, Stack& st
         word = e.x;
         nd = __uint_as_float(e.y);
     }
-    t.btests++;
+    if (kDiag) t.btests++;
     if (nd >= t.ldist) return false;
     if (word & kLeafBit) {
         t.leaf = word & ~kLeafBit;
@@ -860,19 +867,19 @@ __device__ __forceinline__ bool ftrav_node(const DevScene& S, Trav& t, Stack& st
 }
 
 // One triangle test of the fast traversal; with kAny the first hit closes the instance.
-template <bool kAny>
+template <bool kAny, bool kDiag>
 __device__ __forceinline__ void ftrav_tri(const DevScene& S, Trav& t, Stack& st) {
     const float4* P = S.prims + 3ull * (t.prim_base + t.leaf);
     const float4 p0 = P[0], p1 = P[1], p2 = P[2];
     float u, v, tt;
-    t.ttests++;
+    if (kDiag) t.ttests++;
     t.leaf = (__float_as_uint(p2.w) & kPrimLast) ? kNoLeaf : t.leaf + 1;
     if (tri_test(t.lo, t.ld, xyz(p0), xyz(p1), xyz(p2), u, v, tt)) {
         if (tt > 0.0f && tt < t.ldist) {
             t.ldist = tt;
             t.bu = u;
             t.bvv = v;
-            t.bbv = __float_as_uint(p1.w);
+            if (kDiag) t.bbv = __float_as_uint(p1.w);
             t.btri = __float_as_uint(p0.w);
             if (kAny) {
                 t.leaf = kNoLeaf;

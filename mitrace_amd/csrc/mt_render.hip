@@ -35,7 +35,6 @@ using namespace dev;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kLdsDepth = 16;
 // idle lanes of a wave that trigger a refill of the traversal (measured best on the sphere grid:
 // the reference traversal's instance culling runs per refill, so it prefers mid-size refills;
 // the fast traversal's rays are short, so refilling less often wins)
@@ -257,6 +256,9 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
         fetch = io.fetch;
     }
     constexpr bool kAny = kAccel == 1 && (kMode == 1 || kMode == 3);
+    // per-hit diagnostics (the winning instance's tests, the leaf index): counting builds (work
+    // statistics and debug views) and the test hooks only
+    constexpr bool kDiag = kCount || kMode >= 2;
     const uint32_t* q = b.queue[bounce & 1];
     const uint32_t refill_min = c.refill_min;
     const uint64_t lanes_below = (1ull << lane_id()) - 1ull;
@@ -352,24 +354,24 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                 if (run_setup) {
                     kind = kStepSetup;
                     if (kAccel) {
-                        finished = ftrav_setup<kCount, kAny>(S, t, st, w, nm);
+                        finished = ftrav_setup<kCount, kAny, kDiag>(S, t, st, w, nm);
                     } else {
-                        finished = trav_setup<kCount>(S, t, st, w, nm);
+                        finished = trav_setup<kCount, kDiag>(S, t, st, w, nm);
                     }
                 }
             } else if (t.leaf != kNoLeaf) {
                 kind = kStepLeaf;
                 if (kAccel) {
-                    ftrav_tri<kAny>(S, t, st);
+                    ftrav_tri<kAny, kDiag>(S, t, st);
                 } else {
-                    trav_tri(S, t);
+                    trav_tri<kDiag>(S, t);
                 }
             } else if (kAccel) {
-                kind = ftrav_node(S, t, st) ? kStepInner : kStepCull;
+                kind = ftrav_node<kDiag>(S, t, st) ? kStepInner : kStepCull;
             } else if (t.fast) {
-                kind = trav_node<true>(S, t, st) ? kStepInner : kStepCull;
+                kind = trav_node<true, kDiag>(S, t, st) ? kStepInner : kStepCull;
             } else {
-                kind = trav_node<false>(S, t, st) ? kStepInner : kStepCull;
+                kind = trav_node<false, kDiag>(S, t, st) ? kStepInner : kStepCull;
             }
         }
         if (kCount) {
@@ -392,7 +394,7 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                     b.hit_tri[p] = h.tri;
                     b.hit_uv[p] = make_float2(h.u, h.v);
                     b.hit_pos[p] = f4(h.wpos, h.dist);
-                    if (c.debug && h.inst >= 0) {  // the winning instance's tests (Tracer.cpp:170-175)
+                    if (kDiag && c.debug && h.inst >= 0) {  // the winning instance's tests (Tracer.cpp:170-175)
                         uint4 di = b.dbg_i[p];
                         if (bounce == 0) {
                             di.x = h.bvh_tests;
@@ -485,6 +487,7 @@ void launch_trace_as(uint32_t depth, bool count, uint32_t blocks, hipStream_t s,
 template <int kMode>
 void launch_trace(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, const DevScene& S,
                   const ChunkParams& c, const PathBuffers& b, uint32_t k, TestIO io = {}) {
+    count = count || c.debug;  // debug views read the per-hit diagnostics of the counting build
     if (c.accel)
         launch_trace_as<kMode, 1>(depth, count, blocks, s, S, c, b, k, io);
     else
@@ -1277,6 +1280,17 @@ uint32_t stack_capacity(const mt_scene* s) {
     return s->dev.use_tlas ? blas + s->tlas_depth + 1u : blas;
 }
 
+// Measured on the material-spheres grid at 6 blocks per CU: a 12-entry reference stack (24 KB
+// per block, all 6 blocks resident) 447 -> 472 Msamples/s over 16 entries (5 resident); the
+// fast traversal's deeper SAH trees lose 2 % at 12.
+uint32_t lds_stack_depth(bool fast, uint32_t blocks_per_cu) {
+    if (fast) return 16;
+    constexpr uint32_t kLdsBytes = 160u * 1024u, kEntryBytes = 8u * kBlock;
+    for (uint32_t d : {16u, 12u, 8u})
+        if (blocks_per_cu * d * kEntryBytes <= kLdsBytes) return d;
+    return 4;
+}
+
 int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t traversal,
                uint32_t* occluded) {
     if (traversal != MT_TRAVERSAL_REFERENCE && traversal != MT_TRAVERSAL_FAST)
@@ -1526,9 +1540,10 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
 
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     const int cus = device_cu_count();
-    lds_depth_ = env_u32("MT_LDS_DEPTH", kLdsDepth);
+    const uint32_t blocks_per_cu = env_u32("MT_TRACE_BLOCKS_PER_CU", 6);
+    lds_depth_ = env_u32("MT_LDS_DEPTH", lds_stack_depth(st.traversal == MT_TRAVERSAL_FAST, blocks_per_cu));
     if (lds_depth_ != 4 && lds_depth_ != 8 && lds_depth_ != 12) lds_depth_ = 16;
-    trace_blocks_ = static_cast<uint32_t>(cus * env_u32("MT_TRACE_BLOCKS_PER_CU", 6));
+    trace_blocks_ = static_cast<uint32_t>(cus * blocks_per_cu);
     shade_blocks_ = static_cast<uint32_t>(
         std::min<uint64_t>((P + kBlock - 1) / kBlock, cus * static_cast<uint64_t>(env_u32("MT_SHADE_BLOCKS_PER_CU", 16))));
     const uint32_t cap = stack_capacity(scene);

@@ -215,6 +215,11 @@ struct BatchLaunch {
 uint32_t seed_mix(uint32_t x);
 uint32_t seed_key(uint64_t seed);
 uint32_t stack_capacity(const mt_scene* s);
+// traversal stack entries kept in LDS for `blocks_per_cu` persistent traversal blocks of 256
+// threads: the fast traversal keeps 16; the reference traversal the deepest of 16, 12, 8 and 4
+// whose stacks (8 B per entry and thread) let that many blocks share a CU's 160 KB of LDS,
+// since a shallower stack only adds spills once LDS no longer limits the blocks per CU
+uint32_t lds_stack_depth(bool fast, uint32_t blocks_per_cu);
 int cu_count();
 // the per-render parameters of a render of `scene` with this camera and these settings
 void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& cam,

@@ -96,9 +96,10 @@ def test_autotune_picks_from_measured_candidates(scenes):
               None)]
     best, trials = autotune(calib)
     assert len(trials) >= 8
-    assert best["paths_per_iteration"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+    assert best["paths_per_iteration"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24)
     assert best["tile_pixels"] in (256, 1024, 4096, 16384)
     assert best in [t["options"] for t in trials]
-    top = max(t["msamples_per_s"] for t in trials[1:])  # the first trial is the warm-up
-    chosen = max(t["msamples_per_s"] for t in trials[1:] if t["options"] == best)
+    # each trial times its second round (the first, at 1 spp, captures the graphs)
+    top = max(t["msamples_per_s"] for t in trials)
+    chosen = max(t["msamples_per_s"] for t in trials if t["options"] == best)
     assert chosen == top
