@@ -544,8 +544,10 @@ __device__ __forceinline__ void sample_param_texture(const DiffParams& dp, float
 template <int kDiff>
 __device__ void eval_material(const DevScene& S, const DiffParams& dp, uint32_t mi, float u,
                               float v, MatSample& ms) {
+    // the adjoint bookkeeping (tex, *_scale, taps) is written only in the differentiable
+    // builds: in the plain ones its stores stayed in private memory (20 B of scratch per lane)
     const DevMaterial& m = S.materials[mi];
-    ms.tex = 0;
+    if constexpr (kDiff != 0) ms.tex = 0;
     ms.base = V3{m.base[0], m.base[1], m.base[2]};
     if (kDiff && dp.tex && mi >= dp.tex_first && mi - dp.tex_first < dp.tex_count) {
         sample_param_texture(dp, u, v, ms);
@@ -559,12 +561,14 @@ __device__ void eval_material(const DevScene& S, const DiffParams& dp, uint32_t 
         const float4 t = sample_tex(S, m.tex[1], u, v);
         ms.metal = t.z * mf;
         ms.rough = t.y * rf;
-        ms.metal_scale = t.z;
-        ms.rough_scale = t.y;
+        if constexpr (kDiff != 0) {
+            ms.metal_scale = t.z;
+            ms.rough_scale = t.y;
+        }
     } else {
         ms.metal = mf;
         ms.rough = rf;
-        ms.metal_scale = ms.rough_scale = 1.0f;
+        if constexpr (kDiff != 0) ms.metal_scale = ms.rough_scale = 1.0f;
     }
     if (m.tex[2].image >= 0) {
         const float4 t = sample_tex(S, m.tex[2], u, v);
@@ -577,10 +581,10 @@ __device__ void eval_material(const DevScene& S, const DiffParams& dp, uint32_t 
                                      : V3{m.emissive[0], m.emissive[1], m.emissive[2]};
     if (m.tex[3].image >= 0) {
         const float4 t = sample_tex(S, m.tex[3], u, v);
-        ms.em_scale = V3{t.x, t.y, t.z};
-        ms.emission = mul(ms.em_scale, em);
+        if constexpr (kDiff != 0) ms.em_scale = V3{t.x, t.y, t.z};
+        ms.emission = mul(V3{t.x, t.y, t.z}, em);
     } else {
-        ms.em_scale = mk(1.0f);
+        if constexpr (kDiff != 0) ms.em_scale = mk(1.0f);
         ms.emission = em;
     }
 }
