@@ -1,7 +1,8 @@
 // mt_internal.hpp -- internal declarations of the gfx950 Tracer library (not part of the ABI).
 //
 // HBM layout of a scene (all arrays device-resident, built once by mt_scene_create):
-//   instances  DevInstance[I]  192 B: inverse and model matrices, world AABB, mesh/material ids
+//   instances  DevInstance[I]  192 B: inverse and model matrices, world AABB, mesh/material ids,
+//                                     the mesh's shading-attribute offsets
 //   meshes     DevMesh[M]       64 B: root box and word, offsets of the mesh's pair records,
 //                                     leaf-ordered triangles and shading attributes, flags
 //   pairs      float4[4*Q]      64 B per sibling pair of BVH nodes (the reference keeps siblings
@@ -45,7 +46,12 @@ struct alignas(16) DevInstance {
     uint32_t mesh;
     float wmax[3];
     uint32_t material;
-    float pad[8];
+    // the mesh's shading-attribute offsets and flags (DevMesh::attr_base, uv_base, flags),
+    // repeated here so that shading reaches a hit's normals in one dependent load fewer
+    uint32_t attr_base;
+    uint32_t uv_base;
+    uint32_t mesh_flags;
+    uint32_t pad[5];
 };
 static_assert(sizeof(DevInstance) == 192, "DevInstance layout");
 

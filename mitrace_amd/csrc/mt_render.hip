@@ -744,10 +744,9 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 const float2 huv = b.hit_uv[p];
                 const uint32_t tri = b.hit_tri[p];
                 const DevInstance* I = S.instances + inst;
-                const DevMesh m = S.meshes[I->mesh];
                 // RayHitGeometryInfo (Ray.cpp:7-48)
                 const float w = 1.0f - huv.x - huv.y;
-                const float* nn = S.normals + 9ull * (m.attr_base + tri);
+                const float* nn = S.normals + 9ull * (I->attr_base + tri);
                 const V3 n0 = {nn[0], nn[1], nn[2]}, n1 = {nn[3], nn[4], nn[5]}, n2 = {nn[6], nn[7], nn[8]};
                 const V3 ni = normalize(add(add(scale(n0, w), scale(n1, huv.x)), scale(n2, huv.y)));
                 float M[16];
@@ -755,8 +754,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 for (int j = 0; j < 16; ++j) M[j] = I->M[j];
                 const V3 gN = normalize(xform(M, ni, 0.0f));
                 float tu = 0.0f, tv = 0.0f;
-                if (m.flags & kMeshHasUv) {
-                    const float* uv = S.uvs + 6ull * (m.uv_base + tri);
+                if (I->mesh_flags & kMeshHasUv) {
+                    const float* uv = S.uvs + 6ull * (I->uv_base + tri);
                     tu = (w * uv[0] + huv.x * uv[2]) + huv.y * uv[4];
                     tv = (w * uv[1] + huv.x * uv[3]) + huv.y * uv[5];
                 }
@@ -831,7 +830,10 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 }
                 // ProcessRayForward for this step (Tracer.cpp:254-339)
                 const V3 e_term = mul(scale(trad, J.emission_base), ms.emission);
-                {
+                // the sum starts at +0 and e_term is never -0 (throughput and emission are not
+                // negative), so adding an all-zero e_term leaves it bit-identical: skip the
+                // read-modify-write then (NaN components compare unequal and are added)
+                if (e_term.x != 0.0f || e_term.y != 0.0f || e_term.z != 0.0f) {
                     const float4 ae = b.acc_e[p];
                     b.acc_e[p] = make_float4(ae.x + e_term.x, ae.y + e_term.y, ae.z + e_term.z, 0.0f);
                 }

@@ -684,6 +684,9 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         invert4(di.M, di.inv);
         di.mesh = id.mesh;
         di.material = id.material;
+        di.attr_base = meshes[id.mesh].attr_base;
+        di.uv_base = meshes[id.mesh].uv_base;
+        di.mesh_flags = meshes[id.mesh].flags;
         // MeshInstance::CalculateWorldAABB (Mesh.cpp:56-80)
         const auto& bb = local_aabb[id.mesh];
         const float* a = bb.first.data();
