@@ -1532,15 +1532,17 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         return fail(MT_E_INVALID, "the scene was created without fast BVHs (mt_bvh_options.fast)");
     const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
     if (npix_total * 3 > 0xFFFFFFFFull) return fail(MT_E_UNSUPPORTED, "image too large");
-    // Paths per chunk by default: an eighth of the render, at least 16 Mi and at most 64 Mi
-    // (about 256 B of path state per path: 4 to 16 GB per stream slot).  Measured on the
-    // 1920x1080 256-spp sphere grid (530.8 M paths): 4 Mi 375, 8 Mi 416, 16 Mi 473, 24 Mi 481,
-    // 32 Mi 492, 48 Mi 494, 64 Mi 502 Msamples/s -- larger chunks spend less of the frame in
-    // their sparse last bounces -- while eight chunks keep the three stream slots busy on
-    // smaller renders (1920x1080 at 64 spp stays at 16.6 Mi).
+    // Paths per chunk by default: an eighth of the render, at least 16 Mi and at most 32 Mi
+    // (about 256 B of path state per path: 4 to 8 GB per stream slot).  Measured on the
+    // 1920x1080 256-spp sphere grid (530.8 M paths), reference traversal: 4 Mi 375, 8 Mi 416,
+    // 16 Mi 473, 24 Mi 481, 32 Mi 492, 48 Mi 494, 64 Mi 502 Msamples/s -- larger chunks spend
+    // less of the frame in their sparse last bounces -- but the same frame with the fast
+    // traversal fell from ~1200 to 375-910 Msamples/s at 64 Mi (its kernels were not slower:
+    // the time went outside them), so the default stops at 32 Mi.  Eight chunks keep the three
+    // stream slots busy on smaller renders (1920x1080 at 64 spp stays at 16.6 Mi).
     const uint64_t all_paths = npix_total * st.samples;
     uint64_t target = st.paths_in_flight ? st.paths_in_flight
-                                         : std::clamp<uint64_t>(all_paths / 8, 1ull << 24, 1ull << 26);
+                                         : std::clamp<uint64_t>(all_paths / 8, 1ull << 24, 1ull << 25);
     target = std::max<uint64_t>(target, st.samples);
     target = std::min<uint64_t>(target, 1ull << 26);
     npix_chunk_ = static_cast<uint32_t>(std::max<uint64_t>(1, target / st.samples));
