@@ -390,9 +390,10 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                 const Hit& h = t.R;
                 if (kMode == 0) {
                     const uint32_t p = q[item];
-                    b.hit_inst[p] = h.inst;
-                    b.hit_tri[p] = h.tri;
-                    b.hit_uv[p] = make_float2(h.u, h.v);
+                    // one 16-byte record, so that shading gets the instance, triangle and
+                    // barycentrics with a single load
+                    b.hit_rec[p] = make_uint4(static_cast<uint32_t>(h.inst), h.tri, __float_as_uint(h.u),
+                                              __float_as_uint(h.v));
                     b.hit_pos[p] = f4(h.wpos, h.dist);
                     if (kDiag && c.debug && h.inst >= 0) {  // the winning instance's tests (Tracer.cpp:170-175)
                         uint4 di = b.dbg_i[p];
@@ -671,7 +672,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 dl = scale(V3{gi[0], gi[1], gi[2]}, c.inv_spp);
             }
             const V3 d = xyz(b.ray_d[p]);
-            const int inst = b.hit_inst[p];
+            const uint4 hrec = b.hit_rec[p];
+            const int inst = static_cast<int>(hrec.x);
             const V3 trad = xyz(b.t_rad[p]);
             if (kDiff == 0 && !kBatch && kExt && c.cache && c.cache_after && k > c.cache_depth) {
                 // the training record of the previous vertex: everything collected up to and
@@ -741,8 +743,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 const float4 hp = b.hit_pos[p];
                 const V3 wpos = xyz(hp);
                 const float dist = hp.w;
-                const float2 huv = b.hit_uv[p];
-                const uint32_t tri = b.hit_tri[p];
+                const float2 huv = make_float2(__uint_as_float(hrec.z), __uint_as_float(hrec.w));
+                const uint32_t tri = hrec.y;
                 const DevInstance* I = S.instances + inst;
                 // RayHitGeometryInfo (Ray.cpp:7-48)
                 const float w = 1.0f - huv.x - huv.y;
@@ -1496,8 +1498,7 @@ int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, ui
         (rc = alloc(owned, &b.t_rad, P)) || (rc = alloc(owned, &b.t_term, P)) ||
         (rc = alloc(owned, &b.prev_base, P)) || (rc = alloc(owned, &b.acc_e, P)) ||
         (rc = alloc(owned, &b.acc_d, P)) || (rc = alloc(owned, &b.acc_i, P)) ||
-        (rc = alloc(owned, &b.acc_s, P)) || (rc = alloc(owned, &b.hit_inst, P)) ||
-        (rc = alloc(owned, &b.hit_tri, P)) || (rc = alloc(owned, &b.hit_uv, P)) ||
+        (rc = alloc(owned, &b.acc_s, P)) || (rc = alloc(owned, &b.hit_rec, P)) ||
         (rc = alloc(owned, &b.hit_pos, P)) || (rc = alloc(owned, &b.queue[0], P)) ||
         (rc = alloc(owned, &b.queue[1], P)) || (rc = alloc(owned, &b.sh_o, P)) ||
         (rc = alloc(owned, &b.sh_d, P)) || (rc = alloc(owned, &b.sh_c, P)) ||

@@ -87,9 +87,7 @@ struct PathBuffers {
     float4 *ray_o, *ray_d;
     float4 *t_rad, *t_term, *prev_base;
     float4 *acc_e, *acc_d, *acc_i, *acc_s;
-    int32_t* hit_inst;
-    uint32_t* hit_tri;
-    float2* hit_uv;
+    uint4* hit_rec;   // closest hit: {instance (as int, -1 = miss), triangle, u bits, v bits}
     float4* hit_pos;  // world position, world distance
     uint32_t* queue[2];
     float4 *sh_o, *sh_d, *sh_c;  // shadow rays: {o, max dist} {d, path} {contribution, slot}
