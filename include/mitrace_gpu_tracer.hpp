@@ -7,7 +7,8 @@
  *   this header                     reference
  *   ------------------------------  ---------------------------------------------------------
  *   mitrace_gpu::RenderBuffer       RenderBuffer (src/Tracer/RenderBuffer.hpp:11-35,
- *                                     RenderBuffer.cpp:3-58): W x H float RGB, row-major
+ *                                     RenderBuffer.cpp:3-84): W x H float RGB, row-major;
+ *                                     SaveToFilePNG / SaveToFileHDR
  *   mitrace_gpu::Scene              Scene::Scene (src/Scene/Scene.cpp:53-59) as uploaded by
  *                                     mt_scene_create; GetCamera() as Scene::GetCamera
  *   mitrace_gpu::DefaultSettings()  Config::ConfigOptions defaults (src/CLI/Config.hpp:43-80)
@@ -34,8 +35,10 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -109,10 +112,100 @@ class RenderBuffer {
         return out;
     }
 
+    /* RenderBuffer.cpp:60-84 write through stb; these writers are self-contained.  PNG: 8-bit
+     * RGB of GetPixelsRGB8, zlib stream of stored (uncompressed) deflate blocks, so any PNG
+     * reader decodes the same pixels stb's compressed file holds.  HDR: Radiance RGBE with
+     * flat (not run-length encoded) scanlines, top row first, linear floats as the buffer
+     * holds them.  A failure throws std::runtime_error("Failed to save image to file"), the
+     * reference's message.  SaveToFileJPG has no counterpart here (no JPEG encoder). */
+    void SaveToFilePNG(const std::string& filename) const {
+        const auto rgb = GetPixelsRGB8();
+        std::vector<std::uint8_t> raw;  // filter byte 0 (None) + one row
+        raw.reserve((std::size_t(width_) * 3 + 1) * height_);
+        for (unsigned y = 0; y < height_; ++y) {
+            raw.push_back(0);
+            raw.insert(raw.end(), rgb.get() + std::size_t(y) * width_ * 3,
+                       rgb.get() + std::size_t(y + 1) * width_ * 3);
+        }
+        std::vector<std::uint8_t> z = {0x78, 0x01};
+        std::size_t pos = 0;
+        do {
+            const std::size_t n = std::min<std::size_t>(65535, raw.size() - pos);
+            z.push_back(pos + n == raw.size() ? 1 : 0);  // BFINAL, BTYPE 00 (stored)
+            z.push_back(static_cast<std::uint8_t>(n & 0xff));
+            z.push_back(static_cast<std::uint8_t>(n >> 8));
+            z.push_back(static_cast<std::uint8_t>(~n & 0xff));
+            z.push_back(static_cast<std::uint8_t>((~n >> 8) & 0xff));
+            z.insert(z.end(), raw.begin() + pos, raw.begin() + pos + n);
+            pos += n;
+        } while (pos < raw.size());
+        std::uint32_t a = 1, b = 0;  // Adler-32 of the uncompressed data (RFC 1950)
+        for (std::uint8_t c : raw) {
+            a = (a + c) % 65521u;
+            b = (b + a) % 65521u;
+        }
+        PutU32(z, (b << 16) | a);
+        std::vector<std::uint8_t> ihdr;
+        PutU32(ihdr, width_);
+        PutU32(ihdr, height_);
+        ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8-bit, truecolour, deflate, no filter, no interlace
+        std::vector<std::uint8_t> file = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+        PngChunk(file, "IHDR", ihdr);
+        PngChunk(file, "IDAT", z);
+        PngChunk(file, "IEND", {});
+        WriteFile(filename, file);
+    }
+
+    void SaveToFileHDR(const std::string& filename) const {
+        const std::string head = "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y " + std::to_string(height_) +
+                                 " +X " + std::to_string(width_) + "\n";
+        std::vector<std::uint8_t> file(head.begin(), head.end());
+        file.reserve(file.size() + pixels_.size() / 3 * 4);
+        for (std::size_t i = 0; i < pixels_.size(); i += 3) {
+            // RGBE: the largest component's binary exponent is shared (Ward, Graphics Gems II)
+            const float r = pixels_[i], g = pixels_[i + 1], bl = pixels_[i + 2];
+            const float m = std::max(r, std::max(g, bl));
+            if (!(m >= 1e-32f)) {
+                file.insert(file.end(), {0, 0, 0, 0});
+                continue;
+            }
+            int e = 0;
+            const float scale = std::frexp(m, &e) * 256.0f / m;
+            file.push_back(static_cast<std::uint8_t>(std::max(r, 0.0f) * scale));
+            file.push_back(static_cast<std::uint8_t>(std::max(g, 0.0f) * scale));
+            file.push_back(static_cast<std::uint8_t>(std::max(bl, 0.0f) * scale));
+            file.push_back(static_cast<std::uint8_t>(e + 128));
+        }
+        WriteFile(filename, file);
+    }
+
     float* Data() noexcept { return pixels_.data(); }
     const float* Data() const noexcept { return pixels_.data(); }
 
   private:
+    static void PutU32(std::vector<std::uint8_t>& v, std::uint32_t x) {
+        for (int s = 24; s >= 0; s -= 8) v.push_back(static_cast<std::uint8_t>(x >> s));
+    }
+    static void PngChunk(std::vector<std::uint8_t>& file, const char* type,
+                         const std::vector<std::uint8_t>& data) {
+        PutU32(file, static_cast<std::uint32_t>(data.size()));
+        const std::size_t start = file.size();
+        file.insert(file.end(), type, type + 4);
+        file.insert(file.end(), data.begin(), data.end());
+        std::uint32_t crc = 0xffffffffu;  // CRC-32 (ISO 3309) over type and data
+        for (std::size_t i = start; i < file.size(); ++i) {
+            crc ^= file[i];
+            for (int k = 0; k < 8; ++k) crc = (crc >> 1) ^ (0xedb88320u & (0u - (crc & 1u)));
+        }
+        PutU32(file, ~crc);
+    }
+    static void WriteFile(const std::string& filename, const std::vector<std::uint8_t>& bytes) {
+        std::FILE* f = std::fopen(filename.c_str(), "wb");
+        bool ok = f && std::fwrite(bytes.data(), 1, bytes.size(), f) == bytes.size();
+        if (f) ok = (std::fclose(f) == 0) && ok;
+        if (!ok) throw std::runtime_error("Failed to save image to file");
+    }
+
     unsigned width_;
     unsigned height_;
     std::vector<float> pixels_;

@@ -7,8 +7,12 @@ offset/scale, KHR_lights_punctual and EXT_light_attributes area lights, perspect
 orthographic cameras -- and produces a :class:`mitrace_amd.scene.Scene`, i.e. what
 ``Scene::Scene(loader.LoadScene(i))`` plus ``SetCamera`` build in main.cpp:100-107.
 
-The EXT_sky environment is parsed by the reference loader but dropped by ``Scene::Scene``
-(Scene.cpp:53-59), so it is not read here; an environment comes only from ``hdri_override``.
+The EXT_sky environment (doc/gltfs/EXT_sky.md) is parsed by the reference loader
+(GLTF.cpp:902-912) but dropped by ``Scene::Scene`` (Scene.cpp:53-59), so by default it is not
+read here and an environment comes only from ``hdri_override``; ``load(..., ext_sky=True)``
+keeps it.  The reference loads 8-bit images only (Image.cpp:77-104 rejects ``.hdr``/``.exr``);
+``load_env_image`` also reads Radiance ``.hdr`` files as float RGB, which the float
+environment path of the GPU library takes.
 """
 import json
 import os
@@ -42,6 +46,73 @@ def load_image_rgb8(path):
     from PIL import Image as PILImage
     with PILImage.open(path) as im:
         return np.asarray(im.convert("RGB"), dtype=np.uint8).copy()
+
+
+def load_hdr(path):
+    """Radiance RGBE (.hdr) as float32 (H, W, 3): '-Y H +X W' orientation, flat or new-style
+    run-length encoded scanlines; a texel (m, e) decodes to m * 2^(e - 136), 0 when e = 0 (the
+    decode stb_image uses)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if not (data.startswith(b"#?RADIANCE") or data.startswith(b"#?RGBE")):
+        raise RuntimeError("not a Radiance HDR file")
+    pos = 0
+    fmt_ok = False
+    while True:  # header lines up to the blank line
+        end = data.index(b"\n", pos)
+        line = data[pos:end]
+        pos = end + 1
+        if line == b"":
+            break
+        if line.startswith(b"FORMAT="):
+            fmt_ok = line == b"FORMAT=32-bit_rle_rgbe"
+    if not fmt_ok:
+        raise RuntimeError("unsupported HDR format (only 32-bit_rle_rgbe)")
+    end = data.index(b"\n", pos)
+    tok = data[pos:end].split()
+    pos = end + 1
+    if len(tok) != 4 or tok[0] != b"-Y" or tok[2] != b"+X":
+        raise RuntimeError("unsupported HDR orientation (only -Y H +X W)")
+    h, w = int(tok[1]), int(tok[3])
+    out = np.zeros((h, w, 4), np.uint8)
+    for y in range(h):
+        if 8 <= w < 32768 and data[pos] == 2 and data[pos + 1] == 2 and \
+                (data[pos + 2] << 8 | data[pos + 3]) == w:
+            pos += 4
+            for ch in range(4):  # each channel as runs: >128 = repeat, else literal bytes
+                x = 0
+                while x < w:
+                    c = data[pos]
+                    pos += 1
+                    if c > 128:
+                        n = c - 128
+                        if n == 0 or x + n > w:
+                            raise RuntimeError("bad HDR run length")
+                        out[y, x:x + n, ch] = data[pos]
+                        pos += 1
+                    else:
+                        if c == 0 or x + c > w:
+                            raise RuntimeError("bad HDR literal length")
+                        out[y, x:x + c, ch] = np.frombuffer(data, np.uint8, c, pos)
+                        pos += c
+                        n = c
+                    x += n
+        else:
+            row = np.frombuffer(data, np.uint8, w * 4, pos)
+            if w >= 1 and row[0] == 1 and row[1] == 1 and row[2] == 1:
+                raise RuntimeError("old-style HDR run-length encoding is not supported")
+            out[y] = row.reshape(w, 4)
+            pos += w * 4
+    e = out[..., 3].astype(np.int32)
+    scale = np.where(e == 0, 0.0, np.ldexp(1.0, e - 136)).astype(np.float32)
+    return (out[..., :3].astype(np.float32) * scale[..., None]).astype(np.float32)
+
+
+def load_env_image(path):
+    """An environment image: float RGB from a Radiance .hdr, else 8-bit RGB as the reference."""
+    if path.lower().endswith(".hdr"):
+        return load_hdr(path)
+    return load_image_rgb8(path)
 
 
 class GLTF:
@@ -299,11 +370,24 @@ class GLTF:
                      instances=instances, lights=lights, camera=cam)
 
 
-def load(path, scene_index=0, camera_index=0, hdri_override=None, bvh=None):
-    """Loader::GLTF(path).LoadScene + Scene::Scene + optional --hdri-override."""
-    sc = GLTF(path).load_scene(scene_index, camera_index)
+def load(path, scene_index=0, camera_index=0, hdri_override=None, bvh=None, ext_sky=False):
+    """Loader::GLTF(path).LoadScene + Scene::Scene + optional --hdri-override.  ext_sky=True
+    keeps the EXT_sky texture (LoadSceneEnvironmentTexture, GLTF.cpp:902-912) as the
+    environment, which Scene::Scene drops; hdri_override still takes precedence, and the
+    equirect is always sampled bilinearly (Texture.cpp:35-50)."""
+    g = GLTF(path)
+    sc = g.load_scene(scene_index, camera_index)
+    if ext_sky and hdri_override is None:
+        sky = g.j.get("extensions", {}).get("EXT_sky", {})
+        if "sky_texture" in sky:
+            tex = g.j["textures"][sky["sky_texture"]]
+            im = g.j["images"][tex["source"]]
+            if "uri" not in im:
+                raise RuntimeError("Only URI-based images are supported")
+            sc.env = Environment(texels=load_env_image(os.path.join(g.base, im["uri"])),
+                                 filter=abi.MT_FILTER_LINEAR)
     if hdri_override is not None:
-        sc.env = Environment(texels=load_image_rgb8(hdri_override), filter=abi.MT_FILTER_LINEAR)
+        sc.env = Environment(texels=load_env_image(hdri_override), filter=abi.MT_FILTER_LINEAR)
     if bvh is not None:
         sc.bvh = bvh
     return sc

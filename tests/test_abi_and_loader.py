@@ -102,3 +102,70 @@ def test_light_creep_orthographic_camera():
     assert sc.camera.type == abi.MT_CAMERA_ORTHOGONAL
     assert sc.camera.ortho_size == pytest.approx((0.85, 0.478125), rel=1e-6)
     assert sc.camera.aspect == pytest.approx(0.85 / 0.478125, rel=1e-6)
+
+
+def _write_hdr(path, w, rows):
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0\n\n-Y %d +X %d\n" % (len(rows), w))
+        for r in rows:
+            f.write(bytes(r))
+
+
+def test_hdr_reader_flat_and_run_length_scanlines(tmp_path):
+    """Radiance RGBE: a texel (m, e) is m * 2^(e - 136) (stb_image's decode), e = 0 is black.
+    Row 0 is a new-style run-length scanline (2, 2, width; per channel: a byte above 128 repeats
+    the next byte, a byte up to 128 counts literal bytes), row 1 a flat one."""
+    w = 10
+    rle = [2, 2, 0, w,
+           128 + 10, 200,                          # R: one run of 200
+           10, *range(10),                         # G: ten literals 0..9
+           128 + 5, 7, 5, 1, 2, 3, 4, 5,           # B: run of 7, then 5 literals
+           128 + 9, 129, 1, 0]                     # E: run of 129, then a literal 0 (black)
+    flat = []
+    for x in range(w):
+        flat += [x * 10, 255 - x, 3, 140 if x % 2 else 0]
+    p = tmp_path / "t.hdr"
+    _write_hdr(p, w, [rle, flat])
+    img = gltf.load_hdr(str(p))
+    assert img.shape == (2, w, 3) and img.dtype == np.float32
+    s = 2.0 ** (129 - 136)
+    want0 = np.stack([np.full(w, 200.0), np.arange(10.0), np.array([7] * 5 + [1, 2, 3, 4, 5], float)], 1) * s
+    want0[9] = 0.0  # exponent byte 0
+    assert np.array_equal(img[0], want0.astype(np.float32))
+    for x in range(w):
+        e = 140 if x % 2 else 0
+        sc = 2.0 ** (e - 136) if e else 0.0
+        assert np.array_equal(img[1, x], np.float32([x * 10 * sc, (255 - x) * sc, 3 * sc]))
+    q = tmp_path / "bad.hdr"
+    q.write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_xyze\n\n-Y 1 +X 1\n\0\0\0\0")
+    with pytest.raises(RuntimeError):
+        gltf.load_hdr(str(q))
+
+
+def test_ext_sky_is_dropped_by_default_and_kept_on_request(tmp_path):
+    """EXT_sky (doc/gltfs/EXT_sky.md): the reference loader reads sky_texture
+    (GLTF.cpp:902-912) and Scene::Scene drops it (Scene.cpp:53-59).  load() does the same by
+    default; ext_sky=True keeps it as the environment (8-bit PNG or float Radiance .hdr)."""
+    import json
+    from PIL import Image
+    src = fixture_path("cornell_box")
+    j = json.load(open(src))
+    base = os.path.dirname(os.path.abspath(src))
+    for b in j["buffers"]:
+        b["uri"] = os.path.join(base, b["uri"])
+    for im in j.get("images", []):
+        im["uri"] = os.path.join(base, im["uri"])
+    sky = (np.arange(4 * 8 * 3) % 251).astype(np.uint8).reshape(4, 8, 3)
+    Image.fromarray(sky).save(tmp_path / "sky.png")
+    _write_hdr(tmp_path / "sky.hdr", 2, [[128, 64, 32, 129, 0, 0, 0, 0]])
+    for name, want in (("sky.png", sky), ("sky.hdr", np.float32([[[1.0, 0.5, 0.25], [0, 0, 0]]]))):
+        jj = json.loads(json.dumps(j))
+        jj.setdefault("images", []).append({"uri": str(tmp_path / name)})
+        jj.setdefault("textures", []).append({"source": len(jj["images"]) - 1})
+        jj.setdefault("extensions", {})["EXT_sky"] = {"sky_texture": len(jj["textures"]) - 1}
+        path = tmp_path / "scene.gltf"
+        path.write_text(json.dumps(jj))
+        assert gltf.load(str(path)).env is None
+        env = gltf.load(str(path), ext_sky=True).env
+        assert env.texels.dtype == want.dtype and np.array_equal(env.texels, want)
+        assert env.filter == abi.MT_FILTER_LINEAR

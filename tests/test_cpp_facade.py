@@ -4,7 +4,9 @@ tests/cpp/facade_shim.cpp.
 CPU: the driver builds with g++ -Werror against the library and loads; DefaultSettings holds
 the reference's Config defaults; RenderBuffer's RGB8 conversion and DrawSubBuffer follow
 RenderBuffer.cpp:3-58 (checked against values fixed by hand and against the Python mirror);
-a failing C call surfaces as mitrace_gpu::Error with the C-ABI's code and message.
+a failing C call surfaces as mitrace_gpu::Error with the C-ABI's code and message;
+SaveToFilePNG / SaveToFileHDR write files that decode (PIL, zlib, the Radiance reader) to the
+buffer's pixels, with known RGBE bytes pinned by hand.
 GPU: the facade's Start/Wait render equals the synchronous mt_render of the same seed bit for
 bit and the oracle within 3 sigma; StatThread-style polling sees monotone progress; a kill
 ends the render early without an error and a second StartRender completes.
@@ -171,3 +173,83 @@ def test_facade_kill_ends_early_and_restarts(gpu):
     assert out[0] < full, (out[0], full)
     assert out[1] == 1  # IsDone after a killed render's WaitForRender
     assert out[2] == 1  # the next StartRender rendered every sample
+
+
+def _save(px, path, which):
+    h, w, _ = px.shape
+    px = np.ascontiguousarray(px, np.float32)
+    err = C.create_string_buffer(256)
+    L = shim()
+    L.fac_save.argtypes = [C.POINTER(C.c_float), C.c_uint, C.c_uint, C.c_char_p, C.c_int,
+                           C.c_char_p, C.c_int]
+    return L.fac_save(fp(px), w, h, str(path).encode(), which, err, 256), err.value.decode()
+
+
+def test_save_png_decodes_to_the_rgb8_pixels(tmp_path):
+    """SaveToFilePNG (RenderBuffer.cpp:60-66): a standard PNG whose pixels are GetPixelsRGB8's.
+    Decoded twice: by PIL and by hand (chunk CRCs, zlib stream, filter bytes)."""
+    import struct
+    import zlib
+    from PIL import Image
+    rng = np.random.default_rng(5)
+    for w, h in [(1, 1), (7, 3), (300, 250)]:  # 300x250: the scanlines span several stored blocks
+        px = (rng.random((h, w, 3)) * 1.4 - 0.2).astype(np.float32)
+        path = tmp_path / f"img_{w}x{h}.png"
+        rc, msg = _save(px, path, 0)
+        assert rc == 0, msg
+        want = np.clip(px * np.float32(255.0), 0.0, 255.0).astype(np.uint8)
+        with Image.open(path) as im:
+            assert im.mode == "RGB" and im.size == (w, h)
+            assert np.array_equal(np.asarray(im), want)
+        data = path.read_bytes()
+        assert data[:8] == b"\x89PNG\r\n\x1a\n"
+        pos, idat = 8, b""
+        while pos < len(data):
+            n, typ = struct.unpack(">I4s", data[pos:pos + 8])
+            body = data[pos + 8:pos + 8 + n]
+            crc = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0]
+            assert crc == zlib.crc32(typ + body) & 0xffffffff, typ
+            if typ == b"IDAT":
+                idat += body
+            pos += 12 + n
+        raw = zlib.decompress(idat)  # checks the Adler-32 as well
+        rows = np.frombuffer(raw, np.uint8).reshape(h, 1 + 3 * w)
+        assert (rows[:, 0] == 0).all()
+        assert np.array_equal(rows[:, 1:].reshape(h, w, 3), want)
+
+
+def test_save_hdr_rgbe_round_trip(tmp_path):
+    """SaveToFileHDR (RenderBuffer.cpp:77-84): Radiance RGBE.  Known texels: (1, 0.5, 0.25)
+    shares exponent 1 (frexp(1) = 0.5 * 2^1), mantissas 128/64/32, stored exponent 129; black
+    and negative maxima encode as zeros.  Random texels decode within the format's precision
+    (a component loses less than 2^-8 of the texel's largest component)."""
+    from mitrace_amd.gltf import load_hdr
+    px = np.array([[[1.0, 0.5, 0.25], [0.0, 0.0, 0.0], [-1.0, -2.0, -3.0], [3.0, 0.0, 1e-3]]],
+                  np.float32)
+    path = tmp_path / "k.hdr"
+    rc, msg = _save(px, path, 1)
+    assert rc == 0, msg
+    data = path.read_bytes()
+    head = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 4\n"
+    assert data.startswith(head)
+    body = np.frombuffer(data[len(head):], np.uint8).reshape(4, 4)
+    assert body[0].tolist() == [128, 64, 32, 129]
+    assert body[1].tolist() == [0, 0, 0, 0] and body[2].tolist() == [0, 0, 0, 0]
+    assert body[3].tolist() == [192, 0, 0, 130]  # 3 = 0.75 * 2^2
+    back = load_hdr(str(path))
+    assert np.array_equal(back[0, 0], [1.0, 0.5, 0.25]) and not back[0, 1:3].any()
+    rng = np.random.default_rng(9)
+    px = (rng.random((37, 53, 3)) ** 3 * np.float32(50.0)).astype(np.float32)
+    rc, msg = _save(px, tmp_path / "r.hdr", 1)
+    assert rc == 0, msg
+    back = load_hdr(str(tmp_path / "r.hdr"))
+    mx = px.max(axis=2, keepdims=True)
+    assert back.shape == px.shape
+    assert (back <= px).all() and (px - back <= mx / 128.0).all()
+
+
+def test_save_to_unwritable_path_throws_reference_message(tmp_path):
+    px = np.zeros((2, 2, 3), np.float32)
+    for which in (0, 1):
+        rc, msg = _save(px, tmp_path / "missing_dir" / "x.img", which)
+        assert rc == 1 and msg == "Failed to save image to file"
