@@ -366,6 +366,11 @@ typedef struct mt_batch mt_batch;
 int mt_abi_version(void);
 const char* mt_last_error(void);
 int mt_device_count(int* count);
+/* Renders keep their finished path buffers in a per-process device pool for the next render of
+ * the same size (at most MT_POOL_GB GiB, default 64; 0 = off).  This hands every pooled block
+ * back to the device, e.g. before a caller allocates much device memory elsewhere.  No reference
+ * counterpart (the reference tracer holds no device memory). */
+int mt_pool_trim(void);
 
 /* Scene::Scene(loaderScene): builds each mesh's BVH with the reference rules and uploads the
  * flattened scene to HBM of the current device.  The description is copied; the caller may

@@ -190,6 +190,7 @@ SIGNATURES = {
     "mt_abi_version": (C.c_int, []),
     "mt_last_error": (C.c_char_p, []),
     "mt_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "mt_pool_trim": (C.c_int, []),
     "mt_scene_create": (C.c_int, [C.POINTER(mt_scene_desc), C.POINTER(C.c_void_p)]),
     "mt_scene_destroy": (C.c_int, [C.c_void_p]),
     "mt_scene_mesh_bvh": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),

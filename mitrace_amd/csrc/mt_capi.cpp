@@ -168,6 +168,11 @@ int mt_abi_version(void) { return MT_ABI_VERSION; }
 
 const char* mt_last_error(void) { return mt::g_error.c_str(); }
 
+int mt_pool_trim(void) {
+    mt::pool_trim();
+    return MT_OK;
+}
+
 int mt_device_count(int* count) {
     if (!count) return mt::fail(MT_E_INVALID, "null count");
     if (hipGetDeviceCount(count) != hipSuccess) {

@@ -22,6 +22,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "mt_device.hpp"
@@ -1269,11 +1272,99 @@ int device_cu_count() {
     return cus;
 }
 
+// Device blocks of finished renderers are kept for the next renderer that asks for a block of
+// the same size on the same device: consecutive renders of one configuration ask for identical
+// path buffers (up to 48 GB at 1080p), and reusing them spares each render mapping and
+// unmapping that memory (measured: a 1080p 256-spp frame in 64 Mi-path chunks 1.41 s instead
+// of 0.45 s of kernel time without the pool on the fast traversal, 517 against 495
+// Msamples/s with it on the reference one).  At most MT_POOL_GB gibibytes (default 64, 0 =
+// off) stay pooled, oldest released first; a failing hipMalloc empties the pool and retries;
+// mt_pool_trim empties it on request.  Only Renderer::release returns blocks to the pool,
+// after the device has finished every launch that used them.
+struct PoolBlock {
+    int device;
+    size_t bytes;
+    void* ptr;
+};
+std::mutex g_pool_mu;
+std::deque<PoolBlock> g_pool;                           // oldest first
+size_t g_pool_bytes = 0;
+std::unordered_map<void*, std::pair<int, size_t>> g_block;  // blocks made by alloc(): device, bytes
+
+size_t pool_limit() {
+    static const size_t lim = [] {
+        const char* v = std::getenv("MT_POOL_GB");
+        const long gb = (v && *v) ? std::strtol(v, nullptr, 10) : 64;
+        return gb > 0 ? static_cast<size_t>(gb) << 30 : size_t(0);
+    }();
+    return lim;
+}
+
+void pool_flush_locked() {
+    for (const PoolBlock& b : g_pool) {
+        g_block.erase(b.ptr);
+        (void)hipFree(b.ptr);
+    }
+    g_pool.clear();
+    g_pool_bytes = 0;
+}
+
+// return alloc()'s blocks: to the pool (recycle; their users have finished) or to hipFree
+void release_blocks(std::vector<void*>& owned, bool recycle) {
+    std::scoped_lock lk(g_pool_mu);
+    for (void* p : owned) {
+        auto it = g_block.find(p);
+        if (recycle && it != g_block.end() && pool_limit() > 0) {
+            g_pool.push_back({it->second.first, it->second.second, p});
+            g_pool_bytes += it->second.second;
+        } else {
+            if (it != g_block.end()) g_block.erase(it);
+            (void)hipFree(p);
+        }
+    }
+    owned.clear();
+    while (g_pool_bytes > pool_limit() && !g_pool.empty()) {
+        const PoolBlock b = g_pool.front();
+        g_pool.pop_front();
+        g_pool_bytes -= b.bytes;
+        g_block.erase(b.ptr);
+        (void)hipFree(b.ptr);
+    }
+}
+
+}  // namespace
+
+void pool_trim() {
+    std::scoped_lock lk(g_pool_mu);
+    pool_flush_locked();
+}
+
+namespace {
+
 template <typename T>
 int alloc(std::vector<void*>& owned, T** p, size_t n) {
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
     void* q = nullptr;
-    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
-        return fail(MT_E_NOMEM, "hipMalloc of path buffers failed");
+    {
+        std::scoped_lock lk(g_pool_mu);
+        for (auto it = g_pool.begin(); it != g_pool.end(); ++it)
+            if (it->device == dev && it->bytes == bytes) {
+                q = it->ptr;
+                g_pool_bytes -= bytes;
+                g_pool.erase(it);
+                break;
+            }
+        if (!q) {
+            if (hipMalloc(&q, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                pool_flush_locked();
+                if (hipMalloc(&q, bytes) != hipSuccess) return fail(MT_E_NOMEM, "hipMalloc of path buffers failed");
+            }
+            g_block[q] = {dev, bytes};
+        }
+    }
     owned.push_back(q);
     *p = static_cast<T*>(q);
     return MT_OK;
@@ -1309,9 +1400,7 @@ int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t
     std::vector<void*> owned;
     struct Guard {
         std::vector<void*>& v;
-        ~Guard() {
-            for (void* p : v) (void)hipFree(p);
-        }
+        ~Guard() { release_blocks(v, false); }
     } guard{owned};
     const uint32_t stride = occluded ? 7u : 6u;
     float* drays;
@@ -1360,9 +1449,7 @@ int brdf_eval(uint32_t n, const float* in, float* out) {
     std::vector<void*> owned;
     struct Guard {
         std::vector<void*>& v;
-        ~Guard() {
-            for (void* p : v) (void)hipFree(p);
-        }
+        ~Guard() { release_blocks(v, false); }
     } guard{owned};
     float *din, *dout;
     int rc;
@@ -1396,8 +1483,10 @@ void Renderer::release() {
     if (tail_stream_) (void)hipStreamDestroy(tail_stream_);
     tail_stream_ = nullptr;
     if (stream_) (void)hipStreamSynchronize(stream_);
-    for (void* p : owned_) (void)hipFree(p);
-    owned_.clear();
+    // the blocks go back to the pool only when nothing can still use them: the differentiable
+    // renderer launches on its caller's stream, so wait for the device as hipFree would
+    (void)hipDeviceSynchronize();
+    release_blocks(owned_, true);
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
     events_.clear();
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -1533,17 +1622,17 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         return fail(MT_E_INVALID, "the scene was created without fast BVHs (mt_bvh_options.fast)");
     const uint64_t npix_total = static_cast<uint64_t>(st.width) * st.height;
     if (npix_total * 3 > 0xFFFFFFFFull) return fail(MT_E_UNSUPPORTED, "image too large");
-    // Paths per chunk by default: an eighth of the render, at least 16 Mi and at most 32 Mi
-    // (about 256 B of path state per path: 4 to 8 GB per stream slot).  Measured on the
+    // Paths per chunk by default: an eighth of the render, at least 16 Mi and at most 64 Mi
+    // (about 256 B of path state per path: 4 to 16 GB per stream slot).  Measured on the
     // 1920x1080 256-spp sphere grid (530.8 M paths), reference traversal: 4 Mi 375, 8 Mi 416,
     // 16 Mi 473, 24 Mi 481, 32 Mi 492, 48 Mi 494, 64 Mi 502 Msamples/s -- larger chunks spend
-    // less of the frame in their sparse last bounces -- but the same frame with the fast
-    // traversal fell from ~1200 to 375-910 Msamples/s at 64 Mi (its kernels were not slower:
-    // the time went outside them), so the default stops at 32 Mi.  Eight chunks keep the three
+    // less of the frame in their sparse last bounces -- and 517 at 64 Mi once path buffers
+    // are pooled across renders (without the pool, setting up and releasing 48 GB per render
+    // cost the fast traversal's frame two thirds of its rate).  Eight chunks keep the three
     // stream slots busy on smaller renders (1920x1080 at 64 spp stays at 16.6 Mi).
     const uint64_t all_paths = npix_total * st.samples;
     uint64_t target = st.paths_in_flight ? st.paths_in_flight
-                                         : std::clamp<uint64_t>(all_paths / 8, 1ull << 24, 1ull << 25);
+                                         : std::clamp<uint64_t>(all_paths / 8, 1ull << 24, 1ull << 26);
     target = std::max<uint64_t>(target, st.samples);
     target = std::min<uint64_t>(target, 1ull << 26);
     npix_chunk_ = static_cast<uint32_t>(std::max<uint64_t>(1, target / st.samples));

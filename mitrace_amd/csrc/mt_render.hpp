@@ -222,6 +222,8 @@ int cu_count();
 // the per-render parameters of a render of `scene` with this camera and these settings
 void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& cam,
                  const mt_settings& st);
+// hands every pooled device block back to the device (mt_pool_trim)
+void pool_trim();
 int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t paths,
                        uint64_t spill_entries, bool diff, bool batch, bool debug = false);
 // enqueue one batched iteration (schedule, generate, bounces, finalize, resolve): no host

@@ -166,6 +166,25 @@ def test_chunked_render_equals_single_chunk(scenes):
     assert np.array_equal(a, b)
 
 
+def test_renders_on_reused_path_buffers_are_identical(scenes):
+    """A finished render's path buffers go back to a device pool and the next render of the
+    same size takes them, stale contents included: a render repeated after renders of other
+    sizes, other scenes and a debug view equals its first run bit for bit."""
+    sc, g, o = scenes("cornell_gen")
+    st = small_settings(width=64, height=48, samples=16, ev_exposure=2.0, seed=3)
+    a, sa, ka = g.render(st, want_samples=True)
+    _, sg, _ = scenes("sphere_grid")
+    sg.render(small_settings(width=64, height=48, samples=16, seed=9))  # same sizes, other scene
+    g.render(small_settings(width=40, height=30, samples=8, seed=4))
+    g.render(small_settings(width=64, height=48, samples=16, seed=5, debug_mode=abi.MT_DEBUG_BOUNCES))
+    b, sb, kb = g.render(st, want_samples=True)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    assert ka["rays"] == kb["rays"]
+    assert g.L.mt_pool_trim() == 0  # emptied pool: fresh buffers again
+    c, _, _ = g.render(st)
+    assert np.array_equal(a, c)
+
+
 def test_tracer_start_wait_stats_kill(scenes):
     sc, g, o = scenes("cornell_gen")
     st = small_settings(width=64, height=64, samples=16, ev_exposure=2.0, paths_in_flight=4096)
