@@ -95,8 +95,8 @@ __device__ __forceinline__ void init_path(const ChunkParams& J, const PathBuffer
         o = xform(J.cam, V3{px, -py, 0.0f}, 1.0f);
         d = normalize(xform(J.cam, dc, 0.0f));
     }
-    b.ray_o[p] = f4(o, 0.0f);
-    b.ray_d[p] = f4(d, 0.0f);
+    b.ray_o[0][p] = f4(o, 0.0f);  // queue 0 is the identity (below)
+    b.ray_d[0][p] = f4(d, 0.0f);
     b.t_rad[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     b.t_term[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     b.prev_base[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -284,9 +284,8 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                     V3 o, d;
                     float bound = kFltMax;
                     if (kMode == 0) {
-                        const uint32_t p = q[my];
-                        o = xyz(b.ray_o[p]);
-                        d = xyz(b.ray_d[p]);
+                        o = xyz(b.ray_o[bounce & 1][my]);
+                        d = xyz(b.ray_d[bounce & 1][my]);
                     } else if (kMode == 1) {
                         const float4 so = b.sh_o[my];
                         o = xyz(so);
@@ -392,13 +391,13 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                 active = false;
                 const Hit& h = t.R;
                 if (kMode == 0) {
-                    const uint32_t p = q[item];
-                    // one 16-byte record, so that shading gets the instance, triangle and
-                    // barycentrics with a single load
-                    b.hit_rec[p] = make_uint4(static_cast<uint32_t>(h.inst), h.tri, __float_as_uint(h.u),
-                                              __float_as_uint(h.v));
-                    b.hit_pos[p] = f4(h.wpos, h.dist);
+                    // one 16-byte record in queue order, so that shading gets the instance,
+                    // triangle and barycentrics with a single coalesced load
+                    b.hit_rec[item] = make_uint4(static_cast<uint32_t>(h.inst), h.tri, __float_as_uint(h.u),
+                                                 __float_as_uint(h.v));
+                    b.hit_pos[item] = f4(h.wpos, h.dist);
                     if (kDiag && c.debug && h.inst >= 0) {  // the winning instance's tests (Tracer.cpp:170-175)
+                        const uint32_t p = q[item];
                         uint4 di = b.dbg_i[p];
                         if (bounce == 0) {
                             di.x = h.bvh_tests;
@@ -647,6 +646,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
         const uint32_t idx = base + threadIdx.x;
         const bool valid = idx < count;
         bool cont = false, shadow = false;
+        V3 next_o = mk(0.0f), next_d = mk(0.0f);  // the continuing path's next ray
         float4 sho{}, shd{}, shc{};
         uint32_t p = 0;
         // cache query of a path that stops here
@@ -674,8 +674,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 const float* gi = c.grad_img + static_cast<size_t>(px) * 3;
                 dl = scale(V3{gi[0], gi[1], gi[2]}, c.inv_spp);
             }
-            const V3 d = xyz(b.ray_d[p]);
-            const uint4 hrec = b.hit_rec[p];
+            const V3 d = xyz(b.ray_d[k & 1][idx]);
+            const uint4 hrec = b.hit_rec[idx];
             const int inst = static_cast<int>(hrec.x);
             const V3 trad = xyz(b.t_rad[p]);
             if (kDiff == 0 && !kBatch && kExt && c.cache && c.cache_after && k > c.cache_depth) {
@@ -720,7 +720,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 // the cache supplies the radiance leaving this vertex toward the previous one
                 hits++;
                 query = true;
-                qpos = b.hit_pos[p];
+                qpos = b.hit_pos[idx];
                 qdir = f4(neg(d), 0.0f);
             } else {
                 hits++;
@@ -731,7 +731,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                         const size_t r = static_cast<size_t>(slot) * c.train_nrec + (k - c.cache_depth);
                         const V3 sofar = add(add(add(xyz(b.acc_e[p]), xyz(b.acc_d[p])), xyz(b.acc_i[p])),
                                              xyz(b.acc_s[p]));
-                        b.tr_pos[r] = b.hit_pos[p];
+                        b.tr_pos[r] = b.hit_pos[idx];
                         b.tr_dir[r] = f4(neg(d), 0.0f);
                         // after-light records become valid when the path goes on (next bounce)
                         b.tr_w[r] = f4(trad, c.cache_after ? 0.0f : 1.0f);
@@ -742,8 +742,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 coords<kBatch>(c, b, p, pix, s);
                 const uint32_t key = path_key(J.seed_key, pix, s);
                 const uint32_t dim0 = 2u + k * J.light_stride;
-                const V3 o = xyz(b.ray_o[p]);
-                const float4 hp = b.hit_pos[p];
+                const V3 o = xyz(b.ray_o[k & 1][idx]);
+                const float4 hp = b.hit_pos[idx];
                 const V3 wpos = xyz(hp);
                 const float dist = hp.w;
                 const float2 huv = make_float2(__uint_as_float(hrec.z), __uint_as_float(hrec.w));
@@ -945,9 +945,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     // stops here
                 } else if (!stop && !last) {
                     cont = true;
-                    const V3 no = add(add(o, scale(d, dist)), scale(N, kPullout));
-                    b.ray_o[p] = f4(no, 0.0f);
-                    b.ray_d[p] = f4(nd, 0.0f);
+                    next_o = add(add(o, scale(d, dist)), scale(N, kPullout));  // stored below
+                    next_d = nd;
                     b.t_rad[p] = f4(trad_next, 0.0f);
                     b.t_term[p] = f4(tterm, 0.0f);
                     b.prev_base[p] = f4(basec, 0.0f);
@@ -979,7 +978,11 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             }
         }
         const uint32_t cs = wave_push(b.counters + kCntExtend + k + 1, cont);
-        if (cont) qout[cs] = p;
+        if (cont) {
+            qout[cs] = p;
+            b.ray_o[(k + 1) & 1][cs] = f4(next_o, 0.0f);
+            b.ray_d[(k + 1) & 1][cs] = f4(next_d, 0.0f);
+        }
     }
     if (c.count_work) {
         wave_add_u64(b.work + 4, hits);
@@ -1583,7 +1586,8 @@ void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
 int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, uint64_t spill_entries,
                        bool diff, bool batch, bool debug) {
     int rc;
-    if ((rc = alloc(owned, &b.ray_o, P)) || (rc = alloc(owned, &b.ray_d, P)) ||
+    if ((rc = alloc(owned, &b.ray_o[0], P)) || (rc = alloc(owned, &b.ray_d[0], P)) ||
+        (rc = alloc(owned, &b.ray_o[1], P)) || (rc = alloc(owned, &b.ray_d[1], P)) ||
         (rc = alloc(owned, &b.t_rad, P)) || (rc = alloc(owned, &b.t_term, P)) ||
         (rc = alloc(owned, &b.prev_base, P)) || (rc = alloc(owned, &b.acc_e, P)) ||
         (rc = alloc(owned, &b.acc_d, P)) || (rc = alloc(owned, &b.acc_i, P)) ||

@@ -84,11 +84,13 @@ struct ChunkParams {
 
 // SoA path state, queues and scratch of one render (capacity = paths per chunk)
 struct PathBuffers {
-    float4 *ray_o, *ray_d;
+    // rays of bounce k in queue order (slot i = the path at queue[k & 1][i]), double-buffered by
+    // bounce parity, so that extend and shading read them coalesced, not through the queue
+    float4 *ray_o[2], *ray_d[2];
     float4 *t_rad, *t_term, *prev_base;
     float4 *acc_e, *acc_d, *acc_i, *acc_s;
-    uint4* hit_rec;   // closest hit: {instance (as int, -1 = miss), triangle, u bits, v bits}
-    float4* hit_pos;  // world position, world distance
+    uint4* hit_rec;   // closest hit in queue order: {instance (as int, -1 = miss), triangle, u, v bits}
+    float4* hit_pos;  // world position, world distance (queue order)
     uint32_t* queue[2];
     float4 *sh_o, *sh_d, *sh_c;  // shadow rays: {o, max dist} {d, path} {contribution, slot}
     float4* sample_rgb;
