@@ -98,8 +98,8 @@ __device__ __forceinline__ void init_path(const ChunkParams& J, const PathBuffer
     b.ray_o[0][p] = f4(o, 0.0f);  // queue 0 is the identity (below)
     b.ray_d[0][p] = f4(d, 0.0f);
     b.t_rad[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    b.t_term[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    b.prev_base[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.t_term[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.prev_base[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.acc_e[p] = z;
     b.acc_d[p] = z;
@@ -647,6 +647,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
         const bool valid = idx < count;
         bool cont = false, shadow = false;
         V3 next_o = mk(0.0f), next_d = mk(0.0f);  // the continuing path's next ray
+        V3 next_term = mk(0.0f), next_base = mk(0.0f);
         float4 sho{}, shd{}, shc{};
         uint32_t p = 0;
         // cache query of a path that stops here
@@ -702,7 +703,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             }
             if (inst < 0) {  // miss: sky term (Tracer.cpp:244-252)
                 misses++;
-                const V3 col = (k > 0) ? xyz(b.prev_base[p]) : mk(1.0f);
+                const V3 col = (k > 0) ? xyz(b.prev_base[k & 1][idx]) : mk(1.0f);
                 const float mult = (k == 0) ? J.hdri_primary : J.hdri_secondary;
                 const V3 sk = scale(sample_env(S, d, J.hdri_rot_apply, J.hdri_rot_offset), mult);
                 b.acc_s[p] = f4(mul(mul(col, trad), sk), 0.0f);
@@ -815,7 +816,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 if (frozen)
                     energy_t = (pdf > 0.0f) ? scale(eval_brdf(nd, view, N, basec_s, rough_s, metal_s), nl / pdf)
                                             : mk(0.0f);
-                V3 tterm = mul(xyz(b.t_term[p]), energy_t);
+                V3 tterm = mul(xyz(b.t_term[k & 1][idx]), energy_t);
                 bool stop;
                 float rr_q = 1.0f;
                 if (J.mode == MT_MODE_FIXED) {
@@ -948,8 +949,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     next_o = add(add(o, scale(d, dist)), scale(N, kPullout));  // stored below
                     next_d = nd;
                     b.t_rad[p] = f4(trad_next, 0.0f);
-                    b.t_term[p] = f4(tterm, 0.0f);
-                    b.prev_base[p] = f4(basec, 0.0f);
+                    next_term = tterm;
+                    next_base = basec;
                 }
             }
         }
@@ -982,6 +983,8 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             qout[cs] = p;
             b.ray_o[(k + 1) & 1][cs] = f4(next_o, 0.0f);
             b.ray_d[(k + 1) & 1][cs] = f4(next_d, 0.0f);
+            b.t_term[(k + 1) & 1][cs] = f4(next_term, 0.0f);
+            b.prev_base[(k + 1) & 1][cs] = f4(next_base, 0.0f);
         }
     }
     if (c.count_work) {
@@ -1588,8 +1591,9 @@ int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, ui
     int rc;
     if ((rc = alloc(owned, &b.ray_o[0], P)) || (rc = alloc(owned, &b.ray_d[0], P)) ||
         (rc = alloc(owned, &b.ray_o[1], P)) || (rc = alloc(owned, &b.ray_d[1], P)) ||
-        (rc = alloc(owned, &b.t_rad, P)) || (rc = alloc(owned, &b.t_term, P)) ||
-        (rc = alloc(owned, &b.prev_base, P)) || (rc = alloc(owned, &b.acc_e, P)) ||
+        (rc = alloc(owned, &b.t_rad, P)) || (rc = alloc(owned, &b.t_term[0], P)) ||
+        (rc = alloc(owned, &b.t_term[1], P)) || (rc = alloc(owned, &b.prev_base[0], P)) ||
+        (rc = alloc(owned, &b.prev_base[1], P)) || (rc = alloc(owned, &b.acc_e, P)) ||
         (rc = alloc(owned, &b.acc_d, P)) || (rc = alloc(owned, &b.acc_i, P)) ||
         (rc = alloc(owned, &b.acc_s, P)) || (rc = alloc(owned, &b.hit_rec, P)) ||
         (rc = alloc(owned, &b.hit_pos, P)) || (rc = alloc(owned, &b.queue[0], P)) ||

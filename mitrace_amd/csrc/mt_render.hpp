@@ -87,7 +87,9 @@ struct PathBuffers {
     // rays of bounce k in queue order (slot i = the path at queue[k & 1][i]), double-buffered by
     // bounce parity, so that extend and shading read them coalesced, not through the queue
     float4 *ray_o[2], *ray_d[2];
-    float4 *t_rad, *t_term, *prev_base;
+    float4* t_rad;  // radiance throughput by path index (the cache passes read it there)
+    // termination throughput and previous base colour in queue order, by bounce parity
+    float4 *t_term[2], *prev_base[2];
     float4 *acc_e, *acc_d, *acc_i, *acc_s;
     uint4* hit_rec;   // closest hit in queue order: {instance (as int, -1 = miss), triangle, u, v bits}
     float4* hit_pos;  // world position, world distance (queue order)
