@@ -55,12 +55,37 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_threads():
+def host_cpu_info():
+    """What the host offers: os.cpu_count(), the affinity set of this process, the CPU model
+    (lscpu / /proc/cpuinfo) and the thread share the job is given (OMP_NUM_THREADS, which the
+    GPU hosts set to the CPU share of one GPU)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        affinity = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return {"os_cpu_count": os.cpu_count(), "affinity": affinity, "model": model,
+            "omp_num_threads": int(share) if share.isdigit() else None}
+
+
+def cpu_threads():
+    """Worker threads of the CPU baseline: every CPU of the affinity set, capped by the job's
+    thread share when the host states one (OMP_NUM_THREADS): a GPU host shares its cores
+    between the jobs of its GPUs, and one GPU's job is given that many."""
+    info = host_cpu_info()
+    n = info["affinity"]
+    if info["omp_num_threads"]:
+        n = min(n, info["omp_num_threads"])
+    return max(1, n)
 
 
 def cpu_baseline(scene, settings_fn, seconds):
@@ -83,7 +108,7 @@ def cpu_baseline(scene, settings_fn, seconds):
         spp = max(1, min(64 - spp_done, int((seconds - total_s) / max(secs / spp, 1e-3))))
     o.close()
     return {"value": total_n / total_s / 1e6, "unit": "Msamples/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_cpu_info(),
             "sample": f"1920x1080 at {spp_done} spp ({total_n} camera paths, {total_r} rays) in "
                       f"{total_s:.1f} s; oracle with -Ofast -flto -march=x86-64-v3, mt19937 per "
                       f"32x32 tile, {threads} threads",

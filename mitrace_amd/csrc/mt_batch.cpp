@@ -85,6 +85,9 @@ struct mt_batch {
     int device = 0;
     mt_batch_options opt{};
     hipStream_t stream = nullptr, capture = nullptr;  // uploads; graph capture
+    // Device blocks are freed when the batch is destroyed.  The scheduler thread adds tile
+    // queues while submitters add job images, so the vector is guarded by mu (init runs
+    // before the thread exists).
     std::vector<void*> owned;
     std::vector<Lane> lanes;
     uint32_t next_lane = 0;
@@ -93,7 +96,8 @@ struct mt_batch {
     uint32_t trace_blocks = 0, shade_blocks = 0;
     uint32_t lds_ref = 16, lds_fast = 16;  // traversal stack entries in LDS per traversal
     std::chrono::steady_clock::time_point created;
-    // shared with submitters
+    // shared with submitters and readers: jobs, slots, owned and error under mu; status,
+    // iterations and captures are atomics (written by the scheduler thread, read anywhere)
     std::mutex mu;
     std::condition_variable cv;
     std::vector<std::unique_ptr<HostJob>> jobs;
@@ -103,12 +107,12 @@ struct mt_batch {
     std::atomic<bool> finishing{false};
     std::atomic<bool> started{false};
     std::thread worker;
-    int status = MT_OK;
+    std::atomic<int> status{MT_OK};
     std::string error;
     // scheduler state
     hipEvent_t e0 = nullptr;
     double t0_ms = 0.0;
-    uint64_t iterations = 0, captures = 0;
+    std::atomic<uint64_t> iterations{0}, captures{0};
     uint32_t first_open = 0;  // oldest job with tiles not yet scheduled
     std::vector<hipEvent_t> spare;
 
@@ -117,11 +121,29 @@ struct mt_batch {
     int admit(uint32_t id);
     int ensure_graph(SceneSlot& s, uint32_t lane);
     int run();
+    // records the first error of the batch (the scheduler thread stops at it)
     int fail(int code, const std::string& m) {
-        status = code;
-        error = m;
+        {
+            std::scoped_lock lk(mu);
+            if (status.load() == MT_OK) {
+                error = m;
+                status.store(code);
+            }
+        }
         mt::set_error(m);
         return code;
+    }
+    // the batch's error for a caller thread: code, and its message as that thread's last error
+    int report() {
+        std::scoped_lock lk(mu);
+        const int code = status.load();
+        if (code) mt::set_error(error);
+        return code;
+    }
+    // a device block this batch frees at destruction
+    void own(void* p) {
+        std::scoped_lock lk(mu);
+        owned.push_back(p);
     }
 };
 
@@ -154,7 +176,9 @@ mt_batch::~mt_batch() {
     }
     for (hipEvent_t e : spare) (void)hipEventDestroy(e);
     if (e0) (void)hipEventDestroy(e0);
-    for (void* p : owned) (void)hipFree(p);
+    // the lanes' path buffers come from the renderer's block allocator, which keeps track of
+    // them: free_blocks hands every block back and clears that bookkeeping
+    mt::free_blocks(owned);
     if (jobs_host) (void)hipHostFree(jobs_host);
     if (stream) (void)hipStreamDestroy(stream);
     if (capture) (void)hipStreamDestroy(capture);
@@ -182,8 +206,8 @@ int mt_batch::init(const mt_batch_options* o) {
     lds_fast = mt::lds_stack_depth(true, opt.trace_blocks_per_cu);
     const uint64_t spill = static_cast<uint64_t>(kMaxStack - std::min(lds_ref, lds_fast)) * trace_blocks * kBlockSize;
     auto dev_alloc = [&](void** p, size_t bytes) -> int {
-        if (hipMalloc(p, bytes) != hipSuccess) return fail(MT_E_NOMEM, "hipMalloc of batch buffers failed");
-        owned.push_back(*p);
+        if (mt::device_malloc(p, bytes)) return fail(MT_E_NOMEM, "device allocation of batch buffers failed");
+        own(*p);
         return MT_OK;
     };
     int rc;
@@ -213,7 +237,7 @@ int mt_batch::admit(uint32_t id) {
     SceneSlot* s = nullptr;
     // one slot per scene and traversal: the slot's launch constants pick the traversal kernel
     const int32_t accel = j.st.traversal == MT_TRAVERSAL_FAST ? 1 : 0;
-    for (auto& x : slots)
+    for (auto& x : slots)  // only this thread changes slots: reading without the lock is safe
         if (x->scene == j.scene && x->c.accel == accel) s = x.get();
     if (!s) {
         auto ns = std::make_unique<SceneSlot>();
@@ -225,11 +249,14 @@ int mt_batch::admit(uint32_t id) {
         ns->c.count_work = 0;
         for (uint32_t l = 0; l < nl; ++l) {
             SceneQueue& q = ns->q[l];
-            if (hipMalloc(reinterpret_cast<void**>(&q.q_dev), sizeof(uint4) * kQueueTiles) != hipSuccess ||
-                hipMalloc(reinterpret_cast<void**>(&q.q_state), sizeof(uint32_t) * 2) != hipSuccess)
-                return fail(MT_E_NOMEM, "hipMalloc of a tile queue failed");
-            owned.push_back(q.q_dev);
-            owned.push_back(q.q_state);
+            void* qd = nullptr;
+            void* qs = nullptr;
+            if (mt::device_malloc(&qd, sizeof(uint4) * kQueueTiles)) return fail(MT_E_NOMEM, "device allocation of a tile queue failed");
+            own(qd);
+            if (mt::device_malloc(&qs, sizeof(uint32_t) * 2)) return fail(MT_E_NOMEM, "device allocation of a tile queue failed");
+            own(qs);
+            q.q_dev = static_cast<uint4*>(qd);
+            q.q_state = static_cast<uint32_t*>(qs);
             BT_TRY(hipHostMalloc(reinterpret_cast<void**>(&q.q_host), sizeof(uint4) * kQueueTiles));
             mt::launch_set_u32(q.q_state, 0u, stream);
             mt::launch_set_u32(q.q_state + 1, 0u, stream);
@@ -237,8 +264,11 @@ int mt_batch::admit(uint32_t id) {
             q.b.q_tiles = q.q_dev;
             q.b.q_state = q.q_state;
         }
-        slots.push_back(std::move(ns));
-        s = slots.back().get();
+        {
+            std::scoped_lock lk(mu);
+            slots.push_back(std::move(ns));
+            s = slots.back().get();
+        }
     }
     j.slot = 0;
     for (uint32_t i = 0; i < slots.size(); ++i)
@@ -453,7 +483,7 @@ int mt_batch_submit(mt_batch* batch, mt_scene* scene, const mt_camera_desc* came
     if (!batch) return mt::fail(MT_E_INVALID, "null batch");
     if (!valid_job(scene, camera, settings, batch->opt.paths_per_iteration)) return MT_E_INVALID;
     if (batch->finishing.load()) return mt::fail(MT_E_INVALID, "batch is finishing");
-    if (batch->status) return mt::fail(batch->status, batch->error);
+    if (const int rc = batch->report()) return rc;
     try {
         auto j = std::make_unique<HostJob>();
         j->scene = scene;
@@ -464,12 +494,16 @@ int mt_batch_submit(mt_batch* batch, mt_scene* scene, const mt_camera_desc* came
         j->submit_ms = now_ms(batch->created);
         if (hipSetDevice(batch->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
         const size_t bytes = sizeof(float) * 3ull * settings->width * settings->height;
-        if (hipMalloc(reinterpret_cast<void**>(&j->img_dev), bytes) != hipSuccess)
-            return mt::fail(MT_E_NOMEM, "hipMalloc of a job image failed");
+        void* img = nullptr;
+        if (mt::device_malloc(&img, bytes)) return MT_E_NOMEM;
+        j->img_dev = static_cast<float*>(img);
         std::scoped_lock lk(batch->mu);
-        if (batch->jobs.size() >= kMaxJobs) {
+        // checked again under the lock: the scheduler exits once it sees finishing with every
+        // job admitted, so a job pushed after that would never run
+        if (batch->jobs.size() >= kMaxJobs || batch->finishing.load()) {
             (void)hipFree(j->img_dev);
-            return mt::fail(MT_E_UNSUPPORTED, "too many jobs in one batch");
+            return batch->finishing.load() ? mt::fail(MT_E_INVALID, "batch is finishing")
+                                           : mt::fail(MT_E_UNSUPPORTED, "too many jobs in one batch");
         }
         batch->owned.push_back(j->img_dev);
         if (job_id) *job_id = static_cast<uint32_t>(batch->jobs.size());
@@ -489,11 +523,18 @@ int mt_batch_start(mt_batch* batch) {
         hipEventSynchronize(batch->e0) != hipSuccess)
         return mt::fail(MT_E_HIP, "batch start event failed");
     batch->t0_ms = now_ms(batch->created);
+    // the scheduler thread: an error (or a C++ exception, e.g. std::bad_alloc from a vector
+    // it grows) stops it and becomes the batch's status, reported to the caller by
+    // mt_batch_finish; nothing may escape the thread, or the process would terminate
     batch->worker = std::thread([batch]() {
-        const int rc = batch->run();
-        if (rc && !batch->status) {
-            batch->status = rc;
-            batch->error = mt_last_error();
+        int rc;
+        try {
+            rc = batch->run();
+            if (rc) batch->fail(rc, mt_last_error());
+        } catch (const std::bad_alloc&) {
+            batch->fail(MT_E_NOMEM, "host allocation failed in the batch scheduler");
+        } catch (const std::exception& e) {
+            batch->fail(MT_E_INVALID, e.what());
         }
     });
     return MT_OK;
@@ -508,7 +549,7 @@ int mt_batch_finish(mt_batch* batch) {
     batch->finishing.store(true);
     batch->cv.notify_all();
     if (batch->worker.joinable()) batch->worker.join();
-    if (batch->status) return mt::fail(batch->status, batch->error);
+    if (const int rc = batch->report()) return rc;
     if (hipSetDevice(batch->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");
     for (auto& j : batch->jobs) {
         if (!j->rgb_out) continue;
@@ -526,7 +567,7 @@ int mt_batch_get_job_info(const mt_batch* batch, uint32_t job_id, mt_batch_job_i
     if (job_id >= b->jobs.size()) return mt::fail(MT_E_INVALID, "unknown job id");
     const HostJob& j = *b->jobs[job_id];
     std::memset(info, 0, sizeof(*info));
-    info->status = b->status;
+    info->status = b->status.load();
     info->tiles = j.tiles;
     info->samples = j.samples;
     info->submit_ms = j.submit_ms;
@@ -555,8 +596,8 @@ int mt_batch_get_stats(const mt_batch* batch, mt_batch_stats* stats) {
     auto* b = const_cast<mt_batch*>(batch);
     std::scoped_lock lk(b->mu);
     std::memset(stats, 0, sizeof(*stats));
-    stats->iterations = b->iterations;
-    stats->graphs_captured = b->captures;
+    stats->iterations = b->iterations.load();
+    stats->graphs_captured = b->captures.load();
     stats->jobs = static_cast<uint32_t>(b->jobs.size());
     stats->scenes = static_cast<uint32_t>(b->slots.size());
     for (auto& j : b->jobs) stats->samples += j->samples;

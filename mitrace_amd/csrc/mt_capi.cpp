@@ -293,9 +293,19 @@ int mt_render_start(mt_scene* scene, const mt_camera_desc* camera, const mt_sett
         j->rgb_out = rgb_out;
         j->start = std::chrono::steady_clock::now();
         mt_job* raw = j.get();
+        // nothing may escape the render thread (an exception there would terminate the
+        // process): host allocation failures become MT_E_NOMEM, other exceptions MT_E_INVALID
         raw->worker = std::thread([raw]() {
-            raw->status = run_job(raw, nullptr);
-            if (raw->status) raw->error = mt_last_error();
+            int rc;
+            try {
+                rc = run_job(raw, nullptr);
+            } catch (const std::bad_alloc&) {
+                rc = mt::fail(MT_E_NOMEM, "host allocation failed in the render thread");
+            } catch (const std::exception& e) {
+                rc = mt::fail(MT_E_INVALID, e.what());
+            }
+            raw->status = rc;
+            if (rc) raw->error = mt_last_error();
             raw->done.store(true);
         });
         *job = j.release();
@@ -528,8 +538,7 @@ int upload3(std::vector<void*>& owned, uint32_t n, const float* src, float4** ds
     std::vector<float4> tmp(n);
     for (uint32_t i = 0; i < n; ++i) tmp[i] = make_float4(src[i * 3], src[i * 3 + 1], src[i * 3 + 2], 0.0f);
     void* p = nullptr;
-    if (hipMalloc(&p, sizeof(float4) * std::max<uint32_t>(n, 1)) != hipSuccess)
-        return mt::fail(MT_E_NOMEM, "hipMalloc failed");
+    if (mt::device_malloc(&p, sizeof(float4) * std::max<uint32_t>(n, 1))) return MT_E_NOMEM;
     owned.push_back(p);
     *dst = static_cast<float4*>(p);
     if (n && hipMemcpy(p, tmp.data(), sizeof(float4) * n, hipMemcpyHostToDevice) != hipSuccess)

@@ -176,6 +176,12 @@ namespace mt {
 // error reporting (thread-local last error)
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
+// Every device allocation of the library goes through device_malloc: hipMalloc, and when it
+// fails, the pooled path buffers of finished renders are handed back to the device and the
+// allocation is tried once more.  free_blocks frees such blocks (pooled path buffers included,
+// whose bookkeeping it clears) and empties the vector.
+int device_malloc(void** p, size_t bytes);
+void free_blocks(std::vector<void*>& blocks);
 int upload_scene(const mt_scene_desc* desc, mt_scene* scene);
 // top-level BVH over boxes (6 floats each): SAH over 12 bins per axis, up to 4 boxes a leaf.
 // pairs/indices use the device format of DevScene::tlas_*; returns the number of levels.

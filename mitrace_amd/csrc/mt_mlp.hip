@@ -456,8 +456,8 @@ __global__ void k_zero(float* p, uint32_t n) {
 template <typename T>
 int dalloc(std::vector<void*>& owned, T** p, size_t n) {
     void* q = nullptr;
-    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
-        return fail(MT_E_NOMEM, "hipMalloc of cache buffers failed");
+    if (device_malloc(&q, std::max<size_t>(n, 1) * sizeof(T)))
+        return fail(MT_E_NOMEM, "device allocation of cache buffers failed");
     owned.push_back(q);
     *p = static_cast<T*>(q);
     return MT_OK;

@@ -1345,6 +1345,22 @@ void pool_trim() {
     pool_flush_locked();
 }
 
+int device_malloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (hipMalloc(p, std::max<size_t>(bytes, 1)) == hipSuccess) return MT_OK;
+    (void)hipGetLastError();
+    {
+        std::scoped_lock lk(g_pool_mu);
+        pool_flush_locked();
+    }
+    if (hipMalloc(p, std::max<size_t>(bytes, 1)) == hipSuccess) return MT_OK;
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(MT_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+}
+
+void free_blocks(std::vector<void*>& blocks) { release_blocks(blocks, false); }
+
 namespace {
 
 template <typename T>

@@ -526,7 +526,7 @@ int dev_upload(mt_scene* s, const std::vector<T>& host, const T** out) {
     if (host.empty()) return MT_OK;
     void* p = nullptr;
     const size_t bytes = host.size() * sizeof(T);
-    if (hipMalloc(&p, bytes) != hipSuccess) return fail(MT_E_NOMEM, "hipMalloc failed in scene upload");
+    if (device_malloc(&p, bytes)) return fail(MT_E_NOMEM, "device allocation failed in scene upload");
     s->allocations.push_back(p);
     if (hipMemcpy(p, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
         return fail(MT_E_HIP, "hipMemcpy failed in scene upload");
