@@ -18,6 +18,7 @@
  *   main.cpp:121-142 (start + wait, synchronous)      mt_render
  *   IntersectScene (src/Tracer/Intersect.cpp:201-219) mt_trace_closest   (test hook)
  *   BRDF::EvaluateBRDF / DiffusePdf / SpecularPdf     mt_brdf_eval       (test hook)
+ *   sinf/cosf/atan2/acos/pow of the path              mt_math_eval       (test hook)
  *     (src/Tracer/BRDF.hpp:52-97)
  *
  * Conventions: plain C types only; every pointer argument is a HOST pointer unless its name
@@ -426,6 +427,18 @@ int mt_trace_occluded(mt_scene* scene, uint32_t n, const float* rays, uint32_t* 
  * {L.xyz, V.xyz, N.xyz, base.rgb, roughness, metallic}; out: 5 floats per case
  * {f.rgb, diffuse_pdf, specular_pdf}. */
 int mt_brdf_eval(uint32_t n, const float* in, float* out);
+
+/* The path's elementary functions as the kernels compute them (the reference calls the C
+ * library's sinf/cosf (Tracer.cpp:34-93), atan2/acos (Image.cpp:63-75) and glm::pow (BRDF.hpp:9-13)
+ * on the CPU; this build defines them once, identically on the host oracle and on the GPU, so
+ * that renders agree bit for bit).  Test hook: in = n pairs {x, y}, out = n results;
+ * MT_MATH_ATAN2 computes atan2(x, y), the others use x only. */
+#define MT_MATH_SIN 0
+#define MT_MATH_COS 1
+#define MT_MATH_ATAN2 2
+#define MT_MATH_ACOS 3
+#define MT_MATH_POW5 4
+int mt_math_eval(int32_t op, uint32_t n, const float* in, float* out);
 
 /* Path-replay adjoint (no reference counterpart; for inverse rendering).  mt_diff_create
  * sizes per-path storage (16 bytes per camera path: linear radiance and shadow visibility

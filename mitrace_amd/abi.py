@@ -27,6 +27,7 @@ MT_ENV_NONE, MT_ENV_RGB8, MT_ENV_RGB32F = 0, 1, 2
 MT_MODE_PARITY, MT_MODE_FIXED = 0, 1
 MT_OUTPUT_TONEMAPPED, MT_OUTPUT_LINEAR = 0, 1
 MT_TRAVERSAL_REFERENCE, MT_TRAVERSAL_FAST = 0, 1
+MT_MATH_SIN, MT_MATH_COS, MT_MATH_ATAN2, MT_MATH_ACOS, MT_MATH_POW5 = 0, 1, 2, 3, 4
 
 
 class mt_mesh_desc(C.Structure):
@@ -212,6 +213,7 @@ SIGNATURES = {
     "mt_trace_occluded": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_uint32), C.c_int32]),
     "mt_brdf_eval": (C.c_int, [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mt_math_eval": (C.c_int, [C.c_int32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mt_fast_bvh_build": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),

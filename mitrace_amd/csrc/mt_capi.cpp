@@ -379,6 +379,15 @@ int mt_brdf_eval(uint32_t n, const float* in, float* out) {
     }
 }
 
+int mt_math_eval(int32_t op, uint32_t n, const float* in, float* out) {
+    if (n && (!in || !out)) return mt::fail(MT_E_INVALID, "bad arguments");
+    try {
+        return mt::math_eval(op, n, in, out);
+    } catch (const std::exception& e) {
+        return mt::fail(MT_E_INVALID, e.what());
+    }
+}
+
 int mt_diff_create(mt_scene* scene, const mt_settings* settings, mt_diff** out) {
     if (!out) return mt::fail(MT_E_INVALID, "null output handle");
     *out = nullptr;

@@ -49,6 +49,86 @@ __device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1
 __device__ __forceinline__ V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
+// ------------------------------------------------------------------ elementary functions
+// sin, cos, atan2, acos and x^5 of the path, operation by operation as the CPU oracle
+// computes them, so that renders agree with it bit for bit (the device library's sinf/cosf/
+// atan2f/acosf/powf and the C library's differ in the last bits).  sin/cos: Cody & Waite
+// (1980) reduction x - k pi/2 with pi/2 split into three floats (the first two with short
+// mantissas, so k times them is exact for |k| < 2^12), Taylor polynomials of sin (through r^9)
+// and cos (through r^10) on [-pi/4, pi/4].  atan2: octant symmetry to t in [0, 1], atan(t) =
+// pi/4 + atan((t-1)/(t+1)) to |t| <= tan(pi/8), Taylor series through t^19.
+// acos(x) = atan2(sqrt((1-x)(1+x)), x); x^5 = (x^2 x^2) x.
+namespace em {
+constexpr float kTwoOverPi = 0x1.45f306p-1f;
+constexpr float kPio2a = 0x1.92p+0f, kPio2b = 0x1.fb6p-12f, kPio2c = -0x1.777a5cp-25f;
+constexpr float kPiF = 0x1.921fb6p+1f, kPio2 = 0x1.921fb6p+0f, kPio4 = 0x1.921fb6p-1f;
+constexpr float kTanPi8 = 0x1.a8279ap-2f;
+__device__ __forceinline__ float sin_poly(float r) {
+    const float r2 = r * r;
+    float p = 0x1.71de3ap-19f;
+    p = p * r2 + -0x1.a01a02p-13f;
+    p = p * r2 + 0x1.111112p-7f;
+    p = p * r2 + -0x1.555556p-3f;
+    return r + (r * r2) * p;
+}
+__device__ __forceinline__ float cos_poly(float r) {
+    const float r2 = r * r;
+    float p = -0x1.27e4fcp-22f;
+    p = p * r2 + 0x1.a01a02p-16f;
+    p = p * r2 + -0x1.6c16c2p-10f;
+    p = p * r2 + 0x1.555556p-5f;
+    p = p * r2 + -0.5f;
+    return 1.0f + r2 * p;
+}
+__device__ __forceinline__ void sincos(float x, float& s, float& c) {
+    const float k = floorf(x * kTwoOverPi + 0.5f);
+    const float r = ((x - k * kPio2a) - k * kPio2b) - k * kPio2c;
+    const float ps = sin_poly(r), pc = cos_poly(r);
+    const int q = static_cast<int>(k) & 3;
+    s = q == 0 ? ps : (q == 1 ? pc : (q == 2 ? -ps : -pc));
+    c = q == 0 ? pc : (q == 1 ? -ps : (q == 2 ? -pc : ps));
+}
+__device__ __forceinline__ float atan_poly(float t) {
+    const float t2 = t * t;
+    float p = -0x1.af286cp-5f;
+    p = p * t2 + 0x1.e1e1e2p-5f;
+    p = p * t2 + -0x1.111112p-4f;
+    p = p * t2 + 0x1.3b13b2p-4f;
+    p = p * t2 + -0x1.745d18p-4f;
+    p = p * t2 + 0x1.c71c72p-4f;
+    p = p * t2 + -0x1.24924ap-3f;
+    p = p * t2 + 0x1.99999ap-3f;
+    p = p * t2 + -0x1.555556p-2f;
+    return t + (t * t2) * p;
+}
+__device__ __forceinline__ float atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ay < ax ? ax : ay, mn = ay < ax ? ay : ax;
+    float a;
+    if (mx == 0.0f) {
+        a = 0.0f;
+    } else {
+        float t = mn / mx;
+        float off = 0.0f;
+        if (t > kTanPi8) {
+            t = (t - 1.0f) / (t + 1.0f);
+            off = kPio4;
+        }
+        a = off + atan_poly(t);
+    }
+    if (ax < ay) a = kPio2 - a;
+    if (signbit(x)) a = kPiF - a;
+    return signbit(y) ? -a : a;
+}
+__device__ __forceinline__ float acos(float x) {
+    return atan2(__builtin_sqrtf((1.0f - x) * (1.0f + x)), x);
+}
+__device__ __forceinline__ float pow5(float x) {
+    const float x2 = x * x;
+    return (x2 * x2) * x;
+}
+}  // namespace em
+
 // column-major 4x4 times (p, w)
 __device__ __forceinline__ V3 xform(const float* m, V3 p, float w) {
     V3 o;
@@ -134,8 +214,8 @@ __device__ __forceinline__ float4 sample_tex(const DevScene& S, const DevTex& t,
 // Scene::SampleSkybox without the intensity (Scene.cpp:61-73 -> Image.cpp:63-75)
 __device__ V3 sample_env(const DevScene& S, V3 d, int apply_rot, float rot_offset) {
     if (!S.has_env) return mk(0.0f);
-    const float theta = atan2f(d.z, d.x);
-    const float phi = acosf(d.y / len(d));
+    const float theta = em::atan2(d.z, d.x);
+    const float phi = em::acos(d.y / len(d));
     float u = (theta + kPi) / (2.0f * kPi);
     const float v = phi / kPi;
     if (apply_rot) u = u * 1.0f + rot_offset;
@@ -147,7 +227,7 @@ __device__ V3 sample_env(const DevScene& S, V3 d, int apply_rot, float rot_offse
 // ------------------------------------------------------------------ BRDF (BRDF.hpp)
 __device__ __forceinline__ V3 fresnel(float c, V3 f0) {
     c = clampf(c, 0.0f, 1.0f);
-    const float p = powf(1.0f - c, 5.0f);
+    const float p = em::pow5(1.0f - c);
     return {f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
 }
 __device__ __forceinline__ V3 base_refl(V3 base, float m) {
@@ -202,7 +282,7 @@ __device__ BrdfGrad eval_brdf_grad(V3 L, V3 V, V3 N, V3 base, float rough_in, fl
     const float nh = clampf(gmax(dot(N, h), 0.0f), 0.0f, 1.0f);
     const float hv = clampf(gmax(dot(h, V), 0.0f), 0.0f, 1.0f);
     const float mc = clampf(metal, 0.0f, 1.0f);
-    const float p5 = powf(1.0f - hv, 5.0f);
+    const float p5 = em::pow5(1.0f - hv);
     const float a = rough * rough;
     const float a2 = a * a;
     const float den = (nh * nh) * (a2 - 1.0f) + 1.0f;
@@ -271,7 +351,7 @@ __device__ V3 cosine_dir(float u1, float u2, V3 n) {
     V3 t, b;
     tbn(n, t, b);
     float s, c;
-    sincosf(phi, &s, &c);
+    em::sincos(phi, s, c);
     const float lz = __builtin_sqrtf(gmax(0.0f, 1.0f - u1));
     return normalize(add(add(scale(t, r * c), scale(b, r * s)), scale(n, lz)));
 }
@@ -285,7 +365,7 @@ __device__ bool ggx_dir(float u1, float u2, V3 incident, V3 n, float rough, V3& 
     V3 t, b;
     tbn(n, t, b);
     float s, c;
-    sincosf(phi, &s, &c);
+    em::sincos(phi, s, c);
     V3 h = normalize(add(add(scale(t, st * c), scale(b, st * s)), scale(n, ct)));
     if (dot(h, neg(incident)) < 0.0f) h = neg(h);
     const V3 refl = sub(incident, scale(scale(h, dot(h, incident)), 2.0f));  // glm::reflect

@@ -872,8 +872,10 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     if (L.type == MT_LIGHT_POINT || L.type == MT_LIGHT_SPOT) {  // Tracer.cpp:84-93
                         const float ph = draw(key, dim0 + kDimLightPt) * (2.0f * kPi - 0.0f) + 0.0f;
                         const float th = draw(key, dim0 + kDimLightPt + 1) * (kPi * 0.5f - (-kPi * 0.5f)) + (-kPi * 0.5f);
-                        const float ct = cosf(th);
-                        const V3 rd = normalize(V3{cosf(ph) * ct, sinf(th), sinf(ph) * ct});
+                        float sp, cp, st, ct;
+                        em::sincos(ph, sp, cp);
+                        em::sincos(th, st, ct);
+                        const V3 rd = normalize(V3{cp * ct, st, sp * ct});
                         lpos = add(lpos, scale(rd, L.size));
                     }
                     const V3 lvec = sub(lpos, wpos);
@@ -1252,6 +1254,22 @@ __global__ void k_brdf(uint32_t n, const float* in, float* out) {
     o[4] = specular_pdf(L, V, N, q[12]);
 }
 
+__global__ void k_math(int32_t op, uint32_t n, const float* in, float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = in[2 * i], y = in[2 * i + 1];
+    float r = 0.0f, s, c;
+    switch (op) {
+        case MT_MATH_SIN: em::sincos(x, s, c); r = s; break;
+        case MT_MATH_COS: em::sincos(x, s, c); r = c; break;
+        case MT_MATH_ATAN2: r = em::atan2(x, y); break;
+        case MT_MATH_ACOS: r = em::acos(x); break;
+        case MT_MATH_POW5: r = em::pow5(x); break;
+        default: break;
+    }
+    out[i] = r;
+}
+
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
@@ -1482,6 +1500,26 @@ int brdf_eval(uint32_t n, const float* in, float* out) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(out, dout, sizeof(float) * 5 * n, hipMemcpyDeviceToHost));
+    return MT_OK;
+}
+
+int math_eval(int32_t op, uint32_t n, const float* in, float* out) {
+    if (op < MT_MATH_SIN || op > MT_MATH_POW5) return fail(MT_E_INVALID, "unknown elementary function");
+    if (n == 0) return MT_OK;
+    std::vector<void*> owned;
+    struct Guard {
+        std::vector<void*>& v;
+        ~Guard() { release_blocks(v, false); }
+    } guard{owned};
+    float *din, *dout;
+    int rc;
+    if ((rc = alloc(owned, &din, static_cast<size_t>(n) * 2))) return rc;
+    if ((rc = alloc(owned, &dout, static_cast<size_t>(n)))) return rc;
+    HIP_TRY(hipMemcpy(din, in, sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_math, dim3((n + 255) / 256), dim3(256), 0, 0, op, n, din, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, dout, sizeof(float) * n, hipMemcpyDeviceToHost));
     return MT_OK;
 }
 

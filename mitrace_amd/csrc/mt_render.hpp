@@ -242,5 +242,6 @@ void debug_post(int mode, float* rgb, uint64_t npix);
 int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t traversal,
                uint32_t* occluded);
 int brdf_eval(uint32_t n, const float* in, float* out);
+int math_eval(int32_t op, uint32_t n, const float* in, float* out);
 
 }  // namespace mt

@@ -131,6 +131,106 @@ M4 invert4(const M4& M) {
     return out;
 }
 
+// ---------------------------------------------------------------- elementary functions
+// The path's sin, cos, atan2, acos and x^5, written out operation by operation so that the
+// oracle and the GPU kernels (mt_device.hpp, the same operations in the same order) return the
+// same bits; the C library's and the device library's versions differ in the last bits, which
+// made a few per cent of the samples differ.  Rounding is IEEE float32 throughout (no
+// contraction).  sin/cos: the argument reduction of Cody & Waite (1980), x - k pi/2 with pi/2
+// split into three floats whose first two have short mantissas (k (pi/2)_1 and k (pi/2)_2 are
+// exact for |k| < 2^12), then the Taylor polynomials of sin (through r^9) and cos (through
+// r^10) on [-pi/4, pi/4], where the first omitted terms are below 2e-9.  atan2: octant
+// symmetry folds the argument into t in [0, 1], the identity atan(t) = pi/4 + atan((t-1)/(t+1))
+// into |t| <= tan(pi/8), then the Taylor series of atan through t^19 (next term < 5e-10).
+// acos(x) = atan2(sqrt((1-x)(1+x)), x).  x^5 = (x^2 x^2) x.
+namespace em {
+constexpr float kTwoOverPi = 0x1.45f306p-1f;
+constexpr float kPio2a = 0x1.92p+0f, kPio2b = 0x1.fb6p-12f, kPio2c = -0x1.777a5cp-25f;
+constexpr float kPi = 0x1.921fb6p+1f, kPio2 = 0x1.921fb6p+0f, kPio4 = 0x1.921fb6p-1f;
+constexpr float kTanPi8 = 0x1.a8279ap-2f;
+inline float sinPoly(float r) {
+    const float r2 = r * r;
+    float p = 0x1.71de3ap-19f;           //  1/9!
+    p = p * r2 + -0x1.a01a02p-13f;        // -1/7!
+    p = p * r2 + 0x1.111112p-7f;          //  1/5!
+    p = p * r2 + -0x1.555556p-3f;         // -1/3!
+    return r + (r * r2) * p;
+}
+inline float cosPoly(float r) {
+    const float r2 = r * r;
+    float p = -0x1.27e4fcp-22f;           // -1/10!
+    p = p * r2 + 0x1.a01a02p-16f;         //  1/8!
+    p = p * r2 + -0x1.6c16c2p-10f;        // -1/6!
+    p = p * r2 + 0x1.555556p-5f;          //  1/4!
+    p = p * r2 + -0.5f;                   // -1/2!
+    return 1.0f + r2 * p;
+}
+inline void sincos(float x, float& s, float& c) {
+    const float k = std::floor(x * kTwoOverPi + 0.5f);
+    const float r = ((x - k * kPio2a) - k * kPio2b) - k * kPio2c;
+    const float ps = sinPoly(r), pc = cosPoly(r);
+    const int q = static_cast<int>(k) & 3;
+    s = q == 0 ? ps : (q == 1 ? pc : (q == 2 ? -ps : -pc));
+    c = q == 0 ? pc : (q == 1 ? -ps : (q == 2 ? -pc : ps));
+}
+inline float sin(float x) {
+    float s, c;
+    sincos(x, s, c);
+    return s;
+}
+inline float cos(float x) {
+    float s, c;
+    sincos(x, s, c);
+    return c;
+}
+inline float atanPoly(float t) {  // |t| <= tan(pi/8)
+    const float t2 = t * t;
+    float p = -0x1.af286cp-5f;            // -1/19
+    p = p * t2 + 0x1.e1e1e2p-5f;          //  1/17
+    p = p * t2 + -0x1.111112p-4f;         // -1/15
+    p = p * t2 + 0x1.3b13b2p-4f;          //  1/13
+    p = p * t2 + -0x1.745d18p-4f;         // -1/11
+    p = p * t2 + 0x1.c71c72p-4f;          //  1/9
+    p = p * t2 + -0x1.24924ap-3f;         // -1/7
+    p = p * t2 + 0x1.99999ap-3f;          //  1/5
+    p = p * t2 + -0x1.555556p-2f;         // -1/3
+    return t + (t * t2) * p;
+}
+inline float atan2(float y, float x) {
+    const float ax = std::fabs(x), ay = std::fabs(y);
+    const float mx = ay < ax ? ax : ay, mn = ay < ax ? ay : ax;
+    float a;
+    if (mx == 0.0f) {
+        a = 0.0f;
+    } else {
+        float t = mn / mx;
+        float off = 0.0f;
+        if (t > kTanPi8) {
+            t = (t - 1.0f) / (t + 1.0f);
+            off = kPio4;
+        }
+        a = off + atanPoly(t);
+    }
+    if (ax < ay) a = kPio2 - a;
+    if (std::signbit(x)) a = kPi - a;
+    return std::signbit(y) ? -a : a;
+}
+inline float acos(float x) { return atan2(std::sqrt((1.0f - x) * (1.0f + x)), x); }
+inline float pow5(float x) {
+    const float x2 = x * x;
+    return (x2 * x2) * x;
+}
+}  // namespace em
+
+// ---------------------------------------------------------------- planted mistakes
+// A build with -DORC_MUTANT=k (oracle/Makefile, target `mutants`) changes one formula, so that
+// the test-suite can show that each of its pins fails on a plausible mistake.  Every normal
+// build has ORC_MUTANT 0, where every MUT(k) is a compile-time false.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT(k) (ORC_MUTANT == (k))
+
 // ---------------------------------------------------------------- counter RNG
 // Keyed integer hash; the GPU kernels implement the same function.
 inline uint32_t mix32(uint32_t x) {
@@ -637,8 +737,8 @@ RGBA sampleImage(const Image& im, float uvx, float uvy, int filter) {  // Image.
     const float x = uvx - std::floor(uvx);
     const float y = uvy - std::floor(uvy);
     if (filter == MT_FILTER_NEAREST) {  // Image.cpp:17-24
-        int ix = static_cast<int>(x * static_cast<float>(im.w)) % im.w;
-        int iy = static_cast<int>(y * static_cast<float>(im.h)) % im.h;
+        int ix = static_cast<int>(x * static_cast<float>(im.w) + (MUT(1) ? 0.5f : 0.0f)) % im.w;
+        int iy = static_cast<int>(y * static_cast<float>(im.h) + (MUT(1) ? 0.5f : 0.0f)) % im.h;
         if (ix < 0) ix += im.w;
         if (iy < 0) iy += im.h;
         ix = std::max(ix, 0);  // a NaN coordinate converts to INT_MIN on x86
@@ -649,12 +749,18 @@ RGBA sampleImage(const Image& im, float uvx, float uvy, int filter) {  // Image.
     // texel where the reference would read past the row (x*W rounding up to W).
     int x0 = static_cast<int>(std::floor(x * static_cast<float>(im.w)));
     int y0 = static_cast<int>(std::floor(y * static_cast<float>(im.h)));
+    if (MUT(4)) {
+        x0 %= im.w;
+        y0 %= im.h;
+    }
     x0 = std::max(std::min(x0, im.w - 1), 0);
     y0 = std::max(std::min(y0, im.h - 1), 0);
-    const int x1 = std::min(x0 + 1, im.w - 1);
-    const int y1 = std::min(y0 + 1, im.h - 1);
-    const float sx = x - static_cast<float>(x0) / static_cast<float>(im.w);
-    const float sy = y - static_cast<float>(y0) / static_cast<float>(im.h);
+    const int x1 = MUT(3) ? (x0 + 1) % im.w : std::min(x0 + 1, im.w - 1);
+    const int y1 = MUT(3) ? (y0 + 1) % im.h : std::min(y0 + 1, im.h - 1);
+    const float sx = MUT(2) ? x * static_cast<float>(im.w) - static_cast<float>(x0)
+                            : x - static_cast<float>(x0) / static_cast<float>(im.w);
+    const float sy = MUT(2) ? y * static_cast<float>(im.h) - static_cast<float>(y0)
+                            : y - static_cast<float>(y0) / static_cast<float>(im.h);
     const RGBA c0 = mix4(texel(im, x0, y0), texel(im, x1, y0), sx);
     const RGBA c1 = mix4(texel(im, x0, y1), texel(im, x1, y1), sx);
     return mix4(c0, c1, sy);
@@ -665,11 +771,11 @@ RGBA sampleTex(const Scene& S, const Tex& t, float u, float v) {  // Texture.cpp
 }
 V3 sampleEnv(const Scene& S, V3 d, bool applyRot, float rotDeg) {  // Image.cpp:63-75
     if (!S.hasEnv) return v3(0.0f);
-    const float theta = std::atan2(d.z, d.x);
-    const float phi = std::acos(d.y / length(d));
+    const float theta = MUT(5) ? em::atan2(d.x, d.z) : em::atan2(d.z, d.x);
+    const float phi = em::acos(MUT(6) ? d.y : d.y / length(d));
     float u = (theta + kPi) / (2.0f * kPi);
     const float v = phi / kPi;
-    if (applyRot) u = u * 1.0f + rotDeg / 360.0f;  // TransformUv offset that SampleEquirectangular bypasses
+    if (applyRot) u = u * 1.0f + (MUT(7) ? -rotDeg : rotDeg) / 360.0f;  // TransformUv offset that SampleEquirectangular bypasses
     const RGBA c = sampleImage(S.env, u, v, S.envFilter);
     return {c.r, c.g, c.b};
 }
@@ -677,7 +783,7 @@ V3 sampleEnv(const Scene& S, V3 d, bool applyRot, float rotDeg) {  // Image.cpp:
 // ---------------------------------------------------------------- BRDF (BRDF.hpp)
 inline V3 fresnel(float cosT, V3 f0) {  // BRDF.hpp:9-13
     cosT = clampf(cosT, 0.0f, 1.0f);
-    const float p = std::pow(1.0f - cosT, 5.0f);
+    const float p = em::pow5(1.0f - cosT);
     return {f0.x + (1.0f - f0.x) * p, f0.y + (1.0f - f0.y) * p, f0.z + (1.0f - f0.z) * p};
 }
 inline V3 baseRefl(V3 base, float metallic) {  // BRDF.hpp:16-20
@@ -741,7 +847,9 @@ V3 cosineHemisphere(float u1, float u2, V3 n) {  // Tracer.cpp:34-47
     const float phi = 2.0f * kPi * u2;
     V3 t, b;
     buildTBN(n, t, b);
-    const float lx = r * std::cos(phi), ly = r * std::sin(phi);
+    float sp, cp;
+    em::sincos(phi, sp, cp);
+    const float lx = r * cp, ly = r * sp;
     const float lz = std::sqrt(gmax(0.0f, 1.0f - u1));
     return normalize(t * lx + b * ly + n * lz);
 }
@@ -756,7 +864,9 @@ bool ggxDirection(float u1, float u2, V3 incident, V3 n, float rough, V3& out) {
     const float sinT = std::sqrt(gmax(0.0f, 1.0f - cosT * cosT));
     V3 t, b;
     buildTBN(n, t, b);
-    V3 h = normalize(t * (sinT * std::cos(phi)) + b * (sinT * std::sin(phi)) + n * cosT);
+    float sp, cp;
+    em::sincos(phi, sp, cp);
+    V3 h = normalize(t * (sinT * cp) + b * (sinT * sp) + n * cosT);
     if (dot(h, -incident) < 0.0f) h = -h;
     const V3 refl = incident - h * dot(h, incident) * 2.0f;  // glm::reflect
     if (dot(refl, n) > 0.0f) {
@@ -773,8 +883,8 @@ inline float specProbability(V3 view, V3 n, V3 base, float metallic) {  // Trace
 }
 V3 computeNormal(V3 surf, V3 tex) {  // Tracer.cpp:95-105
     const V3 n = normalize(surf);
-    const V3 up = (std::fabs(n.y) < 0.9f) ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
-    const V3 t = normalize(cross(n, up));
+    const V3 up = ((MUT(9) ? n.y : std::fabs(n.y)) < 0.9f) ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+    const V3 t = MUT(8) ? normalize(cross(up, n)) : normalize(cross(n, up));
     const V3 b = normalize(cross(t, n));
     return normalize(t * tex.x + b * tex.y + n * tex.z);
 }
@@ -836,6 +946,53 @@ ShadeOut sampleBounce(V3 view, V3 n, V3 base, float rough, float metal, Rng& rng
     o.energy = (o.pdf > 0.0f) ? f * (nl / o.pdf) : v3(0.0f);
     o.brdfIntensity = (f.x + f.y) + f.z;  // brdf.r + brdf.g + brdf.b (Tracer.cpp:203)
     return o;
+}
+
+// Light selection of one path step (Tracer.cpp:257-279): a one-pass weighted reservoir with one
+// uniform per light, score = PrepareLights' mean intensity / (d^2 + 1).  Returns the chosen
+// light (-1: none) and its probability score / sum of scores.
+int chooseLight(const Scene& S, V3 hitPos, Rng& rng, float& pdf) {
+    int chosen = -1;
+    float chosenScore = 0.0f, totalScore = 0.0f;
+    for (size_t i = 0; i < S.lights.size(); ++i) {  // Tracer.cpp:263-275
+        const V3 lp = S.lights[i].pos - hitPos;
+        const float d2 = dot(lp, lp);
+        const float score = S.lights[i].intensity / (MUT(10) ? d2 : d2 + 1.0f);
+        totalScore += score;
+        const float r = rng.u(kDimLightSel + static_cast<uint32_t>(i)) * totalScore;
+        if (MUT(11) ? r > score : r < score) {
+            chosen = static_cast<int>(i);
+            chosenScore = score;
+        }
+    }
+    pdf = chosenScore / totalScore;
+    return chosen;
+}
+
+// The sampled point of light L seen from hitPos and its colour there (Tracer.cpp:282-315):
+// point and spot lights are jittered by GenerateRandomDirection (Tracer.cpp:84-93; phi and
+// theta uniform, so not uniform on the sphere) times the point size, other types sit at their
+// position; spot lights scale their colour by GetSpotIntensity (Light.hpp:58-63).
+V3 lightPoint(const Light& L, V3 hitPos, Rng& rng, V3& lcol) {
+    V3 lpos = L.pos;
+    if (MUT(15) || L.type == MT_LIGHT_POINT || L.type == MT_LIGHT_SPOT) {
+        const float phi = rng.uRange(kDimLightPt, 0.0f, 2.0f * kPi);
+        const float th = MUT(12) ? std::asin(2.0f * rng.u(kDimLightPt + 1) - 1.0f)
+                                 : rng.uRange(kDimLightPt + 1, -kPi * 0.5f, kPi * 0.5f);
+        float sp, cp, st, ct;
+        em::sincos(phi, sp, cp);
+        em::sincos(th, st, ct);
+        const V3 rd = normalize(V3{cp * ct, st, sp * ct});
+        lpos = L.pos + rd * L.size;
+    }
+    lcol = L.color;
+    if (L.type == MT_LIGHT_SPOT) {
+        const V3 ldir = normalize(lpos - hitPos);
+        const float dd = MUT(13) ? dot(ldir, L.dir) : dot(-ldir, L.dir);
+        const float f = MUT(14) ? (dd - L.inner) / (L.outer - L.inner) : (dd - L.outer) / (L.inner - L.outer);
+        lcol = L.color * clampf(f, 0.0f, 1.0f);
+    }
+    return lpos;
 }
 
 struct Tracer {
@@ -968,38 +1125,13 @@ struct Tracer {
                 break;
             }
             emitted = emitted + cE * st.emission_base_intensity * step.emission;
-            int chosen = -1;
-            float chosenPdf = 0.0f, totalScore = 0.0f;
-            for (size_t i = 0; i < S.lights.size(); ++i) {  // Tracer.cpp:263-275
-                const V3 lp = S.lights[i].pos - step.hitPos;
-                const float d2 = dot(lp, lp);
-                const float score = S.lights[i].intensity / (d2 + 1.0f);
-                totalScore += score;
-                if (rng.u(kDimLightSel + static_cast<uint32_t>(i)) * totalScore < score) {
-                    chosen = static_cast<int>(i);
-                    chosenPdf = score;
-                }
-            }
-            chosenPdf /= totalScore;
+            float chosenPdf = 0.0f;
+            const int chosen = chooseLight(S, step.hitPos, rng, chosenPdf);
             if (chosen == -1) continue;  // Tracer.cpp:278-279: skips the throughput update
-            const Light& L = S.lights[chosen];
-            V3 lpos;
-            if (L.type == MT_LIGHT_POINT || L.type == MT_LIGHT_SPOT) {  // Tracer.cpp:287-298
-                const float phi = rng.uRange(kDimLightPt, 0.0f, 2.0f * kPi);
-                const float th = rng.uRange(kDimLightPt + 1, -kPi * 0.5f, kPi * 0.5f);
-                const V3 rd = normalize(V3{std::cos(phi) * std::cos(th), std::sin(th),
-                                           std::sin(phi) * std::cos(th)});
-                lpos = L.pos + rd * L.size;
-            } else {
-                lpos = L.pos;
-            }
+            V3 lcol;
+            const V3 lpos = lightPoint(S.lights[chosen], step.hitPos, rng, lcol);
             const V3 lvec = lpos - step.hitPos;
             const V3 ldir = normalize(lvec);
-            V3 lcol = L.color;
-            if (L.type == MT_LIGHT_SPOT) {  // Light.hpp:58-63
-                const float dd = dot(-ldir, L.dir);
-                lcol = L.color * clampf((dd - L.outer) / (L.inner - L.outer), 0.0f, 1.0f);
-            }
             Ray sr;
             sr.o = step.hitPos + lvec * 0.0001f;
             sr.d = ldir;
@@ -1159,7 +1291,7 @@ D3 evalBrdfDv(V3 L, V3 V, V3 N, const D3& base, Dv rough, Dv metal) {
     const float nh = clampf(gmax(dot(N, h), 0.0f), 0.0f, 1.0f);
     const float hv = clampf(gmax(dot(h, V), 0.0f), 0.0f, 1.0f);
     const Dv mc = dmin(dmax(metal, 0.0f), 1.0f);
-    const float p5 = std::pow(1.0f - hv, 5.0f);
+    const float p5 = em::pow5(1.0f - hv);
     const Dv a = rough * rough;
     const Dv a2 = a * a;
     Dv den = (a2 - dv(1.0f)) * (nh * nh) + 1.0f;
@@ -1334,33 +1466,13 @@ struct DiffTracer {
                 break;
             }
             L = L + cE * step.emission * st.emission_base_intensity;
-            int chosen = -1;
-            float chosenPdf = 0.0f, totalScore = 0.0f;
-            for (size_t i = 0; i < S.lights.size(); ++i) {
-                const V3 lp = S.lights[i].pos - step.hitPos;
-                const float score = S.lights[i].intensity / (dot(lp, lp) + 1.0f);
-                totalScore += score;
-                if (rng.u(kDimLightSel + static_cast<uint32_t>(i)) * totalScore < score) {
-                    chosen = static_cast<int>(i);
-                    chosenPdf = score;
-                }
-            }
-            chosenPdf /= totalScore;
+            float chosenPdf = 0.0f;
+            const int chosen = chooseLight(S, step.hitPos, rng, chosenPdf);
             if (chosen == -1) continue;
-            const Light& Lt = S.lights[chosen];
-            V3 lpos = Lt.pos;
-            if (Lt.type == MT_LIGHT_POINT || Lt.type == MT_LIGHT_SPOT) {
-                const float phi = rng.uRange(kDimLightPt, 0.0f, 2.0f * kPi);
-                const float th = rng.uRange(kDimLightPt + 1, -kPi * 0.5f, kPi * 0.5f);
-                const V3 rd = normalize(V3{std::cos(phi) * std::cos(th), std::sin(th),
-                                           std::sin(phi) * std::cos(th)});
-                lpos = Lt.pos + rd * Lt.size;
-            }
+            V3 lcol;
+            const V3 lpos = lightPoint(S.lights[chosen], step.hitPos, rng, lcol);
             const V3 lvec = lpos - step.hitPos;
             const V3 ldir = normalize(lvec);
-            V3 lcol = Lt.color;
-            if (Lt.type == MT_LIGHT_SPOT)
-                lcol = Lt.color * clampf((dot(-ldir, Lt.dir) - Lt.outer) / (Lt.inner - Lt.outer), 0.0f, 1.0f);
             Ray sr;
             sr.o = step.hitPos + lvec * 0.0001f;
             sr.d = ldir;
@@ -1785,6 +1897,77 @@ int orc_env_sample(const void* sp, uint32_t n, const float* dirs, int32_t apply_
         rgb[i * 3] = c.x;
         rgb[i * 3 + 1] = c.y;
         rgb[i * 3 + 2] = c.z;
+    }
+    return MT_OK;
+}
+
+void orc_math(int32_t op, uint32_t n, const float* in, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float x = in[2 * i], y = in[2 * i + 1];
+        switch (op) {
+            case MT_MATH_SIN: out[i] = em::sin(x); break;
+            case MT_MATH_COS: out[i] = em::cos(x); break;
+            case MT_MATH_ATAN2: out[i] = em::atan2(x, y); break;
+            case MT_MATH_ACOS: out[i] = em::acos(x); break;
+            case MT_MATH_POW5: out[i] = em::pow5(x); break;
+            default: out[i] = 0.0f;
+        }
+    }
+}
+
+int orc_sample_image(uint32_t w, uint32_t h, const uint8_t* rgb8, uint32_t n, const float* uv,
+                     int32_t filter, float* out) {
+    if (w == 0 || h == 0 || !rgb8) return MT_E_INVALID;
+    Image im;
+    im.w = static_cast<int>(w);
+    im.h = static_cast<int>(h);
+    im.rgb.resize(3ull * w * h);
+    for (size_t i = 0; i < im.rgb.size(); ++i) im.rgb[i] = static_cast<float>(rgb8[i]) / 255.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        const RGBA c = sampleImage(im, uv[2 * i], uv[2 * i + 1], filter);
+        out[4 * i] = c.r;
+        out[4 * i + 1] = c.g;
+        out[4 * i + 2] = c.b;
+        out[4 * i + 3] = c.a;
+    }
+    return MT_OK;
+}
+
+void orc_compute_normal(uint32_t n, const float* surf, const float* tex, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3 r = computeNormal(V3{surf[3 * i], surf[3 * i + 1], surf[3 * i + 2]},
+                                   V3{tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]});
+        out[3 * i] = r.x;
+        out[3 * i + 1] = r.y;
+        out[3 * i + 2] = r.z;
+    }
+}
+
+int orc_light_sample(const void* sp, uint32_t n, const float* pos, uint64_t seed, uint32_t base_dim,
+                     int32_t* chosen, float* pdf, float* lpos, float* lcol) {
+    const Scene& S = *static_cast<const Scene*>(sp);
+    const uint32_t sk = seedKey(seed);
+    for (uint32_t i = 0; i < n; ++i) {
+        Rng rng;
+        rng.key = pathKey(sk, i, 0);
+        rng.base = base_dim;
+        const V3 p{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        float pr = 0.0f;
+        const int c = chooseLight(S, p, rng, pr);
+        chosen[i] = c;
+        if (pdf) pdf[i] = pr;
+        V3 lp{0.0f, 0.0f, 0.0f}, lc{0.0f, 0.0f, 0.0f};
+        if (c >= 0) lp = lightPoint(S.lights[c], p, rng, lc);
+        if (lpos) {
+            lpos[3 * i] = lp.x;
+            lpos[3 * i + 1] = lp.y;
+            lpos[3 * i + 2] = lp.z;
+        }
+        if (lcol) {
+            lcol[3 * i] = lc.x;
+            lcol[3 * i + 1] = lc.y;
+            lcol[3 * i + 2] = lc.z;
+        }
     }
     return MT_OK;
 }

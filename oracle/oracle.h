@@ -65,6 +65,19 @@ void orc_sample_bounce(uint32_t n, const float* in, float* out);
 /* equirect sample of the scene environment, n directions -> rgb (intensity multiplier 1) */
 int orc_env_sample(const void* scene, uint32_t n, const float* dirs, int32_t apply_rotation,
                    float rotation_deg, float* rgb);
+/* the path's elementary functions: in = n pairs {x, y}; MT_MATH_ATAN2 computes atan2(x, y),
+ * the others use x only */
+void orc_math(int32_t op, uint32_t n, const float* in, float* out);
+/* Image::Sample (Image.cpp:11-61) of a w x h RGB8 image at n uv pairs -> n RGBA floats */
+int orc_sample_image(uint32_t w, uint32_t h, const uint8_t* rgb8, uint32_t n, const float* uv,
+                     int32_t filter, float* out);
+/* ComputeNormal (Tracer.cpp:95-105): surface normals and tangent-space normals, 3 floats each */
+void orc_compute_normal(uint32_t n, const float* surf, const float* tex, float* out);
+/* One light sample of ProcessRayForward (Tracer.cpp:257-315) at n hit positions, each with
+ * the counter RNG of (seed, pixel i, sample 0) from dimension base_dim: the chosen light (-1 =
+ * none), its selection probability, the sampled light point and the light colour there. */
+int orc_light_sample(const void* scene, uint32_t n, const float* pos, uint64_t seed, uint32_t base_dim,
+                     int32_t* chosen, float* pdf, float* lpos, float* lcol);
 /* camera rays for (u,v) pairs */
 void orc_camera_rays(const mt_camera_desc* cam, uint32_t n, const float* uv, float aspect,
                      float* rays);

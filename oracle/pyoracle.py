@@ -51,7 +51,11 @@ def _diff_struct(d, keep):
     return out
 
 
-def lib_path(fast=False):
+def lib_path(fast=False, mutant=0):
+    """fast: the release-flag build; mutant k > 0: the strict build with planted mistake k
+    (oracle/Makefile target `mutants`, for the tests that show each pin rejects a mistake)"""
+    if mutant:
+        return os.path.join(_HERE, "build", f"liboracle_m{mutant}.so")
     return os.path.join(_HERE, "build", "liboracle_fast.so" if fast else "liboracle.so")
 
 
@@ -63,10 +67,11 @@ def build():
 _libs = {}
 
 
-def load(fast=False):
-    if fast in _libs:
-        return _libs[fast]
-    p = lib_path(fast)
+def load(fast=False, mutant=0):
+    key = (bool(fast), int(mutant))
+    if key in _libs:
+        return _libs[key]
+    p = lib_path(fast, mutant)
     if not os.path.exists(p):
         raise FileNotFoundError(f"oracle library {p} is not built (make -C oracle)")
     L = C.CDLL(p)
@@ -104,7 +109,15 @@ def load(fast=False):
                                   C.POINTER(abi.mt_settings), C.POINTER(orc_diff_params),
                                   C.POINTER(orc_diff_params), C.c_int32, C.POINTER(C.c_float),
                                   C.POINTER(C.c_float)]
-    _libs[fast] = L
+    L.orc_math.argtypes = [C.c_int32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_sample_image.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                   C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_float)]
+    L.orc_compute_normal.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float)]
+    L.orc_light_sample.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_uint64,
+                                   C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    _libs[key] = L
     return L
 
 
@@ -118,8 +131,8 @@ assert HIT_DTYPE.itemsize == C.sizeof(abi.mt_hit)
 class OracleScene:
     """The oracle's Scene::Scene over a :class:`mitrace_amd.scene.Scene`."""
 
-    def __init__(self, scene, fast=False):
-        self.L = load(fast)
+    def __init__(self, scene, fast=False, mutant=0):
+        self.L = load(fast, mutant)
         self.scene = scene
         desc, keep = scene.to_desc()
         self.h = self.L.orc_scene_create(C.byref(desc))
@@ -165,6 +178,20 @@ class OracleScene:
         self.L.orc_env_sample(self.h, len(dirs), abi.fptr(dirs), int(apply_rotation),
                               rotation_deg, abi.fptr(out))
         return out
+
+    def light_sample(self, pos, seed=0, base_dim=2):
+        """One light choice and light point per hit position (pixel i, sample 0 of `seed`):
+        (chosen light or -1, selection probability, light point, light colour)"""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        n = len(pos)
+        chosen = np.zeros(n, np.int32)
+        pdf = np.zeros(n, np.float32)
+        lpos = np.zeros((n, 3), np.float32)
+        lcol = np.zeros((n, 3), np.float32)
+        self.L.orc_light_sample(self.h, n, abi.fptr(pos), seed, base_dim,
+                                chosen.ctypes.data_as(C.POINTER(C.c_int32)), abi.fptr(pdf),
+                                abi.fptr(lpos), abi.fptr(lcol))
+        return chosen, pdf, lpos, lcol
 
     def render(self, settings, camera=None, threads=0, rng=ORC_RNG_COUNTER, faithful_inverse=True,
                rows=None, want_samples=False, tlas=False):
@@ -283,4 +310,34 @@ def invert4(m_colmajor):
     m = np.ascontiguousarray(m_colmajor, np.float32).reshape(16)
     out = np.zeros(16, np.float32)
     load().orc_invert4(abi.fptr(m), abi.fptr(out))
+    return out
+
+
+def sample_image(rgb8, uv, filter_mode, mutant=0):
+    """Image::Sample of an (H, W, 3) uint8 image at (n, 2) uv pairs -> (n, 4) RGBA floats"""
+    img = np.ascontiguousarray(rgb8, np.uint8)
+    h, w = img.shape[:2]
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((len(uv), 4), np.float32)
+    rc = load(mutant=mutant).orc_sample_image(w, h, img.ctypes.data, len(uv), abi.fptr(uv), filter_mode,
+                                 abi.fptr(out))
+    assert rc == 0
+    return out
+
+
+def compute_normal(surf, tex, mutant=0):
+    surf = np.ascontiguousarray(surf, np.float32).reshape(-1, 3)
+    tex = np.ascontiguousarray(np.broadcast_to(tex, surf.shape), np.float32)
+    out = np.zeros_like(surf)
+    load(mutant=mutant).orc_compute_normal(len(surf), abi.fptr(surf), abi.fptr(tex), abi.fptr(out))
+    return out
+
+
+def math_eval(op, x, y=None):
+    """the oracle's elementary function `op` (abi.MT_MATH_*) at x (and y for ATAN2: atan2(x, y))"""
+    x = np.asarray(x, np.float32).ravel()
+    y = np.zeros_like(x) if y is None else np.asarray(y, np.float32).ravel()
+    inp = np.ascontiguousarray(np.stack([x, y], 1), np.float32)
+    out = np.zeros(len(x), np.float32)
+    load().orc_math(op, len(x), abi.fptr(inp), abi.fptr(out))
     return out

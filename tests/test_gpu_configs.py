@@ -1,0 +1,180 @@
+"""GPU parity on the configurations and forms the smaller suites leave out, each through the
+C-ABI against the oracle or against the GPU's own single renders:
+
+* the path's elementary functions (sin, cos, atan2, acos, x^5) bit-identical to the oracle's;
+* a scene with point (jittered), spot, directional and area lights: every sample, the ray
+  count and the traversal work counters equal the oracle's (Tracer.cpp:257-315, Light.hpp:58-63);
+* Russian roulette after depth 3 with 8 bounces on the generated Cornell box: every sample
+  equals the oracle's roulette;
+* the large terrain scene at its full size (2048x2048-vertex terrain, 5000 icospheres, four
+  2048^2 textures, 16 lights): closest-hit IDs and counters on seeded rays, a 64x36 render at
+  2 spp sample for sample, and the radiance-cache form (every path training = the plain render;
+  a zero cache = the plain render ending after the query vertex);
+* the batched-requests workload (64 seeded jobs over 8 scene instances): every job's image
+  equals its single render.
+"""
+import numpy as np
+import pytest
+
+from helpers import load_scene, seeded_rays, small_settings
+from mitrace_amd import abi, scenegen
+from mitrace_amd.scene import Light
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def test_elementary_functions_bit_identical(gpu):
+    import ctypes as C
+    from mitrace_amd._lib import check, load
+    L = load()
+    rng = np.random.default_rng(4)
+    x = np.concatenate([np.linspace(-4.0, 8.0, 200001), rng.uniform(-1, 1, 100000), [0.0, -0.0, 1.0, -1.0]])
+    y = np.concatenate([rng.normal(size=x.size - 4), [0.0, -0.0, -0.0, 0.0]])
+    for op in (abi.MT_MATH_SIN, abi.MT_MATH_COS, abi.MT_MATH_ATAN2, abi.MT_MATH_ACOS, abi.MT_MATH_POW5):
+        inp = np.ascontiguousarray(np.stack([x, y], 1), np.float32)
+        out = np.zeros(len(x), np.float32)
+        check(L.mt_math_eval(op, len(x), abi.fptr(inp), abi.fptr(out)))
+        ref = po.math_eval(op, inp[:, 0], inp[:, 1])
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), \
+            (op, np.argwhere(out.view(np.uint32) != ref.view(np.uint32))[:5])
+
+
+def _assert_same_render(g, o, st, sc=None):
+    grgb, gs, gk = g.render(st, want_samples=True)
+    orgb, os_, ow, _ = o.render(st, want_samples=True)
+    diff = gs.view(np.uint32) != os_.view(np.uint32)
+    assert not diff.any(), f"{diff.any(axis=2).mean():.4%} of samples differ"
+    assert np.array_equal(grgb, orgb)
+    assert gk["rays"] == ow["rays"] and gk["samples"] == ow["samples"]
+    return gk, ow
+
+
+def lights_scene():
+    """the sphere grid (coarse) under four kinds of lights: a jittered point light, a spot
+    light whose cones cut across the grid, a directional light and an area light (both sampled
+    at their node position), plus a dim point light, so the reservoir chooses among five"""
+    sc = load_scene("sphere_grid_coarse")
+    sc.lights = [
+        Light(abi.MT_LIGHT_POINT, (2.0, 4.0, 2.0), (30.0, 25.0, 20.0), 0.4),
+        Light(abi.MT_LIGHT_SPOT, (3.0, 5.0, 3.0), (80.0, 80.0, 90.0), 0.2,
+              spot_inner_cos=float(np.cos(np.radians(15))), spot_outer_cos=float(np.cos(np.radians(35))),
+              direction=tuple((np.array([0.0, -1.0, 0.2]) / np.linalg.norm([0.0, -1.0, 0.2])).tolist())),
+        Light(abi.MT_LIGHT_DIRECTIONAL, (-2.0, 6.0, 4.0), (10.0, 10.0, 8.0), 0.5, direction=(0.0, -1.0, 0.0)),
+        Light(abi.MT_LIGHT_AREA, (6.0, 3.0, -1.0), (12.0, 6.0, 6.0), 0.3),
+        Light(abi.MT_LIGHT_POINT, (0.0, 1.0, 8.0), (1.0, 1.0, 1.0), 0.1),
+    ]
+    return sc
+
+
+def test_spot_directional_area_and_point_lights(gpu):
+    from mitrace_amd.tracer import GpuScene
+    sc = lights_scene()
+    g, o = GpuScene(sc), po.OracleScene(sc)
+    for kw in (dict(), dict(output=abi.MT_OUTPUT_LINEAR, bounces=3)):
+        st = small_settings(width=48, height=27, samples=8, seed=21, ev_exposure=4.0, **kw)
+        _assert_same_render(g, o, st)
+    st = small_settings(width=32, height=18, samples=4, seed=22, count_work=True)
+    gk, ow = _assert_same_render(g, o, st)
+    for k in ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests", "hits_shaded", "misses"):
+        assert gk[k] == ow[k], (k, gk[k], ow[k])
+    g.close()
+    o.close()
+
+
+def test_russian_roulette_cornell_matches_oracle(gpu):
+    """the Cornell configuration as stated: roulette after depth 3, at most 8 bounces"""
+    from mitrace_amd.tracer import GpuScene
+    sc = scenegen.cornell()
+    g, o = GpuScene(sc), po.OracleScene(sc)
+    st = scenegen.cornell_settings(width=48, height=48, samples=16, seed=8, mode=abi.MT_MODE_FIXED,
+                                   rr_depth=3, bounces=8)
+    _assert_same_render(g, o, st)
+    st.output = abi.MT_OUTPUT_LINEAR
+    _assert_same_render(g, o, st)
+    g.close()
+    o.close()
+
+
+@pytest.fixture(scope="module")
+def large_full(gpu):
+    """the large terrain scene at its stated size, with both BVHs on the GPU"""
+    import sys
+    import time
+    from mitrace_amd.tracer import GpuScene
+    t0 = time.perf_counter()
+    sc = scenegen.large_scene()
+    sc.bvh.fast = True
+    g = GpuScene(sc)
+    o = po.OracleScene(sc)
+    print(f"large scene: {sc.triangle_count} triangles, {len(sc.instances)} instances, built in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    yield sc, g, o
+    g.close()
+    o.close()
+
+
+def test_large_scene_full_size_closest_hits(large_full):
+    sc, g, o = large_full
+    assert sc.triangle_count > 14_700_000 and len(sc.instances) == 5001
+    rays = seeded_rays(sc, 6000, seed=12)
+    gh = g.trace(rays)
+    oh, _ = o.trace(rays)
+    assert (oh["instance"] >= 0).sum() > 1000
+    for f in ("instance", "triangle", "bv_index", "bvh_tests", "triangle_tests"):
+        assert np.array_equal(gh[f], oh[f]), (f, np.argwhere(gh[f] != oh[f])[:5])
+    for f in ("u", "v", "distance"):
+        assert np.array_equal(gh[f].view(np.uint32), oh[f].view(np.uint32)), f
+
+
+def test_large_scene_full_size_render(large_full):
+    sc, g, o = large_full
+    st = scenegen.large_scene_settings(width=64, height=36, samples=2, seed=23, count_work=True)
+    gk, ow = _assert_same_render(g, o, st)
+    for k in ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests"):
+        assert gk[k] == ow[k], (k, gk[k], ow[k])
+
+
+def test_large_scene_full_size_cache_form(large_full):
+    """the radiance-cache form on the large scene (fast traversal, linear radiance, as its
+    comparison runs): with every path training the cached render is the plain render; with a
+    zero cache queried after the light sample of the first bounce's vertex it is the plain
+    render with two bounces"""
+    from mitrace_amd.cache import RadianceCache
+    sc, g, o = large_full
+    kw = dict(width=64, height=36, traversal=abi.MT_TRAVERSAL_FAST, output=abi.MT_OUTPUT_LINEAR)
+    st = scenegen.large_scene_settings(samples=4, seed=24, **kw)
+    ref, _, _ = g.render(st)
+    c = RadianceCache(g, {"train_every": 1, "warmup_passes": 1, "train_batch": 4096})
+    img, stats = c.render(st)
+    np.testing.assert_array_equal(img, ref)
+    assert stats["samples"] == 64 * 36 * 4
+    c.close()
+    c = RadianceCache(g, {"train_every": 0xFFFFFFFF, "query_depth": 1, "query_after_light": 1})
+    c.set_weights(np.zeros(14787, np.float32))
+    img, _ = c.render(st)
+    cut = scenegen.large_scene_settings(samples=4, seed=24, bounces=2, **kw)
+    ref2, _, _ = g.render(cut)
+    np.testing.assert_array_equal(img, ref2)
+    c.close()
+
+
+def test_batched_workload_64_jobs_equal_single_renders(gpu):
+    """the batched-requests workload as the benchmark serves it (64 seeded jobs over 3 Cornell
+    boxes, 3 sphere grids and 2 terrain scenes, 128^2-1024^2 pixels, 16-256 spp, 8-16 bounces,
+    submitted while the batch runs): every job's image equals mt_render's"""
+    from mitrace_amd import batch as B
+    from mitrace_amd.tracer import GpuScene
+    scenes = B.workload_scenes()
+    gs = [GpuScene(s) for s, _ in scenes]
+    fns = [f for _, f in scenes]
+    jobs = B.workload_jobs(rate_hz=200.0)
+    assert len(jobs) == 64
+    rep, images = B.run_stream(gs, fns, jobs, want_images=True)
+    assert rep["jobs"] == 64
+    for j, img in zip(jobs, images):
+        st, cam = B.job_settings(j, fns[j["scene"]], gs[j["scene"]].scene)
+        ref, _, _ = gs[j["scene"]].render(st, camera=cam)
+        assert np.array_equal(img, ref), j
+    for g in gs:
+        g.close()

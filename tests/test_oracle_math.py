@@ -282,3 +282,89 @@ def test_counter_rng_uniform_and_decorrelated():
     assert abs(np.corrcoef(a, b)[0, 1]) < 0.01 and abs(np.corrcoef(a, c)[0, 1]) < 0.01
     # replayable: the same key gives the same draw
     assert np.array_equal(a, po.rng_draws(7, pix, smp, np.zeros(n, np.uint32)))
+
+
+# ---------------------------------------------------------------- elementary functions
+# The path's sin, cos, atan2, acos and x^5 are this build's own (the oracle and the kernels
+# compute them with the same operations); pinned against float64 numpy within a few units in
+# the last place, with the special values and symmetries the C library guarantees.
+from mitrace_amd import abi
+
+
+def _ulp(got, ref):
+    return np.abs(got.astype(np.float64) - ref) / np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+
+
+def test_sin_cos_within_two_ulp():
+    """the angles of the path: 2 pi u (cosine and GGX sampling, light-point longitude) and
+    theta in [-pi/2, pi/2) (light-point latitude), plus a margin; results within 2 ulp of the
+    float64 values (1e-8 absolute where the value crosses zero)"""
+    x = np.linspace(-4.0, 8.0, 1_200_001).astype(np.float32)
+    for op, fn in ((abi.MT_MATH_SIN, np.sin), (abi.MT_MATH_COS, np.cos)):
+        g = po.math_eval(op, x)
+        r = fn(x.astype(np.float64))
+        big = np.abs(r) > 1e-3
+        assert _ulp(g[big], r[big]).max() <= 2.0
+        assert np.abs(g - r).max() < 1e-7
+    for v, s, c in ((0.0, 0.0, 1.0), (-0.0, -0.0, 1.0)):
+        assert po.math_eval(abi.MT_MATH_SIN, [v])[0] == s and po.math_eval(abi.MT_MATH_COS, [v])[0] == c
+    # sin(-x) = -sin(x), cos(-x) = cos(x) inside the first reduction interval (at its end the
+    # rounding of x 2/pi + 1/2 can pick neighbouring quadrants for x and -x)
+    x = np.linspace(0.0, 0.78, 10001).astype(np.float32)
+    assert np.array_equal(po.math_eval(abi.MT_MATH_SIN, -x), -po.math_eval(abi.MT_MATH_SIN, x))
+    assert np.array_equal(po.math_eval(abi.MT_MATH_COS, -x), po.math_eval(abi.MT_MATH_COS, x))
+    # sin^2 + cos^2 = 1 within float rounding
+    x = np.random.default_rng(1).uniform(0, 2 * np.pi, 100000).astype(np.float32)
+    s, c = po.math_eval(abi.MT_MATH_SIN, x).astype(np.float64), po.math_eval(abi.MT_MATH_COS, x).astype(np.float64)
+    assert np.abs(s * s + c * c - 1).max() < 4e-7
+
+
+def test_atan2_within_three_ulp_and_signed_zeros():
+    """atan2 over every octant (random and axis-aligned arguments) within 3 ulp; the signed-zero
+    cases of C's atan2: (+0, +0) = +0, (+0, -0) = pi, (-0, -0) = -pi, (-0, +0) = -0"""
+    rng = np.random.default_rng(2)
+    y = rng.normal(size=1_000_000).astype(np.float32)
+    x = rng.normal(size=1_000_000).astype(np.float32)
+    x[:1000] = 0.0
+    y[1000:2000] = 0.0
+    x[2000:3000] = y[2000:3000]            # the octant boundary t = 1
+    x[3000:4000] = -y[3000:4000]
+    g = po.math_eval(abi.MT_MATH_ATAN2, y, x)
+    r = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    nz = np.abs(r) > 1e-3
+    assert _ulp(g[nz], r[nz]).max() <= 3.0
+    assert np.abs(g - r).max() < 3e-7
+    for (a, b), want in (((0.0, 0.0), 0.0), ((0.0, -0.0), math.pi), ((-0.0, -0.0), -math.pi),
+                         ((-0.0, 0.0), -0.0), ((2.0, 0.0), math.pi / 2), ((0.0, -3.0), math.pi)):
+        got = po.math_eval(abi.MT_MATH_ATAN2, [a], [b])[0]
+        assert got == np.float32(want) and math.copysign(1, got) == math.copysign(1, want)
+    # the longitude of SampleEquirectangular: (atan2(z, x) + pi) / 2 pi stays in [0, 1]
+    u = (g + np.float32(math.pi)) / np.float32(2 * math.pi)
+    assert u.min() >= 0.0 and u.max() <= 1.0
+
+
+def test_acos_within_four_ulp():
+    """acos on [-1, 1] (the polar angle y/|d| of SampleEquirectangular) within 4 ulp, exact at
+    the ends and the middle; NaN outside [-1, 1] like the C library"""
+    x = np.linspace(-1.0, 1.0, 2_000_001).astype(np.float32)
+    g = po.math_eval(abi.MT_MATH_ACOS, x)
+    r = np.arccos(x.astype(np.float64))
+    nz = r > 1e-3
+    assert _ulp(g[nz], r[nz]).max() <= 4.0
+    assert np.abs(g - r).max() < 4e-7
+    ends = po.math_eval(abi.MT_MATH_ACOS, [1.0, -1.0, 0.0])
+    assert ends[0] == 0.0 and ends[1] == np.float32(math.pi) and ends[2] == np.float32(math.pi / 2)
+    assert np.isnan(po.math_eval(abi.MT_MATH_ACOS, [1.0000001, -1.5])).all()
+
+
+def test_pow5_exact_products():
+    """x^5 of Schlick's Fresnel (BRDF.hpp:9-13) as (x^2 x^2) x: within 3 ulp of float64 (three
+    roundings), exact
+    where the products are exact (0, 1, 1/2, 2^-k)"""
+    x = np.linspace(0.0, 1.0, 1_000_001).astype(np.float32)
+    g = po.math_eval(abi.MT_MATH_POW5, x)
+    r = x.astype(np.float64) ** 5
+    nz = r > 1e-30
+    assert _ulp(g[nz], r[nz]).max() <= 3.0
+    assert np.array_equal(po.math_eval(abi.MT_MATH_POW5, [0.0, 1.0, 0.5, 0.25]),
+                          np.float32([0.0, 1.0, 1 / 32, 1 / 1024]))
