@@ -298,12 +298,15 @@ int mt_batch::admit(uint32_t id) {
         std::scoped_lock lk(mu);
         j.tiles = count;
     }
+    // only the tiles go to the device here: the end of the tiles an iteration may take is
+    // written on the lane's own stream right before that iteration (run()), so an iteration
+    // launched before this admission never takes this job's tiles, even when it runs on the
+    // device after the upload has landed
     for (uint32_t l = 0; l < nl; ++l) {
         SceneQueue& q = s->q[l];
         if (q.tail == first[l]) continue;
         BT_TRY(hipMemcpyAsync(q.q_dev + first[l], q.q_host + first[l], sizeof(uint4) * (q.tail - first[l]),
                               hipMemcpyHostToDevice, stream));
-        mt::launch_set_u32(q.q_state + 1, q.tail, stream);
     }
     BT_TRY(hipGetLastError());
     // every lane sees the uploads before its next iteration
@@ -387,17 +390,25 @@ int mt_batch::run() {
         Lane& L = lanes[lane];
         int rc = ensure_graph(s, lane);
         if (rc) return rc;
-        BT_TRY(hipGraphLaunch(q.exec, L.stream));
-        ++iterations;
-        // mirror of k_batch_schedule on this lane's queue
-        uint32_t n = 0;
+        // the tiles this iteration takes: k_batch_schedule's greedy rule (tiles in queue order
+        // while their paths fit, at least one) over the queue as admitted so far; their end
+        // goes to the device on the lane's stream ahead of the iteration, so the device takes
+        // exactly these tiles
+        uint32_t n = 0, end = q.head;
         uint64_t paths = 0;
-        while (q.head < q.tail && n < opt.iteration_tiles) {
-            const uint4 t = q.q_host[q.head];
+        while (end < q.tail && n < opt.iteration_tiles) {
+            const uint4 t = q.q_host[end];
             if (n > 0 && paths + t.w > opt.paths_per_iteration) break;
             paths += t.w;
             ++n;
-            ++q.head;
+            ++end;
+        }
+        mt::launch_set_u32(q.q_state + 1, end, L.stream);
+        BT_TRY(hipGetLastError());
+        BT_TRY(hipGraphLaunch(q.exec, L.stream));
+        ++iterations;
+        for (; q.head < end; ++q.head) {
+            const uint4 t = q.q_host[q.head];
             HostJob& j = *view[t.x];
             --j.tiles_left;
             if (--j.lane_left[lane] == 0) {

@@ -143,9 +143,10 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
 }
 
 // ------------------------------------------------------------------ batched rendering
-// One iteration of a batch: the scheduler takes tiles from the scene's device queue in order
-// while their paths fit the iteration's capacity (always at least one), lays the paths out
-// tile after tile (pixel-major inside a tile) and resets the counters.
+// One iteration of a batch: the scheduler takes tiles from the scene's device queue in order,
+// from its head up to the end the host wrote for this iteration (q_state[1], the same greedy
+// rule: while their paths fit the iteration's capacity, always at least one), lays the paths
+// out tile after tile (pixel-major inside a tile) and resets the counters.
 __global__ void k_batch_schedule(PathBuffers b) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t head = b.q_state[0];

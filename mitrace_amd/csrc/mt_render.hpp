@@ -108,7 +108,7 @@ struct PathBuffers {
     uint4* it_tiles;        // this iteration's tiles {job, first pixel, pixels, first path}
     uint32_t* it_pix_off;   // [tiles + 1] first iteration pixel of every tile
     const uint4* q_tiles;   // the scene's tile queue {job, first pixel, pixels, paths}
-    uint32_t* q_state;      // {head, tail} of that queue
+    uint32_t* q_state;      // {head, end of the tiles the next iteration takes} of that queue
     const struct BatchJob* jobs;
     uint32_t it_capacity;   // paths per iteration
     uint32_t it_tile_cap;   // tiles per iteration

@@ -36,8 +36,12 @@ def test_elementary_functions_bit_identical(gpu):
         out = np.zeros(len(x), np.float32)
         check(L.mt_math_eval(op, len(x), abi.fptr(inp), abi.fptr(out)))
         ref = po.math_eval(op, inp[:, 0], inp[:, 1])
-        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), \
-            (op, np.argwhere(out.view(np.uint32) != ref.view(np.uint32))[:5])
+        # NaN where the C library gives NaN (acos outside [-1, 1]); the sign of a NaN is the
+        # hardware's default (x86 sets it, the GPU does not) and carries nothing
+        nan = np.isnan(ref)
+        assert np.array_equal(np.isnan(out), nan), op
+        assert np.array_equal(out[~nan].view(np.uint32), ref[~nan].view(np.uint32)), \
+            (op, np.argwhere(out[~nan].view(np.uint32) != ref[~nan].view(np.uint32))[:5])
 
 
 def _assert_same_render(g, o, st, sc=None):
@@ -128,11 +132,15 @@ def test_large_scene_full_size_closest_hits(large_full):
 
 
 def test_large_scene_full_size_render(large_full):
+    """sample for sample; the instances entered, node pops and triangle tests equal the
+    reference loop's (the GPU's instance tree only skips instances whose world box the ray
+    misses, so it tests far fewer boxes than the loop's one per instance)"""
     sc, g, o = large_full
     st = scenegen.large_scene_settings(width=64, height=36, samples=2, seed=23, count_work=True)
     gk, ow = _assert_same_render(g, o, st)
-    for k in ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests"):
+    for k in ("inst_entered", "node_pops", "tri_tests", "hits_shaded", "misses"):
         assert gk[k] == ow[k], (k, gk[k], ow[k])
+    assert gk["inst_aabb_tests"] < ow["inst_aabb_tests"] == ow["rays"] * len(sc.instances)
 
 
 def test_large_scene_full_size_cache_form(large_full):
