@@ -417,6 +417,17 @@ int mt_trace_closest(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hit
 int mt_trace_closest_ex(mt_scene* scene, uint32_t n, const float* rays, mt_hit* hits,
                         int32_t traversal);
 
+/* mt_trace_closest_ex without any device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  rays_dev (6 floats per ray) and hits_dev (n
+ * mt_hit) are DEVICE pointers the caller owns; the traversal's scratch (work counters, error
+ * flag, stack spill area) lives in a 16 MiB static device array per GPU, held until the process
+ * ends.  A scene whose BVH needs a deeper stack than that array holds is rejected
+ * (MT_E_UNSUPPORTED).  Runs on `stream` (a hipStream_t, 0 = the null stream) and returns after
+ * the work has completed.  Calls that overlap on one GPU share its scratch and race. */
+int mt_trace_closest_noalloc(mt_scene* scene, uint32_t n, const float* rays_dev, mt_hit* hits_dev,
+                             int32_t traversal, uint64_t stream);
+
 /* Shadow-ray test hook (Tracer.cpp:317-323): rays = n x {ox, oy, oz, dx, dy, dz, dist} host
  * floats; occluded[i] = 1 when the closest hit is at distance <= dist (REFERENCE: the full
  * closest-hit query, FAST: the first occluder found), else 0. */

@@ -213,6 +213,8 @@ SIGNATURES = {
     "mt_trace_occluded": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_uint32), C.c_int32]),
     "mt_brdf_eval": (C.c_int, [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mt_trace_closest_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                           C.c_int32, C.c_uint64]),
     "mt_math_eval": (C.c_int, [C.c_int32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mt_fast_bvh_build": (C.c_int, [C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_float),
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),

@@ -360,6 +360,13 @@ int mt_trace_closest_ex(mt_scene* scene, uint32_t n, const float* rays, mt_hit* 
     }
 }
 
+int mt_trace_closest_noalloc(mt_scene* scene, uint32_t n, const float* rays_dev, mt_hit* hits_dev,
+                             int32_t traversal, uint64_t stream) {
+    if (!scene || (n && (!rays_dev || !hits_dev))) return mt::fail(MT_E_INVALID, "bad arguments");
+    return mt::trace_rays_noalloc(scene, n, rays_dev, hits_dev, traversal,
+                                  reinterpret_cast<hipStream_t>(stream));
+}
+
 int mt_trace_occluded(mt_scene* scene, uint32_t n, const float* rays, uint32_t* occluded,
                       int32_t traversal) {
     if (!scene || (n && (!rays || !occluded))) return mt::fail(MT_E_INVALID, "bad arguments");

@@ -1485,6 +1485,91 @@ int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t
     return MT_OK;
 }
 
+// ------------------------------------------------------------------ closest hits without allocation
+// mt_trace_closest_noalloc keeps its scratch -- the traversal kernel's work counters and error
+// flag, then the per-thread stack spill area -- in this file-scope device array instead of
+// allocating it.  Every GPU that runs the library holds its own copy (kNoAllocScratchBytes)
+// until the process ends; its address on each device comes from hipGetSymbolAddress, taken
+// once per device with that device current.  All calls on one GPU share that GPU's array:
+// calls that overlap on one GPU (other threads or streams) race.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr size_t kNoAllocCounterBytes = 4096;  // >= (kMaxCounters + 1) words, keeps the spill 4 KiB-aligned
+static_assert((kMaxCounters + 1) * sizeof(uint32_t) <= kNoAllocCounterBytes, "counter block");
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// the device address of `scratch` on the current device
+int scratch_address(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(MT_E_UNSUPPORTED, "device index above 63");
+    std::scoped_lock lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MT_E_HIP, std::string("hipGetSymbolAddress of the scratch array failed: ") +
+                                      hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return MT_OK;
+}
+}  // namespace
+
+int trace_rays_noalloc(mt_scene* s, uint32_t n, const float* rays_dev, mt_hit* hits_dev,
+                       int32_t traversal, hipStream_t stream) {
+    if (traversal != MT_TRAVERSAL_REFERENCE && traversal != MT_TRAVERSAL_FAST)
+        return fail(MT_E_INVALID, "unknown traversal");
+    if (traversal == MT_TRAVERSAL_FAST && !s->dev.fast_meshes && s->mesh_count)
+        return fail(MT_E_INVALID, "the scene was created without fast BVHs (mt_bvh_options.fast)");
+    if (n == 0) return MT_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    unsigned char* base = nullptr;
+    int rc = scratch_address(s->device, &base);
+    if (rc) return rc;
+    // the test kernel keeps 16 stack entries in LDS and spills the rest, (cap - 16) entries of
+    // 8 bytes per thread of the grid: the grid is as large as the spill area allows
+    const uint32_t cap = stack_capacity(s);
+    const uint64_t spill_depth = cap > 16u ? cap - 16u : 1u;
+    const uint64_t per_block = spill_depth * kBlock * sizeof(uint2);
+    const uint64_t fit = (kNoAllocScratchBytes - kNoAllocCounterBytes) / per_block;
+    if (fit == 0)
+        return fail(MT_E_UNSUPPORTED, "the scene's BVH needs a traversal stack deeper than the " +
+                                          std::to_string(kNoAllocScratchBytes >> 20) +
+                                          " MiB scratch of mt_trace_closest_noalloc holds");
+    const uint32_t blocks = static_cast<uint32_t>(std::max<uint64_t>(
+        1u, std::min<uint64_t>({(static_cast<uint64_t>(n) + kBlock - 1) / kBlock,
+                                static_cast<uint64_t>(device_cu_count()) * 8u, fit})));
+    PathBuffers b{};
+    b.counters = reinterpret_cast<uint32_t*>(base);
+    b.spill = reinterpret_cast<uint2*>(base + kNoAllocCounterBytes);
+    // the counters start at zero (the last call's values are still there): a fill kernel,
+    // no memset
+    constexpr uint32_t kWords = kMaxCounters + 1;
+    hipLaunchKernelGGL(k_zero_u32, dim3((kWords + 255) / 256), dim3(256), 0, stream, b.counters, kWords);
+    ChunkParams c{};
+    c.stack_cap = cap;
+    c.refill_min = kRefillMin;
+    c.setup_min = kSetupMin;
+    c.accel = traversal == MT_TRAVERSAL_FAST ? 1 : 0;
+    TestIO io{rays_dev, hits_dev, n, b.counters + kMaxCounters, nullptr};
+    launch_trace<2>(16, false, blocks, stream, s->dev, c, b, 0, io);
+    HIP_TRY(hipGetLastError());
+    uint32_t herr = 0;
+    HIP_TRY(hipMemcpyAsync(&herr, b.counters + kCntError, sizeof(herr), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (herr) return fail(MT_E_INVALID, "traversal stack overflow");
+    return MT_OK;
+}
+
 int brdf_eval(uint32_t n, const float* in, float* out) {
     if (n == 0) return MT_OK;
     std::vector<void*> owned;

@@ -241,6 +241,10 @@ void debug_post(int mode, float* rgb, uint64_t npix);
 // (7 floats per ray)
 int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t traversal,
                uint32_t* occluded);
+// closest hits of n device-resident rays into device-resident hits, with the traversal's
+// scratch in a static device array (no allocation): mt_trace_closest_noalloc
+int trace_rays_noalloc(mt_scene* s, uint32_t n, const float* rays_dev, mt_hit* hits_dev,
+                       int32_t traversal, hipStream_t stream);
 int brdf_eval(uint32_t n, const float* in, float* out);
 int math_eval(int32_t op, uint32_t n, const float* in, float* out);
 
