@@ -28,6 +28,18 @@ HBM_PEAK_GBS = 8000.0
 # matrix 64 B per instance entered, 32 B per BVH node popped, 36 B positions + 4 B index per
 # triangle test
 B_AABB, B_ENTER, B_NODE, B_TRI = 24, 64, 32, 40
+# algorithmic bytes of one shaded path step: the scene data the reference reads per hit
+# (vertex normals 36 B, texture coordinates 24 B, material record 64 B; the grid has no
+# textures) or per miss (four float RGB taps of the environment, 48 B)
+B_HIT, B_MISS = 36 + 24 + 64, 4 * 12
+
+
+def load_profile_json(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def parse():
@@ -347,6 +359,13 @@ def main():
         print(f"# step {i}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
     if args.copy_calibration:
         copy_calibration()
+    # one more frame on a single stream: each launch then runs alone, so its HIP-event time is
+    # the kernel's exclusive duration (the timed frames overlap three streams)
+    os.environ["MT_STREAMS"] = "1"
+    _, _, ex = g.render(settings())
+    del os.environ["MT_STREAMS"]
+    print(f"# exclusive frame (one stream): extend {ex['kernel_ms_extend'] / ex['extend_launches']:.2f} "
+          f"ms per launch", file=sys.stderr, flush=True)
     total_t = sum(times)
     samples = sum(s["samples"] for s in stats)
     rays = sum(s["rays"] for s in stats)
@@ -358,11 +377,14 @@ def main():
     bytes_per_launch = bytes_frame / launches_frame
     avg_launch_s = ext_ms / 1e3 / ext_launches
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "extend_traffic_sphere_grid_1080p.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    ext_traffic = load_profile_json("extend_traffic_sphere_grid_1080p.json")
+    traffic = ext_traffic.get("hbm_bytes_per_launch") if ext_traffic else None
+    ex_ext_s = ex["kernel_ms_extend"] / 1e3 / ex["extend_launches"]
+    ex_shade_s = ex["kernel_ms_shade"] / 1e3 / ex["extend_launches"]
+    # shading: one launch per bounce and chunk, as extend
+    shade_bytes_launch = (B_HIT * cnt["hits_shaded"] + B_MISS * cnt["misses"]) / launches_frame
+    shade_traffic = load_profile_json("shade_traffic_sphere_grid_1080p.json")
+    shade_hbm = shade_traffic.get("hbm_bytes_per_launch") if shade_traffic else None
     line = {
         "metric": METRIC,
         "value": samples / total_t / 1e6,
@@ -385,15 +407,40 @@ def main():
         "kernel_ms_per_step": sum(s["kernel_ms_total"] for s in stats) / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_trace<false,0> (extend: closest hit)",
+                     "kernel": "k_trace<false,0,12u,0> (extend: closest hit, reference traversal)",
+                     "level": "algorithmic node/triangle bytes, most served by L2 and MALL "
+                              "(hbm: the measured HBM traffic)",
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3,
                      # chunks run on concurrent streams, so launches overlap: the whole frame's
                      # extend bytes over the frame time is a lower bound of the aggregate rate
                      "achieved_aggregate_lower_bound": bytes_frame / (total_t / args.steps) / 1e9,
+                     "frac_aggregate_lower_bound": bytes_frame / (total_t / args.steps) / 1e9 / HBM_PEAK_GBS,
+                     # the same kernel alone on the GPU (single-stream frame)
+                     "exclusive": {"avg_launch_ms": ex_ext_s * 1e3,
+                                   "achieved": bytes_per_launch / ex_ext_s / 1e9,
+                                   "frac": bytes_per_launch / ex_ext_s / 1e9 / HBM_PEAK_GBS},
+                     "hbm": None if traffic is None else {
+                         "bytes_per_launch": traffic, "achieved": traffic / ex_ext_s / 1e9,
+                         "frac": traffic / ex_ext_s / 1e9 / HBM_PEAK_GBS,
+                         "l2_hit_rate": ext_traffic.get("l2_hit_rate")},
                      "per_ray": {k: cnt[k] / cnt["rays"] for k in
                                  ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}},
         "cpu_baseline": None,
+        # the shading kernel (scenes without point lights), alone on the GPU
+        "shade_roofline": {
+            "kernel": "k_shade_unlit<false> (shade)", "bound": "hbm", "unit": "GB/s",
+            "peak": HBM_PEAK_GBS, "avg_launch_ms": ex_shade_s * 1e3,
+            "avg_launch_ms_overlapped": sum(s["kernel_ms_shade"] for s in stats) / ext_launches,
+            "algorithmic_bytes_per_launch": shade_bytes_launch,
+            "achieved": shade_bytes_launch / ex_shade_s / 1e9,
+            "frac": shade_bytes_launch / ex_shade_s / 1e9 / HBM_PEAK_GBS,
+            "hbm": None if shade_hbm is None else {
+                "bytes_per_launch": shade_hbm, "achieved": shade_hbm / ex_shade_s / 1e9,
+                "frac": shade_hbm / ex_shade_s / 1e9 / HBM_PEAK_GBS,
+                "l2_hit_rate": shade_traffic.get("l2_hit_rate")},
+            "per_launch": {"hits": cnt["hits_shaded"] / launches_frame,
+                           "misses": cnt["misses"] / launches_frame}},
     }
     # the same frame with the fast traversal (true closest hits; not the headline value)
     from mitrace_amd import abi
