@@ -62,6 +62,8 @@ def parse():
                     help="skip the batched-requests workload")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the fwd+adjoint timing of the inverse-rendering workload")
+    ap.add_argument("--no-rmse", action="store_true",
+                    help="skip the RMSE of the plain renders against 4096-spp references")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a device-to-device copy of 1 GiB (counter calibration)")
     return ap.parse_args()
@@ -125,6 +127,29 @@ def cpu_baseline(scene, settings_fn, seconds):
                       f"{total_s:.1f} s; oracle with -Ofast -flto -march=x86-64-v3, mt19937 per "
                       f"32x32 tile, {threads} threads",
             "mrays_per_s": total_r / total_s / 1e6}
+
+
+def converged(g, settings_fn, spp, ref_frames, seed0):
+    """RMSE of one `spp`-sample render against the mean of `ref_frames` renders of `spp` samples
+    with other seeds (a reference of spp * ref_frames samples per pixel; the reference averages
+    tone-mapped samples, as every render does, so both estimate the same per-pixel mean), and
+    the share of pixel channels within 3 sigma of it (sigma of an `spp`-sample mean, from the
+    spread of the reference's frames)"""
+    import numpy as np
+    img, _, _ = g.render(settings_fn(samples=spp, seed=seed0))
+    acc = np.zeros(img.shape, np.float64)
+    acc2 = np.zeros(img.shape, np.float64)
+    for i in range(ref_frames):
+        f, _, _ = g.render(settings_fn(samples=spp, seed=seed0 + 1 + i))
+        acc += f
+        acc2 += f.astype(np.float64) ** 2
+    ref = acc / ref_frames
+    var = np.maximum(acc2 / ref_frames - ref ** 2, 0.0) * ref_frames / (ref_frames - 1)
+    sigma = np.sqrt(var * (1.0 + 1.0 / ref_frames))
+    d = img - ref
+    return {"spp": spp, "reference_spp": spp * ref_frames, "rmse": float(np.sqrt(np.mean(d ** 2))),
+            "mean": float(ref.mean()), "rmse_over_mean": float(np.sqrt(np.mean(d ** 2)) / ref.mean()),
+            "within_3_sigma": float(np.mean(np.abs(d) <= 3.0 * sigma + 1e-7))}
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -458,6 +483,19 @@ def main():
                     ("inst_aabb_tests", "inst_entered", "node_pops", "tri_tests")}}
     print(f"# fast traversal: {line['fast_traversal']['msamples_per_s']:.1f} Msamples/s",
           file=sys.stderr, flush=True)
+    if not args.no_rmse:
+        # the plain tracer's image error at the configurations' sample counts, against
+        # 4096-spp references of the same estimator (the oracle's samples are these bit for bit,
+        # so its RMSE at equal spp is the same)
+        from mitrace_amd import abi as A
+        line["rmse_vs_4096spp"] = {"sphere_grid_1080p": converged(g, settings, args.spp, 4096 // args.spp, 7000)}
+        cg = GpuScene(scenegen.cornell())
+        line["rmse_vs_4096spp"]["cornell_256_rr"] = converged(
+            cg, lambda **kw: scenegen.cornell_settings(mode=A.MT_MODE_FIXED, rr_depth=3, bounces=8,
+                                                       ev_exposure=2.0, **kw),
+            64, 64, 7100)
+        cg.close()
+        print(f"# rmse: {line['rmse_vs_4096spp']}", file=sys.stderr, flush=True)
     if not args.no_large:
         line["large"], cache = large_leg(args)
         if cache is not None:
