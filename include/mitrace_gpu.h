@@ -192,6 +192,9 @@ typedef struct mt_bvh_options {
     uint32_t max_depth;     /* --bvh-max-depth      (default 32) */
     int32_t fast;           /* 1 = also build the SAH BVHs of MT_TRAVERSAL_FAST (no reference
                                counterpart; the reference builds only the trees above)       */
+    uint32_t device_build_min_tris; /* build the reference-rule BVH of every mesh with at least
+                               this many triangles on the GPU, bit-identical to the host
+                               build (BVH.cpp:117-272 rules); 0 = all on the host            */
 } mt_bvh_options;
 
 typedef struct mt_scene_desc {

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmitrace_gpu.so")
-SOURCES = ["mt_scene.cpp", "mt_render.hip", "mt_capi.cpp", "mt_batch.cpp", "mt_mlp.hip"]
+SOURCES = ["mt_scene.cpp", "mt_render.hip", "mt_capi.cpp", "mt_batch.cpp", "mt_mlp.hip", "mt_bvh_build.hip"]
 ARCH = os.environ.get("MITRACE_OFFLOAD_ARCH", "gfx950")
 
 

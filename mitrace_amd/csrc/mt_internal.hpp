@@ -144,6 +144,17 @@ struct HostBvh {
 // per triangle).  max_depth is the value Mesh::Mesh passes (bvhMaxDepth - 1).
 void build_bvh(const float* positions, uint32_t tri_count, uint32_t max_triangles,
                uint32_t max_depth, HostBvh& out);
+// the same tree built on the current GPU (mt_bvh_build.hip); bit-identical to build_bvh.
+// With recs, the mesh's pair records and leaf slots in the scene's device format (see the
+// layout above; words relative to the mesh's pair/prim base) are built on the device too and
+// handed to the caller (free with free_blocks).
+struct DeviceBvhRecords {
+    float4* pairs;
+    float4* prims;
+    uint64_t npairs, nslots;
+};
+int build_bvh_device(const float* positions, uint32_t tri_count, uint32_t max_triangles,
+                     uint32_t max_depth, HostBvh& out, DeviceBvhRecords* recs = nullptr);
 // glm::inverse stand-in; column-major in and out
 void invert4(const float* m, float* out);
 // M * (x, y, z, w) with the library's fixed operation order

@@ -553,6 +553,22 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     uint64_t fast_pairs = 0, fast_slots = 0;
     std::vector<float4> pairs, prims;
     std::vector<uint8_t> last_flags;
+    // meshes whose BVH was built on the device: their pair records and leaf slots stay there and
+    // go after the host-built records in the scene's arrays
+    struct DevPart {
+        uint32_t mesh;
+        DeviceBvhRecords r;
+    };
+    std::vector<DevPart> dev_parts;
+    struct PartsGuard {
+        std::vector<DevPart>& v;
+        ~PartsGuard() {
+            for (DevPart& p : v) {
+                std::vector<void*> b{p.r.pairs, p.r.prims};
+                free_blocks(b);
+            }
+        }
+    } parts_guard{dev_parts};
     std::vector<float> normals, uvs;
     std::vector<std::pair<std::array<float, 3>, std::array<float, 3>>> local_aabb(d->mesh_count);
     s->bvhs.resize(d->mesh_count);
@@ -581,8 +597,17 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         const bool bvh = !d->bvh.disable && md.tri_count > d->bvh.max_triangles;
         std::vector<uint32_t> order;
         std::vector<uint32_t> leaf_of_slot;  // reference node index of every leaf slot
+        const bool on_device = bvh && d->bvh.device_build_min_tris && md.tri_count >= d->bvh.device_build_min_tris;
         if (bvh) {
-            build_bvh(P, md.tri_count, d->bvh.max_triangles, d->bvh.max_depth - 1u, s->bvhs[i]);
+            if (on_device) {
+                DeviceBvhRecords recs{};
+                const int rc = build_bvh_device(P, md.tri_count, d->bvh.max_triangles, d->bvh.max_depth - 1u,
+                                                s->bvhs[i], &recs);
+                if (rc) return rc;
+                dev_parts.push_back({i, recs});
+            } else {
+                build_bvh(P, md.tri_count, d->bvh.max_triangles, d->bvh.max_depth - 1u, s->bvhs[i]);
+            }
             s->mesh_has_bvh[i] = 1;
             s->max_bvh_depth = std::max(s->max_bvh_depth, s->bvhs[i].depth);
             const HostBvh& hb = s->bvhs[i];
@@ -597,7 +622,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             }
             dm.root_word = word(hb.nodes[0]);
             dm.pair_base = static_cast<uint32_t>(pairs.size() / 4);
-            for (size_t q = 0; q < hb.nodes.size() / 2; ++q) {
+            for (size_t q = 0; !on_device && q < hb.nodes.size() / 2; ++q) {
                 const HostNode& A = hb.nodes[2 * q];
                 const HostNode& B = hb.nodes[2 * q + 1];
                 const uint32_t wa = word(A), wb = word(B), zero = 0;
@@ -608,10 +633,10 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
                 r[3].x = B.mx[0]; r[3].y = B.mx[1]; r[3].z = B.mx[2]; std::memcpy(&r[3].w, &zero, 4);
                 pairs.insert(pairs.end(), r, r + 4);
             }
-            order = hb.indices;
+            if (!on_device) order = hb.indices;
             leaf_of_slot.assign(order.size(), 0);
             std::vector<uint8_t> last(order.size(), 0);
-            for (size_t n = 0; n < hb.nodes.size(); ++n) {
+            for (size_t n = 0; !on_device && n < hb.nodes.size(); ++n) {
                 const HostNode& h = hb.nodes[n];
                 if (!(h.a & kLeafBit) || (n == 1)) continue;
                 const uint32_t first = h.a & ~kLeafBit, count = h.b & ~kLeafBit;
@@ -629,7 +654,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             if (!order.empty()) last_flags.back() = 1;
         }
         dm.prim_base = static_cast<uint32_t>(prims.size() / 3);
-        dm.prim_count = static_cast<uint32_t>(order.size());
+        dm.prim_count = static_cast<uint32_t>(on_device ? dev_parts.back().r.nslots : order.size());
         for (size_t k = 0; k < order.size(); ++k) {
             const uint32_t t = order[k];
             const float* v = P + static_cast<uint64_t>(t) * 9;
@@ -667,10 +692,18 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             fast_slots += (prims.size() - s0) / 3;
         }
     }
-    if (pairs.size() / 4 >= kLeafBit || prims.size() / 3 >= kLeafBit)
+    // the device-built parts follow the host-built records
+    uint64_t tot_pairs = pairs.size() / 4, tot_prims = prims.size() / 3;
+    for (DevPart& p : dev_parts) {
+        meshes[p.mesh].pair_base = static_cast<uint32_t>(tot_pairs);
+        meshes[p.mesh].prim_base = static_cast<uint32_t>(tot_prims);
+        tot_pairs += p.r.npairs;
+        tot_prims += p.r.nslots;
+    }
+    if (tot_pairs >= kLeafBit || tot_prims >= kLeafBit)
         return fail(MT_E_UNSUPPORTED, "more than 2^31 BVH node pairs or leaf slots");
-    s->prim_slots = prims.size() / 3 - fast_slots;
-    s->node_count = (pairs.size() / 4 - fast_pairs) * 2;
+    s->prim_slots = tot_prims - fast_slots;
+    s->node_count = (tot_pairs - fast_pairs) * 2;
     s->triangle_count = tri_total;
 
     std::vector<DevInstance> inst(d->instance_count);
@@ -814,8 +847,32 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     int rc;
     if ((rc = dev_upload(s, inst, &s->dev.instances))) return rc;
     if ((rc = dev_upload(s, meshes, &s->dev.meshes))) return rc;
-    if ((rc = dev_upload(s, pairs, &s->dev.pairs))) return rc;
-    if ((rc = dev_upload(s, prims, &s->dev.prims))) return rc;
+    {
+        // host-built records first, then each device-built part
+        void *dp = nullptr, *dq = nullptr;
+        if (device_malloc(&dp, sizeof(float4) * 4 * std::max<uint64_t>(tot_pairs, 1)))
+            return fail(MT_E_NOMEM, "device allocation failed in scene upload");
+        s->allocations.push_back(dp);
+        if (device_malloc(&dq, sizeof(float4) * 3 * std::max<uint64_t>(tot_prims, 1)))
+            return fail(MT_E_NOMEM, "device allocation failed in scene upload");
+        s->allocations.push_back(dq);
+        float4* fp = static_cast<float4*>(dp);
+        float4* fq = static_cast<float4*>(dq);
+        if (!pairs.empty() && hipMemcpy(fp, pairs.data(), sizeof(float4) * pairs.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(MT_E_HIP, "hipMemcpy failed in scene upload");
+        if (!prims.empty() && hipMemcpy(fq, prims.data(), sizeof(float4) * prims.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(MT_E_HIP, "hipMemcpy failed in scene upload");
+        for (const DevPart& p : dev_parts) {
+            const DevMesh& m = meshes[p.mesh];
+            if (hipMemcpy(fp + 4ull * m.pair_base, p.r.pairs, sizeof(float4) * 4 * p.r.npairs,
+                          hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(fq + 3ull * m.prim_base, p.r.prims, sizeof(float4) * 3 * p.r.nslots,
+                          hipMemcpyDeviceToDevice) != hipSuccess)
+                return fail(MT_E_HIP, "hipMemcpy of device-built BVH records failed");
+        }
+        s->dev.pairs = fp;
+        s->dev.prims = fq;
+    }
     if ((rc = dev_upload(s, normals, &s->dev.normals))) return rc;
     if ((rc = dev_upload(s, uvs, &s->dev.uvs))) return rc;
     if ((rc = dev_upload(s, mats, &s->dev.materials))) return rc;

@@ -108,6 +108,10 @@ class BvhOptions:
     max_triangles: int = 32
     max_depth: int = 32
     fast: bool = False  # also build the SAH BVHs of MT_TRAVERSAL_FAST
+    # meshes with at least this many triangles get their reference-rule BVH built on the GPU
+    # (bit-identical to the host build, about 3.5x faster scene creation on the large
+    # terrain); 0 = every mesh on the host
+    device_build_min_tris: int = 65536
 
 
 @dataclass
@@ -206,6 +210,7 @@ class Scene:
         desc.bvh.max_triangles = self.bvh.max_triangles
         desc.bvh.max_depth = self.bvh.max_depth
         desc.bvh.fast = int(self.bvh.fast)
+        desc.bvh.device_build_min_tris = int(self.bvh.device_build_min_tris)
         keep.extend([meshes, images, mats, insts, lights])
         return desc, keep
 
