@@ -294,12 +294,17 @@ typedef struct mt_diff_params {
 } mt_diff_params;
 
 /* Gradient accumulators (DEVICE pointers, same layouts as mt_diff_params; NULL = skip).
- * mt_diff_backward ADDS to them. */
+ * mt_diff_backward ADDS to them.  detach_sampling = 0 (default): the derivative of the expected
+ * image, including how the sampling decisions depend on the parameters (the lobe choice's
+ * probability, the GGX half-vector density and the mixture pdf, Tracer.cpp:49-82, 177-200, in
+ * score-function form); 1: those decisions and pdfs held fixed (the derivative of the
+ * frozen-sampling estimator of mt_diff_forward's sampling_params). */
 typedef struct mt_diff_grads {
     float* base_texture_dev;
     float* roughness_dev;
     float* metallic_dev;
     float* emission_dev;
+    int32_t detach_sampling;
 } mt_diff_grads;
 
 /* Batched rendering (no reference counterpart): knobs of the persistent iteration loop.

@@ -150,7 +150,8 @@ class mt_diff_params(C.Structure):
 
 class mt_diff_grads(C.Structure):
     _fields_ = [("base_texture_dev", C.c_void_p), ("roughness_dev", C.c_void_p),
-                ("metallic_dev", C.c_void_p), ("emission_dev", C.c_void_p)]
+                ("metallic_dev", C.c_void_p), ("emission_dev", C.c_void_p),
+                ("detach_sampling", C.c_int32)]
 
 
 class mt_cache_options(C.Structure):

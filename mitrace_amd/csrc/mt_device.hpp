@@ -355,12 +355,13 @@ __device__ V3 cosine_dir(float u1, float u2, V3 n) {
     const float lz = __builtin_sqrtf(gmax(0.0f, 1.0f - u1));
     return normalize(add(add(scale(t, r * c), scale(b, r * s)), scale(n, lz)));
 }
-__device__ bool ggx_dir(float u1, float u2, V3 incident, V3 n, float rough, V3& out) {
+__device__ bool ggx_dir(float u1, float u2, V3 incident, V3 n, float rough, V3& out, float& hcos) {
     const float alpha = gmax(rough * rough, 0.0025f);
     const float alpha2 = alpha * alpha;
     const float phi = 2.0f * kPi * u2;
     const float tan2 = alpha2 * u1 / gmax(1.0f - u1, 0.0001f);
     const float ct = 1.0f / __builtin_sqrtf(1.0f + tan2);
+    hcos = ct;  // n.h of the sampled half vector, before any flip
     const float st = __builtin_sqrtf(gmax(0.0f, 1.0f - ct * ct));
     V3 t, b;
     tbn(n, t, b);
@@ -375,6 +376,66 @@ __device__ bool ggx_dir(float u1, float u2, V3 incident, V3 n, float rough, V3& 
     }
     return false;
 }
+// How one bounce's sampling depends on base colour (base' = 0.96 base + 0.04), roughness and
+// metallic, for the full derivative of the expected image (score-function form): the tangent
+// factor d log p(decisions) - [throughput updated] d log pdf(direction) per parameter.  The
+// decisions: the lobe, chosen with p = clamp(mean_c F_c, 0.05, 0.95), F_c = f0_c + (1 - f0_c)
+// (1 - n.v)^5, f0_c = mix(0.04, base'_c, clamp(metal, 0, 1)) (Tracer.cpp:74-82); on the
+// specular lobe the half vector, drawn with density D(h) n.h at roughness max(r, 0.05)
+// (Tracer.cpp:49-72; drawn also when its reflection falls below the surface and a cosine
+// direction replaces it).  The estimator divides by the mixture pdf (Tracer.cpp:190-196).
+struct SampleScore {
+    float base, rough, metal;  // base: the same for every colour channel
+};
+__device__ __forceinline__ float dlog_ggx_d(float nh, float r) {  // d log D(h) / d roughness
+    if (r < 0.05f) return 0.0f;
+    nh = clampf(nh, 0.0f, 1.0f);
+    const float a = r * r;
+    const float A = a * a;
+    const float nh2 = nh * nh;
+    const float den = nh2 * (A - 1.0f) + 1.0f;
+    return 4.0f * r * r * r * (1.0f / A - 2.0f * nh2 / den);
+}
+__device__ SampleScore sample_score(V3 view, V3 N, V3 basec, float rough, float metal, float ps,
+                                    bool spec_lobe, float hcos, V3 dir, float pdf, bool pdf_term) {
+    const float c = clampf(gmax(dot(N, view), 0.0f), 0.0f, 1.0f);
+    const float p5 = em::pow5(1.0f - c);
+    const float mc = clampf(metal, 0.0f, 1.0f);
+    const float bc[3] = {basec.x, basec.y, basec.z};
+    float fsum = 0.0f, dm = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float f0 = 0.04f * (1.0f - mc) + bc[k] * mc;
+        fsum += f0 + (1.0f - f0) * p5;
+        dm += (bc[k] - 0.04f) * (1.0f - p5);
+    }
+    const float pbar = fsum / 3.0f;
+    const bool live = !(pbar < 0.05f) && !(0.95f < pbar);  // the clamp passes p through
+    const float dps_m = (live && metal >= 0.0f && metal <= 1.0f) ? dm / 3.0f : 0.0f;
+    const float dps_b = live ? mc * (1.0f - p5) / 3.0f : 0.0f;
+    SampleScore s;
+    if (spec_lobe) {
+        s.metal = dps_m / ps;
+        s.base = dps_b / ps;
+        s.rough = dlog_ggx_d(hcos, rough);
+    } else {
+        s.metal = -dps_m / (1.0f - ps);
+        s.base = -dps_b / (1.0f - ps);
+        s.rough = 0.0f;
+    }
+    if (pdf_term && pdf > 0.0f) {
+        const float dp = diffuse_pdf(dir, N), sp = specular_pdf(dir, view, N, rough);
+        float dsp_r = 0.0f;
+        if (sp > 0.0f) {
+            const V3 h = normalize(add(view, dir));
+            dsp_r = sp * dlog_ggx_d(gmax(dot(N, h), 0.0f), rough);
+        }
+        s.metal -= dps_m * (sp - dp) / pdf;
+        s.base -= dps_b * (sp - dp) / pdf;
+        s.rough -= ps * dsp_r / pdf;
+    }
+    return s;
+}
+
 __device__ __forceinline__ float spec_probability(V3 view, V3 n, V3 base, float metal) {
     const V3 f = fresnel(gmax(dot(n, view), 0.0f), base_refl(base, metal));
     return clampf((f.x + f.y + f.z) / 3.0f, 0.05f, 0.95f);

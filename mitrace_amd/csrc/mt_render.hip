@@ -788,9 +788,12 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 // BRDF sampling (Tracer.cpp:177-200)
                 const float ps = spec_probability(view, N, basec_s, metal_s);
                 V3 nd;
+                bool spec_lobe = false;
+                float hcos = 0.0f;
                 if (draw(key, dim0 + kDimLobe) < ps) {
+                    spec_lobe = true;
                     const float u1 = draw(key, dim0 + kDimDir), u2 = draw(key, dim0 + kDimDir + 1);
-                    if (!ggx_dir(u1, u2, d, N, rough_s, nd))
+                    if (!ggx_dir(u1, u2, d, N, rough_s, nd, hcos))
                         nd = cosine_dir(draw(key, dim0 + kDimFallback), draw(key, dim0 + kDimFallback + 1), N);
                 } else {
                     nd = cosine_dir(draw(key, dim0 + kDimDir), draw(key, dim0 + kDimDir + 1), N);
@@ -931,6 +934,20 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     trad_next = mul(trad, energy);
                 }
                 if (kDiff == 2) {
+                    if (gr.sampling) {
+                        // the bounce's decisions steer everything collected after this vertex:
+                        // (L - S) times the tangent factor of the decisions' density, and of
+                        // the pdf where it divides the throughput (a light was chosen)
+                        const V3 rem = sub(ltot, ssum);
+                        const float wsum = (dl.x * rem.x + dl.y * rem.y) + dl.z * rem.z;
+                        if (wsum != 0.0f) {
+                            const SampleScore sc = sample_score(view, N, basec, rough, metal, ps, spec_lobe, hcos,
+                                                                nd, pdf, chosen >= 0);
+                            g_r += wsum * sc.rough * ms.rough_scale;
+                            g_m += wsum * sc.metal * ms.metal_scale;
+                            g_base = add(g_base, mk(wsum * sc.base));
+                        }
+                    }
                     add_texel_grad(gr, ms, g_base);
                     b.adj_s[p] = f4(ssum, 0.0f);
                     b.adj_prev[p] = make_float4(tu, tv, __uint_as_float(ms.tex ? mi : 0xFFFFFFFFu), 0.0f);

@@ -34,12 +34,14 @@ struct DiffParams {
     const float* metal;
     const float* emis;
 };
-// gradient accumulators of the same layout (null = not wanted)
+// gradient accumulators of the same layout (null = not wanted); sampling = 1: include the
+// sampling decisions' dependence on the parameters (score-function terms)
 struct DiffGradPtrs {
     float* tex;
     float* rough;
     float* metal;
     float* emis;
+    int32_t sampling;
 };
 
 // per-chunk constants, passed to kernels by value

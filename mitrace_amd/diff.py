@@ -3,10 +3,12 @@
 The estimator is the Tracer's own (same counter RNG, same decisions) with linear radiance
 output and no firefly rejection.  The parameters that can be differentiated are a float RGB
 texture that replaces the base colour of a range of materials, and per-material roughness,
-metallic and RGB emission.  Gradients use detached sampling: the sampled directions, lobe and
-light choices and Russian-roulette probabilities are held fixed and only the values the path
-carries are differentiated, which is unbiased for the expectation of the estimator whenever
-the path throughput is updated with f * cos / pdf (scenes with at least one light).
+metallic and RGB emission.  Gradients are those of the expected image: the values the path
+carries are differentiated, and so is the sampling -- the lobe choice's probability, the GGX
+half-vector density and the mixture pdf the throughput divides by -- in score-function form
+(the radiance collected after a bounce times the derivative of the log density of its
+decisions).  With detach_sampling the sampling decisions and pdfs are held fixed instead (the
+derivative of the frozen-sampling estimator of forward(sampling=...)).
 
 ``torch`` is imported before the library is loaded so that both share torch's HIP runtime.
 """
@@ -93,10 +95,11 @@ class DiffRenderer:
         return img
 
     def backward(self, camera, seed, grad_image, tex=None, rough=None, metal=None, emis=None,
-                 want=("tex", "rough", "metal", "emis")):
+                 want=("tex", "rough", "metal", "emis"), detach_sampling=False):
         """gradients of sum(grad_image * image) for the parameters given (and in `want`)"""
         p = self._params(tex, rough, metal, emis)
         g = abi.mt_diff_grads()
+        g.detach_sampling = int(bool(detach_sampling))
         out = {}
         for t, n, field in ((tex, "tex", "base_texture_dev"), (rough, "rough", "roughness_dev"),
                             (metal, "metal", "metallic_dev"), (emis, "emis", "emission_dev")):

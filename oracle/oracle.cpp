@@ -263,7 +263,10 @@ struct Rng {
     // mt mode
     std::mt19937* mt = nullptr;
     std::uniform_real_distribution<float> unit{0.0f, 1.0f};
+    // given uniforms (test hooks): dimension d of the current base reads fixed[d]
+    const float* fixed = nullptr;
     float u(uint32_t dimOffset) {
+        if (fixed) return fixed[dimOffset];
         if (mt) return unit(*mt);
         return drawU(key, base + dimOffset);
     }
@@ -855,12 +858,13 @@ V3 cosineHemisphere(float u1, float u2, V3 n) {  // Tracer.cpp:34-47
 }
 // SampleGGXDirection (Tracer.cpp:49-72).  Returns false when the reflection falls below the
 // surface and the caller must fall back to cosine sampling with two more uniforms.
-bool ggxDirection(float u1, float u2, V3 incident, V3 n, float rough, V3& out) {
+bool ggxDirection(float u1, float u2, V3 incident, V3 n, float rough, V3& out, float* hcos = nullptr) {
     const float alpha = gmax(rough * rough, 0.0025f);
     const float alpha2 = alpha * alpha;
     const float phi = 2.0f * kPi * u2;
     const float tan2 = alpha2 * u1 / gmax(1.0f - u1, 0.0001f);
     const float cosT = 1.0f / std::sqrt(1.0f + tan2);
+    if (hcos) *hcos = cosT;  // n.h of the sampled half vector, before any flip
     const float sinT = std::sqrt(gmax(0.0f, 1.0f - cosT * cosT));
     V3 t, b;
     buildTBN(n, t, b);
@@ -919,6 +923,8 @@ struct ShadeOut {
     float pdf;
     bool spec;
     float brdfIntensity;
+    float hcos = 0.0f;  // specular lobe: n.h of the sampled half vector (also on the fallback)
+    bool fallback = false;  // specular lobe whose reflection fell below the surface
 };
 
 // the BRDF-sampling half of one GeneratePath iteration (Tracer.cpp:177-200)
@@ -929,7 +935,8 @@ ShadeOut sampleBounce(V3 view, V3 n, V3 base, float rough, float metal, Rng& rng
     if (rng.u(kDimLobe) < p) {
         o.spec = true;
         const float u1 = rng.u(kDimDir), u2 = rng.u(kDimDir + 1);
-        if (!ggxDirection(u1, u2, incident, n, rough, o.dir)) {
+        if (!ggxDirection(u1, u2, incident, n, rough, o.dir, &o.hcos)) {
+            o.fallback = true;
             const float f1 = rng.u(kDimFallback), f2 = rng.u(kDimFallback + 1);
             o.dir = cosineHemisphere(f1, f2, n);
         }
@@ -1268,6 +1275,7 @@ inline Dv operator-(float a, Dv b) { return dv(a) - b; }
 inline Dv operator*(Dv a, float b) { return a * dv(b); }
 inline Dv operator*(float a, Dv b) { return dv(a) * b; }
 inline Dv operator/(Dv a, float b) { return a / dv(b); }
+inline Dv operator-(Dv a, float b) { return a - dv(b); }
 inline Dv dmax(Dv a, float b) { return (a.v < b) ? dv(b) : a; }  // same selection as gmax
 inline Dv dmin(Dv a, float b) { return (b < a.v) ? dv(b) : a; }  // same selection as gmin
 struct D3 {
@@ -1313,6 +1321,42 @@ D3 evalBrdfDv(V3 L, V3 V, V3 N, const D3& base, Dv rough, Dv metal) {
     return out;
 }
 
+// The sampling's dependence on the parameters, for the full derivative of the expected image
+// (score-function form): the specular-lobe probability (Tracer.cpp:74-82), the GGX density of
+// the sampled half vector (Tracer.cpp:49-72: tan^2 = alpha^2 u / (1 - u) samples D(h) n.h) and
+// the mixture pdf the estimator divides by (Tracer.cpp:190-196), over tangent numbers.
+Dv distGGXDv(float nh, Dv r) {  // D(h) of BRDF.hpp:22-33 at roughness max(r, 0.05)
+    nh = clampf(nh, 0.0f, 1.0f);
+    const Dv rc = dmax(r, 0.05f);
+    const Dv a = rc * rc;
+    const Dv a2 = a * a;
+    const float nh2 = nh * nh;
+    const Dv den = nh2 * (a2 - 1.0f) + 1.0f;
+    return a2 / (kPi * den * den);
+}
+Dv specProbabilityDv(V3 view, V3 n, const D3& base, Dv metal) {
+    const float c = clampf(gmax(dot(n, view), 0.0f), 0.0f, 1.0f);
+    const float p5 = em::pow5(1.0f - c);
+    const Dv m = dmin(dmax(metal, 0.0f), 1.0f);
+    Dv sum = dv(0.0f);
+    for (int k = 0; k < 3; ++k) {
+        const Dv f0 = dv(0.04f) * (1.0f - m) + base.c[k] * m;
+        sum = sum + (f0 + (1.0f - f0) * p5);
+    }
+    const Dv p = sum / 3.0f;
+    return (p.v < 0.05f) ? dv(0.05f) : ((0.95f < p.v) ? dv(0.95f) : p);
+}
+Dv specularPdfDv(V3 L, V3 V, V3 N, Dv rough) {
+    const float nl = gmax(dot(N, L), 0.0f);
+    const float nv = gmax(dot(N, V), 0.0f);
+    if (nl <= 0.0f || nv <= 0.0f) return dv(0.0f);
+    const V3 h = normalize(V + L);
+    const float nh = gmax(dot(N, h), 0.0f);
+    const float vh = gmax(dot(V, h), 0.0f);
+    if (vh <= 0.0f) return dv(0.0f);
+    return distGGXDv(nh, rough) * (nh / (4.0f * vh));
+}
+
 struct DiffSet {  // host copies of the differentiable parameters (value or tangent)
     bool hasTex = false;
     Image tex;
@@ -1326,6 +1370,7 @@ struct DiffTracer {
     uint32_t lightStride;
     const DiffSet& P;  // values
     const DiffSet& T;  // tangents (same shapes, empty = zero)
+    bool samplingTerms = false;  // add the sampling decisions' dependence (full derivative)
 
     struct MatD {
         D3 base, emission;
@@ -1388,6 +1433,8 @@ struct DiffTracer {
         D3 base, energy, emission;
         Dv rough, metal;
         bool didHit;
+        // full derivative: tangents of log p(decisions of this bounce) and of log pdf(direction)
+        float score = 0.0f, dlogpdf = 0.0f;
     };
 
     // GeneratePath + ProcessRayForward (Tracer.cpp:134-348) for one camera ray, tangent numbers
@@ -1435,6 +1482,21 @@ struct DiffTracer {
             const float nl = gmax(dot(step.normal, so.dir), 0.0f);
             const V3 ed = (so.pdf > 0.0f) ? tangent(fd) * (nl / so.pdf) : v3(0.0f);
             step.energy = d3(so.energy, ed);
+            if (samplingTerms) {
+                // the bounce's decisions: the lobe (probability p_s or 1 - p_s) and, on the
+                // specular lobe, the half vector (density D(h) n.h; drawn also when the
+                // reflection falls below the surface and a cosine direction replaces it)
+                const Dv ps = specProbabilityDv(view, step.normal, step.base, md.metal);
+                if (so.spec) {
+                    const Dv dh = distGGXDv(so.hcos, md.rough);
+                    step.score = ps.d / ps.v + (dh.v > 0.0f ? dh.d / dh.v : 0.0f);
+                } else {
+                    step.score = -ps.d / (1.0f - ps.v);
+                }
+                const Dv pdf = (1.0f - ps) * diffusePdf(so.dir, step.normal) +
+                               ps * specularPdfDv(so.dir, view, step.normal, md.rough);
+                step.dlogpdf = pdf.v > 0.0f ? pdf.d / pdf.v : 0.0f;
+            }
             Ray nr;
             nr.o = h.o + h.d * h.dist + 0.0001f * step.normal;
             nr.d = so.dir;
@@ -1455,6 +1517,20 @@ struct DiffTracer {
         }
         D3 cE = d3(v3(1.0f));
         D3 L = d3(v3(0.0f));
+        // full derivative: (radiance so far, tangent factor) after each bounce decision; what
+        // is collected after the decision times the factor is added at the end
+        struct Mark {
+            V3 sofar;
+            float factor;
+        };
+        std::vector<Mark> marks;
+        auto mark = [&](const StepD& step, bool chosen) {
+            if (!samplingTerms) return;
+            // the pdf divides the throughput only when the step updates it (a light was chosen,
+            // Tracer.cpp:278-279); the decisions steer the rest of the path either way
+            marks.push_back({V3{L.c[0].v, L.c[1].v, L.c[2].v},
+                             step.score - (chosen ? step.dlogpdf : 0.0f)});
+        };
         for (size_t si = 0; si < path.size(); ++si) {
             const StepD& step = path[si];
             rng.base = 2 + static_cast<uint32_t>(si) * lightStride;
@@ -1468,7 +1544,10 @@ struct DiffTracer {
             L = L + cE * step.emission * st.emission_base_intensity;
             float chosenPdf = 0.0f;
             const int chosen = chooseLight(S, step.hitPos, rng, chosenPdf);
-            if (chosen == -1) continue;
+            if (chosen == -1) {
+                mark(step, false);
+                continue;
+            }
             V3 lcol;
             const V3 lpos = lightPoint(S.lights[chosen], step.hitPos, rng, lcol);
             const V3 lvec = lpos - step.hitPos;
@@ -1484,8 +1563,11 @@ struct DiffTracer {
                 const float fall = 1.0f / (llen * llen + 1.0f);
                 L = L + cE * f * d3(lcol * (nl * fall / chosenPdf));
             }
+            mark(step, true);
             cE = cE * step.energy;
         }
+        for (const Mark& mk : marks)
+            for (int c = 0; c < 3; ++c) L.c[c].d = L.c[c].d + (L.c[c].v - get(mk.sofar, c)) * mk.factor;
         return L;
     }
 };
@@ -1915,6 +1997,44 @@ void orc_math(int32_t op, uint32_t n, const float* in, float* out) {
     }
 }
 
+void orc_bounce_terms(uint32_t n, const float* in, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 22ull * i;
+        const V3 view{q[0], q[1], q[2]}, nrm{q[3], q[4], q[5]};
+        const V3 base{q[6], q[7], q[8]};
+        const float rough = q[9], metal = q[10];
+        // the bounce's uniforms in the sampler's dimensions (lobe 0, direction 1-2, fallback 3-4)
+        Rng rng;
+        rng.fixed = q + 16;
+        const ShadeOut so = sampleBounce(view, nrm, base, rough, metal, rng);
+        const D3 bd = d3(base, V3{q[11], q[12], q[13]});
+        const Dv rd{rough, q[14]}, md{metal, q[15]};
+        const D3 f = evalBrdfDv(so.dir, view, nrm, bd, rd, md);
+        const float nl = gmax(dot(nrm, so.dir), 0.0f);
+        const Dv ps = specProbabilityDv(view, nrm, bd, md);
+        float score;
+        if (so.spec) {
+            const Dv dh = distGGXDv(so.hcos, rd);
+            score = ps.d / ps.v + (dh.v > 0.0f ? dh.d / dh.v : 0.0f);
+        } else {
+            score = -ps.d / (1.0f - ps.v);
+        }
+        const Dv pdf = (1.0f - ps) * diffusePdf(so.dir, nrm) + ps * specularPdfDv(so.dir, view, nrm, rd);
+        float* o = out + 13ull * i;
+        o[0] = so.dir.x;
+        o[1] = so.dir.y;
+        o[2] = so.dir.z;
+        for (int c = 0; c < 3; ++c) {
+            o[3 + c] = get(so.energy, c);                                   // f cos / pdf
+            o[6 + c] = so.pdf > 0.0f ? f.c[c].d * (nl / so.pdf) : 0.0f;    // detached tangent
+        }
+        o[9] = score;
+        o[10] = pdf.v > 0.0f ? pdf.d / pdf.v : 0.0f;
+        o[11] = ps.v;
+        o[12] = so.fallback ? 1.0f : 0.0f;
+    }
+}
+
 int orc_sample_image(uint32_t w, uint32_t h, const uint8_t* rgb8, uint32_t n, const float* uv,
                      int32_t filter, float* out) {
     if (w == 0 || h == 0 || !rgb8) return MT_E_INVALID;
@@ -2139,7 +2259,7 @@ void orc_brdf_tangent(uint32_t n, const float* in, const float* tin, float* out)
 
 int orc_render_diff(const void* sp, const mt_camera_desc* cam, const mt_settings* st,
                     const orc_diff_params* params, const orc_diff_params* tangent_dir,
-                    int32_t threads, float* image, float* dimage) {
+                    int32_t sampling_terms, int32_t threads, float* image, float* dimage) {
     const Scene& S = *static_cast<const Scene*>(sp);
     if (!cam || !st || !image || st->width == 0 || st->height == 0 || st->samples == 0) {
         g_err = "invalid render arguments";
@@ -2154,7 +2274,7 @@ int orc_render_diff(const void* sp, const mt_camera_desc* cam, const mt_settings
         g_err = "tangent texture does not match the parameter texture";
         return MT_E_INVALID;
     }
-    const DiffTracer tr{S, *st, 8u + static_cast<uint32_t>(S.lights.size()), P, T};
+    const DiffTracer tr{S, *st, 8u + static_cast<uint32_t>(S.lights.size()), P, T, sampling_terms != 0};
     const uint32_t W = st->width, H = st->height, spp = st->samples;
     const float aspect = static_cast<float>(W) / static_cast<float>(H);
     const float pxW = 1.0f / static_cast<float>(W), pxH = 1.0f / static_cast<float>(H);

@@ -68,6 +68,12 @@ int orc_env_sample(const void* scene, uint32_t n, const float* dirs, int32_t app
 /* the path's elementary functions: in = n pairs {x, y}; MT_MATH_ATAN2 computes atan2(x, y),
  * the others use x only */
 void orc_math(int32_t op, uint32_t n, const float* in, float* out);
+/* One bounce of the differentiable estimator with given uniforms: in = 22 floats per case
+ * {V.xyz, N.xyz, base'.rgb, rough, metal, tangent {dbase'.rgb, drough, dmetal}, u0..u4 (lobe,
+ * direction, fallback), pad}; out {dir.xyz, w.rgb = f cos / pdf, its tangent with
+ * the sampling detached, score (tangent of log p(decisions)), tangent of log pdf(dir), p_spec,
+ * fallback (1 when the specular reflection fell below the surface)}: 13 floats */
+void orc_bounce_terms(uint32_t n, const float* in, float* out);
 /* Image::Sample (Image.cpp:11-61) of a w x h RGB8 image at n uv pairs -> n RGBA floats */
 int orc_sample_image(uint32_t w, uint32_t h, const uint8_t* rgb8, uint32_t n, const float* uv,
                      int32_t filter, float* out);
@@ -113,10 +119,14 @@ void orc_brdf_tangent(uint32_t n, const float* in, const float* tin, float* out)
 
 /* The differentiable estimator (linear radiance, no firefly rejection, counter RNG):
  * image = mean radiance per pixel; dimage = its directional derivative along tangent_dir
- * (same layout as params, NULL members = 0) with the sampling detached (forward mode). */
+ * (same layout as params, NULL members = 0), forward mode.  sampling_terms = 0: the sampling
+ * detached (decisions and pdfs held fixed); 1: plus the dependence of the sampling decisions
+ * on the parameters (the lobe choice's probability, the GGX half-vector density and the
+ * mixture pdf), times the radiance collected after each decision -- the score-function form
+ * of the derivative of the expected image. */
 int orc_render_diff(const void* scene, const mt_camera_desc* cam, const mt_settings* settings,
                     const orc_diff_params* params, const orc_diff_params* tangent_dir,
-                    int32_t threads, float* image, float* dimage);
+                    int32_t sampling_terms, int32_t threads, float* image, float* dimage);
 
 #ifdef __cplusplus
 }

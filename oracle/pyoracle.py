@@ -107,8 +107,9 @@ def load(fast=False, mutant=0):
                                    C.POINTER(C.c_float)]
     L.orc_render_diff.argtypes = [C.c_void_p, C.POINTER(abi.mt_camera_desc),
                                   C.POINTER(abi.mt_settings), C.POINTER(orc_diff_params),
-                                  C.POINTER(orc_diff_params), C.c_int32, C.POINTER(C.c_float),
-                                  C.POINTER(C.c_float)]
+                                  C.POINTER(orc_diff_params), C.c_int32, C.c_int32,
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.orc_bounce_terms.argtypes = [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_math.argtypes = [C.c_int32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.orc_sample_image.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
                                    C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_float)]
@@ -212,10 +213,12 @@ class OracleScene:
         return rgb, samples, w.as_dict(), secs.value
 
 
-    def render_diff(self, settings, params, tangent=None, camera=None, threads=0):
+    def render_diff(self, settings, params, tangent=None, camera=None, threads=0,
+                    sampling_terms=False):
         """The differentiable estimator: (mean linear radiance, directional derivative along
         `tangent`) per pixel.  params/tangent: dicts {tex, tex_first, tex_count, rough, metal,
-        emis} of numpy arrays (missing = the scene's value / zero tangent)."""
+        emis} of numpy arrays (missing = the scene's value / zero tangent).  sampling_terms adds
+        the sampling decisions' dependence on the parameters (full derivative)."""
         cam = (camera or self.scene.camera).to_desc()
         st = settings.to_desc()
         W, H = settings.width, settings.height
@@ -225,7 +228,7 @@ class OracleScene:
         p = _diff_struct(params, keep)
         t = _diff_struct(tangent, keep)
         rc = self.L.orc_render_diff(self.h, C.byref(cam), C.byref(st), C.byref(p), C.byref(t),
-                                    threads, abi.fptr(img), abi.fptr(dimg))
+                                    int(sampling_terms), threads, abi.fptr(img), abi.fptr(dimg))
         if rc != 0:
             raise RuntimeError(self.L.orc_last_error().decode())
         return img, dimg
@@ -340,4 +343,12 @@ def math_eval(op, x, y=None):
     inp = np.ascontiguousarray(np.stack([x, y], 1), np.float32)
     out = np.zeros(len(x), np.float32)
     load().orc_math(op, len(x), abi.fptr(inp), abi.fptr(out))
+    return out
+
+
+def bounce_terms(inp):
+    """orc_bounce_terms over (n, 22) inputs -> (n, 13)"""
+    inp = np.ascontiguousarray(inp, np.float32).reshape(-1, 22)
+    out = np.zeros((len(inp), 13), np.float32)
+    load().orc_bounce_terms(len(inp), abi.fptr(inp), abi.fptr(out))
     return out

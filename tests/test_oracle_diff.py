@@ -128,3 +128,77 @@ def test_diff_tangent_is_linear_in_the_direction():
     _, dw = orc.render_diff(st, p, w)
     np.testing.assert_allclose(dw, 2.0 * du - 0.5 * dv_, rtol=1e-3,
                                atol=1e-4 * (np.abs(dw).max() + 1e-6))
+
+
+# ---------------------------------------------------------------- the sampling's dependence
+def _bounce_expectation(view, n, base, rough, metal, tan, R, G=192, G2=8):
+    """E over one bounce's uniforms of w(x) R(x) (w = f cos / pdf, Tracer.cpp:177-200) by
+    quadrature: the lobe choice weighted by its probability, the direction's two uniforms on a
+    G x G midpoint grid, and for the specular samples whose reflection falls below the surface
+    the cosine fallback's two uniforms on a G2 x G2 grid.  Also the expectations of the two
+    derivative integrands: detached (dw R) and full (dw R + w R (score - dlog pdf))."""
+    def run(u):
+        inp = np.zeros((len(u), 22), np.float32)
+        inp[:, 0:3], inp[:, 3:6], inp[:, 6:9] = view, n, base
+        inp[:, 9], inp[:, 10], inp[:, 11:16], inp[:, 16:21] = rough, metal, tan, u
+        return po.bounce_terms(inp)
+
+    def parts(o):
+        Rx = R(o[:, 0:3])[:, None]
+        val = o[:, 3:6] * Rx
+        return val, o[:, 6:9] * Rx + val * (o[:, 9:10] - o[:, 10:11]), o[:, 6:9] * Rx
+
+    g = (np.arange(G) + 0.5) / G
+    U1, U2 = np.meshgrid(g, g, indexing="ij")
+    res = {}
+    for lobe, u0 in (("spec", 0.0), ("diff", 0.99999)):
+        u = np.zeros((G * G, 5), np.float32)
+        u[:, 0], u[:, 1], u[:, 2], u[:, 3], u[:, 4] = u0, U1.ravel(), U2.ravel(), 0.5, 0.5
+        o = run(u)
+        val, full, det = parts(o)
+        fb = np.nonzero(o[:, 12] > 0.5)[0]
+        if len(fb):
+            g2 = (np.arange(G2) + 0.5) / G2
+            F3, F4 = np.meshgrid(g2, g2, indexing="ij")
+            uu = np.repeat(u[fb], G2 * G2, axis=0)
+            uu[:, 3], uu[:, 4] = np.tile(F3.ravel(), len(fb)), np.tile(F4.ravel(), len(fb))
+            v2, f2, d2 = (x.reshape(len(fb), -1, 3).mean(1) for x in parts(run(uu)))
+            val[fb], full[fb], det[fb] = v2, f2, d2
+        res[lobe] = (o[0, 11], val.mean(0), full.mean(0), det.mean(0), len(fb) / len(o))
+    ps = res["spec"][0]
+    mix = lambda k: ps * res["spec"][k] + (1 - ps) * res["diff"][k]
+    return mix(1), mix(2), mix(3), res["spec"][4]
+
+
+@pytest.mark.parametrize("view,rough,metal", [
+    ((0.2, 0.5, 0.84), 0.7, 0.5),   # a third of the GGX samples fall back to cosine sampling
+    ((0.9, 0.2, 0.1), 0.8, 0.3),    # grazing view: almost half fall back
+    ((0.0, 1.0, 0.0), 0.4, 0.9),    # normal view, shiny metal: few fall back
+])
+def test_full_derivative_of_one_bounce_equals_finite_differences(view, rough, metal):
+    """The sampling's dependence on the parameters (Tracer.cpp:49-82, 177-200): the lobe is
+    chosen with a probability that depends on base colour and metallic, the GGX half vector's
+    density D(h) n.h depends on roughness, and where the reflection falls below the surface a
+    cosine direction replaces it while the estimator still divides by the mixture pdf, so the
+    direction's actual density is not that pdf.  By quadrature over the bounce's uniforms, the
+    expectation of the full derivative integrand (score-function form) equals central
+    differences of the expected bounce contribution for roughness, metallic and base colour,
+    while the detached integrand misses the fallback's share."""
+    n = np.array([0.0, 1.0, 0.0], np.float32)
+    v = np.array(view, np.float32)
+    v /= np.linalg.norm(v)
+    base = np.array([0.7, 0.4, 0.2], np.float32)
+    a = np.array([1.0, 2.0, 0.5]) / np.linalg.norm([1.0, 2.0, 0.5])
+    R = lambda x: 1.0 + 3.0 * np.maximum(0.0, x @ a) ** 3   # a smooth radiance field
+    h = 1e-2
+    for name, tan in (("rough", [0, 0, 0, 1, 0]), ("metal", [0, 0, 0, 0, 1]), ("base", [1, 0.5, -0.3, 0, 0])):
+        tan = np.array(tan, np.float32)
+        _, full, det, fb = _bounce_expectation(v, n, base, rough, metal, tan, R)
+        Ep, *_ = _bounce_expectation(v, n, base + h * tan[:3], rough + h * tan[3], metal + h * tan[4], tan, R)
+        Em, *_ = _bounce_expectation(v, n, base - h * tan[:3], rough - h * tan[3], metal - h * tan[4], tan, R)
+        fd = (Ep - Em) / (2 * h)
+        scale = np.abs(fd).max()
+        # the quadrature resolves the fallback boundary to about 1/G
+        assert np.abs(full - fd).max() <= 0.02 * scale, (name, fd, full)
+        if fb > 0.3 and name != "base":  # there the detached derivative is far off
+            assert np.abs(det - fd).max() > 0.1 * scale, (name, fd, det)
