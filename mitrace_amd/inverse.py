@@ -11,8 +11,9 @@ mean squared error over the 8 images, Adam (lr 1e-2) on texture and roughness.
 
 gradcheck: gradients of that loss at the initial parameters against central differences
 (h = 1e-3): per parameter for 64 random texels and every sphere's roughness and metallic with
-the sampling frozen (the estimator the adjoint differentiates), and along random directions
-with common random numbers only (4096 spp), which also sees the changed sampling decisions.
+the sampling frozen against the detached adjoint (the estimator it differentiates), and along
+random directions with common random numbers only (4096 spp), which also sees the changed
+sampling decisions, against the full adjoint.
 """
 import argparse
 import json
@@ -173,7 +174,7 @@ def gradcheck(h=1e-3, n_texels=64, frozen_spp=64, crn_spp=4096, seed=5, crn_seed
     d = to_dev(init)
     img = dr.forward(cams[0], seed, **d)
     g = {k: v.cpu().numpy().reshape(-1)
-         for k, v in dr.backward(cams[0], seed, 2.0 * (img - tg), **d).items()}
+         for k, v in dr.backward(cams[0], seed, 2.0 * (img - tg), detach_sampling=True, **d).items()}
     # texels that some pixel sees, so that the derivative is not trivially zero
     seen = np.flatnonzero(g["tex"] != 0.0)
     cases = [("tex", int(i)) for i in rng.choice(seen, min(n_texels, len(seen)), replace=False)]
