@@ -8,7 +8,7 @@
  *   ------------------------------  ---------------------------------------------------------
  *   mitrace_gpu::RenderBuffer       RenderBuffer (src/Tracer/RenderBuffer.hpp:11-35,
  *                                     RenderBuffer.cpp:3-84): W x H float RGB, row-major;
- *                                     SaveToFilePNG / SaveToFileHDR
+ *                                     SaveToFilePNG / SaveToFileJPG / SaveToFileHDR
  *   mitrace_gpu::Scene              Scene::Scene (src/Scene/Scene.cpp:53-59) as uploaded by
  *                                     mt_scene_create; GetCamera() as Scene::GetCamera
  *   mitrace_gpu::DefaultSettings()  Config::ConfigOptions defaults (src/CLI/Config.hpp:43-80)
@@ -116,8 +116,9 @@ class RenderBuffer {
      * RGB of GetPixelsRGB8, zlib stream of stored (uncompressed) deflate blocks, so any PNG
      * reader decodes the same pixels stb's compressed file holds.  HDR: Radiance RGBE with
      * flat (not run-length encoded) scanlines, top row first, linear floats as the buffer
-     * holds them.  A failure throws std::runtime_error("Failed to save image to file"), the
-     * reference's message.  SaveToFileJPG has no counterpart here (no JPEG encoder). */
+     * holds them.  JPG: baseline sequential JPEG (ITU-T T.81) of GetPixelsRGB8 in JFIF YCbCr,
+     * see SaveToFileJPG.  A failure throws std::runtime_error("Failed to save image to file"),
+     * the reference's message. */
     void SaveToFilePNG(const std::string& filename) const {
         const auto rgb = GetPixelsRGB8();
         std::vector<std::uint8_t> raw;  // filter byte 0 (None) + one row
@@ -179,10 +180,221 @@ class RenderBuffer {
         WriteFile(filename, file);
     }
 
+    /* RenderBuffer.cpp:69-75 passes width * 3 (the row stride) where stb_image_write expects
+     * the quality, which it clamps to 1..100: the file's quality is min(100, 3 * width), i.e.
+     * 100 from 34 pixels wide on.  This encoder: JFIF YCbCr (T.871 conversion), one 8x8 block
+     * per component and MCU (no chroma subsampling), the example quantisation tables of T.81
+     * Annex K.1 scaled by the IJG quality rule (quality 100: every step 1), the example Huffman
+     * tables of Annex K.3, float DCT-II with rounding to the nearest step, edge blocks padded
+     * by repeating the last column and row. */
+    void SaveToFileJPG(const std::string& filename) const {
+        if (width_ == 0 || height_ == 0 || width_ > 65535 || height_ > 65535)
+            throw std::runtime_error("Failed to save image to file");
+        WriteFile(filename, EncodeJpeg(GetPixelsRGB8().get(), width_, height_,
+                                       static_cast<int>(std::min<unsigned>(100u, width_ * 3u))));
+    }
+
     float* Data() noexcept { return pixels_.data(); }
     const float* Data() const noexcept { return pixels_.data(); }
 
   private:
+    // T.81 Annex K.3 Huffman tables: code counts per length 1..16, then the symbols in order
+    struct HuffSpec {
+        std::uint8_t counts[16];
+        std::vector<std::uint8_t> symbols;
+    };
+    struct HuffCode {
+        std::uint16_t code[256] = {};
+        std::uint8_t len[256] = {};  // 0: symbol absent
+    };
+    static const HuffSpec& JpegTable(int which) {  // 0 DC luma, 1 AC luma, 2 DC chroma, 3 AC chroma
+        static const HuffSpec specs[4] = {
+            {{0, 1, 5, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0}, {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11}},
+            {{0, 2, 1, 3, 3, 2, 4, 3, 5, 5, 4, 4, 0, 0, 1, 0x7d},
+             {0x01, 0x02, 0x03, 0x00, 0x04, 0x11, 0x05, 0x12, 0x21, 0x31, 0x41, 0x06, 0x13, 0x51,
+              0x61, 0x07, 0x22, 0x71, 0x14, 0x32, 0x81, 0x91, 0xa1, 0x08, 0x23, 0x42, 0xb1, 0xc1,
+              0x15, 0x52, 0xd1, 0xf0, 0x24, 0x33, 0x62, 0x72, 0x82, 0x09, 0x0a, 0x16, 0x17, 0x18,
+              0x19, 0x1a, 0x25, 0x26, 0x27, 0x28, 0x29, 0x2a, 0x34, 0x35, 0x36, 0x37, 0x38, 0x39,
+              0x3a, 0x43, 0x44, 0x45, 0x46, 0x47, 0x48, 0x49, 0x4a, 0x53, 0x54, 0x55, 0x56, 0x57,
+              0x58, 0x59, 0x5a, 0x63, 0x64, 0x65, 0x66, 0x67, 0x68, 0x69, 0x6a, 0x73, 0x74, 0x75,
+              0x76, 0x77, 0x78, 0x79, 0x7a, 0x83, 0x84, 0x85, 0x86, 0x87, 0x88, 0x89, 0x8a, 0x92,
+              0x93, 0x94, 0x95, 0x96, 0x97, 0x98, 0x99, 0x9a, 0xa2, 0xa3, 0xa4, 0xa5, 0xa6, 0xa7,
+              0xa8, 0xa9, 0xaa, 0xb2, 0xb3, 0xb4, 0xb5, 0xb6, 0xb7, 0xb8, 0xb9, 0xba, 0xc2, 0xc3,
+              0xc4, 0xc5, 0xc6, 0xc7, 0xc8, 0xc9, 0xca, 0xd2, 0xd3, 0xd4, 0xd5, 0xd6, 0xd7, 0xd8,
+              0xd9, 0xda, 0xe1, 0xe2, 0xe3, 0xe4, 0xe5, 0xe6, 0xe7, 0xe8, 0xe9, 0xea, 0xf1, 0xf2,
+              0xf3, 0xf4, 0xf5, 0xf6, 0xf7, 0xf8, 0xf9, 0xfa}},
+            {{0, 3, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0}, {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11}},
+            {{0, 2, 1, 2, 4, 4, 3, 4, 7, 5, 4, 4, 0, 1, 2, 0x77},
+             {0x00, 0x01, 0x02, 0x03, 0x11, 0x04, 0x05, 0x21, 0x31, 0x06, 0x12, 0x41, 0x51, 0x07,
+              0x61, 0x71, 0x13, 0x22, 0x32, 0x81, 0x08, 0x14, 0x42, 0x91, 0xa1, 0xb1, 0xc1, 0x09,
+              0x23, 0x33, 0x52, 0xf0, 0x15, 0x62, 0x72, 0xd1, 0x0a, 0x16, 0x24, 0x34, 0xe1, 0x25,
+              0xf1, 0x17, 0x18, 0x19, 0x1a, 0x26, 0x27, 0x28, 0x29, 0x2a, 0x35, 0x36, 0x37, 0x38,
+              0x39, 0x3a, 0x43, 0x44, 0x45, 0x46, 0x47, 0x48, 0x49, 0x4a, 0x53, 0x54, 0x55, 0x56,
+              0x57, 0x58, 0x59, 0x5a, 0x63, 0x64, 0x65, 0x66, 0x67, 0x68, 0x69, 0x6a, 0x73, 0x74,
+              0x75, 0x76, 0x77, 0x78, 0x79, 0x7a, 0x82, 0x83, 0x84, 0x85, 0x86, 0x87, 0x88, 0x89,
+              0x8a, 0x92, 0x93, 0x94, 0x95, 0x96, 0x97, 0x98, 0x99, 0x9a, 0xa2, 0xa3, 0xa4, 0xa5,
+              0xa6, 0xa7, 0xa8, 0xa9, 0xaa, 0xb2, 0xb3, 0xb4, 0xb5, 0xb6, 0xb7, 0xb8, 0xb9, 0xba,
+              0xc2, 0xc3, 0xc4, 0xc5, 0xc6, 0xc7, 0xc8, 0xc9, 0xca, 0xd2, 0xd3, 0xd4, 0xd5, 0xd6,
+              0xd7, 0xd8, 0xd9, 0xda, 0xe2, 0xe3, 0xe4, 0xe5, 0xe6, 0xe7, 0xe8, 0xe9, 0xea, 0xf2,
+              0xf3, 0xf4, 0xf5, 0xf6, 0xf7, 0xf8, 0xf9, 0xfa}}};
+        return specs[which];
+    }
+    // canonical codes from the counts (T.81 Annex C): consecutive codes within a length,
+    // shifted left by one at each new length
+    static HuffCode JpegCodes(const HuffSpec& s) {
+        HuffCode h;
+        unsigned code = 0;
+        std::size_t k = 0;
+        for (int l = 1; l <= 16; ++l, code <<= 1)
+            for (int i = 0; i < s.counts[l - 1]; ++i, ++code, ++k) {
+                h.code[s.symbols[k]] = static_cast<std::uint16_t>(code);
+                h.len[s.symbols[k]] = static_cast<std::uint8_t>(l);
+            }
+        return h;
+    }
+    struct BitWriter {
+        std::vector<std::uint8_t>& out;
+        std::uint32_t acc = 0;
+        int n = 0;
+        void Put(std::uint32_t bits, int len) {  // most significant bit first
+            for (int i = len - 1; i >= 0; --i) {
+                acc = (acc << 1) | ((bits >> i) & 1u);
+                if (++n == 8) {
+                    out.push_back(static_cast<std::uint8_t>(acc));
+                    if (acc == 0xff) out.push_back(0);  // byte stuffing (T.81 F.1.2.3)
+                    acc = 0, n = 0;
+                }
+            }
+        }
+        void Flush() {  // pad the last byte with 1-bits
+            while (n) Put(1, 1);
+        }
+    };
+    static void PutSegment(std::vector<std::uint8_t>& f, std::uint8_t marker,
+                           const std::vector<std::uint8_t>& body) {
+        const std::size_t len = body.size() + 2;
+        f.insert(f.end(), {0xff, marker, static_cast<std::uint8_t>(len >> 8),
+                           static_cast<std::uint8_t>(len & 0xff)});
+        f.insert(f.end(), body.begin(), body.end());
+    }
+    static std::vector<std::uint8_t> EncodeJpeg(const std::uint8_t* rgb, unsigned w, unsigned h,
+                                                int quality) {
+        // zig-zag scan: position k -> row-major index, walking the anti-diagonals alternately
+        int zz[64];
+        for (int k = 0, s = 0; s < 15; ++s)
+            for (int i = 0; i <= s; ++i) {
+                const int r = (s & 1) ? i : s - i, c = s - r;
+                if (r < 8 && c < 8) zz[k++] = r * 8 + c;
+            }
+        static const std::uint8_t kLuma[64] = {
+            16, 11, 10, 16, 24,  40,  51,  61,  12, 12, 14, 19, 26,  58,  60,  55,
+            14, 13, 16, 24, 40,  57,  69,  56,  14, 17, 22, 29, 51,  87,  80,  62,
+            18, 22, 37, 56, 68,  109, 103, 77,  24, 35, 55, 64, 81,  104, 113, 92,
+            49, 64, 78, 87, 103, 121, 120, 101, 72, 92, 95, 98, 112, 100, 103, 99};
+        static const std::uint8_t kChroma[64] = {
+            17, 18, 24, 47, 99, 99, 99, 99, 18, 21, 26, 66, 99, 99, 99, 99,
+            24, 26, 56, 99, 99, 99, 99, 99, 47, 66, 99, 99, 99, 99, 99, 99,
+            99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99,
+            99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99, 99};
+        const int scale = quality < 50 ? 5000 / quality : 200 - 2 * quality;  // IJG rule
+        std::uint8_t q[2][64];  // natural (row-major) order
+        for (int i = 0; i < 64; ++i) {
+            q[0][i] = static_cast<std::uint8_t>(std::clamp((kLuma[i] * scale + 50) / 100, 1, 255));
+            q[1][i] = static_cast<std::uint8_t>(std::clamp((kChroma[i] * scale + 50) / 100, 1, 255));
+        }
+        std::vector<std::uint8_t> f = {0xff, 0xd8};  // SOI
+        PutSegment(f, 0xe0, {'J', 'F', 'I', 'F', 0, 1, 1, 0, 0, 1, 0, 1, 0, 0});
+        for (int t = 0; t < 2; ++t) {
+            std::vector<std::uint8_t> dqt = {static_cast<std::uint8_t>(t)};
+            for (int k = 0; k < 64; ++k) dqt.push_back(q[t][zz[k]]);
+            PutSegment(f, 0xdb, dqt);
+        }
+        PutSegment(f, 0xc0, {8, static_cast<std::uint8_t>(h >> 8), static_cast<std::uint8_t>(h & 0xff),
+                             static_cast<std::uint8_t>(w >> 8), static_cast<std::uint8_t>(w & 0xff), 3,
+                             1, 0x11, 0, 2, 0x11, 1, 3, 0x11, 1});
+        const std::uint8_t tclass[4] = {0x00, 0x10, 0x01, 0x11};
+        for (int t = 0; t < 4; ++t) {
+            const HuffSpec& s = JpegTable(t);
+            std::vector<std::uint8_t> dht = {tclass[t]};
+            dht.insert(dht.end(), s.counts, s.counts + 16);
+            dht.insert(dht.end(), s.symbols.begin(), s.symbols.end());
+            PutSegment(f, 0xc4, dht);
+        }
+        PutSegment(f, 0xda, {3, 1, 0x00, 2, 0x11, 3, 0x11, 0, 63, 0});
+        const HuffCode codes[4] = {JpegCodes(JpegTable(0)), JpegCodes(JpegTable(1)),
+                                   JpegCodes(JpegTable(2)), JpegCodes(JpegTable(3))};
+        double basis[8][8];  // C(u) cos((2x + 1) u pi / 16) / 2
+        for (int u = 0; u < 8; ++u)
+            for (int x = 0; x < 8; ++x)
+                basis[u][x] = (u ? 0.5 : 0.5 / std::sqrt(2.0)) *
+                              std::cos((2 * x + 1) * u * 3.14159265358979323846 / 16.0);
+        BitWriter bw{f};
+        auto put_sym = [&](const HuffCode& hc, int sym) {
+            if (!hc.len[sym]) throw std::runtime_error("Failed to save image to file");
+            bw.Put(hc.code[sym], hc.len[sym]);
+        };
+        auto magnitude = [](int v, int& cat) {  // category and its appended bits (T.81 F.1.2.1)
+            const int a = v < 0 ? -v : v;
+            cat = 0;
+            while ((a >> cat) > 0) ++cat;
+            return static_cast<std::uint32_t>(v < 0 ? v + (1 << cat) - 1 : v);
+        };
+        int pred[3] = {0, 0, 0};
+        for (unsigned by = 0; by < h; by += 8)
+            for (unsigned bx = 0; bx < w; bx += 8) {
+                double blk[3][64];
+                for (int y = 0; y < 8; ++y)
+                    for (int x = 0; x < 8; ++x) {
+                        const unsigned sx = std::min(bx + x, w - 1), sy = std::min(by + y, h - 1);
+                        const std::uint8_t* p = rgb + (std::size_t(sy) * w + sx) * 3;
+                        const double r = p[0], g = p[1], b = p[2];
+                        blk[0][y * 8 + x] = 0.299 * r + 0.587 * g + 0.114 * b - 128.0;
+                        blk[1][y * 8 + x] = -0.168735892 * r - 0.331264108 * g + 0.5 * b;
+                        blk[2][y * 8 + x] = 0.5 * r - 0.418687589 * g - 0.081312411 * b;
+                    }
+                for (int c = 0; c < 3; ++c) {
+                    double tmp[64];
+                    int coef[64];
+                    for (int y = 0; y < 8; ++y)  // rows, then columns
+                        for (int u = 0; u < 8; ++u) {
+                            double s = 0;
+                            for (int x = 0; x < 8; ++x) s += basis[u][x] * blk[c][y * 8 + x];
+                            tmp[y * 8 + u] = s;
+                        }
+                    for (int v = 0; v < 8; ++v)
+                        for (int u = 0; u < 8; ++u) {
+                            double s = 0;
+                            for (int y = 0; y < 8; ++y) s += basis[v][y] * tmp[y * 8 + u];
+                            coef[v * 8 + u] = static_cast<int>(std::lround(s / q[c ? 1 : 0][v * 8 + u]));
+                        }
+                    const HuffCode& dc = codes[c ? 2 : 0];
+                    const HuffCode& ac = codes[c ? 3 : 1];
+                    int cat = 0;
+                    std::uint32_t bits = magnitude(coef[0] - pred[c], cat);
+                    pred[c] = coef[0];
+                    put_sym(dc, cat);
+                    bw.Put(bits, cat);
+                    int run = 0;
+                    for (int k = 1; k < 64; ++k) {
+                        const int v = coef[zz[k]];
+                        if (v == 0) {
+                            ++run;
+                            continue;
+                        }
+                        for (; run >= 16; run -= 16) put_sym(ac, 0xf0);  // ZRL
+                        bits = magnitude(v, cat);
+                        put_sym(ac, (run << 4) | cat);
+                        bw.Put(bits, cat);
+                        run = 0;
+                    }
+                    if (run) put_sym(ac, 0x00);  // EOB
+                }
+            }
+        bw.Flush();
+        f.insert(f.end(), {0xff, 0xd9});  // EOI
+        return f;
+    }
+
     static void PutU32(std::vector<std::uint8_t>& v, std::uint32_t x) {
         for (int s = 24; s >= 0; s -= 8) v.push_back(static_cast<std::uint8_t>(x >> s));
     }

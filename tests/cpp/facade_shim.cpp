@@ -135,8 +135,8 @@ int fac_draw_sub(const float* dst, unsigned dw, unsigned dh, const float* src, u
     return 0;
 }
 
-// RenderBuffer::SaveToFilePNG (which = 0) or SaveToFileHDR (which = 1) of a W x H float RGB
-// image; returns 0, or 1 with the exception's message in err.
+// RenderBuffer::SaveToFilePNG (which = 0), SaveToFileHDR (which = 1) or SaveToFileJPG
+// (which = 2) of a W x H float RGB image; returns 0, or 1 with the exception's message in err.
 int fac_save(const float* pixels, unsigned w, unsigned h, const char* path, int which, char* err,
              int errlen) {
     try {
@@ -144,8 +144,10 @@ int fac_save(const float* pixels, unsigned w, unsigned h, const char* path, int 
         std::memcpy(b.Data(), pixels, sizeof(float) * 3 * std::size_t(w) * h);
         if (which == 0)
             b.SaveToFilePNG(path);
-        else
+        else if (which == 1)
             b.SaveToFileHDR(path);
+        else
+            b.SaveToFileJPG(path);
         return 0;
     } catch (const std::runtime_error& e) {
         copy_message(e.what(), err, errlen);

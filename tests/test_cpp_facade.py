@@ -6,7 +6,8 @@ the reference's Config defaults; RenderBuffer's RGB8 conversion and DrawSubBuffe
 RenderBuffer.cpp:3-58 (checked against values fixed by hand and against the Python mirror);
 a failing C call surfaces as mitrace_gpu::Error with the C-ABI's code and message;
 SaveToFilePNG / SaveToFileHDR write files that decode (PIL, zlib, the Radiance reader) to the
-buffer's pixels, with known RGBE bytes pinned by hand.
+buffer's pixels, with known RGBE bytes pinned by hand; SaveToFileJPG writes baseline JFIF that
+PIL decodes, with libjpeg's quantisation tables for the reference's effective quality.
 GPU: the facade's Start/Wait render equals the synchronous mt_render of the same seed bit for
 bit and the oracle within 3 sigma; StatThread-style polling sees monotone progress; a kill
 ends the render early without an error and a second StartRender completes.
@@ -248,8 +249,50 @@ def test_save_hdr_rgbe_round_trip(tmp_path):
     assert (back <= px).all() and (px - back <= mx / 128.0).all()
 
 
+def test_save_jpg_decodes_close_to_the_rgb8_pixels(tmp_path):
+    """SaveToFileJPG (RenderBuffer.cpp:69-75): a baseline JFIF file that PIL (libjpeg) decodes.
+    The reference hands stb the row stride (width * 3) as the quality, so the quality is
+    min(100, 3 * width): from 34 pixels wide every quantisation step is 1 and the decoded
+    pixels stay within a few levels of GetPixelsRGB8's; narrower images carry libjpeg's own
+    tables for that quality (the IJG rule over the T.81 Annex K.1 tables) and lose no more than
+    libjpeg's own 4:4:4 encoding at that quality.  Edge blocks (sizes not multiples of 8)
+    decode to the image's size."""
+    import io
+    from PIL import Image
+    rng = np.random.default_rng(11)
+    yy, xx = np.mgrid[0:250, 0:300]
+    smooth = np.stack([xx / 300.0, yy / 250.0, 0.5 + 0.5 * np.sin(xx / 17.0 + yy / 23.0)], -1)
+    cases = [(smooth + rng.normal(scale=0.05, size=smooth.shape)).astype(np.float32),
+             (rng.random((37, 50, 3)) * 1.4 - 0.2).astype(np.float32),
+             rng.random((3, 7, 3)).astype(np.float32), rng.random((1, 1, 3)).astype(np.float32),
+             smooth[:40, :10].astype(np.float32), smooth[:9, :20].astype(np.float32)]
+    for px in cases:
+        h, w, _ = px.shape
+        path = tmp_path / f"img_{w}x{h}.jpg"
+        rc, msg = _save(px, path, 2)
+        assert rc == 0, msg
+        want = np.clip(px * np.float32(255.0), 0.0, 255.0).astype(np.uint8)
+        quality = min(100, 3 * w)
+        ref = io.BytesIO()
+        Image.fromarray(want).save(ref, "JPEG", quality=quality, subsampling=0)
+        with Image.open(path) as im, Image.open(ref) as lib:
+            assert im.format == "JPEG" and im.mode == "RGB" and im.size == (w, h)
+            assert im.quantization == lib.quantization, (w, h)
+            got = np.asarray(im).astype(np.int32)
+            libpx = np.asarray(lib).astype(np.int32)
+        err = np.abs(got - want.astype(np.int32))
+        if quality == 100:
+            assert all(v == 1 for t in im.quantization.values() for v in t)
+            assert err.max() <= 4 and err.mean() < 1.0, (w, h, err.max(), err.mean())
+        # the same quantisation as libjpeg's encoder (integer DCT): the same distortion, up to
+        # coefficients near a half step that the two DCTs round apart
+        rms = lambda d: float(np.sqrt(np.mean(d.astype(np.float64) ** 2)))
+        assert rms(got - want) <= 1.1 * rms(libpx - want) + 0.5, (w, h, rms(got - want),
+                                                                   rms(libpx - want))
+
+
 def test_save_to_unwritable_path_throws_reference_message(tmp_path):
     px = np.zeros((2, 2, 3), np.float32)
-    for which in (0, 1):
+    for which in (0, 1, 2):
         rc, msg = _save(px, tmp_path / "missing_dir" / "x.img", which)
         assert rc == 1 and msg == "Failed to save image to file"
