@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MT_ABI_VERSION 1
+#define MT_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------------------- */
 #define MT_OK 0
@@ -307,14 +307,19 @@ typedef struct mt_diff_grads {
     int32_t detach_sampling;
 } mt_diff_grads;
 
-/* Batched rendering (no reference counterpart): knobs of the persistent iteration loop.
+/* Batched rendering (no reference counterpart): knobs of the iteration loop.
  * Zero fields take the defaults in brackets. */
+#define MT_BATCH_PERSISTENT 1 /* one cooperative launch serves the batch from a device queue */
+#define MT_BATCH_GRAPHS 2     /* a host thread replays a captured graph per iteration        */
 typedef struct mt_batch_options {
     uint32_t paths_per_iteration; /* wavefront width: camera paths in flight [4194304]   */
     uint32_t tile_pixels;         /* pixels per work item of the device queue [1024]      */
     uint32_t iteration_tiles;     /* queue capacity: tiles one iteration may take [4096] */
-    uint32_t trace_blocks_per_cu; /* persistent traversal blocks per CU [8]              */
-    uint32_t streams;             /* concurrent lanes, each a stream with its path state [3] */
+    uint32_t trace_blocks_per_cu; /* graphs: persistent traversal blocks per CU [8]
+                                     (persistent: as many blocks as the CUs hold)        */
+    uint32_t streams;             /* concurrent lanes, each with its own path state: streams
+                                     (graphs) or groups of the launch's blocks (persistent) [3] */
+    uint32_t mode;                /* MT_BATCH_PERSISTENT or MT_BATCH_GRAPHS [persistent]  */
 } mt_batch_options;
 
 typedef struct mt_batch_job_info {
@@ -328,12 +333,16 @@ typedef struct mt_batch_job_info {
 } mt_batch_job_info;
 
 typedef struct mt_batch_stats {
-    uint64_t iterations;       /* graph launches                                       */
-    uint64_t graphs_captured;  /* captures (one per scene and bounce count)            */
+    uint64_t iterations;       /* wavefront iterations (graphs: graph launches)        */
+    uint64_t graphs_captured;  /* captures (one per scene and bounce count; graphs)    */
     uint64_t samples;          /* camera paths of all jobs                             */
     uint32_t jobs;
     uint32_t scenes;
     mt_batch_options options;  /* the values in effect                                 */
+    uint32_t pad;
+    uint64_t launches;         /* kernel launches of the persistent mode: 1 unless the
+                                  queue stayed empty for 1 s (the launch then ends and
+                                  the next submission starts another)                   */
 } mt_batch_stats;
 
 /* one trial of mt_batch_autotune */
@@ -503,11 +512,15 @@ int mt_cache_train(mt_cache* cache, uint32_t n, const float* pos, const float* d
                    const float* target, float* loss, float* grad);
 
 /* Batched rendering (no reference counterpart; for serving many render requests).  Jobs of
- * any scene, resolution, sample count and depth are split into tiles of tile_pixels pixels on
- * per-scene device queues (one per lane); one host thread replays, per iteration, a captured
- * graph of the whole wavefront step (schedule tiles from the queue, generate, extend/shade/shadow per
- * bounce, finalize, resolve into each job's image) for the scene of the oldest unfinished
- * job.  Every job's image equals mt_render's with the same scene, camera and settings.
+ * any scene, resolution, sample count and depth are split into tiles of tile_pixels pixels.
+ * MT_BATCH_PERSISTENT: one cooperative kernel launch serves the whole batch; the host
+ * appends job records and tiles to a queue in pinned memory, and each lane (a group of the
+ * launch's blocks with its own path state) takes the next tiles of one scene, runs the whole
+ * wavefront iteration (generate, extend/shade/shadow per bounce, finalize, resolve into each
+ * job's image) between group barriers and marks the jobs it completed in pinned memory.
+ * MT_BATCH_GRAPHS: per-scene device queues (one per lane); one host thread replays, per
+ * iteration, a captured graph of the same steps for the scene of the oldest unfinished job.
+ * Every job's image equals mt_render's with the same scene, camera and settings.
  * mt_batch_submit may be called from any thread before or while the batch runs; rgb_out
  * (host, width*height*3 floats, may be NULL) is written by mt_batch_finish. */
 int mt_batch_create(const mt_batch_options* options, mt_batch** out);
@@ -519,10 +532,10 @@ int mt_batch_finish(mt_batch* batch);
 int mt_batch_get_job_info(const mt_batch* batch, uint32_t job_id, mt_batch_job_info* info);
 int mt_batch_get_stats(const mt_batch* batch, mt_batch_stats* stats);
 int mt_batch_destroy(mt_batch* batch);
-/* Picks paths_per_iteration, tile_pixels, iteration_tiles and trace_blocks_per_cu by
- * coordinate descent over measured runs of the n calibration jobs.  Each trial creates a
- * batch, serves the jobs once at 1 sample per pixel (graph captures, first touches; untimed),
- * then submits them as given, all at once, and times that round to its finish.
+/* Picks mode, paths_per_iteration, tile_pixels, trace_blocks_per_cu (graphs), iteration_tiles
+ * and streams by coordinate descent over measured runs of the n calibration jobs.  Each trial
+ * creates a batch, serves the jobs once at 1 sample per pixel (graph captures, first touches;
+ * untimed), then submits them as given, all at once, and times that round to its finish.
  * trials (may be NULL) receives up to max_trials measurements. */
 int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
                       const mt_settings* settings, uint32_t n, mt_batch_options* best,

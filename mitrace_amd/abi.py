@@ -164,7 +164,7 @@ class mt_cache_options(C.Structure):
 class mt_batch_options(C.Structure):
     _fields_ = [("paths_per_iteration", C.c_uint32), ("tile_pixels", C.c_uint32),
                 ("iteration_tiles", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
-                ("streams", C.c_uint32)]
+                ("streams", C.c_uint32), ("mode", C.c_uint32)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -179,7 +179,7 @@ class mt_batch_job_info(C.Structure):
 class mt_batch_stats(C.Structure):
     _fields_ = [("iterations", C.c_uint64), ("graphs_captured", C.c_uint64),
                 ("samples", C.c_uint64), ("jobs", C.c_uint32), ("scenes", C.c_uint32),
-                ("options", mt_batch_options)]
+                ("options", mt_batch_options), ("pad", C.c_uint32), ("launches", C.c_uint64)]
 
 
 class mt_autotune_trial(C.Structure):

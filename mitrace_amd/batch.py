@@ -1,6 +1,7 @@
 """Batched rendering over the C-ABI (mt_batch_*): many render jobs of uneven size, sample count,
-depth and scene served by one device work queue per scene and one captured iteration graph per
-scene, plus the autotuner of the iteration's shape.
+depth and scene served either by one persistent kernel launch fed from a pinned queue
+(options["mode"] = PERSISTENT, the default) or by one device work queue per scene and a captured
+iteration graph per scene (GRAPHS), plus the autotuner of the iteration's shape.
 
     b = Batch(options)            # options: dict or None (defaults)
     j = b.submit(gpu_scene, settings, camera)
@@ -17,7 +18,8 @@ from . import abi, scenegen
 from ._lib import check, load
 
 OPTION_NAMES = ("paths_per_iteration", "tile_pixels", "iteration_tiles", "trace_blocks_per_cu",
-                "streams")
+                "streams", "mode")
+PERSISTENT, GRAPHS = 1, 2  # MT_BATCH_PERSISTENT, MT_BATCH_GRAPHS
 
 
 def _options(d):
@@ -78,7 +80,7 @@ class Batch:
         s = abi.mt_batch_stats()
         check(self.L.mt_batch_get_stats(self.h, C.byref(s)))
         return {"iterations": s.iterations, "graphs_captured": s.graphs_captured,
-                "samples": s.samples, "jobs": s.jobs, "scenes": s.scenes,
+                "launches": s.launches, "samples": s.samples, "jobs": s.jobs, "scenes": s.scenes,
                 "options": s.options.as_dict()}
 
 

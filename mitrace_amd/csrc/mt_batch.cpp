@@ -1,7 +1,15 @@
-// mt_batch.cpp -- batched rendering: per-scene device tile queues, one captured graph per scene
-// for a whole wavefront iteration, a host scheduler thread, and the autotuner.
+// mt_batch.cpp -- batched rendering, two ways: one persistent cooperative launch fed from
+// pinned memory (MT_BATCH_PERSISTENT), or per-scene device tile queues with one captured graph
+// per scene for a whole wavefront iteration (MT_BATCH_GRAPHS); a host scheduler thread; and
+// the autotuner.
 //
-// Each scene has one queue and one captured graph per lane (a stream with its own path state);
+// Persistent: the scheduler thread appends job records and tiles to pinned coherent memory
+// and publishes their counts; the kernel's lane groups copy them to the device and take tiles
+// (k_batch_persistent).  The thread watches the per-job completion flags the kernel writes,
+// starts a new launch when one ended (after 1 s with an empty queue) while work is pending,
+// and closes the queue at mt_batch_finish, upon which the launch drains it and ends.
+//
+// Graphs: each scene has one queue and one captured graph per lane (a stream with its own path state);
 // a job's tiles are dealt round-robin over the lanes, so iterations of one scene run
 // concurrently and one lane's sparse last bounces overlap another's full ones.
 // The scheduler keeps an exact host mirror of the device queues: the device scheduler kernel
@@ -14,6 +22,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -28,7 +37,11 @@
 namespace {
 
 constexpr uint32_t kMaxJobs = 1u << 14;
-constexpr uint32_t kQueueTiles = 1u << 20;  // per scene
+constexpr uint32_t kQueueTiles = 1u << 20;  // per scene (graphs), per batch (persistent)
+constexpr uint32_t kMaxSlots = 256;         // scene slots of a persistent batch
+constexpr uint64_t kStallMs = 60000;        // persistent: a group barrier waiting this long aborts
+                                            // (MT_BATCH_STALL_MS overrides, at least 1000)
+constexpr uint64_t kIdleMs = 1000;          // persistent: a lane group exits after this long idle
 constexpr uint32_t kMaxStack = 64;          // traversal stack entries (LDS + spill)
 constexpr uint32_t kBlockSize = 256;
 constexpr uint32_t kInflight = 3;           // iterations queued ahead of the device
@@ -37,7 +50,9 @@ double now_ms(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-constexpr uint32_t kMaxLanes = 8;
+constexpr uint32_t kMaxLanes = 8;         // graphs: streams
+constexpr uint32_t kMaxGroups = 32;       // persistent: lane groups
+constexpr uint64_t kMaxPersistentPaths = 1ull << 28;  // persistent: paths in flight over all groups
 
 struct HostJob {
     mt_scene* scene = nullptr;
@@ -52,7 +67,27 @@ struct HostJob {
     double submit_ms = 0.0;
     std::vector<hipEvent_t> done_evs;         // one per lane that had tiles; complete before `recorded`
     std::atomic<bool> recorded{false};
+    // persistent mode: the completion flag seen by the scheduler thread, and when
+    std::atomic<bool> finished{false};
+    double complete_ms = -1.0;  // written before `finished`
 };
+
+// coherent pinned host memory that the device reads (and partly writes) while it runs
+template <typename T>
+int pinned_alloc(T** p, size_t n) {
+    void* q = nullptr;
+    if (hipHostMalloc(&q, sizeof(T) * n, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+        return mt::fail(MT_E_NOMEM, "pinned host allocation of the batch queue failed");
+    std::memset(q, 0, sizeof(T) * n);
+    *p = static_cast<T*>(q);
+    return MT_OK;
+}
+template <typename T>
+T* device_view(T* host) {
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, host, 0) != hipSuccess) return nullptr;
+    return static_cast<T*>(d);
+}
 
 // one stream with its own path state; every scene has a queue and a graph per lane
 struct Lane {
@@ -115,12 +150,31 @@ struct mt_batch {
     std::atomic<uint64_t> iterations{0}, captures{0};
     uint32_t first_open = 0;  // oldest job with tiles not yet scheduled
     std::vector<hipEvent_t> spare;
+    // persistent mode
+    bool persistent = false;
+    mt::PersistCtl ctl{};
+    mt::PersistHost* ph = nullptr;        // pinned records and counts (host addresses)
+    uint4* p_tiles = nullptr;
+    mt::BatchJob* p_jobs = nullptr;
+    uint32_t *p_job_slot = nullptr, *p_job_tiles = nullptr, *p_job_done = nullptr;
+    mt::PersistSlot* p_slots = nullptr;
+    unsigned long long* p_iters = nullptr;
+    std::vector<std::pair<mt_scene*, int32_t>> p_slot_keys;  // (scene, traversal) per slot
+    uint32_t p_tail = 0;
+    uint32_t p_first_undone = 0;
+    hipEvent_t p_end = nullptr;           // recorded after the running launch
+    bool p_running = false;
+    std::atomic<uint64_t> launches{0};
+    uint64_t stall_ms = kStallMs;         // MT_BATCH_STALL_MS
 
     ~mt_batch();
     int init(const mt_batch_options* o);
+    int init_persistent();
     int admit(uint32_t id);
+    int admit_persistent(uint32_t id);
     int ensure_graph(SceneSlot& s, uint32_t lane);
     int run();
+    int run_persistent();
     // records the first error of the batch (the scheduler thread stops at it)
     int fail(int code, const std::string& m) {
         {
@@ -159,6 +213,9 @@ mt_batch::~mt_batch() {
         cv.notify_all();
         worker.join();
     }
+    // a launch still running (the scheduler stopped at an error) drains what was published
+    // and ends once the queue is closed
+    if (ph) __atomic_store_n(&ph->closed, 1u, __ATOMIC_RELEASE);
     if (stream) (void)hipStreamSynchronize(stream);
     for (Lane& L : lanes)
         if (L.stream) (void)hipStreamSynchronize(L.stream);
@@ -180,6 +237,11 @@ mt_batch::~mt_batch() {
     // them: free_blocks hands every block back and clears that bookkeeping
     mt::free_blocks(owned);
     if (jobs_host) (void)hipHostFree(jobs_host);
+    for (void* h : {static_cast<void*>(ph), static_cast<void*>(p_tiles), static_cast<void*>(p_jobs),
+                    static_cast<void*>(p_job_slot), static_cast<void*>(p_job_tiles),
+                    static_cast<void*>(p_job_done), static_cast<void*>(p_slots), static_cast<void*>(p_iters)})
+        if (h) (void)hipHostFree(h);
+    if (p_end) (void)hipEventDestroy(p_end);
     if (stream) (void)hipStreamDestroy(stream);
     if (capture) (void)hipStreamDestroy(capture);
 }
@@ -191,11 +253,17 @@ int mt_batch::init(const mt_batch_options* o) {
     if (!opt.tile_pixels) opt.tile_pixels = 1024;
     if (!opt.iteration_tiles) opt.iteration_tiles = 4096;
     if (!opt.trace_blocks_per_cu) opt.trace_blocks_per_cu = 8;
-    if (!opt.streams) opt.streams = 3;
-    if (opt.paths_per_iteration > (1u << 26) || opt.iteration_tiles > (1u << 20) ||
-        opt.trace_blocks_per_cu > 32 || opt.streams > kMaxLanes)
-        return fail(MT_E_UNSUPPORTED, "batch options out of range");
+    if (!opt.mode) opt.mode = MT_BATCH_PERSISTENT;
+    persistent = opt.mode == MT_BATCH_PERSISTENT;
     BT_TRY(hipGetDevice(&device));
+    uint32_t xcds = 1;
+    if (persistent && mt::xcd_count(&xcds)) return fail(MT_E_HIP, mt_last_error());
+    if (!opt.streams) opt.streams = persistent ? xcds : 3;  // persistent: one lane group per XCD
+    if (opt.paths_per_iteration > (1u << 26) || opt.iteration_tiles > (1u << 20) ||
+        opt.trace_blocks_per_cu > 32 || opt.streams > (persistent ? kMaxGroups : kMaxLanes) ||
+        (persistent && uint64_t(opt.streams) * opt.paths_per_iteration > kMaxPersistentPaths) ||
+        (opt.mode != MT_BATCH_PERSISTENT && opt.mode != MT_BATCH_GRAPHS))
+        return fail(MT_E_UNSUPPORTED, "batch options out of range");
     BT_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     BT_TRY(hipStreamCreateWithFlags(&capture, hipStreamNonBlocking));
     const int cus = mt::cu_count();
@@ -204,7 +272,25 @@ int mt_batch::init(const mt_batch_options* o) {
     shade_blocks = static_cast<uint32_t>(std::min<uint64_t>((P + kBlockSize - 1) / kBlockSize, cus * 16ull));
     lds_ref = mt::lds_stack_depth(false, opt.trace_blocks_per_cu);
     lds_fast = mt::lds_stack_depth(true, opt.trace_blocks_per_cu);
-    const uint64_t spill = static_cast<uint64_t>(kMaxStack - std::min(lds_ref, lds_fast)) * trace_blocks * kBlockSize;
+    uint64_t spill = static_cast<uint64_t>(kMaxStack - std::min(lds_ref, lds_fast)) * trace_blocks * kBlockSize;
+    if (persistent) {
+        // the launch holds as many blocks as the CUs do, split evenly into the lane groups
+        uint32_t per_cu = 0;
+        if (mt::persistent_blocks_per_cu(&per_cu)) return fail(MT_E_HIP, mt_last_error());
+        const uint32_t blocks = static_cast<uint32_t>(cus) * per_cu;
+        ctl.groups = opt.streams;
+        // lane groups inside XCDs when the counts divide evenly, else groups of consecutive blocks
+        if (xcds > 1 && opt.streams % xcds == 0 && blocks % xcds == 0 &&
+            (blocks / xcds) % (opt.streams / xcds) == 0) {
+            ctl.xcds = xcds;
+            ctl.group_blocks = blocks / opt.streams;
+        } else {
+            ctl.xcds = 0;
+            ctl.group_blocks = blocks / opt.streams;
+        }
+        if (ctl.group_blocks == 0) return fail(MT_E_UNSUPPORTED, "more lanes than resident blocks");
+        spill = static_cast<uint64_t>(kMaxStack - mt::kPersistDepth) * ctl.group_blocks * kBlockSize;
+    }
     auto dev_alloc = [&](void** p, size_t bytes) -> int {
         if (mt::device_malloc(p, bytes)) return fail(MT_E_NOMEM, "device allocation of batch buffers failed");
         own(*p);
@@ -214,7 +300,7 @@ int mt_batch::init(const mt_batch_options* o) {
     if ((rc = dev_alloc(reinterpret_cast<void**>(&jobs_dev), sizeof(mt::BatchJob) * kMaxJobs))) return rc;
     lanes.resize(opt.streams);
     for (Lane& L : lanes) {
-        BT_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        if (!persistent) BT_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
         mt::PathBuffers& b = L.buf;
         if ((rc = mt::alloc_path_buffers(owned, b, P, spill, false, true))) return fail(rc, mt_last_error());
         if ((rc = dev_alloc(reinterpret_cast<void**>(&b.it_tiles), sizeof(uint4) * opt.iteration_tiles)) ||
@@ -226,6 +312,177 @@ int mt_batch::init(const mt_batch_options* o) {
     }
     BT_TRY(hipHostMalloc(reinterpret_cast<void**>(&jobs_host), sizeof(mt::BatchJob) * kMaxJobs));
     BT_TRY(hipEventCreate(&e0));
+    return persistent ? init_persistent() : MT_OK;
+}
+
+int mt_batch::init_persistent() {
+    int rc;
+    auto dev_alloc = [&](void** p, size_t bytes) -> int {
+        if (mt::device_malloc(p, bytes)) return fail(MT_E_NOMEM, "device allocation of batch buffers failed");
+        own(*p);
+        return MT_OK;
+    };
+    if ((rc = pinned_alloc(&ph, 1)) || (rc = pinned_alloc(&p_tiles, kQueueTiles)) ||
+        (rc = pinned_alloc(&p_jobs, kMaxJobs)) || (rc = pinned_alloc(&p_job_slot, kMaxJobs)) ||
+        (rc = pinned_alloc(&p_job_tiles, kMaxJobs)) || (rc = pinned_alloc(&p_job_done, kMaxJobs)) ||
+        (rc = pinned_alloc(&p_slots, kMaxSlots)) || (rc = pinned_alloc(&p_iters, kMaxGroups)))
+        return fail(rc, mt_last_error());
+    ctl.host = device_view(ph);
+    ctl.host_tiles = device_view(p_tiles);
+    ctl.host_jobs = device_view(p_jobs);
+    ctl.host_job_slot = device_view(p_job_slot);
+    ctl.host_job_tiles = device_view(p_job_tiles);
+    ctl.host_slots = device_view(p_slots);
+    ctl.job_done = device_view(p_job_done);
+    ctl.host_iters = device_view(p_iters);
+    if (!ctl.host || !ctl.host_tiles || !ctl.host_jobs || !ctl.host_job_slot || !ctl.host_job_tiles ||
+        !ctl.host_slots || !ctl.job_done || !ctl.host_iters)
+        return fail(MT_E_HIP, "device addresses of the pinned batch queue failed");
+    void* p = nullptr;
+    if ((rc = dev_alloc(&p, sizeof(uint4) * kQueueTiles))) return rc;
+    ctl.tiles = static_cast<uint4*>(p);
+    ctl.jobs = jobs_dev;
+    if ((rc = dev_alloc(&p, sizeof(uint32_t) * kMaxJobs))) return rc;
+    ctl.job_slot = static_cast<uint32_t*>(p);
+    if ((rc = dev_alloc(&p, sizeof(uint32_t) * kMaxJobs))) return rc;
+    ctl.tiles_left = static_cast<uint32_t*>(p);
+    if ((rc = dev_alloc(&p, sizeof(mt::PersistSlot) * kMaxSlots))) return rc;
+    ctl.slots = static_cast<mt::PersistSlot*>(p);
+    if ((rc = dev_alloc(&p, sizeof(uint32_t) * mt::kPsWords))) return rc;
+    ctl.state = static_cast<uint32_t*>(p);
+    if ((rc = dev_alloc(&p, sizeof(uint32_t) * mt::kPsBarWords * kMaxGroups))) return rc;
+    ctl.bar = static_cast<uint32_t*>(p);
+    if ((rc = dev_alloc(&p, sizeof(mt::PathBuffers) * lanes.size()))) return rc;
+    ctl.lanes = static_cast<const mt::PathBuffers*>(p);
+    std::vector<mt::PathBuffers> lb;
+    for (const Lane& L : lanes) lb.push_back(L.buf);
+    BT_TRY(hipMemcpyAsync(p, lb.data(), sizeof(mt::PathBuffers) * lb.size(), hipMemcpyHostToDevice, stream));
+    mt::launch_zero_u32(ctl.state, mt::kPsWords, stream);
+    mt::launch_zero_u32(ctl.bar, mt::kPsBarWords * kMaxGroups, stream);
+    BT_TRY(hipGetLastError());
+    BT_TRY(hipStreamSynchronize(stream));
+    uint64_t per_ms = 0;
+    if (mt::wall_ticks_per_ms(&per_ms)) return fail(MT_E_HIP, mt_last_error());
+    ctl.idle_ticks = kIdleMs * per_ms;
+    if (const char* v = std::getenv("MT_BATCH_STALL_MS"))
+        stall_ms = std::max<uint64_t>(1000, std::strtoull(v, nullptr, 10));
+    ctl.stall_ticks = stall_ms * per_ms;
+    BT_TRY(hipEventCreateWithFlags(&p_end, hipEventDisableTiming));
+    return MT_OK;
+}
+
+// writes job `id`'s scene slot (new scenes only), record and tiles into the pinned queue; the
+// counts are published by run_persistent after each round of admissions
+int mt_batch::admit_persistent(uint32_t id) {
+    HostJob& j = *view[id];
+    const int32_t accel = j.st.traversal == MT_TRAVERSAL_FAST ? 1 : 0;
+    uint32_t slot = static_cast<uint32_t>(p_slot_keys.size());
+    for (uint32_t i = 0; i < p_slot_keys.size(); ++i)
+        if (p_slot_keys[i].first == j.scene && p_slot_keys[i].second == accel) slot = i;
+    if (slot == p_slot_keys.size()) {
+        if (slot >= kMaxSlots) return fail(MT_E_UNSUPPORTED, "too many scenes in one batch");
+        if (mt::stack_capacity(j.scene) > kMaxStack)
+            return fail(MT_E_UNSUPPORTED, "scene BVH too deep for batched rendering");
+        mt::PersistSlot& ps = p_slots[slot];
+        ps.S = j.scene->dev;
+        mt::fill_params(ps.c, j.scene, j.cam, j.st);
+        ps.c.count_work = 0;
+        ps.lights = j.scene->light_count ? 1u : 0u;
+        ps.accel = static_cast<uint32_t>(accel);
+        std::scoped_lock lk(mu);
+        p_slot_keys.emplace_back(j.scene, accel);
+    }
+    j.slot = slot;
+    mt::BatchJob& rec = p_jobs[id];
+    mt::fill_params(rec.p, j.scene, j.cam, j.st);
+    rec.p.count_work = 0;
+    rec.img = j.img_dev;
+    p_job_slot[id] = slot;
+    const uint32_t npix = j.st.width * j.st.height;
+    const uint32_t tp = std::max<uint32_t>(1u, std::min<uint32_t>(opt.tile_pixels, opt.paths_per_iteration / j.st.samples));
+    uint32_t count = 0;
+    for (uint32_t p0 = 0; p0 < npix; p0 += tp, ++count) {
+        if (p_tail >= kQueueTiles) return fail(MT_E_UNSUPPORTED, "tile queue of the batch is full");
+        const uint32_t n = std::min(tp, npix - p0);
+        p_tiles[p_tail++] = make_uint4(id, p0, n, n * j.st.samples);
+    }
+    p_job_tiles[id] = count;
+    j.tiles_left = count;
+    std::scoped_lock lk(mu);
+    j.tiles = count;
+    return MT_OK;
+}
+
+int mt_batch::run_persistent() {
+    if (hipSetDevice(device) != hipSuccess) return fail(MT_E_HIP, "hipSetDevice failed");
+    bool closed = false;
+    while (true) {
+        {
+            std::scoped_lock lk(mu);
+            for (size_t i = view.size(); i < jobs.size(); ++i) view.push_back(jobs[i].get());
+        }
+        const uint32_t total = static_cast<uint32_t>(view.size());
+        if (admitted < total) {
+            while (admitted < total) {
+                const int rc = admit_persistent(admitted);
+                if (rc) return rc;
+                ++admitted;
+            }
+            // records before counts, and the tail last: a group that sees a tile has its
+            // job's record and scene slot
+            __atomic_store_n(&ph->slots, static_cast<uint32_t>(p_slot_keys.size()), __ATOMIC_RELEASE);
+            __atomic_store_n(&ph->jobs, admitted, __ATOMIC_RELEASE);
+            __atomic_store_n(&ph->tail, p_tail, __ATOMIC_RELEASE);
+        }
+        // whether the launch has ended first, then the completion flags: a launch that ended
+        // wrote every flag of the tiles it took before it ended
+        if (p_running) {
+            const hipError_t q = hipEventQuery(p_end);
+            if (q == hipSuccess) {
+                p_running = false;
+                uint32_t abort_flag = 0;
+                BT_TRY(hipMemcpy(&abort_flag, ctl.state + mt::kPsAbort, sizeof(uint32_t), hipMemcpyDeviceToHost));
+                if (abort_flag)
+                    return fail(MT_E_HIP, "persistent batch kernel aborted: a lane group waited " +
+                                              std::to_string(stall_ms) + " ms at a barrier");
+            } else if (q != hipErrorNotReady) {
+                return fail(MT_E_HIP, std::string("persistent batch kernel failed: ") + hipGetErrorString(q));
+            }
+        }
+        for (uint32_t i = p_first_undone; i < admitted; ++i) {
+            HostJob& j = *view[i];
+            if (!j.finished.load() && __atomic_load_n(p_job_done + i, __ATOMIC_ACQUIRE)) {
+                j.complete_ms = now_ms(created);
+                j.finished.store(true, std::memory_order_release);
+            }
+        }
+        while (p_first_undone < admitted && view[p_first_undone]->finished.load()) ++p_first_undone;
+        const bool pending = p_first_undone < admitted;
+        if (!p_running && pending) {
+            // a new launch: the queue state (head, copied counts) carries over; the lock, the
+            // abort word and the group barriers start at zero
+            mt::launch_zero_u32(ctl.state + mt::kPsLock, 2, stream);
+            mt::launch_zero_u32(ctl.bar, mt::kPsBarWords * kMaxGroups, stream);
+            BT_TRY(hipGetLastError());
+            if (const int rc = mt::launch_batch_persistent(ctl, stream)) return fail(rc, mt_last_error());
+            BT_TRY(hipEventRecord(p_end, stream));
+            p_running = true;
+            ++launches;
+        }
+        std::unique_lock lk(mu);
+        if (finishing.load() && jobs.size() == admitted) {
+            if (!closed) {
+                __atomic_store_n(&ph->closed, 1u, __ATOMIC_RELEASE);
+                closed = true;
+            }
+            if (!pending && !p_running) break;
+        }
+        cv.wait_for(lk, std::chrono::microseconds(100));
+    }
+    uint64_t it = 0;
+    for (uint32_t g = 0; g < ctl.groups; ++g) it += __atomic_load_n(p_iters + g, __ATOMIC_ACQUIRE);
+    iterations.store(it);
+    BT_TRY(hipStreamSynchronize(stream));
     return MT_OK;
 }
 
@@ -540,7 +797,7 @@ int mt_batch_start(mt_batch* batch) {
     batch->worker = std::thread([batch]() {
         int rc;
         try {
-            rc = batch->run();
+            rc = batch->persistent ? batch->run_persistent() : batch->run();
             if (rc) batch->fail(rc, mt_last_error());
         } catch (const std::bad_alloc&) {
             batch->fail(MT_E_NOMEM, "host allocation failed in the batch scheduler");
@@ -583,7 +840,12 @@ int mt_batch_get_job_info(const mt_batch* batch, uint32_t job_id, mt_batch_job_i
     info->samples = j.samples;
     info->submit_ms = j.submit_ms;
     info->complete_ms = -1.0;
-    if (j.recorded.load(std::memory_order_acquire)) {
+    if (b->persistent) {
+        if (j.finished.load(std::memory_order_acquire)) {
+            info->done = 1;
+            info->complete_ms = j.complete_ms;
+        }
+    } else if (j.recorded.load(std::memory_order_acquire)) {
         bool all = true;
         float latest = 0.0f;
         for (hipEvent_t e : j.done_evs) {
@@ -608,9 +870,15 @@ int mt_batch_get_stats(const mt_batch* batch, mt_batch_stats* stats) {
     std::scoped_lock lk(b->mu);
     std::memset(stats, 0, sizeof(*stats));
     stats->iterations = b->iterations.load();
+    if (b->persistent && b->p_iters) {
+        uint64_t it = 0;  // the device's running counts
+        for (uint32_t g = 0; g < b->ctl.groups; ++g) it += __atomic_load_n(b->p_iters + g, __ATOMIC_ACQUIRE);
+        stats->iterations = it;
+    }
     stats->graphs_captured = b->captures.load();
+    stats->launches = b->launches.load();
     stats->jobs = static_cast<uint32_t>(b->jobs.size());
-    stats->scenes = static_cast<uint32_t>(b->slots.size());
+    stats->scenes = static_cast<uint32_t>(b->persistent ? b->p_slot_keys.size() : b->slots.size());
     for (auto& j : b->jobs) stats->samples += j->samples;
     stats->options = b->opt;
     return MT_OK;
@@ -668,7 +936,10 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
         if (n_trials) *n_trials = std::min(done, max_trials);
         return MT_OK;
     };
-    mt_batch_options cur{1u << 22, 1024, 4096, 8, 3};
+    // streams 0: the mode's default (graphs 3 streams, persistent one lane group per XCD)
+    mt_batch_options cur{1u << 22, 1024, 4096, 8, 0, MT_BATCH_PERSISTENT};
+    uint32_t xcds = 1;
+    if (mt::xcd_count(&xcds)) return MT_E_HIP;
     double cur_v = 0.0;
     int rc = measure(cur, cur_v);
     if (rc) return rc;
@@ -677,12 +948,20 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
     const uint32_t blocks[] = {4, 8, 12};
     const uint32_t itiles[] = {1024, 4096, 16384};
     const uint32_t nstreams[] = {1, 2, 3, 4};
+    const uint32_t ngroups[] = {xcds, 2 * xcds, 4 * xcds};  // persistent: groups within XCDs
+    const uint32_t modes[] = {MT_BATCH_PERSISTENT, MT_BATCH_GRAPHS};
     auto sweep = [&](auto field, const uint32_t* vals, size_t nv) -> int {
         mt_batch_options b0 = cur;
         for (size_t i = 0; i < nv; ++i) {
             mt_batch_options o = b0;
             o.*field = vals[i];
             if (o.*field == b0.*field) continue;
+            // lanes of the mode (0 = its default), and only shapes whose path state fits
+            const uint32_t lanes = o.streams ? o.streams : (o.mode == MT_BATCH_GRAPHS ? 3u : xcds);
+            if (o.mode == MT_BATCH_GRAPHS && lanes > kMaxLanes) continue;
+            if (o.mode != MT_BATCH_GRAPHS && (lanes > kMaxGroups ||
+                                              uint64_t(lanes) * o.paths_per_iteration > kMaxPersistentPaths))
+                continue;
             double v = 0.0;
             const int r = measure(o, v);
             if (r) return r;
@@ -693,11 +972,16 @@ int mt_batch_autotune(mt_scene* const* scenes, const mt_camera_desc* cameras,
         }
         return MT_OK;
     };
-    if ((rc = sweep(&mt_batch_options::paths_per_iteration, paths, 5)) ||
+    // the launch mode first (each trial of the others then refines the better one); the
+    // persistent launch holds as many blocks as the CUs do, so traversal blocks per CU only
+    // matter to the graphs
+    if ((rc = sweep(&mt_batch_options::mode, modes, 2)) ||
+        (rc = sweep(&mt_batch_options::paths_per_iteration, paths, 5)) ||
         (rc = sweep(&mt_batch_options::tile_pixels, tiles, 4)) ||
-        (rc = sweep(&mt_batch_options::trace_blocks_per_cu, blocks, 3)) ||
+        (cur.mode == MT_BATCH_GRAPHS && (rc = sweep(&mt_batch_options::trace_blocks_per_cu, blocks, 3))) ||
         (rc = sweep(&mt_batch_options::iteration_tiles, itiles, 3)) ||
-        (rc = sweep(&mt_batch_options::streams, nstreams, 4)))
+        (rc = cur.mode == MT_BATCH_GRAPHS ? sweep(&mt_batch_options::streams, nstreams, 4)
+                                          : sweep(&mt_batch_options::streams, ngroups, 3)))
         return rc;
     *best = cur;
     return MT_OK;

@@ -142,6 +142,13 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
     }
 }
 
+// a queue size that atomics (wave_push) update: read it with an agent-scope atomic load, which
+// sees every earlier atomic on it even where only L1 is invalidated between the writer and the
+// reader (the persistent batch kernel's barriers)
+__device__ __forceinline__ uint32_t counter(const PathBuffers& b, uint32_t i) {
+    return __hip_atomic_load(b.counters + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------------ batched rendering
 // One iteration of a batch: the scheduler takes tiles from the scene's device queue in order,
 // from its head up to the end the host wrote for this iteration (q_state[1], the same greedy
@@ -206,9 +213,8 @@ __device__ __forceinline__ void coords(const ChunkParams& c, const PathBuffers& 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_generate(PathBuffers b) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= b.counters[kCntPaths]) return;
+// path p (< the iteration's paths) of a batched iteration: its tile, job, pixel and sample
+__device__ __forceinline__ void batch_generate_path(const PathBuffers& b, uint32_t p) {
     const uint4 t = b.it_tiles[tile_of_path(b, b.counters[kCntTiles], p)];
     const ChunkParams& J = b.jobs[t.x].p;
     const uint32_t local = p - t.w;
@@ -219,6 +225,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_generate(PathBuffers b) {
     b.path_pix[p] = pix;
     b.path_smp[p] = s;
     init_path(J, b, p, pix, s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_generate(PathBuffers b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= b.counters[kCntPaths]) return;
+    batch_generate_path(b, p);
 }
 
 // rays of the test hooks (mode 2: closest hits, mode 3: occlusion up to a distance)
@@ -239,21 +251,21 @@ struct TestIO {
 #ifndef MT_TRACE_WAVES_PER_SIMD
 #define MT_TRACE_WAVES_PER_SIMD 6
 #endif
+// The body runs in k_trace (gtid/nthreads over the grid) and in the batch's persistent kernel
+// (over one lane group's blocks); lds_stack holds kDepth * kBlock entries of the block.
 template <bool kCount, int kMode, uint32_t kDepth, int kAccel>
-__global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(DevScene S, ChunkParams c, PathBuffers b,
-                                                  uint32_t bounce, TestIO io) {
-    __shared__ uint2 lds_stack[kDepth * kBlock];
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams& c, const PathBuffers& b,
+                                           uint32_t bounce, const TestIO& io, uint2* lds_stack,
+                                           uint32_t gtid, uint32_t nthreads) {
     Stack st{lds_stack + threadIdx.x, kBlock, kDepth, b.spill + gtid,
-             gridDim.x * blockDim.x, c.stack_cap, b.counters + kCntError, 0};
+             nthreads, c.stack_cap, b.counters + kCntError, 0};
     uint32_t count;
     uint32_t* fetch;
     if (kMode == 0) {
-        count = b.counters[kCntExtend + bounce];
+        count = counter(b, kCntExtend + bounce);
         fetch = b.counters + kCntFetchExt + bounce;
     } else if (kMode == 1) {
-        count = b.counters[kCntShadow + bounce];
+        count = counter(b, kCntShadow + bounce);
         fetch = b.counters + kCntFetchSh + bounce;
     } else {
         count = io.n;
@@ -462,6 +474,15 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
     }
 }
 
+template <bool kCount, int kMode, uint32_t kDepth, int kAccel>
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(DevScene S, ChunkParams c, PathBuffers b,
+                                                  uint32_t bounce, TestIO io) {
+    __shared__ uint2 lds_stack[kDepth * kBlock];
+    trace_body<kCount, kMode, kDepth, kAccel>(S, c, b, bounce, io, lds_stack,
+                                              blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 template <int kMode, int kAccel>
 void launch_trace_as(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, const DevScene& S,
                      const ChunkParams& c, const PathBuffers& b, uint32_t k, TestIO io) {
@@ -633,17 +654,19 @@ __device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const Mat
 #ifndef MT_SHADE_WAVES_PER_SIMD
 #define MT_SHADE_WAVES_PER_SIMD 4
 #endif
+// blk / nblk: this block and the blocks sharing the work (the grid, or one lane group of the
+// batch's persistent kernel)
 template <int kDiff, bool kBatch, bool kExt, bool kLights>
 __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams& c, const PathBuffers& b,
-                                           uint32_t k) {
-    const uint32_t count = b.counters[kCntExtend + k];
+                                           uint32_t k, uint32_t blk, uint32_t nblk) {
+    const uint32_t count = counter(b, kCntExtend + k);
     const uint32_t* qin = b.queue[k & 1];
     uint32_t* qout = b.queue[(k + 1) & 1];
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t stride = nblk * blockDim.x;
     uint32_t hits = 0, misses = 0;
     const DiffParams& dp = c.dp;
     const DiffGradPtrs& gr = c.gr;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    for (uint32_t base = blk * blockDim.x; base < count; base += stride) {
         const uint32_t idx = base + threadIdx.x;
         const bool valid = idx < count;
         bool cont = false, shadow = false;
@@ -1016,14 +1039,14 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
 template <int kDiff, bool kBatch, bool kExt, bool kLights = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT_SHADE_WAVES_PER_SIMD, 8)))
 void k_shade(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
-    shade_body<kDiff, kBatch, kExt, kLights>(S, c, b, k);
+    shade_body<kDiff, kBatch, kExt, kLights>(S, c, b, k, blockIdx.x, gridDim.x);
 }
 
 // plain shading of a scene without point lights: small enough to run five waves per SIMD
 template <bool kBatch>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void k_shade_unlit(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
-    shade_body<0, kBatch, false, false>(S, c, b, k);
+    shade_body<0, kBatch, false, false>(S, c, b, k, blockIdx.x, gridDim.x);
 }
 
 // RandomColorFromU16 (Tracer.cpp:350-364)
@@ -1076,9 +1099,7 @@ __device__ V3 debug_color(const ChunkParams& c, const PathBuffers& b, uint32_t p
 }
 
 template <bool kBatch>
-__global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers b) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (kBatch ? b.counters[kCntPaths] : c.paths)) return;
+__device__ __forceinline__ void finalize_path(const ChunkParams& c, const PathBuffers& b, uint32_t p) {
     const ChunkParams& J = job_params<kBatch>(c, b, p);
     const V3 e = xyz(b.acc_e[p]), d = xyz(b.acc_d[p]), i = xyz(b.acc_i[p]), s = xyz(b.acc_s[p]);
     const V3 L = add(add(add(e, d), i), s);  // Tracer.cpp:344-345
@@ -1100,6 +1121,13 @@ __global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers 
         path_coords(c, p, pix, s);
         c.path_store[static_cast<size_t>(pix) * c.spp + s] = make_float4(L.x, L.y, L.z, __uint_as_float(b.vis[p]));
     }
+}
+
+template <bool kBatch>
+__global__ __launch_bounds__(kBlock) void k_finalize(ChunkParams c, PathBuffers b) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (kBatch ? b.counters[kCntPaths] : c.paths)) return;
+    finalize_path<kBatch>(c, b, p);
 }
 
 // per-pixel mean of the samples sp[0..n) in sample order, with CleanupFireflies
@@ -1156,16 +1184,256 @@ __global__ __launch_bounds__(kBlock) void k_resolve(ChunkParams c, PathBuffers b
     resolve_pixel(c, b.sample_rgb + static_cast<size_t>(lp) * c.spp, img + (c.pix_begin + lp) * 3ull);
 }
 
-// batched: one thread per pixel of the iteration, written into its job's image
-__global__ __launch_bounds__(kBlock) void k_batch_resolve(PathBuffers b) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= b.counters[kCntPixels]) return;
+// batched: pixel q of the iteration, written into its job's image
+__device__ __forceinline__ void batch_resolve_pixel(const PathBuffers& b, uint32_t q) {
     const uint32_t ti = tile_of_pixel(b, b.counters[kCntTiles], q);
     const uint4 t = b.it_tiles[ti];
     const BatchJob& job = b.jobs[t.x];
     const uint32_t lp = q - b.it_pix_off[ti];
     resolve_pixel(job.p, b.sample_rgb + t.w + static_cast<size_t>(lp) * job.p.spp,
                   job.img + (t.y + lp) * 3ull);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_resolve(PathBuffers b) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= b.counters[kCntPixels]) return;
+    batch_resolve_pixel(b, q);
+}
+
+// ------------------------------------------------------------------ one persistent launch
+// The grid is split into lane groups of group_blocks consecutive blocks, each with its own
+// path state; every group loops: its first block takes tiles from the shared queue (scene of
+// the first tile, in queue order while their paths fit), then the group runs generate, the
+// bounces (extend, shade, shadow), finalize and resolve of that iteration with a group
+// barrier between phases, and marks the jobs whose last tile it resolved.  The launch is
+// cooperative, so all blocks are resident and the barriers can be waited on.  Every wait
+// ends: a group exits when the host has closed the queue and it is drained, or after
+// idle_ticks without tiles, and a barrier that waits longer than stall_ticks sets the abort
+// word, on which every waiting block returns.
+constexpr uint32_t kSelNone = 0xFFFFFFFFu;  // no tiles yet
+constexpr uint32_t kSelExit = 0xFFFFFFFEu;  // the group leaves the kernel
+
+// polls are relaxed loads (an acquire load invalidates the cache hierarchy on every poll,
+// evicting the BVH the other lane groups are traversing); one acquire fence follows the wait
+__device__ __forceinline__ uint32_t load_sys(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the XCD (accelerator complex die) this wave runs on
+__device__ __forceinline__ uint32_t xcc_id() {
+    uint32_t v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xfu;
+}
+
+// barrier of one lane group's blocks; false when the launch is aborting (the caller returns).
+// Release: every wave of the block makes its own stores visible at agent scope (L2 write-back)
+// before the block barrier (__syncthreads waits only for LDS and scalar operations), then one
+// thread arrives.  Acquire: that thread invalidates L1 and L2 and waits for the invalidation
+// before the block's other waves go on.  The wait polls with relaxed loads: an acquire load
+// would invalidate the caches on every poll, evicting what the other groups are working on.
+__device__ bool group_sync(const PersistCtl& ctl, uint32_t* bar, uint32_t* flag) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1;
+        const uint32_t gen = load_agent(bar + 1);
+        if (atomicAdd(bar, 1u) == ctl.group_blocks - 1u) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar + 1, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while (load_agent(bar + 1) == gen) {
+                if (load_agent(ctl.state + kPsAbort)) {
+                    ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > ctl.stall_ticks) {
+                    atomicExch(ctl.state + kPsAbort, 1u);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(16);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        *flag = ok;
+    }
+    __syncthreads();
+    return *flag != 0;
+}
+
+// 16-byte words [first, end) of the published host records into the device copies
+__device__ __forceinline__ void copy_records(uint4* dst, const uint4* src, uint64_t first, uint64_t end) {
+    for (uint64_t i = first + threadIdx.x; i < end; i += kBlock) dst[i] = src[i];
+}
+
+// the group's first block: copy what the host published since the last look, then take the
+// next iteration's tiles; returns the scene slot, or kSelExit
+__device__ uint32_t persist_take(const PersistCtl& ctl, const PathBuffers& b, uint32_t* sh) {
+    const uint64_t t0 = wall_clock64();
+    constexpr uint32_t kJobWords = sizeof(BatchJob) / 16, kSlotWords = sizeof(PersistSlot) / 16;
+    static_assert(sizeof(BatchJob) % 16 == 0 && sizeof(PersistSlot) % 16 == 0, "record layout");
+    while (true) {
+        if (threadIdx.x == 0) {
+            uint32_t ok = 1;
+            while (atomicCAS(ctl.state + kPsLock, 0u, 1u) != 0u) {
+                if (load_agent(ctl.state + kPsAbort)) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+            __threadfence();
+            sh[0] = ok;
+            if (ok) {
+                // the tail first: the host publishes it after the job and slot counts, so
+                // every tile read below has its job's record and scene slot
+                sh[5] = ctl.state[kPsTiles];
+                sh[6] = load_sys(&ctl.host->tail);
+                sh[3] = ctl.state[kPsJobs];
+                sh[4] = load_sys(&ctl.host->jobs);
+                sh[1] = ctl.state[kPsSlots];
+                sh[2] = load_sys(&ctl.host->slots);
+                __atomic_thread_fence(__ATOMIC_ACQUIRE);  // system scope: the records behind the counts
+            }
+        }
+        __syncthreads();
+        if (!sh[0]) return kSelExit;
+        copy_records(reinterpret_cast<uint4*>(ctl.slots), reinterpret_cast<const uint4*>(ctl.host_slots),
+                     uint64_t(sh[1]) * kSlotWords, uint64_t(sh[2]) * kSlotWords);
+        copy_records(reinterpret_cast<uint4*>(ctl.jobs), reinterpret_cast<const uint4*>(ctl.host_jobs),
+                     uint64_t(sh[3]) * kJobWords, uint64_t(sh[4]) * kJobWords);
+        for (uint32_t j = sh[3] + threadIdx.x; j < sh[4]; j += kBlock) {
+            ctl.job_slot[j] = ctl.host_job_slot[j];
+            ctl.tiles_left[j] = ctl.host_job_tiles[j];
+        }
+        copy_records(ctl.tiles, ctl.host_tiles, sh[5], sh[6]);
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ctl.state[kPsSlots] = sh[2];
+            ctl.state[kPsJobs] = sh[4];
+            ctl.state[kPsTiles] = sh[6];
+            uint32_t head = ctl.state[kPsHead];
+            uint32_t n = 0, paths = 0, npix = 0, slot = kSelNone;
+            while (head < sh[6] && n < b.it_tile_cap) {
+                const uint4 t = ctl.tiles[head];
+                const uint32_t s = ctl.job_slot[t.x];
+                if (n > 0 && (s != slot || paths + t.w > b.it_capacity)) break;
+                slot = s;
+                b.it_tiles[n] = make_uint4(t.x, t.y, t.z, paths);
+                b.it_pix_off[n] = npix;
+                paths += t.w;
+                npix += t.z;
+                ++n;
+                ++head;
+            }
+            b.it_pix_off[n] = npix;
+            ctl.state[kPsHead] = head;
+            __threadfence();
+            __hip_atomic_store(ctl.state + kPsLock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t sel = kSelNone;
+            if (n > 0) {
+                for (uint32_t k = 0; k < kMaxCounters; ++k) b.counters[k] = 0;
+                b.counters[kCntExtend] = paths;
+                b.counters[kCntPaths] = paths;
+                b.counters[kCntPixels] = npix;
+                b.counters[kCntTiles] = n;
+                sel = slot;
+            } else if (load_sys(&ctl.host->closed) && load_sys(&ctl.host->tail) == sh[6]) {
+                sel = kSelExit;  // closed after its last tiles, and every tile taken
+            } else if (wall_clock64() - t0 > ctl.idle_ticks) {
+                sel = kSelExit;
+            }
+            sh[7] = sel;
+        }
+        __syncthreads();
+        if (sh[7] != kSelNone) return sh[7];
+        if (threadIdx.x == 0)
+            for (int i = 0; i < 16; ++i) __builtin_amdgcn_s_sleep(127);
+        __syncthreads();
+    }
+}
+
+template <uint32_t kDepth>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_batch_persistent(PersistCtl ctl) {
+    __shared__ uint2 lds_stack[kDepth * kBlock];
+    __shared__ uint32_t sh[8];
+    uint32_t g, blk;
+    if (ctl.xcds) {
+        // groups within XCDs: the hardware deals blocks round-robin over the XCDs, so block b
+        // is the (b / xcds)-th block of XCD b % xcds; each XCD's blocks are dealt over its
+        // groups.  A group's barriers then write back and invalidate only the L2 its own
+        // blocks use, not the other groups'.  (The placement affects speed only: the barriers
+        // are agent-scope either way.)
+        const uint32_t x = blockIdx.x % ctl.xcds, w = blockIdx.x / ctl.xcds;
+        const uint32_t gpx = ctl.groups / ctl.xcds;
+        g = x * gpx + w % gpx;
+        blk = w / gpx;
+    } else {
+        g = blockIdx.x / ctl.group_blocks;
+        blk = blockIdx.x - g * ctl.group_blocks;
+    }
+    if (g >= ctl.groups || blk >= ctl.group_blocks) return;  // the grid is groups * group_blocks blocks
+    const PathBuffers& b = ctl.lanes[g];
+    uint32_t* bar = ctl.bar + g * kPsBarWords;
+    const uint32_t nblk = ctl.group_blocks;
+    const uint32_t nthreads = nblk * kBlock;
+    const uint32_t gtid = blk * kBlock + threadIdx.x;
+    const TestIO io{};
+    while (true) {
+        if (blk == 0) {
+            const uint32_t sel = persist_take(ctl, b, sh);
+            if (threadIdx.x == 0) bar[2] = sel;
+        }
+        if (!group_sync(ctl, bar, sh)) return;
+        const uint32_t slot = __builtin_amdgcn_readfirstlane(bar[2]);
+        if (slot == kSelExit) return;
+        const PersistSlot& P = ctl.slots[slot];
+        const bool lights = __builtin_amdgcn_readfirstlane(P.lights) != 0;
+        const bool fast = __builtin_amdgcn_readfirstlane(P.accel) != 0;
+        const uint32_t paths = __builtin_amdgcn_readfirstlane(b.counters[kCntPaths]);
+        for (uint32_t p = gtid; p < paths; p += nthreads) batch_generate_path(b, p);
+        if (!group_sync(ctl, bar, sh)) return;
+        for (uint32_t k = 0; k < kMaxBounces; ++k) {
+            if (__builtin_amdgcn_readfirstlane(counter(b, kCntExtend + k)) == 0) break;
+            if (fast) trace_body<false, 0, kDepth, 1>(P.S, P.c, b, k, io, lds_stack, gtid, nthreads);
+            else trace_body<false, 0, kDepth, 0>(P.S, P.c, b, k, io, lds_stack, gtid, nthreads);
+            if (!group_sync(ctl, bar, sh)) return;
+            if (lights) shade_body<0, true, false, true>(P.S, P.c, b, k, blk, nblk);
+            else shade_body<0, true, false, false>(P.S, P.c, b, k, blk, nblk);
+            if (!group_sync(ctl, bar, sh)) return;
+            if (lights) {
+                if (fast) trace_body<false, 1, kDepth, 1>(P.S, P.c, b, k, io, lds_stack, gtid, nthreads);
+                else trace_body<false, 1, kDepth, 0>(P.S, P.c, b, k, io, lds_stack, gtid, nthreads);
+                if (!group_sync(ctl, bar, sh)) return;
+            }
+        }
+        for (uint32_t p = gtid; p < paths; p += nthreads) finalize_path<true>(P.c, b, p);
+        if (!group_sync(ctl, bar, sh)) return;
+        const uint32_t npix = __builtin_amdgcn_readfirstlane(b.counters[kCntPixels]);
+        for (uint32_t q = gtid; q < npix; q += nthreads) batch_resolve_pixel(b, q);
+        if (!group_sync(ctl, bar, sh)) return;
+        if (blk == 0 && threadIdx.x == 0) {
+            // the jobs whose last tile this iteration resolved are final
+            const uint32_t n = b.counters[kCntTiles];
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t job = b.it_tiles[i].x;
+                if (atomicSub(ctl.tiles_left + job, 1u) == 1u)
+                    __hip_atomic_store(ctl.job_done + job, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            // only this thread writes the group's count: it survives relaunches
+            const unsigned long long it = __hip_atomic_load(ctl.host_iters + g, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(ctl.host_iters + g, it + 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ radiance cache passes
@@ -1541,6 +1809,10 @@ int scratch_address(int dev, unsigned char** out) {
 }
 }  // namespace
 
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+}
+
 int trace_rays_noalloc(mt_scene* s, uint32_t n, const float* rays_dev, mt_hit* hits_dev,
                        int32_t traversal, hipStream_t stream) {
     if (traversal != MT_TRAVERSAL_REFERENCE && traversal != MT_TRAVERSAL_FAST)
@@ -1741,6 +2013,53 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v) {
 
 void launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, p, v);
+}
+
+int persistent_blocks_per_cu(uint32_t* blocks) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_persistent<kPersistDepth>, kBlock, 0) !=
+            hipSuccess || n <= 0)
+        return fail(MT_E_HIP, "occupancy query of the persistent batch kernel failed");
+    *blocks = static_cast<uint32_t>(n);
+    return MT_OK;
+}
+
+int launch_batch_persistent(const PersistCtl& ctl, hipStream_t s) {
+    void* args[] = {const_cast<PersistCtl*>(&ctl)};
+    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_batch_persistent<kPersistDepth>),
+                                                    dim3(ctl.groups * ctl.group_blocks), dim3(kBlock), args, 0, s);
+    if (e != hipSuccess)
+        return fail(MT_E_HIP, std::string("cooperative launch of the persistent batch kernel failed: ") +
+                                  hipGetErrorString(e));
+    return MT_OK;
+}
+
+__global__ void k_xcc_ids(uint32_t* out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = xcc_id();
+}
+
+int xcd_count(uint32_t* xcds) {
+    const uint32_t n = static_cast<uint32_t>(device_cu_count()) * 4u;
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, sizeof(uint32_t) * n) != hipSuccess) return fail(MT_E_NOMEM, "XCD probe allocation failed");
+    std::vector<uint32_t> h(n);
+    hipLaunchKernelGGL(k_xcc_ids, dim3(n), dim3(64), 0, nullptr, d);
+    const hipError_t e = hipMemcpy(h.data(), d, sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(MT_E_HIP, "XCD probe failed");
+    uint32_t mx = 0;
+    for (uint32_t v : h) mx = std::max(mx, v);
+    *xcds = mx + 1u;
+    return MT_OK;
+}
+
+int wall_ticks_per_ms(uint64_t* ticks) {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+        return fail(MT_E_HIP, "wall clock rate query failed");
+    *ticks = static_cast<uint64_t>(khz);
+    return MT_OK;
 }
 
 int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, uint64_t spill_entries,

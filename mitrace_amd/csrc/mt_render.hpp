@@ -135,7 +135,7 @@ struct PathBuffers {
 };
 
 // one job of a batch, resident in device memory
-struct BatchJob {
+struct alignas(16) BatchJob {
     ChunkParams p;  // the job's settings and camera (pix_begin/npix/paths unused)
     float* img;     // width * height * 3, device
 };
@@ -207,6 +207,70 @@ class Renderer {
     mt_stats stats_{};
     std::vector<StreamSlot> slots_;  // slots_[0] uses buf_, stream_, host_counters_, events_
 };
+
+// ---- one persistent launch for a whole batch (mt_batch_options.mode = MT_BATCH_PERSISTENT)
+// The host publishes scene slots, job records and tiles in coherent pinned memory and bumps
+// the published counts last; the kernel's lane groups copy what is new into device memory
+// under the queue lock, take tiles in queue order and run whole wavefront iterations between
+// group barriers.
+struct alignas(16) PersistSlot {  // a scene and traversal: what an iteration's kernels need
+    DevScene S;
+    ChunkParams c;           // scene-level launch constants (per-job ones: the job table)
+    uint32_t lights;         // point lights: lit shading and shadow rays
+    uint32_t accel;          // 1 = fast traversal
+    uint32_t pad[2];
+};
+struct PersistHost {         // pinned, written by the host (published counts, then closed)
+    uint32_t tail;           // tiles published
+    uint32_t jobs;           // job records published
+    uint32_t slots;          // scene slots published
+    uint32_t closed;         // 1: nothing more will be published; groups exit once drained
+};
+// device state words (PersistCtl::state)
+constexpr uint32_t kPsHead = 0;    // next tile to take
+constexpr uint32_t kPsLock = 1;    // queue lock (group leaders)
+constexpr uint32_t kPsAbort = 2;   // set when a barrier waited past stall_ticks
+constexpr uint32_t kPsTiles = 3;   // tiles copied to the device queue
+constexpr uint32_t kPsJobs = 4;    // job records copied
+constexpr uint32_t kPsSlots = 5;   // scene slots copied
+constexpr uint32_t kPsWords = 16;
+constexpr uint32_t kPsBarWords = 64;  // per lane group: {arrived, generation, selection}, own 256 B
+struct PersistCtl {
+    const PersistHost* host;
+    const uint4* host_tiles;         // {job, first pixel, pixels, paths}
+    const BatchJob* host_jobs;
+    const uint32_t* host_job_slot;   // per job: its scene slot
+    const uint32_t* host_job_tiles;  // per job: its tile count
+    const PersistSlot* host_slots;
+    uint32_t* job_done;              // pinned, written by the device: 1 once the job's image is final
+    unsigned long long* host_iters;  // pinned, written by the device: iterations per lane group
+    uint4* tiles;                    // device copies of the published records
+    BatchJob* jobs;
+    uint32_t* job_slot;
+    uint32_t* tiles_left;            // per job: tiles not yet resolved
+    PersistSlot* slots;
+    uint32_t* state;                 // kPs* words
+    uint32_t* bar;                   // kPsBarWords per lane group
+    const PathBuffers* lanes;        // per lane group
+    uint32_t groups, group_blocks;
+    // xcds > 0: lane groups within XCDs (groups a multiple of xcds; the hardware deals blocks
+    // round-robin over the XCDs, block b to XCD b % xcds), so that a group's barrier fences
+    // touch only its own XCD's L2.  0: groups of consecutive blocks (spanning XCDs).
+    uint32_t xcds;
+    uint64_t idle_ticks;             // a group exits after this long without tiles (wall clock)
+    uint64_t stall_ticks;            // a barrier wait this long aborts the launch
+};
+constexpr uint32_t kPersistDepth = 16;  // traversal stack entries in LDS in the persistent kernel
+// blocks of the persistent kernel one CU holds (occupancy of its registers and LDS)
+int persistent_blocks_per_cu(uint32_t* blocks);
+// cooperative launch of groups * group_blocks blocks; fails instead of launching a grid whose
+// blocks could not all be resident (the group barriers need every block running)
+int launch_batch_persistent(const PersistCtl& ctl, hipStream_t s);
+// wall-clock ticks per millisecond of the device's constant-rate counter
+int wall_ticks_per_ms(uint64_t* ticks);
+// XCDs (accelerator complex dies) of the current device, from the XCC_ID of a grid of blocks
+int xcd_count(uint32_t* xcds);
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);
 
 // launch shape of one batched iteration
 struct BatchLaunch {

@@ -15,6 +15,9 @@ if ROOT not in sys.path:
 TESTS = os.path.dirname(os.path.abspath(__file__))
 if TESTS not in sys.path:
     sys.path.insert(0, TESTS)
+# a persistent batch launch whose lane-group barrier waits this long gives up with an error
+# (the library's default is 60 s; the test scenes need milliseconds)
+os.environ.setdefault("MT_BATCH_STALL_MS", "20000")
 
 
 def pytest_configure(config):

@@ -1,6 +1,8 @@
-"""Batched rendering (mt_batch_*): every job's image equals the single-job render with the same
-scene, camera and settings, bit for bit, whatever the batch's tiling and iteration size, and
-jobs submitted while the batch runs complete after their submission."""
+"""Batched rendering (mt_batch_*), both launch modes: every job's image equals the single-job
+render with the same scene, camera and settings, bit for bit, whatever the batch's tiling and
+iteration size, and jobs submitted while the batch runs complete after their submission.  The
+persistent mode serves a batch with one kernel launch (a new one only after the queue stayed
+empty for its idle time), the graph mode with a captured graph per iteration."""
 import time
 
 import numpy as np
@@ -35,14 +37,34 @@ def _jobs():
     ]
 
 
+def _check_launches(b, mode):
+    from mitrace_amd.batch import PERSISTENT
+    s = b.stats()
+    if mode == PERSISTENT:
+        assert s["launches"] == 1 and s["graphs_captured"] == 0, s
+    else:
+        assert s["launches"] == 0 and s["graphs_captured"] >= 1, s
+    return s
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["persistent", "graphs"])
 @pytest.mark.parametrize("options", [
     {"paths_per_iteration": 1 << 14, "tile_pixels": 16, "iteration_tiles": 64},
     {"paths_per_iteration": 1 << 20, "tile_pixels": 1024, "iteration_tiles": 4096,
      "trace_blocks_per_cu": 4},
+    {"paths_per_iteration": 1 << 16, "tile_pixels": 256, "streams": 1},
+    {"paths_per_iteration": 1 << 15, "tile_pixels": 64, "streams": 16},
 ])
-def test_batch_images_equal_single_job_renders(scenes, options):
-    from mitrace_amd.batch import Batch
-    b = Batch(options)
+def test_batch_images_equal_single_job_renders(scenes, options, mode):
+    from mitrace_amd._lib import MiTraceError
+    from mitrace_amd.batch import GRAPHS, Batch
+    print(f"batch mode {mode} options {options}", flush=True)
+    if mode == GRAPHS and options.get("streams", 3) > 8:
+        with pytest.raises(MiTraceError) as e:  # the graph mode has at most 8 streams
+            Batch(dict(options, mode=mode))
+        assert e.value.code == abi.MT_E_UNSUPPORTED
+        return
+    b = Batch(dict(options, mode=mode))
     ids = []
     for name, st in _jobs():
         ids.append(b.submit(scenes[name], st))
@@ -54,14 +76,16 @@ def test_batch_images_equal_single_job_renders(scenes, options):
         info = b.info(j)
         assert info["done"] and info["complete_ms"] >= info["submit_ms"]
         assert info["samples"] == st.width * st.height * st.samples
-    s = b.stats()
+    s = _check_launches(b, mode)
     assert s["jobs"] == len(ids) and s["scenes"] == 4 and s["iterations"] >= 4
     b.close()
 
 
-def test_jobs_submitted_while_running(scenes):
+@pytest.mark.parametrize("mode", [1, 2], ids=["persistent", "graphs"])
+def test_jobs_submitted_while_running(scenes, mode):
     from mitrace_amd.batch import Batch
-    b = Batch({"paths_per_iteration": 1 << 16, "tile_pixels": 64})
+    print(f"batch mode {mode}", flush=True)
+    b = Batch({"paths_per_iteration": 1 << 16, "tile_pixels": 64, "mode": mode})
     b.start()
     ids = []
     for k, (name, st) in enumerate(_jobs()):
@@ -73,6 +97,30 @@ def test_jobs_submitted_while_running(scenes):
         np.testing.assert_array_equal(b.image(j), ref)
         info = b.info(j)
         assert info["done"] and info["complete_ms"] >= info["submit_ms"]
+    _check_launches(b, mode)
+    b.close()
+
+
+def test_persistent_launch_ends_when_idle_and_restarts(scenes):
+    """after the queue stays empty for the idle time (1 s) the launch ends; a later submission
+    starts a second launch, which continues the same queue"""
+    from mitrace_amd.batch import Batch
+    b = Batch({"paths_per_iteration": 1 << 16, "tile_pixels": 64})
+    b.start()
+    jobs = _jobs()[:3]
+    ids = [b.submit(scenes[jobs[0][0]], jobs[0][1])]
+    t0 = time.perf_counter()
+    while not b.info(ids[0])["done"]:
+        assert time.perf_counter() - t0 < 60
+        time.sleep(0.01)
+    time.sleep(2.0)
+    for name, st in jobs[1:]:
+        ids.append(b.submit(scenes[name], st))
+    b.finish()
+    for (name, st), j in zip(jobs, ids):
+        ref, _, _ = scenes[name].render(st)
+        np.testing.assert_array_equal(b.image(j), ref)
+    assert b.stats()["launches"] == 2
     b.close()
 
 

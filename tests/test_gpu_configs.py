@@ -180,6 +180,8 @@ def test_batched_workload_64_jobs_equal_single_renders(gpu):
     assert len(jobs) == 64
     rep, images = B.run_stream(gs, fns, jobs, want_images=True)
     assert rep["jobs"] == 64
+    # served by one persistent launch (the default mode)
+    assert rep["stats"]["launches"] == 1 and rep["stats"]["graphs_captured"] == 0, rep["stats"]
     for j, img in zip(jobs, images):
         st, cam = B.job_settings(j, fns[j["scene"]], gs[j["scene"]].scene)
         ref, _, _ = gs[j["scene"]].render(st, camera=cam)
