@@ -274,6 +274,29 @@ def batch_leg(rate_hz=16.0):
     return rep
 
 
+def inverse_loop_timing(iters=8):
+    """ms per iteration of the inverse-rendering loop as it runs (8 poses at 512x512, each a
+    32-spp forward for the residual plus a 32-spp forward and full adjoint through the torch
+    autograd wrapper, then the Adam step), median over `iters` iterations.  It runs in a child
+    process (the loop uses torch; this script does not), with 64-spp targets and no image
+    evaluation, which are outside the timed iterations."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "inverse.json")
+        r = subprocess.run([sys.executable, "-m", "mitrace_amd.inverse", "optimize", "--iters", str(iters),
+                            "--tex-lr", "1e-3", "--target-spp", "64", "--eval-spp", "0", "--json", out],
+                           cwd=os.path.dirname(os.path.abspath(__file__)), capture_output=True, text=True,
+                           timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("inverse-rendering loop failed: " + r.stderr[-2000:])
+        with open(out) as f:
+            rep = json.load(f)
+    return {"workload": "inverse rendering: 8 poses at 512x512, 32 spp, residual forward + forward "
+                        "+ full adjoint per pose, Adam", "iters_timed": iters,
+            "ms_per_iter": rep["ms_per_iter"], "poses": rep["poses"]}
+
+
 def large_cpu(sc, settings_fn, seconds, threads, tlas):
     """The oracle (release flags, mt19937 per 32x32 tile, faithful per-ray inverse) on a band of
     rows of the 1080p frame at 1 spp, grown until about `seconds` of CPU time; with `tlas` the
@@ -521,6 +544,7 @@ def main():
               f"{line['batch']['completion_ms_p99']:.0f} ms", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_timing()
+        line["adjoint"]["inverse_loop"] = inverse_loop_timing()
         print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(scene, settings, args.cpu_seconds)

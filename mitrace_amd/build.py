@@ -1,4 +1,5 @@
-"""Builds the gfx950 Tracer library ``mitrace_amd/lib/libmitrace_gpu.so`` with hipcc.
+"""Builds the gfx950 Tracer library ``mitrace_amd/lib/libmitrace_gpu.so`` with hipcc, and the
+command-line renderer ``mitrace_amd/bin/mitrace_gpu_cli`` over it with g++.
 
 Every translation unit is compiled with ``-ffp-contract=off`` (the traversal and shading
 arithmetic follow a fixed operation order, see mt_device.hpp) and with packed-FP32 code
@@ -60,7 +61,29 @@ def build(verbose=False, jobs=4, extra_flags=(), lib_path=None):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, out_lib)
+    if lib_path is None:
+        build_cli(verbose)
     return out_lib
+
+
+BINDIR = os.path.join(HERE, "bin")
+CLI = os.path.join(BINDIR, "mitrace_gpu_cli")
+
+
+def build_cli(verbose=False):
+    """The command-line renderer (cli/mitrace_gpu_cli.cpp): g++ over the C++ host headers,
+    linked against the library (found next to it at run time through its rpath)."""
+    os.makedirs(BINDIR, exist_ok=True)
+    include = os.path.join(os.path.dirname(HERE), "include")
+    tmp = CLI + ".tmp"
+    cmd = [shutil.which("g++") or "g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-Werror",
+           "-I", include, os.path.join(HERE, "cli", "mitrace_gpu_cli.cpp"), "-o", tmp,
+           "-L", LIBDIR, "-lmitrace_gpu", "-Wl,-rpath,$ORIGIN/../lib", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -62,7 +62,7 @@ def targets(gs, truth, cams, spp=1024, width=512, height=512):
 
 
 def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024, decorrelated=True,
-             tex_lr=None):
+             tex_lr=None, eval_spp=128):
     """decorrelated: the residual (image - target) comes from an independent render (another
     seed) than the one whose adjoint is taken.  With one render for both, the gradient of the
     squared error carries 2 Cov(I, dI/dtheta), which for a base colour (I roughly linear in it)
@@ -91,8 +91,11 @@ def optimize(iters=500, spp=32, lr=1e-2, log_every=10, target_spp=1024, decorrel
             return (float(torch.sqrt(torch.mean((tex - gt["tex"]) ** 2))),
                     float(torch.mean(torch.abs(rough[:49] - gt["rough"][:49]))))
 
-    def image_rmse(spp_eval=128):
-        """RMSE of the 8 poses rendered with the current parameters against the targets"""
+    def image_rmse(spp_eval=eval_spp):
+        """RMSE of the 8 poses rendered with the current parameters against the targets
+        (None when eval_spp is 0)"""
+        if not spp_eval:
+            return None
         ev = DiffRenderer(gs, scenegen.inverse_settings(samples=spp_eval), truth["tex_first"],
                           truth["tex_count"])
         with torch.no_grad():
@@ -242,11 +245,15 @@ def main(argv=None):
     ap.add_argument("--crn-spp", type=int, default=4096)
     ap.add_argument("--json", default=None)
     ap.add_argument("--tex-lr", type=float, default=None, help="Adam step for the texture")
+    ap.add_argument("--target-spp", type=int, default=1024, help="samples of the target images")
+    ap.add_argument("--eval-spp", type=int, default=128,
+                    help="samples of the image-RMSE evaluation before and after (0: none)")
     ap.add_argument("--correlated", action="store_true",
                     help="one render per pose for both the residual and the adjoint")
     a = ap.parse_args(argv)
     if a.mode == "optimize":
-        out = optimize(iters=a.iters, spp=a.spp, decorrelated=not a.correlated, tex_lr=a.tex_lr)
+        out = optimize(iters=a.iters, spp=a.spp, decorrelated=not a.correlated, tex_lr=a.tex_lr,
+                       target_spp=a.target_spp, eval_spp=a.eval_spp)
     else:
         out = gradcheck(crn_spp=a.crn_spp)
     summary = {k: v for k, v in out.items() if k not in ("log", "rows")}
