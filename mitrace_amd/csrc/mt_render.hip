@@ -97,7 +97,7 @@ __device__ __forceinline__ void init_path(const ChunkParams& J, const PathBuffer
     }
     b.ray_o[0][p] = f4(o, 0.0f);  // queue 0 is the identity (below)
     b.ray_d[0][p] = f4(d, 0.0f);
-    b.t_rad[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.t_rad[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);  // queue slot p at bounce 0
     b.t_term[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     b.prev_base[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -671,12 +671,12 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
         const bool valid = idx < count;
         bool cont = false, shadow = false;
         V3 next_o = mk(0.0f), next_d = mk(0.0f);  // the continuing path's next ray
-        V3 next_term = mk(0.0f), next_base = mk(0.0f);
+        V3 next_term = mk(0.0f), next_base = mk(0.0f), next_trad = mk(0.0f);
         float4 sho{}, shd{}, shc{};
         uint32_t p = 0;
         // cache query of a path that stops here
         bool query = false;
-        float4 qpos{}, qdir{};
+        float4 qpos{}, qdir{}, qthr{};  // and the throughput at the query vertex
         // adjoint addends for the per-material wave reductions
         uint32_t gmat = 0;
         float g_r = 0.0f, g_m = 0.0f;
@@ -702,7 +702,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             const V3 d = xyz(b.ray_d[k & 1][idx]);
             const uint4 hrec = b.hit_rec[idx];
             const int inst = static_cast<int>(hrec.x);
-            const V3 trad = xyz(b.t_rad[p]);
+            const V3 trad = xyz(b.t_rad[k & 1][idx]);
             if (kDiff == 0 && !kBatch && kExt && c.cache && c.cache_after && k > c.cache_depth) {
                 // the training record of the previous vertex: everything collected up to and
                 // including its light sample (the rest is what the bounce ray brought)
@@ -747,6 +747,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 query = true;
                 qpos = b.hit_pos[idx];
                 qdir = f4(neg(d), 0.0f);
+                qthr = f4(trad, 0.0f);
             } else {
                 hits++;
                 if (kDiff == 0 && !kBatch && kExt && c.cache && k >= c.cache_depth) {
@@ -984,6 +985,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     query = true;
                     qpos = hp;
                     qdir = f4(view, 0.0f);
+                    qthr = f4(trad, 0.0f);
                 }
                 if (after_q && b.train_slot[p] == kNoSlot) {
                     // stops here
@@ -991,7 +993,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     cont = true;
                     next_o = add(add(o, scale(d, dist)), scale(N, kPullout));  // stored below
                     next_d = nd;
-                    b.t_rad[p] = f4(trad_next, 0.0f);
+                    next_trad = trad_next;
                     next_term = tterm;
                     next_base = basec;
                 }
@@ -1018,6 +1020,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     b.cq_path[qs] = p;
                     b.cq_pos[qs] = qpos;
                     b.cq_dir[qs] = qdir;
+                    b.cq_t[qs] = qthr;
                 }
             }
         }
@@ -1027,6 +1030,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             b.ray_o[(k + 1) & 1][cs] = f4(next_o, 0.0f);
             b.ray_d[(k + 1) & 1][cs] = f4(next_d, 0.0f);
             b.t_term[(k + 1) & 1][cs] = f4(next_term, 0.0f);
+            b.t_rad[(k + 1) & 1][cs] = f4(next_trad, 0.0f);
             b.prev_base[(k + 1) & 1][cs] = f4(next_base, 0.0f);
         }
     }
@@ -1444,7 +1448,7 @@ __global__ __launch_bounds__(kBlock) void k_cache_apply(ChunkParams c, PathBuffe
     const uint32_t p = b.cq_path[q];
     const float4 o = b.cq_out[q];
     const V3 pred = {gmax(o.x, 0.0f), gmax(o.y, 0.0f), gmax(o.z, 0.0f)};
-    const V3 add_ = mul(xyz(b.t_rad[p]), pred);
+    const V3 add_ = mul(xyz(b.cq_t[q]), pred);
     if (c.cache_after && b.train_slot[p] < c.train_cap) {
         // a training path keeps its full radiance for its target; its image value is formed
         // in k_finalize from the radiance up to the query vertex and this estimate
@@ -2067,7 +2071,8 @@ int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, ui
     int rc;
     if ((rc = alloc(owned, &b.ray_o[0], P)) || (rc = alloc(owned, &b.ray_d[0], P)) ||
         (rc = alloc(owned, &b.ray_o[1], P)) || (rc = alloc(owned, &b.ray_d[1], P)) ||
-        (rc = alloc(owned, &b.t_rad, P)) || (rc = alloc(owned, &b.t_term[0], P)) ||
+        (rc = alloc(owned, &b.t_rad[0], P)) || (rc = alloc(owned, &b.t_rad[1], P)) ||
+        (rc = alloc(owned, &b.t_term[0], P)) ||
         (rc = alloc(owned, &b.t_term[1], P)) || (rc = alloc(owned, &b.prev_base[0], P)) ||
         (rc = alloc(owned, &b.prev_base[1], P)) || (rc = alloc(owned, &b.acc_e, P)) ||
         (rc = alloc(owned, &b.acc_d, P)) || (rc = alloc(owned, &b.acc_i, P)) ||
@@ -2396,6 +2401,7 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
             (rc = alloc(owned_, &b.tr_w, tslots * nrec)) || (rc = alloc(owned_, &b.tr_s, tslots * nrec)) ||
             (rc = alloc(owned_, &b.cq_path, P)) || (rc = alloc(owned_, &b.cq_pos, P)) ||
             (rc = alloc(owned_, &b.cq_dir, P)) || (rc = alloc(owned_, &b.cq_out, P)) ||
+            (rc = alloc(owned_, &b.cq_t, P)) ||
             (rc = alloc(owned_, &b.tr_img, P)) ||
             (rc = alloc(owned_, &b.tp_pos, tslots * nrec)) || (rc = alloc(owned_, &b.tp_dir, tslots * nrec)) ||
             (rc = alloc(owned_, &b.tp_tgt, tslots * nrec)))

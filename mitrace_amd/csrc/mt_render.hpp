@@ -89,9 +89,10 @@ struct PathBuffers {
     // rays of bounce k in queue order (slot i = the path at queue[k & 1][i]), double-buffered by
     // bounce parity, so that extend and shading read them coalesced, not through the queue
     float4 *ray_o[2], *ray_d[2];
-    float4* t_rad;  // radiance throughput by path index (the cache passes read it there)
-    // termination throughput and previous base colour in queue order, by bounce parity
-    float4 *t_term[2], *prev_base[2];
+    // radiance throughput, termination throughput and previous base colour in queue order, by
+    // bounce parity (shading reads bounce k's at its queue slot and writes the survivors' next
+    // values at their slots in queue k + 1)
+    float4 *t_rad[2], *t_term[2], *prev_base[2];
     float4 *acc_e, *acc_d, *acc_i, *acc_s;
     uint4* hit_rec;   // closest hit in queue order: {instance (as int, -1 = miss), triangle, u, v bits}
     float4* hit_pos;  // world position, world distance (queue order)
@@ -120,8 +121,8 @@ struct PathBuffers {
     float4 *tr_pos, *tr_dir, *tr_w, *tr_s;  // per slot and recorded bounce: vertex, direction
                                             // to the previous vertex, throughput (w = valid),
                                             // radiance collected before the vertex
-    uint32_t* cq_path;      // cache queries: path, position, direction, prediction
-    float4 *cq_pos, *cq_dir, *cq_out;
+    uint32_t* cq_path;      // cache queries: path, position, direction, prediction and the
+    float4 *cq_pos, *cq_dir, *cq_out, *cq_t;  // throughput at the query vertex
     float4* tr_img;         // per path (after-light mode): throughput x prediction of a
                             // training path's own query, the cached estimate its image uses
     float4 *tp_pos, *tp_dir, *tp_tgt;       // compact training samples
