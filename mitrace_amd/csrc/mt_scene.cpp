@@ -11,6 +11,7 @@
 #include <array>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <thread>
@@ -753,9 +754,12 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             s->world_hi[k] = fmax_g(s->world_hi[k], inst[i].wmax[k]);
         }
     // top-level BVH for scenes with more instances than one wave-wide mask covers
+    // (MT_TLAS_MIN: above how many instances, default 64)
     std::vector<float4> tlas_pairs;
     std::vector<uint32_t> tlas_inst;
-    s->dev.use_tlas = d->instance_count > 64 ? 1u : 0u;
+    uint32_t tlas_min = 64;
+    if (const char* v = std::getenv("MT_TLAS_MIN")) tlas_min = static_cast<uint32_t>(std::strtoul(v, nullptr, 10));
+    s->dev.use_tlas = d->instance_count > tlas_min ? 1u : 0u;
     if (s->dev.use_tlas) {
         std::vector<float> boxes(static_cast<size_t>(d->instance_count) * 6);
         for (uint32_t i = 0; i < d->instance_count; ++i) {
