@@ -40,7 +40,7 @@ constexpr uint32_t kMaxJobs = 1u << 14;
 constexpr uint32_t kQueueTiles = 1u << 20;  // per scene (graphs), per batch (persistent)
 constexpr uint32_t kMaxSlots = 256;         // scene slots of a persistent batch
 constexpr uint64_t kStallMs = 60000;        // persistent: a group barrier waiting this long aborts
-                                            // (MT_BATCH_STALL_MS overrides, at least 1000)
+                                            // (MT_BATCH_STALL_MS overrides, at least 1)
 constexpr uint64_t kIdleMs = 1000;          // persistent: a lane group exits after this long idle
 constexpr uint32_t kMaxStack = 64;          // traversal stack entries (LDS + spill)
 constexpr uint32_t kBlockSize = 256;
@@ -365,7 +365,7 @@ int mt_batch::init_persistent() {
     if (mt::wall_ticks_per_ms(&per_ms)) return fail(MT_E_HIP, mt_last_error());
     ctl.idle_ticks = kIdleMs * per_ms;
     if (const char* v = std::getenv("MT_BATCH_STALL_MS"))
-        stall_ms = std::max<uint64_t>(1000, std::strtoull(v, nullptr, 10));
+        stall_ms = std::max<uint64_t>(1, std::strtoull(v, nullptr, 10));
     ctl.stall_ticks = stall_ms * per_ms;
     BT_TRY(hipEventCreateWithFlags(&p_end, hipEventDisableTiming));
     return MT_OK;

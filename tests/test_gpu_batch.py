@@ -124,6 +124,34 @@ def test_persistent_launch_ends_when_idle_and_restarts(scenes):
     b.close()
 
 
+def test_persistent_launch_aborts_on_its_stall_limit(scenes, monkeypatch):
+    """the persistent kernel's barrier wait is bounded: one tile for eight lane groups leaves
+    seven groups whose first block polls the queue for tiles while the group's other blocks wait
+    at its barrier; held open for 200 ms, they wait beyond a 1 ms limit, set the abort word,
+    every block returns (the pollers once the queue closes), and the batch reports the error
+    instead of hanging; a batch created afterwards with the normal limit renders correctly"""
+    import time
+    from mitrace_amd._lib import MiTraceError
+    from mitrace_amd.batch import Batch
+    monkeypatch.setenv("MT_BATCH_STALL_MS", "1")
+    b = Batch({"paths_per_iteration": 1 << 20, "tile_pixels": 16384, "streams": 8})
+    monkeypatch.setenv("MT_BATCH_STALL_MS", "20000")
+    b.submit(scenes["cornell_gen"], small_settings(width=128, height=128, samples=64, bounces=8, seed=9))
+    b.start()
+    time.sleep(0.2)
+    with pytest.raises(MiTraceError) as e:
+        b.finish()
+    assert e.value.code == abi.MT_E_HIP and "aborted" in str(e.value)
+    b.close()
+    name, st = _jobs()[0]
+    b = Batch({"paths_per_iteration": 1 << 16})
+    j = b.submit(scenes[name], st)
+    b.finish()
+    ref, _, _ = scenes[name].render(st)
+    np.testing.assert_array_equal(b.image(j), ref)
+    b.close()
+
+
 def test_batch_rejects_jobs_beyond_its_limits(scenes):
     import ctypes as C
     from mitrace_amd.batch import Batch
