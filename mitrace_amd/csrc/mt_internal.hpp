@@ -177,8 +177,9 @@ struct mt_scene {
     uint32_t light_count = 0;
     uint32_t material_count = 0;
     float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};  // union of the instance boxes
-    uint64_t prim_slots = 0;
-    uint64_t node_count = 0;
+    uint64_t prim_slots = 0;        // leaf slots (48 bytes) of the reference-rule trees
+    uint64_t node_count = 0;        // nodes of the reference-rule trees (64 bytes per pair)
+    uint64_t fast_record_bytes = 0; // pair and slot records of the fast traversal's trees
     uint64_t triangle_count = 0;
     ~mt_scene();
 };

@@ -149,6 +149,109 @@ __device__ __forceinline__ uint32_t counter(const PathBuffers& b, uint32_t i) {
     return __hip_atomic_load(b.counters + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ------------------------------------------------------------------ ray binning
+// Before the extend of a secondary bounce the queue's rays are binned by direction octant and
+// origin cell, so that the rays a wave takes up together start close to each other and head
+// the same way.  Only the order in which extend takes the queue's slots changes: it reads the
+// ray of slot perm[i] and writes that slot's hit record, so hits and shading are unchanged.
+// Counting sort over at most kBinMax bins: per-block histograms in LDS over contiguous slot
+// ranges, one global histogram, its exclusive scan, then every block reserves its run of each
+// bin with one atomic and places its slots (in any order inside the run).
+constexpr uint32_t kBinMax = 4096;
+constexpr uint32_t kBinKeyBits = 12;  // direction bits (3 per axis step) + origin bits
+
+__device__ __forceinline__ uint32_t ray_bin(const ChunkParams& c, V3 o, V3 d) {
+    uint32_t key = 0;
+    const float dv[3] = {d.x, d.y, d.z}, ov[3] = {o.x, o.y, o.z};
+    const uint32_t dtop = (1u << c.bin_dir_bits) - 1u;
+    const float dscale = 0.5f * static_cast<float>(1u << c.bin_dir_bits);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float f = (dv[a] + 1.0f) * dscale;
+        key = (key << c.bin_dir_bits) |
+              (f > 0.0f ? (f < static_cast<float>(dtop) ? static_cast<uint32_t>(f) : dtop) : 0u);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t top = (1u << c.bin_bits[a]) - 1u;
+        const float f = (ov[a] - c.bin_lo[a]) * c.bin_scale[a];
+        // cells outside the box (and NaN) clamp to the edge cells
+        const uint32_t q = f > 0.0f ? (f < static_cast<float>(top) ? static_cast<uint32_t>(f) : top) : 0u;
+        key = (key << c.bin_bits[a]) | q;
+    }
+    return key;
+}
+
+// the contiguous slot range [lo, hi) of this block out of n
+__device__ __forceinline__ void bin_range(uint32_t n, uint32_t& lo, uint32_t& hi) {
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    lo = min(blockIdx.x * per, n);
+    hi = min(lo + per, n);
+}
+
+__device__ __forceinline__ void bin_block_histogram(const PathBuffers& b, uint32_t lo, uint32_t hi,
+                                                    uint32_t* h) {
+    for (uint32_t j = threadIdx.x; j < kBinMax; j += kBlock) h[j] = 0;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) atomicAdd(&h[b.bin_key[i] & (kBinMax - 1)], 1u);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_count(PathBuffers b, uint32_t k) {
+    __shared__ uint32_t h[kBinMax];
+    uint32_t lo, hi;
+    bin_range(counter(b, kCntExtend + k), lo, hi);
+    bin_block_histogram(b, lo, hi, h);
+    for (uint32_t j = threadIdx.x; j < kBinMax; j += kBlock)
+        if (h[j]) atomicAdd(b.bin_hist + j, h[j]);
+}
+
+// one block: bin_cur = exclusive scan of bin_hist, and bin_hist back to zero for the next bounce
+__global__ __launch_bounds__(kBlock) void k_bin_scan(PathBuffers b) {
+    constexpr uint32_t kPer = kBinMax / kBlock;
+    __shared__ uint32_t part[kBlock];
+    uint32_t v[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        v[j] = b.bin_hist[threadIdx.x * kPer + j];
+        sum += v[j];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t i = 0; i < kBlock; ++i) {
+            const uint32_t t = part[i];
+            part[i] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        b.bin_cur[threadIdx.x * kPer + j] = run;
+        b.bin_hist[threadIdx.x * kPer + j] = 0;
+        run += v[j];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(PathBuffers b, uint32_t k) {
+    __shared__ uint32_t h[kBinMax];
+    const uint32_t n = counter(b, kCntExtend + k);
+    uint32_t lo, hi;
+    bin_range(n, lo, hi);
+    bin_block_histogram(b, lo, hi, h);
+    for (uint32_t j = threadIdx.x; j < kBinMax; j += kBlock)
+        if (h[j]) h[j] = atomicAdd(b.bin_cur + j, h[j]);  // this block's first position in bin j
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const uint32_t pos = atomicAdd(&h[b.bin_key[i] & (kBinMax - 1)], 1u);
+        if (pos < n) b.perm[pos] = i;
+    }
+}
+
 // ------------------------------------------------------------------ batched rendering
 // One iteration of a batch: the scheduler takes tiles from the scene's device queue in order,
 // from its head up to the end the host wrote for this iteration (q_state[1], the same greedy
@@ -276,6 +379,7 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
     // statistics and debug views) and the test hooks only
     constexpr bool kDiag = kCount || kMode >= 2;
     const uint32_t* q = b.queue[bounce & 1];
+    const bool binned = kMode == 0 && c.bin && bounce > 0;
     const uint32_t refill_min = c.refill_min;
     const uint64_t lanes_below = (1ull << lane_id()) - 1ull;
     Work w{};
@@ -293,12 +397,15 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
             if (!active) {
                 const uint32_t my = base + static_cast<uint32_t>(__popcll(idle & lanes_below));
                 if (my < count) {
-                    item = my;
+                    // binned bounces: the my-th slot in bin order (clamped: a slot index
+                    // never leaves the queue)
+                    if (kMode == 0 && binned) item = min(b.perm[my], count - 1u);
+                    else item = my;
                     V3 o, d;
                     float bound = kFltMax;
                     if (kMode == 0) {
-                        o = xyz(b.ray_o[bounce & 1][my]);
-                        d = xyz(b.ray_d[bounce & 1][my]);
+                        o = xyz(b.ray_o[bounce & 1][item]);
+                        d = xyz(b.ray_d[bounce & 1][item]);
                     } else if (kMode == 1) {
                         const float4 so = b.sh_o[my];
                         o = xyz(so);
@@ -1029,6 +1136,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             qout[cs] = p;
             b.ray_o[(k + 1) & 1][cs] = f4(next_o, 0.0f);
             b.ray_d[(k + 1) & 1][cs] = f4(next_d, 0.0f);
+            if (!kBatch && c.bin) b.bin_key[cs] = ray_bin(c, next_o, next_d);
             b.t_term[(k + 1) & 1][cs] = f4(next_term, 0.0f);
             b.t_rad[(k + 1) & 1][cs] = f4(next_trad, 0.0f);
             b.prev_base[(k + 1) & 1][cs] = f4(next_base, 0.0f);
@@ -2097,6 +2205,45 @@ int alloc_path_buffers(std::vector<void*>& owned, PathBuffers& b, uint64_t P, ui
     return MT_OK;
 }
 
+// the ray binning's buffers of one slot; the histogram starts at zero (the scan keeps it there)
+int alloc_bins(std::vector<void*>& owned, PathBuffers& b, uint64_t P, hipStream_t s) {
+    int rc;
+    if ((rc = alloc(owned, &b.bin_key, P)) || (rc = alloc(owned, &b.perm, P)) ||
+        (rc = alloc(owned, &b.bin_hist, kBinMax)) || (rc = alloc(owned, &b.bin_cur, kBinMax)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(b.bin_hist, 0, sizeof(uint32_t) * kBinMax, s));
+    return MT_OK;
+}
+
+// bins: dir_bits per direction component, the remaining key bits for the origin's cell in the
+// scene's box, each bit to the axis with the longest cells (no binning for an empty box)
+bool bin_grid(const mt_scene* scene, uint32_t dir_bits, uint32_t key_bits, ChunkParams& c) {
+    c.bin_dir_bits = dir_bits;
+    float ext[3];
+    for (int a = 0; a < 3; ++a) {
+        ext[a] = scene->world_hi[a] - scene->world_lo[a];
+        if (!(ext[a] >= 0.0f) || !std::isfinite(ext[a])) return false;
+        c.bin_bits[a] = 0;
+        c.bin_lo[a] = scene->world_lo[a];
+    }
+    for (uint32_t i = 3 * dir_bits; i < key_bits; ++i) {
+        int best = -1;
+        float longest = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            const float cell = ext[a] / static_cast<float>(1u << c.bin_bits[a]);
+            if (cell > longest) {
+                longest = cell;
+                best = a;
+            }
+        }
+        if (best < 0) break;
+        ++c.bin_bits[best];
+    }
+    for (int a = 0; a < 3; ++a)
+        c.bin_scale[a] = c.bin_bits[a] ? static_cast<float>(1u << c.bin_bits[a]) / ext[a] : 0.0f;
+    return true;
+}
+
 int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st, bool diff,
                     uint32_t max_slots) {
     scene_ = scene;
@@ -2154,6 +2301,22 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
 
     ChunkParams& c = params_;
     fill_params(c, scene, cam, st);
+    // ray binning for the plain chunk loop (enqueue_chunk sets c.bin in its own copy): by
+    // default for scenes whose traversal records exceed the 256 MB last-level cache (measured:
+    // the large terrain scene +2 % reference / +5 % fast traversal; the sphere grid's 12 MB of
+    // records, 98 % served by L2, gains nothing), MT_BIN_RAYS=0/1 to force it
+    const uint64_t records = st.traversal == MT_TRAVERSAL_FAST
+                                 ? scene->fast_record_bytes
+                                 : scene->node_count * 32 + scene->prim_slots * 48;
+    bool bin = records > (256ull << 20);
+    if (const char* v = std::getenv("MT_BIN_RAYS"); v && *v) bin = std::strtol(v, nullptr, 10) != 0;
+    bin_blocks_ = 0;
+    if (!diff && bin &&
+        bin_grid(scene, std::min<uint32_t>(4u, std::max<uint32_t>(1u, env_u32("MT_BIN_DIR_BITS", 1))),
+                 std::min<uint32_t>(kBinKeyBits, env_u32("MT_BIN_KEY_BITS", kBinKeyBits)), c)) {
+        bin_blocks_ = static_cast<uint32_t>(cus) * 2u;
+        if ((rc = alloc_bins(owned_, b, P, stream_))) return rc;
+    }
     c.stack_cap = cap;
     c.inv_spp = 1.0f / static_cast<float>(st.samples);
     if (diff) {  // the differentiable image is the linear radiance, no firefly rejection
@@ -2181,6 +2344,7 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         if ((rc = alloc_path_buffers(owned_, S.b, P, spill_depth * trace_threads, false, false,
                                      st.debug_mode != 0)))
             return rc;
+        if (bin_blocks_ && (rc = alloc_bins(owned_, S.b, P, S.stream))) return rc;
         HIP_TRY(hipMemsetAsync(S.b.work, 0, sizeof(unsigned long long) * 16, S.stream));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&S.host_counters), sizeof(uint32_t) * kMaxCounters));
         for (uint32_t k = 0; k < 2 + 4 * st.bounces; ++k) {
@@ -2213,6 +2377,7 @@ int Renderer::enqueue_chunk(uint32_t slot, uint32_t chunk) {
     c.pix_begin = chunk * npix_chunk_;
     c.npix = static_cast<uint32_t>(std::min<uint64_t>(npix_chunk_, npix_total - c.pix_begin));
     c.paths = c.npix * st_.samples;
+    c.bin = bin_blocks_ != 0;
     const uint32_t gen_blocks = (c.paths + kBlock - 1) / kBlock;
     hipStream_t s = L.stream;
     const PathBuffers& b = L.b;
@@ -2223,6 +2388,11 @@ int Renderer::enqueue_chunk(uint32_t slot, uint32_t chunk) {
     hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, c, b);
     for (uint32_t k = 0; k < st_.bounces; ++k) {
         hipEvent_t* ev = &events[2 + 4 * k];
+        if (c.bin && k > 0) {  // the bins of the rays shading queued for this bounce
+            hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks_), dim3(kBlock), 0, s, b, k);
+            hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kBlock), 0, s, b);
+            hipLaunchKernelGGL(k_bin_scatter, dim3(bin_blocks_), dim3(kBlock), 0, s, b, k);
+        }
         HIP_TRY(hipEventRecord(ev[0], s));
         launch_trace<0>(lds_depth_, c.count_work != 0, trace_blocks_, s, scene_->dev, c, b, k);
         HIP_TRY(hipEventRecord(ev[1], s));

@@ -82,6 +82,13 @@ struct ChunkParams {
     int32_t cache_after;  // the query follows the vertex's emission and light sample
     int32_t debug;  // MT_DEBUG_* view (plain renders)
     int32_t accel;  // 1 = fast traversal (MT_TRAVERSAL_FAST)
+    // ray binning before the extend of bounces >= 1 (plain renders): shading writes every
+    // continuing ray's bin (direction octant, then the origin's cell on a grid of
+    // 2^bin_bits[a] cells per axis over the scene's box), extend takes rays bin by bin
+    int32_t bin;
+    uint32_t bin_dir_bits;  // cells per direction component: 2^bin_dir_bits over [-1, 1]
+    uint32_t bin_bits[3];
+    float bin_lo[3], bin_scale[3];
 };
 
 // SoA path state, queues and scratch of one render (capacity = paths per chunk)
@@ -97,6 +104,9 @@ struct PathBuffers {
     uint4* hit_rec;   // closest hit in queue order: {instance (as int, -1 = miss), triangle, u, v bits}
     float4* hit_pos;  // world position, world distance (queue order)
     uint32_t* queue[2];
+    // ray binning: bin of every queue slot of the next bounce, the order in which extend takes
+    // the slots, and the per-bin counts and cursors
+    uint32_t *bin_key, *perm, *bin_hist, *bin_cur;
     float4 *sh_o, *sh_d, *sh_c;  // shadow rays: {o, max dist} {d, path} {contribution, slot}
     float4* sample_rgb;
     uint2* spill;
@@ -205,6 +215,7 @@ class Renderer {
     float4* store_ = nullptr;
     uint32_t capacity_ = 0, npix_chunk_ = 0, chunks_ = 0;
     uint32_t trace_blocks_ = 0, shade_blocks_ = 0, lds_depth_ = 16;
+    uint32_t bin_blocks_ = 0;  // > 0: plain chunks bin the rays of bounces >= 1 (MT_BIN_RAYS)
     mt_stats stats_{};
     std::vector<StreamSlot> slots_;  // slots_[0] uses buf_, stream_, host_counters_, events_
 };

@@ -705,6 +705,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         return fail(MT_E_UNSUPPORTED, "more than 2^31 BVH node pairs or leaf slots");
     s->prim_slots = tot_prims - fast_slots;
     s->node_count = (tot_pairs - fast_pairs) * 2;
+    s->fast_record_bytes = fast_pairs * 64 + fast_slots * 48;
     s->triangle_count = tri_total;
 
     std::vector<DevInstance> inst(d->instance_count);
