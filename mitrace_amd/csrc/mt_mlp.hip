@@ -6,8 +6,9 @@
 // leaves lane (g = lane/32, n = lane%32) with rows q*8 + g*4 + j (q, j < 4) of sample n; the
 // next layer's B operand wants 8 k-slots per lane group, so the input features of every layer
 // are numbered in that order (mlp::perm) and the packed weights follow it: activations go from
-// one product to the next without leaving registers.  The weight gradient sums over samples
-// with the samples as K, from [feature][sample] slabs staged in LDS.
+// one product to the next without leaving registers.  The training pass keeps activations and
+// gradients in [feature][32 samples] slabs per group of 32 samples, so that the weight gradient
+// (samples as K) loads its operand fragments straight from them.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -138,21 +139,15 @@ __device__ __forceinline__ void grad_chunks(const f32x16 acc[2], uint32_t mask, 
     }
 }
 
-// stores the 4 chunks of a 64-feature vector (natural feature order) for sample n
-__device__ __forceinline__ void store_chunks(uint16_t* row, const bf16x8 v[], int nchunks, int g) {
+// stores the chunks of a feature vector of sample s (0..31 in its group) into the group's
+// [feature][32 samples] slab; the 32 lanes of a lane group write 64 contiguous bytes per feature
+__device__ __forceinline__ void store_slab(uint16_t* slab, const bf16x8 v[], int nchunks, int g, int s) {
     for (int c = 0; c < nchunks; ++c)
-        for (int half = 0; half < 2; ++half) {
-            // k-slots half*4 .. half*4+3 are 4 consecutive features
-            const int f = perm(c, g, half * 4);
-            uint16_t u[4];
-            for (int j = 0; j < 4; ++j) {
-                const __bf16 h = v[c][half * 4 + j];
-                __builtin_memcpy(&u[j], &h, 2);
-            }
-            uint2 w;
-            w.x = static_cast<uint32_t>(u[0]) | (static_cast<uint32_t>(u[1]) << 16);
-            w.y = static_cast<uint32_t>(u[2]) | (static_cast<uint32_t>(u[3]) << 16);
-            *reinterpret_cast<uint2*>(row + f) = w;
+        for (int t = 0; t < 8; ++t) {
+            const __bf16 h = v[c][t];
+            uint16_t u;
+            __builtin_memcpy(&u, &h, 2);
+            slab[perm(c, g, t) * 32 + s] = u;
         }
 }
 
@@ -256,12 +251,12 @@ __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n_h
         uint32_t masks[4];
         bf16x8 h[4][4];
         forward(N, enc, o, masks, h, lane);
-        uint16_t* Hrow = T.H + static_cast<size_t>(i) * kHWidth;
-        uint16_t* Drow = T.DZ + static_cast<size_t>(i) * kDZWidth;
-        if (ok) {
-            store_chunks(Hrow + kHOff[0], enc, 2, g);
-            for (int l = 1; l < 5; ++l) store_chunks(Hrow + kHOff[l], h[l - 1], 4, g);
-        }
+        // the group's slabs: every lane stores, lanes past n with dZ = 0 (finite activations
+        // of a zero input), so that they add nothing to the weight gradients
+        uint16_t* Hs = T.H + static_cast<size_t>(base / 32) * kHWidth * 32;
+        uint16_t* Ds = T.DZ + static_cast<size_t>(base / 32) * kDZWidth * 32;
+        store_slab(Hs + kHOff[0] * 32, enc, 2, g, s);
+        for (int l = 1; l < 5; ++l) store_slab(Hs + kHOff[l] * 32, h[l - 1], 4, g, s);
         // output gradient (lanes g == 0 hold outputs 0..2 in o[0..2])
         f32x16 dz5 = {};
         if (g == 0 && ok) {
@@ -282,12 +277,8 @@ __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n_h
         // dZ5 chunk 0 of the output layer: k-slots t<4 of group 0 are outputs 0..3
         bf16x8 dzc[4];
         for (int t = 0; t < 8; ++t) dzc[0][t] = static_cast<__bf16>(t < 4 ? dz5[t] : 0.0f);
-        if (ok && g == 0) {
-            uint2 w;
-            w.x = static_cast<uint32_t>(bf16_bits(dz5[0])) | (static_cast<uint32_t>(bf16_bits(dz5[1])) << 16);
-            w.y = static_cast<uint32_t>(bf16_bits(dz5[2]));
-            *reinterpret_cast<uint2*>(Drow + kDZOff[4]) = w;
-        }
+        if (g == 0)
+            for (int c = 0; c < 4; ++c) Ds[(kDZOff[4] + c) * 32 + s] = c < 3 ? bf16_bits(dz5[c]) : 0;
         // layer 5 backward: dH4 = W5 dZ5 (K = one chunk of the 3 outputs)
         f32x16 acc[2];
         for (int mt = 0; mt < 2; ++mt) {
@@ -298,7 +289,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n_h
         for (int l = 3; l >= 0; --l) {
             bf16x8 dz[4];
             grad_chunks(acc, masks[l], dz);
-            if (ok) store_chunks(Drow + kDZOff[l], dz, 4, g);
+            store_slab(Ds + kDZOff[l] * 32, dz, 4, g, s);
             if (l == 0) break;
             for (int mt = 0; mt < 2; ++mt) {
                 f32x16 a = {};
@@ -316,14 +307,15 @@ __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n_h
     if (threadIdx.x == 0) atomicAdd(T.loss, (red[0] + red[1] + red[2] + red[3]) * inv_count / 3.0f);
 }
 
-// weight gradients: every block sums dW = dZ^T H over its sample range with the samples as K,
-// from 32-sample slabs transposed into LDS ([feature][sample]); writes its partial sums
-constexpr int kSlab = 32;
+// weight gradients: every block sums dW = dZ^T H over its range of 32-sample groups with the
+// samples as K.  The slabs are [feature][32 samples], so the A fragment of lane (g, r) for an
+// output tile (layer l, rows o) is dZ feature o over samples c*16 + g*8 .. +7 -- 16 contiguous
+// bytes -- and the B fragment is H feature i over the same samples: both load straight from
+// memory.  The bias gradient of output o is the sum of the A fragments of the tile with i
+// tile 0.  Each block writes its partial sums.
 __global__ __launch_bounds__(256) void k_mlp_wgrad(CacheTrainBuffers T, uint32_t n_host, const uint32_t* n_dev,
                                                    uint32_t offset) {
     const uint32_t n = step_count(n_dev, offset, n_host);
-    __shared__ __attribute__((aligned(16))) uint16_t sH[kHWidth][kSlab];
-    __shared__ __attribute__((aligned(16))) uint16_t sD[kDZWidth][kSlab];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 5, r = lane & 31;
     // 16 output tiles: layer, o tile, i tile -- 4 per wave
     int tl[4], to[4], ti[4];
@@ -340,47 +332,32 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(CacheTrainBuffers T, uint32_t
                     }
     }
     f32x16 acc[4] = {};
-    float bias_acc[2] = {0.0f, 0.0f};
-    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-    const uint32_t s0 = blockIdx.x * per, s1 = min(n, s0 + per);
-    for (uint32_t sb = s0; sb < s1; sb += kSlab) {
-        __syncthreads();
-        // transpose the slab into LDS (zeros past the end)
-        for (int e = threadIdx.x; e < kSlab * kHWidth; e += 256) {
-            const int smp = e / kHWidth, f = e - smp * kHWidth;
-            const uint32_t si = sb + smp;
-            sH[f][smp] = (si < s1) ? T.H[static_cast<size_t>(si) * kHWidth + f] : 0;
-        }
-        for (int e = threadIdx.x; e < kSlab * kDZWidth; e += 256) {
-            const int smp = e / kDZWidth, f = e - smp * kDZWidth;
-            const uint32_t si = sb + smp;
-            sD[f][smp] = (si < s1) ? T.DZ[static_cast<size_t>(si) * kDZWidth + f] : 0;
-        }
-        __syncthreads();
+    float bias_acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t groups = (n + 31) / 32;
+    const uint32_t per = (groups + gridDim.x - 1) / gridDim.x;
+    const uint32_t g0 = min(groups, blockIdx.x * per), g1 = min(groups, g0 + per);
+    for (uint32_t grp = g0; grp < g1; ++grp) {
+        const uint16_t* Hs = T.H + static_cast<size_t>(grp) * kHWidth * 32;
+        const uint16_t* Ds = T.DZ + static_cast<size_t>(grp) * kDZWidth * 32;
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int l = tl[k];
-            const int o = to[k] * 32 + r;     // A row: output feature
-            const int i = ti[k] * 32 + r;     // B column: input feature
-            for (int c = 0; c < kSlab / 16; ++c) {
-                bf16x8 a, b;
+            const int o = to[k] * 32 + r;  // A row: output feature
+            const int i = ti[k] * 32 + r;  // B column: input feature
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
                 const int sm = c * 16 + g * 8;
+                bf16x8 a;
                 if (o < kOut[l]) {
-                    a = *reinterpret_cast<const bf16x8*>(&sD[kDZOff[l] + o][sm]);
+                    a = *reinterpret_cast<const bf16x8*>(Ds + (kDZOff[l] + o) * 32 + sm);
+                    if (ti[k] == 0)
+                        for (int t = 0; t < 8; ++t) bias_acc[k] += static_cast<float>(a[t]);
                 } else {
                     for (int t = 0; t < 8; ++t) a[t] = static_cast<__bf16>(0.0f);
                 }
-                b = *reinterpret_cast<const bf16x8*>(&sH[kHOff[l] + i][sm]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(Hs + (kHOff[l] + i) * 32 + sm);
                 acc[k] = mfma(a, b, acc[k]);
             }
-        }
-        // bias gradients: thread t owns bias entries t and t + 256 of the 259
-        for (int q = 0; q < 2; ++q) {
-            const int e = threadIdx.x + q * 256;
-            if (e >= 259) continue;
-            const int f = e < 256 ? e : 256 + (e - 256);
-            float sum = 0.0f;
-            for (int smp = 0; smp < kSlab; ++smp) sum += bf16_float(sD[f][smp]);
-            bias_acc[q] += sum;
         }
     }
     float* out = T.partial + static_cast<size_t>(blockIdx.x) * kParams;
@@ -392,27 +369,33 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(CacheTrainBuffers T, uint32_t
                 const int i = ti[k] * 32 + r;
                 if (o < kOut[l]) out[kWOff[l] + o * kIn[l] + i] = acc[k][q * 4 + j];
             }
-    }
-    for (int q = 0; q < 2; ++q) {
-        const int e = threadIdx.x + q * 256;
-        if (e >= 259) continue;
-        const int l = e < 256 ? e / 64 : 4;
-        const int o = e < 256 ? e % 64 : e - 256;
-        out[kBOff[l] + o] = bias_acc[q];
+        // lane groups 0 and 1 hold the two halves of every group's samples
+        const float bsum = bias_acc[k] + __shfl_xor(bias_acc[k], 32);
+        const int o = to[k] * 32 + r;
+        if (ti[k] == 0 && g == 0 && o < kOut[l]) out[kBOff[l] + o] = bsum;
     }
 }
 
 // Adam on the fp32 master weights from the sum of the blocks' partial gradients
-__global__ void k_mlp_adam(float* master, float* m, float* v, const float* partial, uint32_t blocks,
-                           float lr, float b1, float b2, float eps, float c1, float c2,
-                           float* grad_out, uint32_t n_host, const uint32_t* n_dev, uint32_t offset,
-                           unsigned long long* trained) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+// (64 parameters per block; its four waves sum a quarter of the partials each)
+__global__ __launch_bounds__(256) void k_mlp_adam(float* master, float* m, float* v, const float* partial,
+                                                  uint32_t blocks, float lr, float b1, float b2, float eps,
+                                                  float c1, float c2, float* grad_out, uint32_t n_host,
+                                                  const uint32_t* n_dev, uint32_t offset,
+                                                  unsigned long long* trained) {
     const uint32_t n = step_count(n_dev, offset, n_host);
-    if (n == 0 || p >= static_cast<uint32_t>(kParams)) return;  // an empty step changes nothing
+    if (n == 0) return;  // an empty step changes nothing (uniform over the grid)
+    const uint32_t p = blockIdx.x * 64 + (threadIdx.x & 63), quarter = threadIdx.x >> 6;
+    __shared__ float red[4][64];
+    float part = 0.0f;
+    if (p < static_cast<uint32_t>(kParams))
+        for (uint32_t b = quarter; b < blocks; b += 4) part += partial[static_cast<size_t>(b) * kParams + p];
+    red[quarter][threadIdx.x & 63] = part;
+    __syncthreads();
+    if (quarter != 0 || p >= static_cast<uint32_t>(kParams)) return;
     if (p == 0 && trained) atomicAdd(trained, static_cast<unsigned long long>(n));
-    float gsum = 0.0f;
-    for (uint32_t b = 0; b < blocks; ++b) gsum += partial[static_cast<size_t>(b) * kParams + p];
+    const uint32_t t = threadIdx.x & 63;
+    const float gsum = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
     if (grad_out) grad_out[p] = gsum;
     const float mm = b1 * m[p] + (1.0f - b1) * gsum;
     const float vv = b2 * v[p] + (1.0f - b2) * gsum * gsum;
@@ -556,7 +539,7 @@ int Mlp::train(uint32_t n, const float4* pos, const float4* dir, const float4* t
     const uint32_t blocks = std::min<uint32_t>((groups + 3) / 4, 2048);
     hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(256), 0, s, N, E, n, n_dev, offset, pos, dir,
                        target, tb_);
-    const uint32_t wb = std::min<uint32_t>(tb_.blocks, (n + kSlab - 1) / kSlab);
+    const uint32_t wb = std::min<uint32_t>(tb_.blocks, (n + 31) / 32);
     hipLaunchKernelGGL(k_mlp_wgrad, dim3(wb), dim3(256), 0, s, tb_, n, n_dev, offset);
     ++step_;
     const float b1 = 0.9f, b2 = 0.999f;
@@ -564,7 +547,7 @@ int Mlp::train(uint32_t n, const float4* pos, const float4* dir, const float4* t
     const float c2 = 1.0f - std::pow(b2, static_cast<float>(step_));
     float* gdev = nullptr;
     if (grad_host && dalloc(owned_, &gdev, kParams)) return MT_E_NOMEM;
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, master_, adam_m_,
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 63) / 64), dim3(256), 0, s, master_, adam_m_,
                        adam_v_, tb_.partial, wb, lr_, b1, b2, 1e-8f, c1, c2, gdev, n, n_dev, offset,
                        trained_);
     int rc = pack(s);

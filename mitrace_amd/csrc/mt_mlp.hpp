@@ -28,7 +28,7 @@ constexpr int kBwdTiles = 2;
 constexpr int kBwdChunks[kLayers] = {0, 4, 4, 4, 1};
 constexpr int kBwdOff[kLayers] = {0, 0, 4096, 8192, 12288};
 constexpr int kBwdSize = 13312;
-// per-sample activations kept by the training pass (bf16, [sample][feature]):
+// per-sample activations kept by the training pass (bf16, [group][feature][32 samples]):
 // H0 (32, the encoding) H1..H4 (64 each), and dZ1..dZ4 (64 each) dZ5 (3, stored as 4)
 constexpr int kHWidth = 32 + 4 * 64;      // 288
 constexpr int kDZWidth = 4 * 64 + 4;      // 260
@@ -44,8 +44,8 @@ inline __host__ __device__ int perm(int c, int g, int t) {
 }  // namespace mlp
 
 struct CacheTrainBuffers {
-    uint16_t* H;          // [n][kHWidth] bf16
-    uint16_t* DZ;         // [n][kDZWidth] bf16
+    uint16_t* H;          // [n / 32][kHWidth][32] bf16: per group of 32 samples, feature-major
+    uint16_t* DZ;         // [n / 32][kDZWidth][32] bf16
     float* partial;       // [blocks][kParams]
     float* loss;          // [blocks]
     uint32_t capacity;    // samples

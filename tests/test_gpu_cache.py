@@ -46,7 +46,12 @@ def test_inference_matches_numpy(cornell):
     scale = np.abs(ref).max()
     assert scale > 1e-3
     err = np.abs(out - ref)
-    assert np.percentile(err, 99) < 1e-2 * scale and err.max() < 5e-2 * scale, err.max() / scale
+    # the reference rounds to BF16 where the kernels do, so outputs agree to float32
+    # accumulation order (p99 measured <= 5e-8 of scale) except where such an order difference
+    # flips one BF16 rounding of a hidden activation (one BF16 ulp, 2^-8 relative; max measured
+    # 1.2e-3 of scale over three seeds); most outputs are bit-identical
+    assert np.percentile(err, 99) < 1e-5 * scale and err.max() < 5e-3 * scale, err.max() / scale
+    assert np.mean(err == 0) > 0.8
 
 
 def test_training_gradient_matches_numpy(cornell):
@@ -59,7 +64,7 @@ def test_training_gradient_matches_numpy(cornell):
     loss, grad = c.train(pos, d, target, want_grad=True)
     lo, hi = _bounds(cornell)
     ref_loss, ref_grad = loss_and_grad(w, encode(pos, d, lo, hi), target)
-    assert abs(loss - ref_loss) < 1e-3 * abs(ref_loss)
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss)  # measured <= 3e-6
     from mitrace_amd.cache import LAYERS
     o = 0
     for i, n in LAYERS:
@@ -67,7 +72,9 @@ def test_training_gradient_matches_numpy(cornell):
             g, r = grad[o:o + size], ref_grad[o:o + size]
             o += size
             rel = np.linalg.norm(g - r) / (np.linalg.norm(r) + 1e-12)
-            assert rel < 3e-2, (i, n, size, rel)
+            # float32 sums in another order plus rare BF16 rounding flips of the stored
+            # activations: measured <= 5.3e-5 per block over three seeds
+            assert rel < 5e-4, (i, n, size, rel)
     # one Adam step moved every weight with a gradient by about the learning rate
     moved = np.abs(c.weights() - w)
     live = np.abs(ref_grad) > 1e-6 * np.abs(ref_grad).max()
