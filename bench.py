@@ -375,12 +375,19 @@ def large_leg(args):
         def fn(**kw):
             kw.setdefault("traversal", trav)
             return scenegen.large_scene_settings(**kw)
+        ref = cache_eval.reference(g, fn, 4096)
         cache = {"workload": out["workload"] + ", 64 spp, linear radiance",
-                 "traversal": "reference" if args.cache_reference else "fast", "configs": []}
-        for opts in ({"query_depth": 1, "query_after_light": 1},
+                 "traversal": "reference" if args.cache_reference else "fast",
+                 "reference": "plain render at 4096 spp, 1920x1080 (%.1f s)" % ref[1], "configs": []}
+        # the camera ray's hit queries the cache after its light sample; the training paths
+        # estimate the rest of the path with environment samples (same expectation, far less
+        # variance) and the network learns it with the mean-keeping L2 loss; beside it the
+        # same query trained on the reference estimator's own paths with the relative loss
+        for opts in ({"query_depth": 0, "query_after_light": 1, "train_nee": 1, "loss": 1,
+                      "learning_rate": 2e-3, "warmup_passes": 4, "steps_per_pass": 16},
                      {"query_depth": 0, "query_after_light": 1}):
             cache["configs"].append(cache_eval.compare(
-                g, fn, cache_opts=opts, log=lambda m: print(m, file=sys.stderr, flush=True)))
+                g, fn, cache_opts=opts, log=lambda m: print(m, file=sys.stderr, flush=True), ref=ref))
     g.close()
     return out, cache
 

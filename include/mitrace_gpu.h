@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MT_ABI_VERSION 2
+#define MT_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------------------------- */
 #define MT_OK 0
@@ -372,6 +372,24 @@ typedef struct mt_cache_options {
                                    through that bounce; training paths record only their
                                    query_depth vertex, and query_depth 0 (the camera ray's
                                    hit) is allowed [0]                                      */
+    uint32_t train_nee;       /* 1 (needs query_after_light): the query is made whenever the
+                                 path has a bounce left (not only when it would go on), so
+                                 the cache stands for the expected rest of the path, and the
+                                 training paths estimate that rest with the same expectation
+                                 but less variance: from their query vertex on, each vertex's
+                                 shadow ray goes to the environment (drawn in proportion to
+                                 its radiance) or, with probability 1/2 when the scene has
+                                 point lights, to the reference's light choice (weight 2);
+                                 environment samples and bounce rays that reach the sky are
+                                 combined by the balance heuristic with the reference's true
+                                 sampling density [0]                                       */
+    uint32_t loss;            /* training loss per output channel: 0 = relative L2,
+                                 ((y - t) / sqrt(y^2 + 0.01))^2 with the denominator held,
+                                 Huber-clipped at 10; 1 = L2, (y - t)^2, which keeps the mean:
+                                 at a stationary point the output biases make the mean residual
+                                 over the training samples zero; the render passes then train
+                                 on targets divided by the first training chunk's mean target
+                                 and multiply the predictions by it [0]                        */
 } mt_cache_options;
 
 typedef struct mt_scene mt_scene;

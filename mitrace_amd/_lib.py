@@ -31,7 +31,7 @@ def load():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    if L.mt_abi_version() != 2:
+    if L.mt_abi_version() != 3:
         raise ImportError("libmitrace_gpu ABI version mismatch")
     _lib = L
     return L

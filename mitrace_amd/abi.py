@@ -158,7 +158,8 @@ class mt_cache_options(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("train_batch", C.c_uint32), ("learning_rate", C.c_float),
                 ("train_every", C.c_uint32), ("query_depth", C.c_uint32),
                 ("warmup_passes", C.c_uint32), ("steps_per_pass", C.c_uint32),
-                ("spp_per_pass", C.c_uint32), ("query_after_light", C.c_uint32)]
+                ("spp_per_pass", C.c_uint32), ("query_after_light", C.c_uint32),
+                ("train_nee", C.c_uint32), ("loss", C.c_uint32)]
 
 
 class mt_batch_options(C.Structure):

@@ -1,12 +1,12 @@
 """Plain vs radiance-cached rendering of one scene at equal RMSE.
 
-RMSE is measured against a high-spp plain render of the same view at a reduced resolution
-(the per-pixel error depends on the samples per pixel, not on the image size); throughput is
-measured at the full resolution, wall clock around each render call (the cached one includes
-its warm-up passes and all training).  The samples the cached form needs to reach the plain
-form's RMSE come from log-log interpolation of its RMSE ladder; the speed-up at equal RMSE is
-the plain render's time at `spp` over the cached render's time at that many samples (rounded
-up), both at full resolution.
+RMSE and bias are measured against a high-spp plain render of the same view at the same
+resolution (the cache's training depends on the number of paths per pass, so the ladder is not
+taken at a reduced size); time is the wall clock around each render call (the cached one
+includes its warm-up passes and all training).  The samples the cached form needs to reach the
+plain form's RMSE come from log-log interpolation of its RMSE ladder; the speed-up at equal
+RMSE is the plain render's time at `spp` over the cached render's time at that many samples
+(whole passes).
 """
 import time
 
@@ -37,8 +37,19 @@ def _spp_for(rmse_target, ladder):
     return None  # never reached: the cache's bias is above the target
 
 
-def compare(gscene, settings_fn, spp=64, small=(320, 180), ref_spp=4096, ladder=(4, 8, 16, 32, 64),
-            cache_opts=None, log=print):
+def reference(gscene, settings_fn, ref_spp=4096):
+    """the plain render that judges the cached form: (linear image as float64, seconds)"""
+    t0 = time.perf_counter()
+    ref, _, _ = gscene.render(settings_fn(samples=ref_spp, seed=999, output=abi.MT_OUTPUT_LINEAR))
+    return ref.astype(np.float64), time.perf_counter() - t0
+
+
+def compare(gscene, settings_fn, spp=64, ref_spp=4096, ladder=(4, 8, 16, 32, 64), cache_opts=None,
+            log=print, ref=None):
+    """Plain and cached renders at the full resolution of settings_fn for every sample count of
+    the ladder (each cached render with a fresh cache: warm-up and training included in its
+    time), their RMSE and relative bias against a ref_spp plain render of the same view, and
+    the speed-up at the plain render's RMSE at `spp`."""
     opts = dict(DEFAULT_CACHE, **(cache_opts or {}))
     out = {"cache_options": opts, "spp": spp, "output": "linear"}
     # linear radiance on both sides: the reference tone-maps every sample before averaging
@@ -48,63 +59,67 @@ def compare(gscene, settings_fn, spp=64, small=(320, 180), ref_spp=4096, ladder=
     def settings_fn(**kw):
         kw.setdefault("output", abi.MT_OUTPUT_LINEAR)
         return user_fn(**kw)
-    # throughput at full resolution
-    st = settings_fn(samples=spp, seed=5)
     gscene.render(settings_fn(samples=1, seed=4))  # first-touch warm-up
-    t0 = time.perf_counter()
-    _, _, sp = gscene.render(st)
-    t_plain = time.perf_counter() - t0
-    cache = RadianceCache(gscene, opts)
-    t0 = time.perf_counter()
-    _, sc = cache.render(st)
-    t_cached = time.perf_counter() - t0
-    cache.close()
-    n = st.width * st.height * spp
-    out["plain"] = {"seconds": t_plain, "msamples_per_s": n / t_plain / 1e6,
-                    "mrays_per_s": sp["rays"] / t_plain / 1e6}
-    out["cached"] = {"seconds": t_cached, "msamples_per_s": n / t_cached / 1e6,
-                     "mrays_per_s": sc["rays"] / t_cached / 1e6, "trained_samples": int(sc["diag"][8])}
-    log(f"# cache: 1080p plain {out['plain']['msamples_per_s']:.1f} Msamples/s, cached "
-        f"{out['cached']['msamples_per_s']:.1f} Msamples/s")
-    # RMSE ladders at reduced resolution
-    W, H = small
-    ref, _, _ = gscene.render(settings_fn(width=W, height=H, samples=ref_spp, seed=999))
-    lad_p, lad_c = [], []
+    ref, t_ref = ref if ref is not None else reference(gscene, user_fn, ref_spp)
+    ref_mean = float(np.mean(ref))
+    st0 = settings_fn(samples=1, seed=0)
+    out["reference"] = {"width": st0.width, "height": st0.height, "spp": ref_spp, "mean": ref_mean,
+                        "seconds": t_ref}
+    lad_p, lad_c, bias_c, t_p, t_c = [], [], [], {}, {}
     for s in ladder:
-        img, _, _ = gscene.render(settings_fn(width=W, height=H, samples=s, seed=17))
+        t0 = time.perf_counter()
+        img, _, sp = gscene.render(settings_fn(samples=s, seed=17))
+        t_p[s] = time.perf_counter() - t0
         lad_p.append((s, rmse(img, ref)))
         cache = RadianceCache(gscene, opts)
-        img_c, _ = cache.render(settings_fn(width=W, height=H, samples=s, seed=17))
+        t0 = time.perf_counter()
+        img_c, sc = cache.render(settings_fn(samples=s, seed=17))
+        t_c[s] = time.perf_counter() - t0
         cache.close()
         lad_c.append((s, rmse(img_c, ref)))
-        if s == ladder[-1]:
-            out["cached_bias"] = float(np.mean(img_c.astype(np.float64) - ref))
-            out["plain_bias"] = float(np.mean(img.astype(np.float64) - ref))
+        bias_c.append((s, float(np.mean(img_c, dtype=np.float64)) / ref_mean - 1.0 if ref_mean else None))
+        if s == spp:
+            n = st0.width * st0.height * spp
+            out["plain"] = {"seconds": t_p[s], "msamples_per_s": n / t_p[s] / 1e6,
+                            "mrays_per_s": sp["rays"] / t_p[s] / 1e6}
+            out["cached"] = {"seconds": t_c[s], "msamples_per_s": n / t_c[s] / 1e6,
+                             "mrays_per_s": sc["rays"] / t_c[s] / 1e6,
+                             "trained_samples": int(sc["diag"][8])}
+            out["plain_bias_relative"] = float(np.mean(img, dtype=np.float64)) / ref_mean - 1.0
+            out["cached_bias_relative"] = bias_c[-1][1]
+        log(f"# cache: {s} spp: plain {t_p[s]:.3f} s RMSE {lad_p[-1][1]:.1f}, cached {t_c[s]:.3f} s "
+            f"RMSE {lad_c[-1][1]:.1f} bias {bias_c[-1][1]:+.4f}")
     out["rmse_plain"] = lad_p
     out["rmse_cached"] = lad_c
-    ref_mean = float(np.mean(ref.astype(np.float64)))
-    out["reference"] = {"width": W, "height": H, "spp": ref_spp, "mean": ref_mean}
-    out["cached_bias_relative"] = out["cached_bias"] / ref_mean if ref_mean else None
-    target = lad_p[-1][1]
+    out["cached_bias_relative_by_spp"] = bias_c
+    out["seconds_plain_by_spp"] = [(s, t_p[s]) for s in ladder]
+    out["seconds_cached_by_spp"] = [(s, t_c[s]) for s in ladder]
+    target = dict(lad_p)[spp]
     need = _spp_for(target, lad_c)
-    out["equal_rmse"] = {"rmse": target, "plain_spp": ladder[-1], "cached_spp": need,
+    out["equal_rmse"] = {"rmse": target, "plain_spp": spp, "cached_spp": need,
                          "cached_spp_is_upper_bound": bool(need is not None and need <= ladder[0]
                                                            and lad_c[0][1] <= target)}
     if need:
-        # the cached render at full resolution with the samples it needs for that RMSE
-        # (warm-up and training included), against the plain render at `spp`
+        # the cached render with the samples it needs for that RMSE (whole passes), against the
+        # plain render at `spp`
         per = opts["spp_per_pass"]
-        k = max(per, int(np.ceil(need / per)) * per)
-        cache = RadianceCache(gscene, opts)
-        t0 = time.perf_counter()
-        cache.render(settings_fn(samples=k, seed=5))
-        t_c = time.perf_counter() - t0
-        cache.close()
-        out["equal_rmse"].update({"cached_spp_rendered": k, "cached_seconds": t_c,
-                                  "plain_seconds": t_plain, "speedup": t_plain / t_c})
+        k = max(per, int(np.ceil(need / per - 1e-9)) * per)
+        if k in t_c:
+            tk = t_c[k]
+            bk = dict(bias_c)[k]
+        else:
+            cache = RadianceCache(gscene, opts)
+            t0 = time.perf_counter()
+            img_k, _ = cache.render(settings_fn(samples=k, seed=17))
+            tk = time.perf_counter() - t0
+            cache.close()
+            bk = float(np.mean(img_k, dtype=np.float64)) / ref_mean - 1.0
+        out["equal_rmse"].update({"cached_spp_rendered": k, "cached_seconds": tk,
+                                  "cached_bias_relative": bk, "plain_seconds": t_p[spp],
+                                  "speedup": t_p[spp] / tk})
     else:
         out["equal_rmse"]["speedup"] = None
-    log(f"# cache: RMSE plain {lad_p} cached {lad_c} -> {out['equal_rmse']}")
+    log(f"# cache: -> {out['equal_rmse']}")
     return out
 
 

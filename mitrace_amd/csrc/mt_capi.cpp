@@ -500,6 +500,7 @@ int mt_cache_create(mt_scene* scene, const mt_cache_options* options, mt_cache**
         const int rc = c->mlp.init(c->opt.seed, std::max<uint32_t>(c->opt.train_batch, 1u << 16),
                                    c->opt.learning_rate);
         if (rc) return rc;
+        c->mlp.set_loss(c->opt.loss);
         c->mlp.set_bounds(scene->world_lo, scene->world_hi);
         *out = c.release();
         return MT_OK;

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(Net N, EncCtx E, uint32_t n_h
     }
 }
 
-// training pass for 32 samples per wave: forward keeping H0..H4, relative-L2 loss gradient,
+// training pass for 32 samples per wave: forward keeping H0..H4, loss gradient (relative L2 or L2),
 // backward through the layers keeping dZ1..dZ5 (bf16, natural feature order)
 __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n_host, const uint32_t* n_dev,
                                                    uint32_t offset, const float4* pos,
@@ -265,6 +265,12 @@ __global__ __launch_bounds__(256) void k_mlp_train(Net N, EncCtx E, uint32_t n_h
             const float tv[3] = {t.x, t.y, t.z};
             for (int c = 0; c < 3; ++c) {
                 const float y = o[c] + b[c];
+                if (T.l2) {  // L2: mean over samples and channels of (y - t)^2
+                    const float e = y - tv[c];
+                    loss += e * e;
+                    dz5[c] = 2.0f * e * inv_count / 3.0f;
+                    continue;
+                }
                 const float sd = sqrtf(y * y + kLossEps);
                 const float r = (y - tv[c]) / sd;
                 const float ar = fabsf(r);
@@ -464,8 +470,15 @@ int Mlp::init(uint32_t seed, uint32_t train_capacity, float lr) {
         (rc = dalloc(owned_, &tb_.H, static_cast<size_t>(train_capacity) * kHWidth)) ||
         (rc = dalloc(owned_, &tb_.DZ, static_cast<size_t>(train_capacity) * kDZWidth)) ||
         (rc = dalloc(owned_, &tb_.partial, static_cast<size_t>(tb_.blocks) * kParams)) ||
-        (rc = dalloc(owned_, &tb_.loss, 1)))
+        (rc = dalloc(owned_, &tb_.loss, 1)) || (rc = dalloc(owned_, &trained_, 1)) ||
+        (rc = dalloc(owned_, &scale_, 2)))
         return rc;
+    // the counter of trained samples starts at zero and the output scale at {1, not set}
+    // (device memory is not cleared)
+    const float sc0[2] = {1.0f, 0.0f};
+    if (hipMemset(trained_, 0, sizeof(*trained_)) != hipSuccess ||
+        hipMemcpy(scale_, sc0, sizeof(sc0), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(MT_E_HIP, "cache counter reset failed");
     // He-uniform initial weights, zero biases
     std::vector<float> w(kParams, 0.0f);
     std::mt19937 rng(seed);
@@ -531,7 +544,6 @@ int Mlp::train(uint32_t n, const float4* pos, const float4* dir, const float4* t
                float* loss_host, float* grad_host, const uint32_t* n_dev, uint32_t offset) {
     if (n == 0) return MT_OK;
     if (n > tb_.capacity) return fail(MT_E_INVALID, "training batch exceeds the cache's capacity");
-    if (!trained_ && dalloc(owned_, &trained_, 1)) return MT_E_NOMEM;
     const Net N = make_net(fwd_, bwd_, master_);
     EncCtx E{{lo_[0], lo_[1], lo_[2]}, {inv_ext_[0], inv_ext_[1], inv_ext_[2]}};
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, tb_.loss, 1u);

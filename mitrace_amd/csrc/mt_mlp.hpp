@@ -50,6 +50,7 @@ struct CacheTrainBuffers {
     float* loss;          // [blocks]
     uint32_t capacity;    // samples
     uint32_t blocks;      // weight-gradient blocks
+    int32_t l2;           // loss: 0 relative L2 (Huber-clipped), 1 L2
 };
 
 class Mlp {
@@ -71,7 +72,12 @@ class Mlp {
     int set_master(const float* in);
     void set_bounds(const float lo[3], const float hi[3]);
     void set_lr(float lr) { lr_ = lr; }
+    void set_loss(uint32_t l2) { tb_.l2 = l2 ? 1 : 0; }
     uint64_t steps() const { return step_; }
+    // the render passes' output scale: {scale, set}; the cache learns targets divided by the
+    // scale and its predictions are multiplied by it; {1, 0} until the first training chunk of
+    // a cached render sets it to that chunk's mean target
+    float* scale() const { return scale_; }
 
    private:
     int pack(hipStream_t s);
@@ -86,6 +92,7 @@ class Mlp {
     float lr_ = 1e-2f;
     uint64_t step_ = 0;
     unsigned long long* trained_ = nullptr;  // samples of all non-empty steps
+    float* scale_ = nullptr;
 };
 
 }  // namespace mt

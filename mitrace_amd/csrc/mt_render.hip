@@ -129,6 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers 
         uint32_t slot = kNoSlot;
         if (c.train_every != kNoSlot && q % c.train_every == 0) slot = q / c.train_every;
         b.train_slot[p] = slot;
+        if (c.cache_nee) b.acc_n[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (slot < c.train_cap) {
             b.tr_path[slot] = p;
             for (uint32_t j = 0; j < c.train_nrec; ++j)
@@ -535,7 +536,8 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
                     if (h.inst < 0 || h.dist > so.w) {  // Tracer.cpp:322
                         const uint32_t p = __float_as_uint(b.sh_d[item].w);
                         const float4 cc = b.sh_c[item];
-                        float4* acc = (__float_as_uint(cc.w) == 0u) ? b.acc_d : b.acc_i;
+                        const uint32_t kind = __float_as_uint(cc.w);  // direct, indirect, environment sample
+                        float4* acc = kind == 0u ? b.acc_d : (kind == 1u ? b.acc_i : b.acc_n);
                         const float4 a = acc[p];
                         acc[p] = make_float4(a.x + cc.x, a.y + cc.y, a.z + cc.z, 0.0f);
                         // one shadow ray per path and bounce: no other lane writes this word
@@ -746,6 +748,114 @@ __device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const Mat
     }
 }
 
+// ------------------------------------------------------------------ environment samples
+// (training paths of a cached render with cache_nee)
+constexpr int kPbCos = 129, kPbRough = 65;
+constexpr uint32_t kDimNee = 0x9E000000u;  // dimensions of the training paths' extra draws
+
+// Chance that the specular lobe's half-vector draw (Tracer.cpp:49-72) reflects the view below
+// the surface, so that a cosine direction replaces it: entry (ri, ci) for roughness
+// ri / (kPbRough - 1) and cos(theta_o) = ci / (kPbCos - 1).  The half vector's polar angle t
+// follows from u1 (tan^2 t = a^2 u1 / max(1 - u1, 1e-4), a = max(r^2, 0.0025)) and its azimuth
+// is uniform; flipping h toward the view does not move the reflection, which lies below the
+// surface where 2 (h.v)(h.n) <= cos(theta_o), i.e. where
+// sin(2t) sin(theta_o) cos(phi) <= -cos(theta_o) cos(2t): a fraction 1 - acos(c) / pi of the
+// azimuths, c = the right side over sin(2t) sin(theta_o).  Midpoint rule over u1, in double.
+__global__ void k_pbelow_table(float* table) {
+    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= kPbCos * kPbRough) return;
+    const int ri = e / kPbCos, ci = e % kPbCos;
+    const double co = static_cast<double>(ci) / (kPbCos - 1), r = static_cast<double>(ri) / (kPbRough - 1);
+    const double so = sqrt(fmax(0.0, 1.0 - co * co));
+    const double a = fmax(r * r, 0.0025), a2 = a * a;
+    constexpr int n = 8192;
+    double below = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double u = (i + 0.5) / n;
+        const double t2 = a2 * u / fmax(1.0 - u, 1e-4);
+        const double ct = 1.0 / sqrt(1.0 + t2), st = sqrt(fmax(0.0, 1.0 - ct * ct));
+        const double num = -co * (ct * ct - st * st), den = 2.0 * st * ct * so;
+        if (den > 0.0) below += 1.0 - acos(fmin(1.0, fmax(-1.0, num / den))) / 3.14159265358979323846;
+        else below += (num >= 0.0) ? 1.0 : 0.0;
+    }
+    table[ri * kPbCos + ci] = static_cast<float>(below / n);
+}
+
+__device__ __forceinline__ float pbelow(const float* tab, float co, float r) {
+    const float x = clampf(co, 0.0f, 1.0f) * static_cast<float>(kPbCos - 1);
+    const float y = clampf(r, 0.0f, 1.0f) * static_cast<float>(kPbRough - 1);
+    const int x0 = min(static_cast<int>(x), kPbCos - 2), y0 = min(static_cast<int>(y), kPbRough - 2);
+    const float fx = x - static_cast<float>(x0), fy = y - static_cast<float>(y0);
+    const float* t = tab + y0 * kPbCos + x0;
+    return mixf(mixf(t[0], t[1], fx), mixf(t[kPbCos], t[kPbCos + 1], fx), fy);
+}
+
+// The true density of the reference's bounce direction at l (Tracer.cpp:177-200): the mixture
+// pdf it divides by, plus the cosine directions that replace specular draws reflected below.
+__device__ __forceinline__ float bounce_density(const ChunkParams& c, V3 l, V3 view, V3 N, float ps, float rough,
+                                                float pdf_ref) {
+    const float nl = dot(N, l);
+    if (!(nl > 0.0f)) return 0.0f;
+    return pdf_ref + ps * pbelow(c.pbelow, dot(N, view), rough) * nl * (1.0f / kPi);
+}
+
+// The environment's sampling distribution: cell (i, j) of its W x H texel grid with chance
+// (marg[j+1] - marg[j]) (cond_j[i+1] - cond_j[i]), uniform in (u, v) inside the cell, where
+// u = (atan2(d.z, d.x) + pi) / 2pi (+ the rotation offset) and v = acos(d.y) / pi as in
+// sample_env; solid-angle density = cell chance * W * H / (2 pi^2 sin(pi v)).
+__device__ __forceinline__ float env_cell_chance(const ChunkParams& c, uint32_t W, int i, int j) {
+    const float* row = c.env_cond + static_cast<size_t>(j) * (W + 1);
+    return (c.env_marg[j + 1] - c.env_marg[j]) * (row[i + 1] - row[i]);
+}
+
+__device__ float env_pdf(const ChunkParams& c, const DevScene& S, const ChunkParams& J, V3 d) {
+    const int W = static_cast<int>(S.env_w), H = static_cast<int>(S.env_h);
+    const float l = len(d);
+    const float cy = d.y / l;
+    float u = (em::atan2(d.z, d.x) + kPi) / (2.0f * kPi);
+    if (J.hdri_rot_apply) u = u * 1.0f + J.hdri_rot_offset;
+    const float v = em::acos(cy) / kPi;
+    const float x = u - floorf(u), y = v - floorf(v);
+    const int i = min(max(static_cast<int>(x * static_cast<float>(W)), 0), W - 1);
+    const int j = min(max(static_cast<int>(y * static_cast<float>(H)), 0), H - 1);
+    const float sphi = __builtin_sqrtf(gmax(0.0f, 1.0f - cy * cy));
+    if (!(sphi > 0.0f)) return 0.0f;
+    return env_cell_chance(c, S.env_w, i, j) * static_cast<float>(W) * static_cast<float>(H) /
+           (2.0f * kPi * kPi * sphi);
+}
+
+__device__ bool env_sample(const ChunkParams& c, const DevScene& S, const ChunkParams& J, float u1, float u2,
+                           float u3, float u4, V3& dir, float& pdf) {
+    const int W = static_cast<int>(S.env_w), H = static_cast<int>(S.env_h);
+    int lo = 0, hi = H;  // the last row j with marg[j] <= u1 (marg[0] = 0, marg[H] = 1)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (c.env_marg[mid] <= u1) lo = mid; else hi = mid;
+    }
+    const int j = lo;
+    const float* row = c.env_cond + static_cast<size_t>(j) * (W + 1);
+    lo = 0;
+    hi = W;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (row[mid] <= u2) lo = mid; else hi = mid;
+    }
+    const int i = lo;
+    const float chance = (c.env_marg[j + 1] - c.env_marg[j]) * (row[i + 1] - row[i]);
+    if (!(chance > 0.0f)) return false;
+    float u = (static_cast<float>(i) + u3) / static_cast<float>(W);
+    const float v = (static_cast<float>(j) + u4) / static_cast<float>(H);
+    if (J.hdri_rot_apply) u -= J.hdri_rot_offset;
+    u -= floorf(u);
+    float sp, cp, sth, cth;
+    em::sincos(v * kPi, sp, cp);
+    em::sincos(u * (2.0f * kPi) - kPi, sth, cth);
+    if (!(sp > 0.0f)) return false;
+    dir = V3{sp * cth, cp, sp * sth};
+    pdf = chance * static_cast<float>(W) * static_cast<float>(H) / (2.0f * kPi * kPi * sp);
+    return true;
+}
+
 // The shading of one bounce for every path in the extension queue (Tracer.cpp:134-340).
 // kDiff 0: plain render; 1: differentiable forward (parameter overrides, shadow visibility
 // bits); 2: adjoint replay -- the same decisions, no shadow rays (visibility comes from the
@@ -779,6 +889,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
         bool cont = false, shadow = false;
         V3 next_o = mk(0.0f), next_d = mk(0.0f);  // the continuing path's next ray
         V3 next_term = mk(0.0f), next_base = mk(0.0f), next_trad = mk(0.0f);
+        float next_q = 0.0f;  // training paths (cache_nee): true density of the bounce direction
         float4 sho{}, shd{}, shc{};
         uint32_t p = 0;
         // cache query of a path that stops here
@@ -836,7 +947,16 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 misses++;
                 const V3 col = (k > 0) ? xyz(b.prev_base[k & 1][idx]) : mk(1.0f);
                 const float mult = (k == 0) ? J.hdri_primary : J.hdri_secondary;
-                const V3 sk = scale(sample_env(S, d, J.hdri_rot_apply, J.hdri_rot_offset), mult);
+                V3 sk = scale(sample_env(S, d, J.hdri_rot_apply, J.hdri_rot_offset), mult);
+                if (kDiff == 0 && !kBatch && kExt && c.env_pc > 0.0f && k > c.cache_depth &&
+                    b.train_slot[p] < c.train_cap) {
+                    // a training path's bounce ray from a vertex that also drew environment
+                    // samples: balance heuristic against them (the true density of this
+                    // direction was stored with the previous base colour)
+                    const float q = b.prev_base[k & 1][idx].w;
+                    const float pe = c.env_pc * env_pdf(c, S, J, d);
+                    if (pe > 0.0f) sk = scale(sk, q / (q + pe));
+                }
                 b.acc_s[p] = f4(mul(mul(col, trad), sk), 0.0f);
                 if (kDiff == 2 && k > 0) {
                     const float4 pv = b.adj_prev[p];
@@ -985,6 +1105,13 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     gvalid = true;
                     g_e = mul(mul(dl, scale(trad, J.emission_base)), ms.em_scale);
                 }
+                // a training path's vertex from its query vertex on (cache_nee): its shadow ray
+                // goes to an environment sample with probability env_pc, else to the
+                // reference's light choice with weight 1 / (1 - env_pc)
+                const bool nee_v = kDiff == 0 && !kBatch && kExt && c.env_pc > 0.0f && k >= c.cache_depth &&
+                                   b.train_slot[p] < c.train_cap;
+                const bool use_env = nee_v && (c.env_pc >= 1.0f || draw(key, kDimNee + k * 8u) < c.env_pc);
+                if (nee_v) next_q = bounce_density(c, nd, view, N, ps, rough_s, pdf);
                 int chosen = -1;
                 float chosen_pdf = 0.0f, total = 0.0f;
                 const uint32_t n_lights = kLights ? S.light_count : 0u;
@@ -1056,13 +1183,60 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                             }
                             g_base = add(g_base, V3{gb[0], gb[1], gb[2]});
                         }
-                    } else {
+                    } else if (!use_env) {
                         shadow = true;
+                        const float wl = nee_v ? 1.0f / (1.0f - c.env_pc) : 1.0f;
                         sho = f4(add(wpos, scale(lvec, kPullout)), llen);
                         shd = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(p));
-                        shc = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(k == 0 ? 0u : 1u));
+                        shc = make_float4(contrib.x * wl, contrib.y * wl, contrib.z * wl,
+                                          __uint_as_float(k == 0 ? 0u : 1u));
                     }
                     trad_next = mul(trad, energy);
+                }
+                if (use_env && !last) {
+                    // environment sample: the sky term the next bounce would add
+                    // (Tracer.cpp:244-252: base' x throughput x sky x secondary intensity, the
+                    // throughput updated only where a light was chosen, Tracer.cpp:273, 336),
+                    // where the path would go on past this vertex (Tracer.cpp:206-208), by the
+                    // balance heuristic with the bounce's own draw of the same term
+                    V3 ed;
+                    float pe;
+                    const uint32_t dn = kDimNee + k * 8u;
+                    if (env_sample(c, S, J, draw(key, dn + 1), draw(key, dn + 2), draw(key, dn + 3),
+                                   draw(key, dn + 4), ed, pe)) {
+                        const float nle = dot(N, ed);
+                        const float pref = (1.0f - ps) * diffuse_pdf(ed, N) + ps * specular_pdf(ed, view, N, rough_s);
+                        if (nle > 0.0f && pref > 0.0f) {
+                            const float q = bounce_density(c, ed, view, N, ps, rough_s, pref);
+                            const V3 ee = scale(eval_brdf(ed, view, N, basec, rough, metal), nle / pref);
+                            // chance that the path goes on past this vertex in direction ed:
+                            // the energy cut-off, or the roulette on the termination throughput
+                            const V3 tt = mul(xyz(b.t_term[k & 1][idx]), ee);
+                            float keep = 1.0f;
+                            if (J.mode == MT_MODE_FIXED) {
+                                if (k + 1 >= J.rr_depth) keep = clampf(gmax(gmax(tt.x, tt.y), tt.z), 0.05f, 1.0f);
+                            } else if (len(tt) < J.terminate_energy) {
+                                keep = 0.0f;
+                            }
+                            if (keep > 0.0f && q > 0.0f) {
+                                // where a light was chosen the throughput takes the energy over
+                                // the roulette's chance (which cancels it), elsewhere it stays
+                                // and only the chance of going on weighs the term
+                                const V3 thr = (chosen >= 0) ? mul(trad, ee) : scale(trad, keep);
+                                const V3 sky = scale(sample_env(S, ed, J.hdri_rot_apply, J.hdri_rot_offset),
+                                                     J.hdri_secondary);
+                                const V3 val = scale(mul(mul(basec, thr), sky), q / (q + c.env_pc * pe));
+                                if (val.x > 0.0f || val.y > 0.0f || val.z > 0.0f) {
+                                    shadow = true;
+                                    // no light distance: any occluder counts (kFltMax, whose
+                                    // next float, the any-hit bound, is +inf)
+                                    sho = f4(add(add(o, scale(d, dist)), scale(N, kPullout)), kFltMax);
+                                    shd = make_float4(ed.x, ed.y, ed.z, __uint_as_float(p));
+                                    shc = make_float4(val.x, val.y, val.z, __uint_as_float(2u));
+                                }
+                            }
+                        }
+                    }
                 }
                 if (kDiff == 2) {
                     if (gr.sampling) {
@@ -1083,8 +1257,15 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     b.adj_s[p] = f4(ssum, 0.0f);
                     b.adj_prev[p] = make_float4(tu, tv, __uint_as_float(ms.tex ? mi : 0xFFFFFFFFu), 0.0f);
                 }
-                const bool after_q = !stop && !last && kDiff == 0 && !kBatch && kExt && c.cache &&
-                                     c.cache_after && k == c.cache_depth;
+                const bool after_q = (!stop || c.cache_nee) && !last && kDiff == 0 && !kBatch && kExt &&
+                                     c.cache && c.cache_after && k == c.cache_depth;
+                if (after_q && stop && b.train_slot[p] < c.train_cap) {
+                    // cache_nee: a training path that stops here has its record completed when
+                    // the path ends (all it collects after this point is environment samples)
+                    const size_t r = static_cast<size_t>(b.train_slot[p]) * c.train_nrec;
+                    const float4 tw = b.tr_w[r];
+                    b.tr_w[r] = make_float4(tw.x, tw.y, tw.z, -1.0f);
+                }
                 if (after_q) {
                     // a path that would go on asks the cache for what the bounce ray would
                     // bring (scaled by the throughput at this vertex); a training path asks too
@@ -1139,7 +1320,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             if (!kBatch && c.bin) b.bin_key[cs] = ray_bin(c, next_o, next_d);
             b.t_term[(k + 1) & 1][cs] = f4(next_term, 0.0f);
             b.t_rad[(k + 1) & 1][cs] = f4(next_trad, 0.0f);
-            b.prev_base[(k + 1) & 1][cs] = f4(next_base, 0.0f);
+            b.prev_base[(k + 1) & 1][cs] = f4(next_base, next_q);
         }
     }
     if (c.count_work) {
@@ -1222,9 +1403,20 @@ __device__ __forceinline__ void finalize_path(const ChunkParams& c, const PathBu
         const uint32_t slot = b.train_slot[p];
         if (slot < c.train_cap) {
             const size_t r = static_cast<size_t>(slot) * c.train_nrec;
-            if (b.tr_w[r].w > 0.0f) Limg = add(xyz(b.tr_s[r]), xyz(b.tr_img[p]));
+            const float4 tw = b.tr_w[r];
+            if (tw.w < 0.0f) {
+                // cache_nee: the path stopped after its query vertex, so everything it collected
+                // (environment samples aside) came up to and including that vertex's light sample
+                b.tr_s[r] = f4(L, 0.0f);
+                b.tr_w[r] = make_float4(tw.x, tw.y, tw.z, 1.0f);
+                Limg = add(L, xyz(b.tr_img[p]));
+            } else if (tw.w > 0.0f) {
+                Limg = add(xyz(b.tr_s[r]), xyz(b.tr_img[p]));
+            }
         }
     }
+    if (!kBatch && c.cache_full_image && b.train_slot[p] < c.train_cap)
+        Limg = c.cache_nee ? add(L, xyz(b.acc_n[p])) : L;
     V3 col = J.output_linear ? Limg : aces(Limg, J.exposure);
     if (!kBatch && c.debug) col = debug_color(c, b, p, e, d, i, s, col);
     b.sample_rgb[p] = f4(col, 0.0f);
@@ -1555,7 +1747,8 @@ __global__ __launch_bounds__(kBlock) void k_cache_apply(ChunkParams c, PathBuffe
     if (q >= b.counters[kCntQuery]) return;
     const uint32_t p = b.cq_path[q];
     const float4 o = b.cq_out[q];
-    const V3 pred = {gmax(o.x, 0.0f), gmax(o.y, 0.0f), gmax(o.z, 0.0f)};
+    const float sc = c.cache_scale[0];
+    const V3 pred = {gmax(o.x, 0.0f) * sc, gmax(o.y, 0.0f) * sc, gmax(o.z, 0.0f) * sc};
     const V3 add_ = mul(xyz(b.cq_t[q]), pred);
     if (c.cache_after && b.train_slot[p] < c.train_cap) {
         // a training path keeps its full radiance for its target; its image value is formed
@@ -1591,6 +1784,7 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(ChunkParams c, PathBuf
     if (ok_slot) {
         p = b.tr_path[slot];
         L = add(add(add(xyz(b.acc_e[p]), xyz(b.acc_d[p])), xyz(b.acc_i[p])), xyz(b.acc_s[p]));
+        if (c.cache_nee) L = add(L, xyz(b.acc_n[p]));
     }
     for (uint32_t j = 0; j < c.train_nrec; ++j) {
         bool ok = false;
@@ -1616,6 +1810,48 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(ChunkParams c, PathBuf
             b.tp_tgt[ts] = tgt;
         }
     }
+}
+
+// The scale of the cache's targets, set on the first training chunk that the cache sees: with
+// the L2 loss the mean target (over channels and samples; 1 if that is not positive and
+// finite), with the relative loss 1.  One block.
+__global__ __launch_bounds__(1024) void k_cache_scale_init(ChunkParams c, PathBuffers b) {
+    __shared__ double part[1024];
+    if (c.cache_scale[1] != 0.0f) return;  // set already (uniform over the block)
+    const uint32_t n = b.counters[kCntTrainRec];
+    double sum = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const float4 t = b.tp_tgt[i];
+        sum += (static_cast<double>(t.x) + t.y) + t.z;
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double m = n ? part[0] / (3.0 * n) : 0.0;
+        // the relative loss does not depend on the units (but for its 0.01): no scale there
+        c.cache_scale[0] = (c.cache_l2 && m > 0.0 && m < 3.0e38) ? static_cast<float>(m) : 1.0f;
+        c.cache_scale[1] = 1.0f;
+    }
+}
+
+// The chunk's training samples in the order the Adam steps take them, targets in the cache's
+// units.  The records come in path order (pixel rows), and a pass's steps may use only part of
+// them: sample i is record (i * (2^31 - 1)) mod n, a permutation for every n < 2^31 (the stride
+// is prime) that spreads every step's consecutive samples over the whole chunk.
+__global__ __launch_bounds__(kBlock) void k_cache_shuffle(ChunkParams c, PathBuffers b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = b.counters[kCntTrainRec];
+    if (i >= n) return;
+    const uint32_t j = static_cast<uint32_t>((static_cast<uint64_t>(i) * 2147483647ull) % n);
+    const float inv = 1.0f / c.cache_scale[0];
+    const float4 t = b.tp_tgt[j];
+    b.ts_pos[i] = b.tp_pos[j];
+    b.ts_dir[i] = b.tp_dir[j];
+    b.ts_tgt[i] = make_float4(t.x * inv, t.y * inv, t.z * inv, t.w);
 }
 
 __global__ __launch_bounds__(kBlock) void k_cache_resolve(const float* sum, uint32_t n, float spp,
@@ -2548,6 +2784,8 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
     if (copt_.query_depth >= st.bounces)
         return fail(MT_E_INVALID, "query_depth must be below the bounce count");
     if (st.debug_mode) return fail(MT_E_INVALID, "debug views are rendered by mt_render");
+    if (copt_.train_nee && !copt_.query_after_light)
+        return fail(MT_E_INVALID, "train_nee needs query_after_light");
     total_spp_ = st.samples;
     mt_settings pass = st;
     pass.samples = copt_.spp_per_pass;
@@ -2574,7 +2812,9 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
             (rc = alloc(owned_, &b.cq_t, P)) ||
             (rc = alloc(owned_, &b.tr_img, P)) ||
             (rc = alloc(owned_, &b.tp_pos, tslots * nrec)) || (rc = alloc(owned_, &b.tp_dir, tslots * nrec)) ||
-            (rc = alloc(owned_, &b.tp_tgt, tslots * nrec)))
+            (rc = alloc(owned_, &b.tp_tgt, tslots * nrec)) ||
+            (rc = alloc(owned_, &b.ts_pos, tslots * nrec)) || (rc = alloc(owned_, &b.ts_dir, tslots * nrec)) ||
+            (rc = alloc(owned_, &b.ts_tgt, tslots * nrec)))
             return rc;
         b.img_sum = img_sum;
         HIP_TRY(hipEventCreateWithFlags(&S.ev_bounce, hipEventDisableTiming));
@@ -2585,8 +2825,77 @@ int Renderer::setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_
     params_.cache = 1;
     params_.cache_depth = copt_.query_depth;
     params_.cache_after = copt_.query_after_light ? 1 : 0;
+    params_.cache_l2 = copt_.loss ? 1 : 0;
     params_.train_cap = static_cast<uint32_t>(tslots);
     params_.train_nrec = nrec;
+    // test hook: every training path's image sample is the full estimate its target comes from
+    params_.cache_full_image = env_u32("MT_CACHE_TRAINING_IMAGE", 0) ? 1 : 0;
+    if (copt_.train_nee) {
+        params_.cache_nee = 1;
+        for (StreamSlot& S : slots_) {
+            if ((rc = alloc(owned_, &S.b.acc_n, P))) return rc;
+        }
+        buf_ = slots_[0].b;
+        if ((rc = setup_env_sampling())) return rc;
+    }
+    return MT_OK;
+}
+
+// The training paths' environment sampling (cache_nee): the P_below table of the bounce
+// density and, for a scene with a non-black environment, the two-level CDF over its texels.
+// A texel's weight is its mean channel value, plus (linear filter) the three neighbours the
+// filter blends in with weights up to one texel in uv units, times sin(theta) of its row.
+int Renderer::setup_env_sampling() {
+    int rc;
+    float* tab = nullptr;
+    if ((rc = alloc(owned_, &tab, static_cast<size_t>(kPbCos) * kPbRough))) return rc;
+    hipLaunchKernelGGL(k_pbelow_table, dim3((kPbCos * kPbRough + 255) / 256), dim3(256), 0, stream_, tab);
+    HIP_TRY(hipGetLastError());
+    params_.pbelow = tab;
+    params_.env_pc = 0.0f;
+    const DevScene& D = scene_->dev;
+    if (!D.has_env || !D.env_w || !D.env_h) return MT_OK;
+    const uint32_t W = D.env_w, H = D.env_h;
+    std::vector<float4> env(static_cast<size_t>(W) * H);
+    HIP_TRY(hipMemcpy(env.data(), D.env, sizeof(float4) * env.size(), hipMemcpyDeviceToHost));
+    auto lum = [&](uint32_t i, uint32_t j) {
+        const float4 t = env[static_cast<size_t>(std::min(j, H - 1)) * W + std::min(i, W - 1)];
+        const double m = (static_cast<double>(t.x) + t.y + t.z) / 3.0;
+        return (m > 0.0 && std::isfinite(m)) ? m : 0.0;
+    };
+    const double blend = D.env_filter == MT_FILTER_NEAREST ? 0.0 : 1.0 / std::min(W, H);
+    std::vector<float> marg(H + 1), cond(static_cast<size_t>(H) * (W + 1));
+    std::vector<double> rows(H), cum(W + 1);
+    double total = 0.0;
+    for (uint32_t j = 0; j < H; ++j) {
+        const double sth = std::sin(3.14159265358979323846 * (j + 0.5) / H);
+        cum[0] = 0.0;
+        for (uint32_t i = 0; i < W; ++i) {
+            const double w = (lum(i, j) + blend * (lum(i + 1, j) + lum(i, j + 1) + lum(i + 1, j + 1))) * sth;
+            cum[i + 1] = cum[i] + w;
+        }
+        float* row = cond.data() + static_cast<size_t>(j) * (W + 1);
+        for (uint32_t i = 0; i <= W; ++i)
+            row[i] = cum[W] > 0.0 ? static_cast<float>(cum[i] / cum[W]) : static_cast<float>(i) / W;
+        row[W] = 1.0f;
+        rows[j] = cum[W];
+        total += cum[W];
+    }
+    if (!(total > 0.0)) return MT_OK;  // a black environment: nothing to sample
+    double acc = 0.0;
+    marg[0] = 0.0f;
+    for (uint32_t j = 0; j < H; ++j) {
+        acc += rows[j];
+        marg[j + 1] = static_cast<float>(acc / total);
+    }
+    marg[H] = 1.0f;
+    float *dm = nullptr, *dc = nullptr;
+    if ((rc = alloc(owned_, &dm, marg.size())) || (rc = alloc(owned_, &dc, cond.size()))) return rc;
+    HIP_TRY(hipMemcpy(dm, marg.data(), sizeof(float) * marg.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dc, cond.data(), sizeof(float) * cond.size(), hipMemcpyHostToDevice));
+    params_.env_marg = dm;
+    params_.env_cond = dc;
+    params_.env_pc = scene_->light_count ? 0.5f : 1.0f;
     return MT_OK;
 }
 
@@ -2613,7 +2922,8 @@ int Renderer::run_cache_chunk(Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t 
     for (uint32_t k = 0; k < st_.bounces; ++k) {
         launch_trace<0>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, b, k);
         hipLaunchKernelGGL((k_shade<0, false, true>), dim3(shade_blocks_), dim3(kBlock), 0, s, scene_->dev, c, b, k);
-        if (scene_->light_count) launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, b, k);
+        if (scene_->light_count || c.env_pc > 0.0f)
+            launch_trace<1>(lds_depth_, false, trace_blocks_, s, scene_->dev, c, b, k);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(S.ev_bounce, s));
@@ -2634,9 +2944,12 @@ int Renderer::run_cache_chunk(Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t 
     if (nslots) {
         hipLaunchKernelGGL(k_cache_targets, dim3((nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, m, c, b,
                            slot0, nslots);
+        hipLaunchKernelGGL(k_cache_scale_init, dim3(1), dim3(1024), 0, m, c, b);
+        hipLaunchKernelGGL(k_cache_shuffle, dim3((nslots * c.train_nrec + kBlock - 1) / kBlock), dim3(kBlock),
+                           0, m, c, b);
         const uint32_t batch = copt_.train_batch;
         for (uint32_t st = 0; st < copt_.steps_per_pass; ++st)
-            if ((rc = mlp.train(batch, b.tp_pos, b.tp_dir, b.tp_tgt, m, nullptr, nullptr,
+            if ((rc = mlp.train(batch, b.ts_pos, b.ts_dir, b.ts_tgt, m, nullptr, nullptr,
                                 b.counters + kCntTrainRec, st * batch)))
                 return rc;
     }
@@ -2650,6 +2963,7 @@ int Renderer::render_cached(Mlp& mlp, float* rgb_out) {
     const uint64_t npix_total = static_cast<uint64_t>(st_.width) * st_.height;
     hipStream_t m = tail_stream_;
     mlp.set_bounds(scene_->world_lo, scene_->world_hi);
+    params_.cache_scale = mlp.scale();
     const uint64_t trained0 = mlp.trained_samples();
     HIP_TRY(hipDeviceSynchronize());
     hipLaunchKernelGGL(k_fill, dim3(static_cast<uint32_t>((npix_total * 3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, m,

@@ -80,6 +80,19 @@ struct ChunkParams {
     int32_t cache;
     uint32_t cache_depth, train_every, train_offset, train_cap, train_nrec;
     int32_t cache_after;  // the query follows the vertex's emission and light sample
+    // cache_nee: the query is made whenever a bounce is left, a stopped training path's record
+    // is completed at the end of the path, and from its query vertex on a training path draws
+    // environment samples with probability env_pc per vertex (0: no environment sampling)
+    int32_t cache_nee;
+    float env_pc;
+    const float* env_marg;  // [env_h + 1] CDF of the environment's sampling weight over rows
+    const float* env_cond;  // [env_h][env_w + 1] CDF over the cells of every row
+    const float* pbelow;    // [kPbRough][kPbCos] chance that a specular draw reflects below
+    int32_t cache_full_image;  // test hook: a training path's sample is its full estimate
+    // {scale, set}: the cache learns targets divided by the scale (the first training chunk's
+    // mean target) and its predictions are multiplied by it
+    float* cache_scale;
+    int32_t cache_l2;  // the cache trains with the L2 loss (scaled targets)
     int32_t debug;  // MT_DEBUG_* view (plain renders)
     int32_t accel;  // 1 = fast traversal (MT_TRAVERSAL_FAST)
     // ray binning before the extend of bounces >= 1 (plain renders): shading writes every
@@ -136,6 +149,8 @@ struct PathBuffers {
     float4* tr_img;         // per path (after-light mode): throughput x prediction of a
                             // training path's own query, the cached estimate its image uses
     float4 *tp_pos, *tp_dir, *tp_tgt;       // compact training samples
+    float4 *ts_pos, *ts_dir, *ts_tgt;       // the same in the order the Adam steps take them
+    float4* acc_n;          // per path (cache_nee): radiance of the environment samples
     float* img_sum;         // per pixel: sum of the accumulated passes' samples
     // debug views (first-hit records of ProcessDebugRay)
     float4 *dbg_a, *dbg_b, *dbg_c, *dbg_d, *dbg_e;  // {base', metal} {rough, geom normal}
@@ -188,6 +203,7 @@ class Renderer {
     int setup_cached(mt_scene* scene, const mt_camera_desc& cam, const mt_settings& st,
                      const mt_cache_options& opt);
     int render_cached(class Mlp& mlp, float* rgb_out);
+    int setup_env_sampling();
     int finish(float* rgb_out);
     uint32_t chunks() const { return chunks_; }
     const mt_stats& stats() const { return stats_; }

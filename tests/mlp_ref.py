@@ -3,7 +3,7 @@ BF16 weights, BF16 inputs to every product (the encoding and each hidden activat
 backward gradients), FP32 accumulation and biases.  Layer l: z = W_l h + b_l, ReLU on the
 four hidden layers, linear output; loss = mean over samples and channels of the
 Huber function (r^2 up to |r| = 10, linear beyond) of r = (y - t) / sqrt(y^2 + 0.01) with the
-denominator held constant."""
+denominator held constant, or (l2=True) of (y - t)^2."""
 import numpy as np
 
 LAYERS = [(32, 64), (64, 64), (64, 64), (64, 64), (64, 3)]
@@ -65,14 +65,19 @@ def forward(w, enc):
     return h, hs, zs
 
 
-def loss_and_grad(w, enc, target):
+def loss_and_grad(w, enc, target, l2=False):
     y, hs, zs = forward(w, enc)
     n = len(enc)
-    sd = np.sqrt(y * y + EPS)
-    r = (y - target) / sd
-    ar = np.abs(r)
-    loss = float(np.where(ar <= HUBER, r * r, HUBER * (2 * ar - HUBER)).sum() / (3 * n))
-    dz = bf16((2 * np.clip(r, -HUBER, HUBER) / sd / n / 3).astype(DTYPE))
+    if l2:
+        e = y - target
+        loss = float((e * e).sum() / (3 * n))
+        dz = bf16((2 * e / n / 3).astype(DTYPE))
+    else:
+        sd = np.sqrt(y * y + EPS)
+        r = (y - target) / sd
+        ar = np.abs(r)
+        loss = float(np.where(ar <= HUBER, r * r, HUBER * (2 * ar - HUBER)).sum() / (3 * n))
+        dz = bf16((2 * np.clip(r, -HUBER, HUBER) / sd / n / 3).astype(DTYPE))
     grads = []
     layers = unpack(w)
     for l in range(4, -1, -1):

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_function():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(abi.SIGNATURES), set(names) ^ set(abi.SIGNATURES)
-    assert L.mt_abi_version() == 2
+    assert L.mt_abi_version() == 3
 
 
 def test_struct_layouts_match_the_header():

@@ -54,16 +54,17 @@ def test_inference_matches_numpy(cornell):
     assert np.mean(err == 0) > 0.8
 
 
-def test_training_gradient_matches_numpy(cornell):
+@pytest.mark.parametrize("l2", [0, 1])
+def test_training_gradient_matches_numpy(cornell, l2):
     from mitrace_amd.cache import RadianceCache
-    c = RadianceCache(cornell, {"seed": 5, "learning_rate": 1e-3})
+    c = RadianceCache(cornell, {"seed": 5, "learning_rate": 1e-3, "loss": l2})
     w = c.weights()
     pos, d = _inputs(cornell, 2000, 2)
     rng = np.random.default_rng(3)
     target = rng.uniform(0.0, 2.0, size=(2000, 3)).astype(np.float32)
     loss, grad = c.train(pos, d, target, want_grad=True)
     lo, hi = _bounds(cornell)
-    ref_loss, ref_grad = loss_and_grad(w, encode(pos, d, lo, hi), target)
+    ref_loss, ref_grad = loss_and_grad(w, encode(pos, d, lo, hi), target, l2=bool(l2))
     assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss)  # measured <= 3e-6
     from mitrace_amd.cache import LAYERS
     o = 0

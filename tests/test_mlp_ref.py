@@ -37,6 +37,33 @@ def test_reference_gradient_matches_central_differences(monkeypatch):
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
 
 
+def test_reference_l2_gradient_matches_central_differences(monkeypatch):
+    """the L2 loss (mean of (y - t)^2): its gradient by central differences, and the property
+    that makes it mean-preserving: the output biases' gradient is twice the mean residual"""
+    monkeypatch.setattr(mlp_ref, "bf16", lambda x: np.asarray(x, np.float64))
+    monkeypatch.setattr(mlp_ref, "DTYPE", np.float64)
+    rng = np.random.default_rng(2)
+    w = rng.normal(0, 0.3, 14787)
+    enc = rng.normal(size=(40, 32))
+    t = rng.uniform(0, 3, size=(40, 3))
+    loss, g = mlp_ref.loss_and_grad(w, enc, t, l2=True)
+
+    def f(wv):
+        y, _, _ = mlp_ref.forward(wv, enc)
+        return ((y - t) ** 2).sum() / (3 * len(enc))
+
+    assert abs(f(w) - loss) < 1e-12
+    idx = np.concatenate([rng.choice(14787, 60, replace=False), [14784, 14785, 14786]])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(w)
+        e[i] = h
+        fd = (f(w + e) - f(w - e)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+    y, _, _ = mlp_ref.forward(w, enc)
+    np.testing.assert_allclose(g[-3:], 2 * (y - t).mean(0) / 3, rtol=1e-10)
+
+
 def test_reference_forward_is_the_layer_stack():
     rng = np.random.default_rng(1)
     w = rng.normal(0, 0.3, 14787).astype(np.float32)
