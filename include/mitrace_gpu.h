@@ -195,6 +195,11 @@ typedef struct mt_bvh_options {
     uint32_t device_build_min_tris; /* build the reference-rule BVH of every mesh with at least
                                this many triangles on the GPU, bit-identical to the host
                                build (BVH.cpp:117-272 rules); 0 = all on the host            */
+    uint32_t fast_device_build_min_tris; /* with fast: build the fast traversal's tree of every
+                               mesh with at least this many triangles on the GPU as a linear
+                               BVH (triangles in Morton order of their box centres, subtrees of
+                               at most 4 triangles as leaves) instead of the host's binned SAH;
+                               0 = all on the host (no reference counterpart)                */
 } mt_bvh_options;
 
 typedef struct mt_scene_desc {

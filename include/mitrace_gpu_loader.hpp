@@ -710,7 +710,7 @@ struct LoadedScene {
     std::vector<uint8_t> env_rgb8;
     std::vector<float> env_rgbf;
     uint32_t env_width = 0, env_height = 0;
-    mt_bvh_options bvh{0, 32, 32, 0, 65536};
+    mt_bvh_options bvh{0, 32, 32, 0, 65536, 0};
 
     /* --hdri-override (main.cpp:108-112): an equirect from an 8-bit image (as the reference),
      * or float RGB from a Radiance .hdr; always sampled bilinearly (Texture.cpp:35-50) */
@@ -1183,7 +1183,7 @@ inline mt_settings ToSettings(const CliOptions& o) {
  * Mesh.cpp:37-40) */
 inline mt_bvh_options ToBvhOptions(const CliOptions& o) {
     mt_bvh_options b{o.bvhDisable ? 1 : 0, o.bvhMaxTriangles, o.bvhMaxDepth,
-                     o.traversal == MT_TRAVERSAL_FAST ? 1 : 0, 65536};
+                     o.traversal == MT_TRAVERSAL_FAST ? 1 : 0, 65536, 0};
     return b;
 }
 

@@ -89,7 +89,8 @@ class mt_env_desc(C.Structure):
 
 class mt_bvh_options(C.Structure):
     _fields_ = [("disable", C.c_int32), ("max_triangles", C.c_uint32), ("max_depth", C.c_uint32),
-                ("fast", C.c_int32), ("device_build_min_tris", C.c_uint32)]
+                ("fast", C.c_int32), ("device_build_min_tris", C.c_uint32),
+                ("fast_device_build_min_tris", C.c_uint32)]
 
 
 class mt_scene_desc(C.Structure):

@@ -495,6 +495,185 @@ int read_last(const T* scan, const T* vals, uint32_t n, T& total) {
     return MT_OK;
 }
 
+// ------------------------------------------------------------------ the fast traversal's tree
+// A linear BVH (Karras 2012, "Maximizing Parallelism in the Construction of BVHs, Octrees, and
+// k-d Trees"): triangles sorted by the 30-bit Morton code of their box centre on the grid of
+// the centres' bounds (ties by index), the binary radix tree over the sorted keys, every
+// subtree of at most kFastLeaf triangles collapsed into one leaf, boxes fitted from the leaves
+// upward a level at a time, records in the layout of the host's SAH build.
+constexpr uint32_t kFastLeaf = 4;
+constexpr uint32_t kLeafChild = 0x80000000u;  // child word of the radix tree: a sorted position
+
+// 10-bit value -> its bits at every third position
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void k_lbvh_boxes(const float* P, uint32_t n, float4* lo, float4* hi, float4* cen) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* v = P + 9ull * i;
+    float l[3], h[3];
+    for (int k = 0; k < 3; ++k) {
+        l[k] = dmin(v[k], dmin(v[3 + k], v[6 + k]));
+        h[k] = dmax(v[k], dmax(v[3 + k], v[6 + k]));
+    }
+    lo[i] = make_float4(l[0], l[1], l[2], 0.0f);
+    hi[i] = make_float4(h[0], h[1], h[2], 0.0f);
+    cen[i] = make_float4(0.5f * (l[0] + h[0]), 0.5f * (l[1] + h[1]), 0.5f * (l[2] + h[2]), 0.0f);
+}
+
+__global__ void k_lbvh_keys(const float4* cen, uint32_t n, const float4* cmn, const float4* cmx, uint32_t* key,
+                            uint32_t* id) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = cen[i], a = *cmn, b = *cmx;
+    const float cc[3] = {c.x, c.y, c.z}, lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        const float ext = hi[k] - lo[k];
+        const float u = ext > 0.0f ? (cc[k] - lo[k]) / ext : 0.0f;
+        q[k] = static_cast<uint32_t>(fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f));
+    }
+    key[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+    id[i] = i;
+}
+
+// common prefix length of sorted keys i and j (ties broken by the positions); -1 outside
+__device__ __forceinline__ int lbvh_delta(const uint32_t* key, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = key[i], b = key[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz(static_cast<uint32_t>(i) ^ static_cast<uint32_t>(j));
+}
+
+// internal node i of the radix tree: its range of sorted positions [first, last], its split
+// and its children (kLeafChild | position for a single triangle); parents of both kinds
+__global__ void k_lbvh_nodes(const uint32_t* key, int n, uint32_t* child, uint32_t* first, uint32_t* last,
+                             uint32_t* par_node, uint32_t* par_leaf) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n - 1) return;
+    const int d = (lbvh_delta(key, n, i, i + 1) - lbvh_delta(key, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin_ = lbvh_delta(key, n, i, i - d);
+    int lmax = 2;
+    while (lbvh_delta(key, n, i, i + lmax * d) > dmin_) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (lbvh_delta(key, n, i, i + (l + t) * d) > dmin_) l += t;
+    const int j = i + l * d;
+    const int dnode = lbvh_delta(key, n, i, j);
+    int sp = 0;
+    for (int t = (l + 1) / 2;; t = (t + 1) / 2) {
+        if (lbvh_delta(key, n, i, i + (sp + t) * d) > dnode) sp += t;
+        if (t == 1) break;
+    }
+    const int g = i + sp * d + (d < 0 ? -1 : 0);
+    const int lo = d > 0 ? i : j, hi = d > 0 ? j : i;
+    const uint32_t a = (lo == g) ? (kLeafChild | static_cast<uint32_t>(g)) : static_cast<uint32_t>(g);
+    const uint32_t b = (hi == g + 1) ? (kLeafChild | static_cast<uint32_t>(g + 1)) : static_cast<uint32_t>(g + 1);
+    child[2 * i] = a;
+    child[2 * i + 1] = b;
+    first[i] = static_cast<uint32_t>(lo);
+    last[i] = static_cast<uint32_t>(hi);
+    if (a & kLeafChild) par_leaf[a & ~kLeafChild] = static_cast<uint32_t>(i); else par_node[a] = static_cast<uint32_t>(i);
+    if (b & kLeafChild) par_leaf[b & ~kLeafChild] = static_cast<uint32_t>(i); else par_node[b] = static_cast<uint32_t>(i);
+}
+
+// depth of every internal node (the root's is 0), by walking to the root
+__global__ void k_lbvh_depth(const uint32_t* par_node, uint32_t m, uint32_t* depth, uint32_t* max_depth) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    uint32_t d = 0, p = i;
+    while (p != 0 && d < m) {
+        p = par_node[p];
+        ++d;
+    }
+    depth[i] = d;
+    atomicMax(max_depth, d);
+}
+
+// boxes of the internal nodes at one depth from their children's (deeper levels are done)
+__global__ void k_lbvh_fit(const uint32_t* child, const uint32_t* depth, uint32_t m, uint32_t level,
+                           const uint32_t* id, const float4* tlo, const float4* thi, float4* nmn, float4* nmx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || depth[i] != level) return;
+    float4 a0, a1, b0, b1;
+    const uint32_t ca = child[2 * i], cb = child[2 * i + 1];
+    if (ca & kLeafChild) { a0 = tlo[id[ca & ~kLeafChild]]; a1 = thi[id[ca & ~kLeafChild]]; } else { a0 = nmn[ca]; a1 = nmx[ca]; }
+    if (cb & kLeafChild) { b0 = tlo[id[cb & ~kLeafChild]]; b1 = thi[id[cb & ~kLeafChild]]; } else { b0 = nmn[cb]; b1 = nmx[cb]; }
+    nmn[i] = make_float4(dmin(a0.x, b0.x), dmin(a0.y, b0.y), dmin(a0.z, b0.z), 0.0f);
+    nmx[i] = make_float4(dmax(a1.x, b1.x), dmax(a1.y, b1.y), dmax(a1.z, b1.z), 0.0f);
+}
+
+// internal nodes that stay nodes: more than kFastLeaf triangles below them
+__global__ void k_lbvh_keep(const uint32_t* first, const uint32_t* last, uint32_t m, uint32_t* keep) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) keep[i] = (last[i] - first[i] + 1u > kFastLeaf) ? 1u : 0u;
+}
+
+// one pair record per kept node (index: the scan of keep); children of at most kFastLeaf
+// triangles become leaves over their sorted positions, marked for the prim records
+__global__ void k_lbvh_pairs(const uint32_t* child, const uint32_t* first, const uint32_t* last,
+                             const uint32_t* keep, const uint32_t* cidx, const uint32_t* depth, uint32_t m,
+                             const uint32_t* id, const float4* tlo, const float4* thi, const float4* nmn,
+                             const float4* nmx, float4* pairs, uint32_t* leaf_of, uint32_t* last_of,
+                             uint32_t* max_level) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || !keep[i]) return;
+    uint32_t w[2];
+    float4 mn[2], mx[2];
+    bool leaf_child = false;
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t c = child[2 * i + side];
+        uint32_t f, l;
+        if (c & kLeafChild) {
+            f = l = c & ~kLeafChild;
+            mn[side] = tlo[id[f]];
+            mx[side] = thi[id[f]];
+        } else {
+            f = first[c];
+            l = last[c];
+            mn[side] = nmn[c];
+            mx[side] = nmx[c];
+        }
+        if (!(c & kLeafChild) && keep[c]) {
+            w[side] = cidx[c];
+        } else {
+            leaf_child = true;
+            w[side] = kLeafBit | f;
+            const uint32_t lid = 2u * i + static_cast<uint32_t>(side);
+            for (uint32_t k = f; k <= l; ++k) {
+                leaf_of[k] = lid;
+                last_of[k] = (k == l) ? kPrimLast : 0u;
+            }
+        }
+    }
+    // levels as the host build counts them: the root is level 1, a leaf one below its parent
+    atomicMax(max_level, depth[i] + (leaf_child ? 2u : 1u));
+    float4* r = pairs + 4ull * cidx[i];
+    r[0] = make_float4(mn[0].x, mn[0].y, mn[0].z, __uint_as_float(w[0]));
+    r[1] = make_float4(mx[0].x, mx[0].y, mx[0].z, __uint_as_float(w[1]));
+    r[2] = make_float4(mn[1].x, mn[1].y, mn[1].z, 0.0f);
+    r[3] = make_float4(mx[1].x, mx[1].y, mx[1].z, 0.0f);
+}
+
+// prim slot k: the triangle at sorted position k {v0, triangle} {v1 - v0, leaf} {v2 - v0, last}
+__global__ void k_lbvh_prims(const float* P, const uint32_t* id, uint32_t n, const uint32_t* leaf_of,
+                             const uint32_t* last_of, float4* prims) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = id[k];
+    const float* v = P + 9ull * t;
+    float4* r = prims + 3ull * k;
+    r[0] = make_float4(v[0], v[1], v[2], __uint_as_float(t));
+    r[1] = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], __uint_as_float(leaf_of[k]));
+    r[2] = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], __uint_as_float(last_of[k]));
+}
+
 }  // namespace
 
 int build_bvh_device(const float* positions, uint32_t tri_count, uint32_t max_triangles,
@@ -707,6 +886,101 @@ int build_bvh_device(const float* positions, uint32_t tri_count, uint32_t max_tr
         BB_TRY(hipGetLastError());
         BB_TRY(hipDeviceSynchronize());
     }
+    return MT_OK;
+}
+
+
+int build_fast_bvh_device(const float* positions, uint32_t tri_count, DevMesh& dm, DeviceBvhRecords& recs,
+                          uint32_t& levels) {
+    recs = DeviceBvhRecords{};
+    levels = 0;
+    const uint32_t n = tri_count;
+    if (n <= kFastLeaf) return fail(MT_E_INVALID, "device build of a mesh that is one leaf");
+    if (n >= (1u << 31)) return fail(MT_E_UNSUPPORTED, "more than 2^31 triangles in one mesh");
+    hipStream_t s = nullptr;
+    Bufs B;
+    int rc;
+    const uint32_t m = n - 1;  // internal nodes of the radix tree
+    float* P;
+    float4 *tlo, *thi, *cen, *cmn, *cmx, *nmn, *nmx;
+    uint32_t *key, *key2, *id, *id2, *child, *first, *last, *par_node, *par_leaf, *depth, *keep, *cidx, *leaf_of,
+        *last_of, *counters;
+    if ((rc = B.get(&P, 9ull * n)) || (rc = B.get(&tlo, n)) || (rc = B.get(&thi, n)) || (rc = B.get(&cen, n)) ||
+        (rc = B.get(&cmn, 1)) || (rc = B.get(&cmx, 1)) || (rc = B.get(&key, n)) || (rc = B.get(&key2, n)) ||
+        (rc = B.get(&id, n)) || (rc = B.get(&id2, n)) || (rc = B.get(&child, 2ull * m)) ||
+        (rc = B.get(&first, m)) || (rc = B.get(&last, m)) || (rc = B.get(&par_node, m)) ||
+        (rc = B.get(&par_leaf, n)) || (rc = B.get(&depth, m)) || (rc = B.get(&keep, m)) || (rc = B.get(&cidx, m)) ||
+        (rc = B.get(&nmn, m)) || (rc = B.get(&nmx, m)) || (rc = B.get(&leaf_of, n)) || (rc = B.get(&last_of, n)) ||
+        (rc = B.get(&counters, 2)))
+        return fail(rc, "device allocation failed in the fast BVH build");
+    BB_TRY(hipMemcpy(P, positions, sizeof(float) * 9ull * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_lbvh_boxes, dim3(grid(n)), dim3(kThreads), 0, s, P, n, tlo, thi, cen);
+    {  // bounds of the box centres
+        const uint32_t parts = std::min<uint32_t>(65536u, std::max<uint32_t>(1u, n / 64u));
+        const uint32_t per = (n + parts - 1) / parts;
+        float4 *pmn, *pmx;
+        if ((rc = B.get(&pmn, parts)) || (rc = B.get(&pmx, parts))) return fail(rc, "device allocation failed");
+        hipLaunchKernelGGL(k_box_partial, dim3((parts + kThreads - 1) / kThreads), dim3(kThreads), 0, s, cen, cen,
+                           n, per, parts, pmn, pmx);
+        hipLaunchKernelGGL(k_box_final, dim3(1), dim3(1), 0, s, pmn, pmx, parts, cmn, cmx);
+    }
+    hipLaunchKernelGGL(k_lbvh_keys, dim3(grid(n)), dim3(kThreads), 0, s, cen, n, cmn, cmx, key, id);
+    BB_TRY(hipGetLastError());
+    {  // stable sort by key: equal keys keep the triangle order
+        size_t bytes = 0;
+        BB_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key, key2, id, id2, static_cast<int>(n), 0, 30, s));
+        unsigned char* tmp = nullptr;
+        if ((rc = B.get(&tmp, bytes))) return fail(rc, "device allocation failed in the fast BVH build");
+        BB_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, key, key2, id, id2, static_cast<int>(n), 0, 30, s));
+    }
+    const uint32_t zero2[2] = {0, 0};
+    BB_TRY(hipMemcpy(counters, zero2, sizeof(zero2), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fill, dim3(grid(m)), dim3(kThreads), 0, s, par_node, m, 0u);
+    hipLaunchKernelGGL(k_lbvh_nodes, dim3(grid(m)), dim3(kThreads), 0, s, key2, static_cast<int>(n), child, first,
+                       last, par_node, par_leaf);
+    hipLaunchKernelGGL(k_lbvh_depth, dim3(grid(m)), dim3(kThreads), 0, s, par_node, m, depth, counters);
+    BB_TRY(hipGetLastError());
+    uint32_t max_depth = 0;
+    BB_TRY(hipMemcpy(&max_depth, counters, 4, hipMemcpyDeviceToHost));
+    for (uint32_t lv = max_depth + 1; lv-- > 0;)
+        hipLaunchKernelGGL(k_lbvh_fit, dim3(grid(m)), dim3(kThreads), 0, s, child, depth, m, lv, id2, tlo, thi, nmn,
+                           nmx);
+    hipLaunchKernelGGL(k_lbvh_keep, dim3(grid(m)), dim3(kThreads), 0, s, first, last, m, keep);
+    BB_TRY(hipGetLastError());
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    if ((rc = scan(B, tmp, tmp_bytes, keep, cidx, m, s))) return rc;
+    uint32_t npairs = 0;
+    if ((rc = read_last(cidx, keep, m, npairs))) return rc;
+    void *dp = nullptr, *dq = nullptr;
+    if (device_malloc(&dp, sizeof(float4) * 4 * std::max<uint32_t>(npairs, 1))) return MT_E_NOMEM;
+    if (device_malloc(&dq, sizeof(float4) * 3 * n)) {
+        std::vector<void*> one{dp};
+        free_blocks(one);
+        return MT_E_NOMEM;
+    }
+    recs.pairs = static_cast<float4*>(dp);
+    recs.prims = static_cast<float4*>(dq);
+    recs.npairs = npairs;
+    recs.nslots = n;
+    hipLaunchKernelGGL(k_lbvh_pairs, dim3(grid(m)), dim3(kThreads), 0, s, child, first, last, keep, cidx, depth, m,
+                       id2, tlo, thi, nmn, nmx, recs.pairs, leaf_of, last_of, counters + 1);
+    hipLaunchKernelGGL(k_lbvh_prims, dim3(grid(n)), dim3(kThreads), 0, s, P, id2, n, leaf_of, last_of, recs.prims);
+    BB_TRY(hipGetLastError());
+    float4 rmn, rmx;
+    BB_TRY(hipMemcpy(&rmn, nmn, sizeof(float4), hipMemcpyDeviceToHost));
+    BB_TRY(hipMemcpy(&rmx, nmx, sizeof(float4), hipMemcpyDeviceToHost));
+    BB_TRY(hipMemcpy(&levels, counters + 1, 4, hipMemcpyDeviceToHost));
+    BB_TRY(hipDeviceSynchronize());
+    dm.prim_count = n;
+    dm.root_word = 0;  // the root is kept (n > kFastLeaf) and numbered first
+    dm.flags |= kMeshHasBvh;
+    dm.root_min[0] = rmn.x;
+    dm.root_min[1] = rmn.y;
+    dm.root_min[2] = rmn.z;
+    dm.root_max[0] = rmx.x;
+    dm.root_max[1] = rmx.y;
+    dm.root_max[2] = rmx.z;
     return MT_OK;
 }
 

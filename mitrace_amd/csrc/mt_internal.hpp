@@ -155,6 +155,11 @@ struct DeviceBvhRecords {
 };
 int build_bvh_device(const float* positions, uint32_t tri_count, uint32_t max_triangles,
                      uint32_t max_depth, HostBvh& out, DeviceBvhRecords* recs = nullptr);
+// the fast traversal's tree of one mesh (more than 4 triangles) built on the device as a linear
+// BVH: records in recs (owned by the caller, mesh-local words), root box/word and flags in dm,
+// levels = the tree's depth as build_fast_bvh counts it
+int build_fast_bvh_device(const float* positions, uint32_t tri_count, DevMesh& dm, DeviceBvhRecords& recs,
+                          uint32_t& levels);
 // glm::inverse stand-in; column-major in and out
 void invert4(const float* m, float* out);
 // M * (x, y, z, w) with the library's fixed operation order

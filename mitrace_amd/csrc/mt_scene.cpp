@@ -559,6 +559,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
     struct DevPart {
         uint32_t mesh;
         DeviceBvhRecords r;
+        bool fast;  // a record set of the fast traversal's trees (fast_meshes[mesh])
     };
     std::vector<DevPart> dev_parts;
     struct PartsGuard {
@@ -605,7 +606,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
                 const int rc = build_bvh_device(P, md.tri_count, d->bvh.max_triangles, d->bvh.max_depth - 1u,
                                                 s->bvhs[i], &recs);
                 if (rc) return rc;
-                dev_parts.push_back({i, recs});
+                dev_parts.push_back({i, recs, false});
             } else {
                 build_bvh(P, md.tri_count, d->bvh.max_triangles, d->bvh.max_depth - 1u, s->bvhs[i]);
             }
@@ -687,17 +688,30 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
             DevMesh& fm = fast_meshes[i];
             fm = dm;
             fm.flags &= ~kMeshHasBvh;
-            const size_t p0 = pairs.size(), s0 = prims.size();
-            s->max_fast_depth = std::max(s->max_fast_depth, build_fast_bvh(P, md.tri_count, pairs, prims, fm));
-            fast_pairs += (pairs.size() - p0) / 4;
-            fast_slots += (prims.size() - s0) / 3;
+            if (d->bvh.fast_device_build_min_tris && md.tri_count >= d->bvh.fast_device_build_min_tris &&
+                md.tri_count > 4) {
+                DeviceBvhRecords recs{};
+                uint32_t levels = 0;
+                const int rc = build_fast_bvh_device(P, md.tri_count, fm, recs, levels);
+                if (rc) return rc;
+                dev_parts.push_back({i, recs, true});
+                s->max_fast_depth = std::max(s->max_fast_depth, levels);
+                fast_pairs += recs.npairs;
+                fast_slots += recs.nslots;
+            } else {
+                const size_t p0 = pairs.size(), s0 = prims.size();
+                s->max_fast_depth = std::max(s->max_fast_depth, build_fast_bvh(P, md.tri_count, pairs, prims, fm));
+                fast_pairs += (pairs.size() - p0) / 4;
+                fast_slots += (prims.size() - s0) / 3;
+            }
         }
     }
     // the device-built parts follow the host-built records
     uint64_t tot_pairs = pairs.size() / 4, tot_prims = prims.size() / 3;
     for (DevPart& p : dev_parts) {
-        meshes[p.mesh].pair_base = static_cast<uint32_t>(tot_pairs);
-        meshes[p.mesh].prim_base = static_cast<uint32_t>(tot_prims);
+        DevMesh& m = p.fast ? fast_meshes[p.mesh] : meshes[p.mesh];
+        m.pair_base = static_cast<uint32_t>(tot_pairs);
+        m.prim_base = static_cast<uint32_t>(tot_prims);
         tot_pairs += p.r.npairs;
         tot_prims += p.r.nslots;
     }
@@ -868,7 +882,7 @@ int upload_scene(const mt_scene_desc* d, mt_scene* s) {
         if (!prims.empty() && hipMemcpy(fq, prims.data(), sizeof(float4) * prims.size(), hipMemcpyHostToDevice) != hipSuccess)
             return fail(MT_E_HIP, "hipMemcpy failed in scene upload");
         for (const DevPart& p : dev_parts) {
-            const DevMesh& m = meshes[p.mesh];
+            const DevMesh& m = p.fast ? fast_meshes[p.mesh] : meshes[p.mesh];
             if (hipMemcpy(fp + 4ull * m.pair_base, p.r.pairs, sizeof(float4) * 4 * p.r.npairs,
                           hipMemcpyDeviceToDevice) != hipSuccess ||
                 hipMemcpy(fq + 3ull * m.prim_base, p.r.prims, sizeof(float4) * 3 * p.r.nslots,

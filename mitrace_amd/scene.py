@@ -112,6 +112,9 @@ class BvhOptions:
     # (bit-identical to the host build, about 3.5x faster scene creation on the large
     # terrain); 0 = every mesh on the host
     device_build_min_tris: int = 65536
+    # with fast: meshes with at least this many triangles get the fast traversal's tree built on
+    # the GPU as a linear BVH instead of the host's binned SAH; 0 = every mesh on the host
+    fast_device_build_min_tris: int = 0
 
 
 @dataclass
@@ -211,6 +214,7 @@ class Scene:
         desc.bvh.max_depth = self.bvh.max_depth
         desc.bvh.fast = int(self.bvh.fast)
         desc.bvh.device_build_min_tris = int(self.bvh.device_build_min_tris)
+        desc.bvh.fast_device_build_min_tris = int(self.bvh.fast_device_build_min_tris)
         keep.extend([meshes, images, mats, insts, lights])
         return desc, keep
 
