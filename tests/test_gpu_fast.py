@@ -1,6 +1,6 @@
 """Fast traversal (MT_TRAVERSAL_FAST; a GPU speed mode with no reference counterpart): per-mesh
-SAH BVHs with tight boxes, nodes and instances culled by entry distance, shadow rays ended at
-their first occluder.
+SAH BVHs with tight boxes (built on the host, or on the GPU as linear BVHs), nodes and
+instances culled by entry distance, shadow rays ended at their first occluder.
 
 Bars: every closest hit is the exact closest hit of the oracle's float64 brute-force search
 (same instance and triangle, or a hit at the same distance within 1e-4 relative: a shared
@@ -23,7 +23,10 @@ pytestmark = pytest.mark.gpu
 
 FAST = abi.MT_TRAVERSAL_FAST
 REF = abi.MT_TRAVERSAL_REFERENCE
-SCENES = ["cornell_gen", "material_spheres_bvh4", "sphere_grid", "large_small"]
+# ":lbvh": the fast trees of every mesh above 4 triangles built on the GPU as linear BVHs
+# (mt_bvh_options.fast_device_build_min_tris) instead of the host's binned SAH
+SCENES = ["cornell_gen", "material_spheres_bvh4", "sphere_grid", "large_small", "sphere_grid:lbvh",
+          "large_small:lbvh", "material_spheres_bvh4:lbvh"]
 
 
 @pytest.fixture(scope="module")
@@ -32,8 +35,11 @@ def scenes(gpu):
 
     def get(name):
         if name not in cache:
-            sc = load_scene(name)
+            base, _, variant = name.partition(":")
+            sc = load_scene(base)
             sc.bvh.fast = True
+            if variant == "lbvh":
+                sc.bvh.fast_device_build_min_tris = 1
             cache[name] = (sc, GpuScene(sc), po.OracleScene(sc))
         return cache[name]
     yield get
@@ -91,8 +97,9 @@ def test_occlusion_queries(scenes, name):
     assert exact.sum() > 100 and (~exact).sum() > 100
 
 
-def test_fast_render_matches_reference_traversal(scenes):
-    sc, g, _ = scenes("sphere_grid")
+@pytest.mark.parametrize("name", ["sphere_grid", "sphere_grid:lbvh"])
+def test_fast_render_matches_reference_traversal(scenes, name):
+    sc, g, _ = scenes(name)
     for extra in ({}, {"mode": abi.MT_MODE_FIXED, "bounces": 8}):
         st = small_settings(width=48, height=32, samples=64, output=abi.MT_OUTPUT_LINEAR,
                             hdri_rotation=30.0, hdri_rotation_apply=True, **extra)
