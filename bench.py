@@ -383,9 +383,7 @@ def large_leg(args):
         # estimate the rest of the path with environment samples (same expectation, far less
         # variance) and the network learns it with the mean-keeping L2 loss; beside it the
         # same query trained on the reference estimator's own paths with the relative loss
-        for opts in ({"query_depth": 0, "query_after_light": 1, "train_nee": 1, "loss": 1,
-                      "learning_rate": 2e-3, "warmup_passes": 4, "steps_per_pass": 16},
-                     {"query_depth": 0, "query_after_light": 1}):
+        for opts in (cache_eval.LARGE_SCENE_CACHE, {"query_depth": 0, "query_after_light": 1}):
             cache["configs"].append(cache_eval.compare(
                 g, fn, cache_opts=opts, log=lambda m: print(m, file=sys.stderr, flush=True), ref=ref))
     g.close()

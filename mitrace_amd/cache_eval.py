@@ -150,10 +150,44 @@ def mlp_bench(gscene, n_infer=1 << 22, n_train=1 << 16, reps=10, log=print):
     return out
 
 
+# the bench's configuration for the large scene: the camera ray's hit queries the cache after
+# its light sample, training paths with environment samples, L2 loss
+LARGE_SCENE_CACHE = {"query_depth": 0, "query_after_light": 1, "train_nee": 1, "loss": 1,
+                     "learning_rate": 2e-3, "warmup_passes": 4, "steps_per_pass": 16}
+
+
+def render_large(spp=16, log=print):
+    """one plain and one cached render of the large scene at 1080p (fast traversal, linear
+    radiance, the bench's cache configuration): the workload of the cache's kernel trace"""
+    from . import scenegen
+    from .tracer import GpuScene
+    sc = scenegen.large_scene()
+    sc.bvh.fast = True
+    g = GpuScene(sc)
+
+    def st(**kw):
+        return scenegen.large_scene_settings(traversal=abi.MT_TRAVERSAL_FAST, output=abi.MT_OUTPUT_LINEAR, **kw)
+    g.render(st(samples=1, seed=4))
+    t0 = time.perf_counter()
+    _, _, sp = g.render(st(samples=spp, seed=5))
+    t_plain = time.perf_counter() - t0
+    cache = RadianceCache(g, dict(DEFAULT_CACHE, **LARGE_SCENE_CACHE))
+    t0 = time.perf_counter()
+    _, sc_ = cache.render(st(samples=spp, seed=5))
+    t_cached = time.perf_counter() - t0
+    cache.close()
+    g.close()
+    log(f"# large scene 1080p {spp} spp: plain {t_plain:.3f} s ({sp['rays']} rays), cached "
+        f"{t_cached:.3f} s ({sc_['rays']} rays, {int(sc_['diag'][8])} trained samples)")
+
+
 if __name__ == "__main__":
     import json
     import sys
-    from . import scenegen
-    from .tracer import GpuScene
-    g = GpuScene(scenegen.cornell())
-    print(json.dumps(mlp_bench(g, reps=int(sys.argv[1]) if len(sys.argv) > 1 else 10)))
+    if len(sys.argv) > 1 and sys.argv[1] == "large":
+        render_large(int(sys.argv[2]) if len(sys.argv) > 2 else 16)
+    else:
+        from . import scenegen
+        from .tracer import GpuScene
+        g = GpuScene(scenegen.cornell())
+        print(json.dumps(mlp_bench(g, reps=int(sys.argv[1]) if len(sys.argv) > 1 else 10)))
