@@ -482,19 +482,26 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
                         finished = trav_setup<kCount, kDiag>(S, t, st, w, nm);
                     }
                 }
-            } else if (t.leaf != kNoLeaf) {
-                kind = kStepLeaf;
-                if (kAccel) {
-                    ftrav_tri<kAny, kDiag>(S, t, st);
-                } else {
-                    trav_tri<kDiag>(S, t);
-                }
-            } else if (kAccel) {
-                kind = ftrav_node<kDiag>(S, t, st) ? kStepInner : kStepCull;
-            } else if (t.fast) {
-                kind = trav_node<true, kDiag>(S, t, st) ? kStepInner : kStepCull;
             } else {
-                kind = trav_node<false, kDiag>(S, t, st) ? kStepInner : kStepCull;
+                // a node pop, and in the same step the first triangle of a leaf it reaches (the
+                // order of the tests is the traversal's: only the steps are grouped)
+                if (t.leaf == kNoLeaf) {
+                    if (kAccel) {
+                        kind = ftrav_node<kDiag>(S, t, st) ? kStepInner : kStepCull;
+                    } else if (t.fast) {
+                        kind = trav_node<true, kDiag>(S, t, st) ? kStepInner : kStepCull;
+                    } else {
+                        kind = trav_node<false, kDiag>(S, t, st) ? kStepInner : kStepCull;
+                    }
+                }
+                if (t.leaf != kNoLeaf) {
+                    if (kind == 0xFFu) kind = kStepLeaf;
+                    if (kAccel) {
+                        ftrav_tri<kAny, kDiag>(S, t, st);
+                    } else {
+                        trav_tri<kDiag>(S, t);
+                    }
+                }
             }
         }
         if (kCount) {
