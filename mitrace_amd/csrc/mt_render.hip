@@ -502,6 +502,14 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
                         trav_tri<kDiag>(S, t);
                     }
                 }
+                if (t.leaf != kNoLeaf) {  // and the leaf's next triangle (measured: a third or
+                                          // fourth test per step loses, the wave runs each block)
+                    if (kAccel) {
+                        ftrav_tri<kAny, kDiag>(S, t, st);
+                    } else {
+                        trav_tri<kDiag>(S, t);
+                    }
+                }
             }
         }
         if (kCount) {
