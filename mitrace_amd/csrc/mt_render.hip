@@ -68,8 +68,10 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
 
 __device__ __forceinline__ void path_coords(const ChunkParams& c, uint32_t p, uint32_t& pix,
                                             uint32_t& s) {
-    const uint32_t lp = p / c.spp;  // pixel-major: a wave holds the samples of one pixel
-    s = p - lp * c.spp + c.sample_base;
+    // a training-only pass of a cached render runs every path_stride-th path of the chunk
+    const uint32_t q = c.path_stride > 1u ? p * c.path_stride + c.path_first : p;
+    const uint32_t lp = q / c.spp;  // pixel-major: a wave holds the samples of one pixel
+    s = q - lp * c.spp + c.sample_base;
     pix = c.pix_begin + lp;
 }
 
@@ -2932,6 +2934,19 @@ int Renderer::run_cache_chunk(Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t 
     c.sample_base = sample_base;
     c.train_every = train_every;
     c.train_offset = train_offset % train_every;
+    if (!accumulate) {
+        // a training-only (warm-up) pass: only its training paths, the chunk's paths p with
+        // (p + offset) % train_every == 0, run, as paths 0, 1, ... of the pass, each its own
+        // training slot (the other paths' image samples would be thrown away)
+        if (train_every == kNoSlot) return MT_OK;
+        const uint32_t first = (train_every - c.train_offset) % train_every;
+        if (c.paths <= first) return MT_OK;
+        c.path_stride = train_every;
+        c.path_first = first;
+        c.paths = (c.paths - 1 - first) / train_every + 1;
+        c.train_every = 1;
+        c.train_offset = 0;
+    }
     const uint32_t gen_blocks = (c.paths + kBlock - 1) / kBlock;
     hipStream_t s = S.stream, m = tail_stream_;
     const PathBuffers& b = S.b;
@@ -2955,10 +2970,10 @@ int Renderer::run_cache_chunk(Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t 
         hipLaunchKernelGGL(k_cache_accum, dim3((c.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, m, c, b);
     // training slots of this chunk: paths p with (p + offset) % train_every == 0, numbered
     // (p + offset) / train_every: [slot0, slot0 + nslots)
-    const uint32_t off = c.train_offset;
-    const uint32_t first = (train_every - off) % train_every;
-    const uint32_t nslots = (train_every != kNoSlot && c.paths > first) ? (c.paths - 1 - first) / train_every + 1 : 0;
-    const uint32_t slot0 = (first + off) / train_every;
+    const uint32_t te = c.train_every, off = c.train_offset;
+    const uint32_t first = (te - off) % te;
+    const uint32_t nslots = (te != kNoSlot && c.paths > first) ? (c.paths - 1 - first) / te + 1 : 0;
+    const uint32_t slot0 = (first + off) / te;
     if (nslots) {
         hipLaunchKernelGGL(k_cache_targets, dim3((nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, m, c, b,
                            slot0, nslots);

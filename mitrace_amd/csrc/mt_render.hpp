@@ -93,6 +93,9 @@ struct ChunkParams {
     // mean target) and its predictions are multiplied by it
     float* cache_scale;
     int32_t cache_l2;  // the cache trains with the L2 loss (scaled targets)
+    // a training-only pass runs the chunk's paths path_first + k * path_stride (stride 0 or 1:
+    // every path)
+    uint32_t path_stride, path_first;
     int32_t debug;  // MT_DEBUG_* view (plain renders)
     int32_t accel;  // 1 = fast traversal (MT_TRAVERSAL_FAST)
     // ray binning before the extend of bounces >= 1 (plain renders): shading writes every
