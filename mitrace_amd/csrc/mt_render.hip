@@ -357,6 +357,19 @@ struct TestIO {
 #ifndef MT_TRACE_WAVES_PER_SIMD
 #define MT_TRACE_WAVES_PER_SIMD 6
 #endif
+// The unit steps of one wave step of the traversal, in order: 'N' a node pop (for lanes not in
+// a leaf), 'T' a triangle test (for lanes in a leaf); the reference and the fast traversal
+#ifndef MT_STEP_SCHEDULE
+#define MT_STEP_SCHEDULE "NTTNTNT"
+#endif
+#ifndef MT_STEP_SCHEDULE_FAST
+#define MT_STEP_SCHEDULE_FAST "NNTNNT"
+#endif
+constexpr char kStepSchedule[] = MT_STEP_SCHEDULE;
+constexpr char kStepScheduleFast[] = MT_STEP_SCHEDULE_FAST;
+constexpr int kStepUnits = static_cast<int>(sizeof(kStepSchedule)) - 1;
+constexpr int kStepUnitsFast = static_cast<int>(sizeof(kStepScheduleFast)) - 1;
+static_assert(kStepSchedule[0] == 'N' && kStepScheduleFast[0] == 'N', "a wave step starts with a node pop");
 // The body runs in k_trace (gtid/nthreads over the grid) and in the batch's persistent kernel
 // (over one lane group's blocks); lds_stack holds kDepth * kBlock entries of the block.
 template <bool kCount, int kMode, uint32_t kDepth, int kAccel>
@@ -486,15 +499,16 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
                 }
             } else {
                 // A step is a fixed schedule of unit steps, each taken by the lanes it applies
-                // to: node pop, triangle, triangle, node pop, triangle, node pop, triangle.  A
-                // lane's tests keep the traversal's order; only more of them share one wave
-                // iteration, whose refill and set-up bookkeeping they amortise (sphere grid:
-                // 527 Msamples/s with one unit per iteration, 632 with this schedule; longer
-                // ones measured no faster, a third triangle in a row slower).
-                constexpr char kSchedule[] = "NTTNTNT";
+                // to (reference traversal: node pop, triangle, triangle, node pop, triangle,
+                // node pop, triangle; fast traversal: node, node, triangle, node, node,
+                // triangle).  A lane's tests keep the traversal's order; only more of them
+                // share one wave iteration, whose refill and set-up bookkeeping they amortise
+                // (sphere grid: 527 Msamples/s with one unit per iteration, 632 with the
+                // schedule; longer ones measured no faster, a third triangle in a row slower;
+                // the fast traversal's schedule: 1396 -> 1459 on the sphere grid).
 #pragma unroll
-                for (int u = 0; u < static_cast<int>(sizeof(kSchedule)) - 1; ++u) {
-                    if (kSchedule[u] == 'N') {
+                for (int u = 0; u < (kAccel ? kStepUnitsFast : kStepUnits); ++u) {
+                    if ((kAccel ? kStepScheduleFast[u] : kStepSchedule[u]) == 'N') {
                         if (t.leaf == kNoLeaf && (u == 0 || !trav_needs_setup(t, st))) {
                             bool inner;
                             if (kAccel) {
