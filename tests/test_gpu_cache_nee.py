@@ -136,3 +136,30 @@ def test_trained_nee_cache_on_the_sunlit_terrain(scenes):
     # measured: RMSE 272 against 3058, bias -1.0 % to -1.4 % over three seeds
     assert e_cache < 0.2 * e_plain
     assert abs(bias) < 0.04
+
+
+def test_train_nee_needs_after_light_queries(scenes):
+    from mitrace_amd._lib import MiTraceError
+    from mitrace_amd.cache import RadianceCache
+    g = scenes("cornell_gen")
+    c = RadianceCache(g, {"train_nee": 1, "query_depth": 1})
+    with pytest.raises(MiTraceError):
+        c.render(small_settings(width=8, height=8, samples=4))
+    c.close()
+
+
+@pytest.mark.parametrize("mode", [abi.MT_MODE_PARITY, abi.MT_MODE_FIXED])
+def test_training_estimate_without_environment_is_the_plain_estimate(scenes, mode, monkeypatch):
+    """No environment to sample: the training paths run the reference estimator itself, so with
+    every path training their full estimates are the plain render's samples bit for bit."""
+    from mitrace_amd.cache import RadianceCache
+    g = scenes("cornell_gen")
+    kw = dict(width=24, height=20, samples=8, bounces=8, seed=31, mode=mode, rr_depth=3,
+              output=abi.MT_OUTPUT_LINEAR)
+    ref, _, _ = g.render(small_settings(**kw))
+    monkeypatch.setenv("MT_CACHE_TRAINING_IMAGE", "1")
+    c = RadianceCache(g, {"train_every": 1, "query_depth": 0, "query_after_light": 1, "train_nee": 1,
+                          "loss": 1, "spp_per_pass": 8, "steps_per_pass": 1, "train_batch": 1024})
+    img, _ = c.render(small_settings(**kw))
+    c.close()
+    np.testing.assert_array_equal(img, ref)
