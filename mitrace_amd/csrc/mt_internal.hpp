@@ -37,6 +37,7 @@ namespace mt {
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kMeshHasBvh = 1u;
 constexpr uint32_t kMeshHasUv = 2u;
+constexpr uint32_t kTlasReference = 1u, kTlasFast = 2u;  // DevScene::use_tlas bits
 constexpr uint32_t kPrimLast = 1u;  // prims[3i+2].w: last slot of its leaf
 
 struct alignas(16) DevInstance {
@@ -123,7 +124,7 @@ struct DevScene {
     const uint32_t* tlas_inst;
     float tlas_root_min[3], tlas_root_max[3];
     uint32_t tlas_root_word;
-    uint32_t use_tlas;
+    uint32_t use_tlas;  // kTlasReference | kTlasFast: the traversals that start at the top level
     uint32_t env_w, env_h;
     int32_t env_filter;
     int32_t has_env;

@@ -439,7 +439,7 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
                     // an occluder counts up to and including the light distance (Tracer.cpp:322)
                     if (kAny) t.R.dist = bound >= 0.0f ? __uint_as_float(__float_as_uint(bound) + 1u) : bound;
                     active = true;
-                    if (S.use_tlas && finite3(t.invd) && finite3(o)) {
+                    if ((S.use_tlas & (kAccel ? kTlasFast : kTlasReference)) && finite3(t.invd) && finite3(o)) {
                         if (kAccel) {
                             ftrav_tlas_begin(S, t, st);
                         } else {
