@@ -446,6 +446,15 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
                             trav_tlas_begin(S, t, st);
                         }
                         need = false;
+                    } else if (kMode <= 1 && !kAccel && !kCount && b.inst_mask) {
+                        // the mask computed before the traversal (k_instance_masks), and the
+                        // start of the instance walk as the culling below would begin it
+                        t.mask = b.inst_mask[item];
+                        t.cur = -1;
+                        t.leaf = kNoLeaf;
+                        t.has_next = false;
+                        st.sp = 0;
+                        need = false;
                     } else {
                         need = true;
                     }
@@ -626,6 +635,29 @@ __attribute__((amdgpu_waves_per_eu(MT_TRACE_WAVES_PER_SIMD, 8))) void k_trace(De
                                               blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
+// The instance masks of a queue's rays, computed before the reference traversal of a scene of
+// at most 64 instances: bit j of slot i's mask is set when ray i enters instance j's world box
+// (Intersect.cpp:206-216 tests every instance's world box).  The traversal kernel would
+// otherwise test the 64 boxes in its refill steps with only the refilled lanes active.
+template <int kMode>
+__global__ __launch_bounds__(kBlock) void k_instance_masks(DevScene S, PathBuffers b, uint32_t bounce) {
+    const uint32_t count = counter(b, (kMode == 0 ? kCntExtend : kCntShadow) + bounce);
+    Work w{};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        V3 o, d;
+        if (kMode == 0) {
+            o = xyz(b.ray_o[bounce & 1][i]);
+            d = xyz(b.ray_d[bounce & 1][i]);
+        } else {
+            o = xyz(b.sh_o[i]);
+            d = xyz(b.sh_d[i]);
+        }
+        const V3 invd = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};  // as trav_begin
+        b.inst_mask[i] = (finite3(invd) && finite3(o)) ? instance_mask<false, true>(S, 0, o, invd, w)
+                                                       : instance_mask<false, false>(S, 0, o, invd, w);
+    }
+}
+
 template <int kMode, int kAccel>
 void launch_trace_as(uint32_t depth, bool count, uint32_t blocks, hipStream_t s, const DevScene& S,
                      const ChunkParams& c, const PathBuffers& b, uint32_t k, TestIO io) {
@@ -640,6 +672,8 @@ void launch_trace_as(uint32_t depth, bool count, uint32_t blocks, hipStream_t s,
         else                                                                                      \
             hipLaunchKernelGGL((k_trace<false, kMode, D, kAccel>), dim3(blocks), dim3(kBlock), 0, s, S, c, b, k, io); \
         break;
+        if (kMode <= 1 && !kAccel && !count && b.inst_mask)
+            hipLaunchKernelGGL(k_instance_masks<kMode>, dim3(blocks), dim3(kBlock), 0, s, S, b, k);
         switch (depth) {
             MT_TRACE_CASE(4)
             MT_TRACE_CASE(8)
@@ -2561,8 +2595,12 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
     const size_t trace_threads = static_cast<size_t>(trace_blocks_) * kBlock;
     PathBuffers& b = buf_;
     int rc;
+    // instance masks computed before the reference traversal: scenes of 1 to 64 instances that
+    // do not start at an instance tree
+    premask_ = st.traversal == MT_TRAVERSAL_REFERENCE && scene->instance_count >= 1 &&
+               scene->instance_count <= 64 && !(scene->dev.use_tlas & kTlasReference);
     if ((rc = alloc_path_buffers(owned_, b, P, spill_depth * trace_threads, diff, false, st.debug_mode != 0)) ||
-        (rc = alloc(owned_, &img_, npix_total * 3)))
+        (premask_ && (rc = alloc(owned_, &b.inst_mask, P))) || (rc = alloc(owned_, &img_, npix_total * 3)))
         return rc;
     if (diff && (rc = alloc(owned_, &store_, npix_total * st.samples))) return rc;
     HIP_TRY(hipMemsetAsync(b.work, 0, sizeof(unsigned long long) * 16, stream_));
@@ -2612,7 +2650,8 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         slots_.push_back(L);  // registered first, so that release() frees what follows
         StreamSlot& S = slots_.back();
         if ((rc = alloc_path_buffers(owned_, S.b, P, spill_depth * trace_threads, false, false,
-                                     st.debug_mode != 0)))
+                                     st.debug_mode != 0)) ||
+            (premask_ && (rc = alloc(owned_, &S.b.inst_mask, P))))
             return rc;
         if (bin_blocks_ && (rc = alloc_bins(owned_, S.b, P, S.stream))) return rc;
         HIP_TRY(hipMemsetAsync(S.b.work, 0, sizeof(unsigned long long) * 16, S.stream));

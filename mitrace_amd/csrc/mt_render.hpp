@@ -124,6 +124,9 @@ struct PathBuffers {
     // the slots, and the per-bin counts and cursors
     uint32_t *bin_key, *perm, *bin_hist, *bin_cur;
     float4 *sh_o, *sh_d, *sh_c;  // shadow rays: {o, max dist} {d, path} {contribution, slot}
+    // reference traversal of scenes of at most 64 instances: every queue slot's mask of the
+    // instances whose world box its ray enters, computed before the traversal (or null)
+    uint64_t* inst_mask;
     float4* sample_rgb;
     uint2* spill;
     uint32_t* counters;
@@ -229,6 +232,7 @@ class Renderer {
     std::vector<void*> owned_;
     std::vector<hipEvent_t> events_;
     hipStream_t stream_ = nullptr;
+    bool premask_ = false;  // the path buffers carry instance masks (PathBuffers::inst_mask)
     uint32_t* host_counters_ = nullptr;
     float* img_ = nullptr;
     float4* store_ = nullptr;
