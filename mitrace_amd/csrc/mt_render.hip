@@ -1404,8 +1404,11 @@ void k_shade(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
 }
 
 // plain shading of a scene without point lights: small enough to run five waves per SIMD
+#ifndef MT_SHADE_UNLIT_WAVES_PER_SIMD
+#define MT_SHADE_UNLIT_WAVES_PER_SIMD 5
+#endif
 template <bool kBatch>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MT_SHADE_UNLIT_WAVES_PER_SIMD, 8)))
 void k_shade_unlit(DevScene S, ChunkParams c, PathBuffers b, uint32_t k) {
     shade_body<0, kBatch, false, false>(S, c, b, k, blockIdx.x, gridDim.x);
 }
