@@ -97,17 +97,16 @@ __device__ __forceinline__ void init_path(const ChunkParams& J, const PathBuffer
         o = xform(J.cam, V3{px, -py, 0.0f}, 1.0f);
         d = normalize(xform(J.cam, dc, 0.0f));
     }
-    b.ray_o[0][p] = f4(o, 0.0f);  // queue 0 is the identity (below)
+    // bounce 0's queue is the identity (slot p holds path p) and every path starts with
+    // throughputs and previous base colour 1: the kernels of bounce 0 take those from the slot
+    // index and constants, so only the camera ray and the accumulators are written here
+    b.ray_o[0][p] = f4(o, 0.0f);
     b.ray_d[0][p] = f4(d, 0.0f);
-    b.t_rad[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);  // queue slot p at bounce 0
-    b.t_term[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    b.prev_base[0][p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.acc_e[p] = z;
     b.acc_d[p] = z;
     b.acc_i[p] = z;
     b.acc_s[p] = z;
-    b.queue[0][p] = p;
 }
 
 __global__ __launch_bounds__(kBlock) void k_generate(ChunkParams c, PathBuffers b) {
@@ -561,7 +560,7 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
                                                  __float_as_uint(h.v));
                     b.hit_pos[item] = f4(h.wpos, h.dist);
                     if (kDiag && c.debug && h.inst >= 0) {  // the winning instance's tests (Tracer.cpp:170-175)
-                        const uint32_t p = q[item];
+                        const uint32_t p = bounce == 0 ? item : q[item];
                         uint4 di = b.dbg_i[p];
                         if (bounce == 0) {
                             di.x = h.bvh_tests;
@@ -969,7 +968,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
         V3 g_e = mk(0.0f);
         bool gvalid = false;
         if (valid) {
-            p = qin[idx];
+            p = k == 0 ? idx : qin[idx];
             // the render this path belongs to (batched: its job's settings)
             const ChunkParams& J = job_params<kBatch>(c, b, p);
             const bool last = k + 1 >= J.bounces;
@@ -988,7 +987,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             const V3 d = xyz(b.ray_d[k & 1][idx]);
             const uint4 hrec = b.hit_rec[idx];
             const int inst = static_cast<int>(hrec.x);
-            const V3 trad = xyz(b.t_rad[k & 1][idx]);
+            const V3 trad = k == 0 ? mk(1.0f) : xyz(b.t_rad[k & 1][idx]);
             if (kDiff == 0 && !kBatch && kExt && c.cache && c.cache_after && k > c.cache_depth) {
                 // the training record of the previous vertex: everything collected up to and
                 // including its light sample (the rest is what the bounce ray brought)
@@ -1139,7 +1138,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                 if (frozen)
                     energy_t = (pdf > 0.0f) ? scale(eval_brdf(nd, view, N, basec_s, rough_s, metal_s), nl / pdf)
                                             : mk(0.0f);
-                V3 tterm = mul(xyz(b.t_term[k & 1][idx]), energy_t);
+                V3 tterm = mul(k == 0 ? mk(1.0f) : xyz(b.t_term[k & 1][idx]), energy_t);
                 bool stop;
                 float rr_q = 1.0f;
                 if (J.mode == MT_MODE_FIXED) {
@@ -1279,7 +1278,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                             const V3 ee = scale(eval_brdf(ed, view, N, basec, rough, metal), nle / pref);
                             // chance that the path goes on past this vertex in direction ed:
                             // the energy cut-off, or the roulette on the termination throughput
-                            const V3 tt = mul(xyz(b.t_term[k & 1][idx]), ee);
+                            const V3 tt = mul(k == 0 ? mk(1.0f) : xyz(b.t_term[k & 1][idx]), ee);
                             float keep = 1.0f;
                             if (J.mode == MT_MODE_FIXED) {
                                 if (k + 1 >= J.rr_depth) keep = clampf(gmax(gmax(tt.x, tt.y), tt.z), 0.05f, 1.0f);
