@@ -44,6 +44,8 @@ constexpr int kBlock = 256;
 constexpr uint32_t kRefillMin = 24;
 constexpr uint32_t kRefillMinFast = 48;
 constexpr uint32_t kSetupMin = 12;   // lanes needing an instance set-up that trigger one
+constexpr uint32_t kGrab = 256;      // queue positions a traversal wave takes per atomic
+constexpr uint32_t kGrabFast = 1024;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -350,9 +352,10 @@ struct TestIO {
 // Persistent closest-hit kernel.  mode 0: extend (queue of path ids), 1: shadow records,
 // 2/3: test rays.  kAccel 0 runs the reference traversal, 1 the fast one (for shadow rays an
 // any-hit query bounded by the light distance).  A wave keeps 64 lanes busy: whenever at least refill_min lanes are idle it
-// takes that many new work items with one atomic (ballot + prefix for the lane slots), runs
-// the instance culling for them together, and every lane then advances its own ray by one
-// unit of work per loop iteration.  Waves leave the loop when the queue is drained.
+// hands them that many new work items (ballot + prefix for the lane slots) from its own batch
+// of queue positions, taken `grab` at a time with one atomic, runs the instance culling for
+// them together, and every lane then advances its own ray by one unit of work per loop
+// iteration.  Waves leave the loop when the queue is drained.
 #ifndef MT_TRACE_WAVES_PER_SIMD
 #define MT_TRACE_WAVES_PER_SIMD 6
 #endif
@@ -396,21 +399,43 @@ __device__ __forceinline__ void trace_body(const DevScene& S, const ChunkParams&
     const uint32_t* q = b.queue[bounce & 1];
     const bool binned = kMode == 0 && c.bin && bounce > 0;
     const uint32_t refill_min = c.refill_min;
+    // a wave takes work items from the queue in batches (one atomic per batch) and hands them
+    // to its idle lanes from that batch: one atomic per refill on one counter for every wave of
+    // the chip serialises at the L2 when rays are short.  A batch is at most c.grab positions
+    // and at most an eighth of a wave's share of the queue, so short queues still spread over
+    // every wave; never below 64 (one refill of a whole wave).
+    const uint32_t grab = max(64u, min(c.grab, count / (max(nthreads / 64u, 1u) * 8u)));
     const uint64_t lanes_below = (1ull << lane_id()) - 1ull;
     Work w{};
     Trav t;
-    bool active = false, need = false, exhausted = false;
-    uint32_t item = 0;
+    bool active = false, need = false, exhausted = false, gdone = false;
+    uint32_t item = 0, lb = 0, le = 0;  // the wave's batch [lb, le) of queue positions
     while (true) {
         const uint64_t idle = __ballot(!active);
         const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle));
         if (!exhausted && n_idle >= refill_min) {
-            uint32_t base = 0;
-            if (lane_id() == 0) base = atomicAdd(fetch, n_idle);
-            base = __shfl(base, 0);
-            if (base + n_idle >= count) exhausted = true;
+            const uint32_t rem = le - lb;
+            uint32_t nb = 0;
+            const bool take = rem < n_idle && !gdone;
+            if (take) {
+                if (lane_id() == 0) nb = atomicAdd(fetch, grab);
+                nb = __shfl(nb, 0);
+                if (nb >= count || count - nb <= grab) gdone = true;
+            }
+            // idle lane of rank r: position lb + r from the batch, past its end the new batch's
+            const uint32_t r = static_cast<uint32_t>(__popcll(idle & lanes_below));
+            const uint32_t my = r < rem ? lb + r : (take ? nb + (r - rem) : count);
+            if (rem >= n_idle) {
+                lb += n_idle;
+            } else if (take) {
+                lb = nb + (n_idle - rem);
+                le = nb < count ? (count - nb < grab ? count : nb + grab) : nb;
+                if (lb > le) lb = le;
+            } else {
+                lb = le;
+            }
+            if (gdone && lb >= le) exhausted = true;
             if (!active) {
-                const uint32_t my = base + static_cast<uint32_t>(__popcll(idle & lanes_below));
                 if (my < count) {
                     // binned bounces: the my-th slot in bin order (clamped: a slot index
                     // never leaves the queue)
@@ -2170,6 +2195,7 @@ int trace_rays(mt_scene* s, uint32_t n, const float* rays, mt_hit* hits, int32_t
     c.stack_cap = cap;
     c.refill_min = kRefillMin;
     c.setup_min = kSetupMin;
+    c.grab = traversal == MT_TRAVERSAL_FAST ? kGrabFast : kGrab;
     c.accel = traversal == MT_TRAVERSAL_FAST ? 1 : 0;
     TestIO io{drays, dhits, n, b.counters + kMaxCounters, docc};
     if (occluded)
@@ -2266,6 +2292,7 @@ int trace_rays_noalloc(mt_scene* s, uint32_t n, const float* rays_dev, mt_hit* h
     c.stack_cap = cap;
     c.refill_min = kRefillMin;
     c.setup_min = kSetupMin;
+    c.grab = traversal == MT_TRAVERSAL_FAST ? kGrabFast : kGrab;
     c.accel = traversal == MT_TRAVERSAL_FAST ? 1 : 0;
     TestIO io{rays_dev, hits_dev, n, b.counters + kMaxCounters, nullptr};
     launch_trace<2>(16, false, blocks, stream, s->dev, c, b, 0, io);
@@ -2405,6 +2432,8 @@ void fill_params(ChunkParams& c, const mt_scene* scene, const mt_camera_desc& ca
     c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, c.accel ? env_u32("MT_REFILL_MIN_FAST", kRefillMinFast)
                                                                           : env_u32("MT_REFILL_MIN", kRefillMin)));
     c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMin)));
+    c.grab = std::min<uint32_t>(1u << 20, std::max<uint32_t>(64u, c.accel ? env_u32("MT_GRAB_FAST", kGrabFast)
+                                                                         : env_u32("MT_GRAB", kGrab)));
     c.inv_spp = 1.0f / static_cast<float>(st.samples);
 }
 

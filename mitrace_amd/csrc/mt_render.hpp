@@ -65,6 +65,7 @@ struct ChunkParams {
     uint32_t stack_cap;
     uint32_t refill_min;  // idle lanes of a wave that trigger a refill of the traversal
     uint32_t setup_min;   // lanes needing an instance set-up that trigger a set-up step
+    uint32_t grab;        // queue positions a wave of the traversal takes per atomic (>= 64)
     // differentiable rendering: 0 plain, 1 forward with per-path store, 2 adjoint replay
     int32_t diff;
     int32_t frozen;        // differentiable forward: decisions from dps, values from dp
