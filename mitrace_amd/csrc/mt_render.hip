@@ -41,7 +41,7 @@ constexpr int kBlock = 256;
 // idle lanes of a wave that trigger a refill of the traversal (measured best on the sphere grid:
 // the reference traversal's instance culling runs per refill, so it prefers mid-size refills;
 // the fast traversal's rays are short, so refilling less often wins)
-constexpr uint32_t kRefillMin = 24;
+constexpr uint32_t kRefillMin = 16;
 constexpr uint32_t kRefillMinFast = 48;
 constexpr uint32_t kSetupMin = 12;   // lanes needing an instance set-up that trigger one
 constexpr uint32_t kGrab = 256;      // queue positions a traversal wave takes per atomic
