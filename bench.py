@@ -248,9 +248,14 @@ def batch_leg(rate_hz=16.0):
     jobs = B.workload_jobs(rate_hz=rate_hz)
     # the workload served both ways with the tuned shape: one persistent launch (the main
     # figures) and a captured graph per iteration (beside it)
+    # ("streams" is the graph mode's concurrent streams and the persistent mode's lane groups:
+    # a count tuned for the other mode is not carried over, 0 = that mode's default)
     modes = {}
     for name, mode in (("graphs", B.GRAPHS), ("persistent", B.PERSISTENT)):
-        r, _ = B.run_stream(gscenes, fns, jobs, options=dict(best, mode=mode))
+        opts = dict(best, mode=mode)
+        if best["mode"] != mode:
+            opts["streams"] = 0
+        r, _ = B.run_stream(gscenes, fns, jobs, options=opts)
         modes[name] = r
         print(f"# batch ({name}): {r['msamples_per_s']:.1f} Msamples/s, p50 "
               f"{r['completion_ms_p50']:.0f} ms, p99 {r['completion_ms_p99']:.0f} ms, "
