@@ -461,6 +461,7 @@ int mt_diff_backward(mt_diff* diff, const mt_camera_desc* camera, const mt_diff_
         gr.metal = grads->metallic_dev;
         gr.emis = grads->emission_dev;
         gr.sampling = grads->detach_sampling ? 0 : 1;
+        gr.materials = diff->scene->material_count;
         if ((gr.rough || gr.metal || gr.emis) && (!params || params->material_count != diff->scene->material_count))
             return mt::fail(MT_E_INVALID, "per-material gradients need material_count in the params");
         if (hipSetDevice(diff->scene->device) != hipSuccess) return mt::fail(MT_E_HIP, "hipSetDevice failed");

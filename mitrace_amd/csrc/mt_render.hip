@@ -863,6 +863,21 @@ __device__ __forceinline__ void wave_atomic_by_key(float* base, uint32_t key, fl
     }
 }
 
+// Per-material gradient sums of one block of the adjoint shading kernel, in LDS: roughness,
+// metallic and the three emission channels of material m at [5m .. 5m + 4].  Each hit adds
+// its five addends with LDS atomics; the block adds its nonzero sums to the gradient arrays
+// once, after its last path (one wave-level reduction and global atomic per material present
+// in a wave, five times per hit, had cost the adjoint 9 of its 37 ms on the inverse-rendering
+// sphere grid).  Scenes with more materials keep the wave-level reduction.
+constexpr uint32_t kGradLdsMats = 256;
+__device__ __forceinline__ float* grad_lds() {
+    __shared__ float s[kGradLdsMats * 5];
+    return s;
+}
+__device__ __forceinline__ bool grad_in_lds(const DiffGradPtrs& gr) {
+    return gr.materials > 0 && gr.materials <= kGradLdsMats && (gr.rough || gr.metal || gr.emis);
+}
+
 // adjoint of the base colour b' = 0.96 * texel + 0.04 at a hit: d/d(texel) = 0.96 * tap weight
 __device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const MatSample& ms, V3 g) {
     if (!gr.tex || !ms.tex) return;
@@ -1009,6 +1024,13 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
     uint32_t hits = 0, misses = 0;
     const DiffParams& dp = c.dp;
     const DiffGradPtrs& gr = c.gr;
+    if constexpr (kDiff == 2) {
+        if (grad_in_lds(gr)) {
+            float* sg = grad_lds();
+            for (uint32_t i = threadIdx.x; i < gr.materials * 5u; i += blockDim.x) sg[i] = 0.0f;
+            __syncthreads();
+        }
+    }
     for (uint32_t base = blk * blockDim.x; base < count; base += stride) {
         const uint32_t idx = base + threadIdx.x;
         const bool valid = idx < count;
@@ -1414,12 +1436,25 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             }
         }
         if (kDiff == 2) {
-            if (gr.rough) wave_atomic_by_key(gr.rough, gmat, g_r, gvalid && g_r != 0.0f);
-            if (gr.metal) wave_atomic_by_key(gr.metal, gmat, g_m, gvalid && g_m != 0.0f);
-            if (gr.emis) {
-                wave_atomic_by_key(gr.emis, gmat * 3 + 0, g_e.x, gvalid && g_e.x != 0.0f);
-                wave_atomic_by_key(gr.emis, gmat * 3 + 1, g_e.y, gvalid && g_e.y != 0.0f);
-                wave_atomic_by_key(gr.emis, gmat * 3 + 2, g_e.z, gvalid && g_e.z != 0.0f);
+            if (grad_in_lds(gr)) {
+                if (gvalid) {
+                    float* sg = grad_lds() + gmat * 5u;
+                    if (gr.rough && g_r != 0.0f) atomicAdd(sg + 0, g_r);
+                    if (gr.metal && g_m != 0.0f) atomicAdd(sg + 1, g_m);
+                    if (gr.emis) {
+                        if (g_e.x != 0.0f) atomicAdd(sg + 2, g_e.x);
+                        if (g_e.y != 0.0f) atomicAdd(sg + 3, g_e.y);
+                        if (g_e.z != 0.0f) atomicAdd(sg + 4, g_e.z);
+                    }
+                }
+            } else {
+                if (gr.rough) wave_atomic_by_key(gr.rough, gmat, g_r, gvalid && g_r != 0.0f);
+                if (gr.metal) wave_atomic_by_key(gr.metal, gmat, g_m, gvalid && g_m != 0.0f);
+                if (gr.emis) {
+                    wave_atomic_by_key(gr.emis, gmat * 3 + 0, g_e.x, gvalid && g_e.x != 0.0f);
+                    wave_atomic_by_key(gr.emis, gmat * 3 + 1, g_e.y, gvalid && g_e.y != 0.0f);
+                    wave_atomic_by_key(gr.emis, gmat * 3 + 2, g_e.z, gvalid && g_e.z != 0.0f);
+                }
             }
         } else {
             const uint32_t ss = block_push(b.counters + kCntShadow + k, shadow);
@@ -1447,6 +1482,20 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             b.t_term[(k + 1) & 1][cs] = f4(next_term, 0.0f);
             b.t_rad[(k + 1) & 1][cs] = f4(next_trad, 0.0f);
             b.prev_base[(k + 1) & 1][cs] = f4(next_base, next_q);
+        }
+    }
+    if constexpr (kDiff == 2) {
+        if (grad_in_lds(gr)) {  // the block's sums, once
+            __syncthreads();
+            const float* sg = grad_lds();
+            for (uint32_t i = threadIdx.x; i < gr.materials * 5u; i += blockDim.x) {
+                const float v = sg[i];
+                if (v == 0.0f) continue;
+                const uint32_t m = i / 5u, j = i - m * 5u;
+                if (j == 0) atomicAdd(gr.rough + m, v);
+                else if (j == 1) atomicAdd(gr.metal + m, v);
+                else atomicAdd(gr.emis + m * 3u + (j - 2u), v);
+            }
         }
     }
     if (c.count_work) {

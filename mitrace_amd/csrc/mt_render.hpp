@@ -42,6 +42,7 @@ struct DiffGradPtrs {
     float* metal;
     float* emis;
     int32_t sampling;
+    uint32_t materials;  // the scene's material count (per-material gradients are indexed by it)
 };
 
 // per-chunk constants, passed to kernels by value
