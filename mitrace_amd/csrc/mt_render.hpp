@@ -219,7 +219,12 @@ class Renderer {
 
    private:
     void release();
-    int run_diff_chunk(uint32_t chunk, int mode, float* img, hipStream_t s);
+    // differentiable chunks: enqueue one on a slot's stream; wait for a slot's chunk and take
+    // its counters; all chunks of one pass over the slots, after the caller's stream
+    int run_diff_chunk(uint32_t chunk, int mode, float* img, StreamSlot& L);
+    int finish_diff_chunk(StreamSlot& L, int mode);
+    int run_diff(int mode, float* img, hipStream_t s);
+    hipEvent_t diff_ev_ = nullptr;
     int run_cache_chunk(class Mlp& mlp, uint32_t slot, uint32_t chunk, uint32_t sample_base,
                         uint32_t train_every, uint32_t train_offset, bool accumulate);
     hipStream_t tail_stream_ = nullptr;
