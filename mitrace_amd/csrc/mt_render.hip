@@ -879,15 +879,53 @@ __device__ __forceinline__ bool grad_in_lds(const DiffGradPtrs& gr) {
     return gr.materials > 0 && gr.materials <= kGradLdsMats && (gr.rough || gr.metal || gr.emis);
 }
 
-// adjoint of the base colour b' = 0.96 * texel + 0.04 at a hit: d/d(texel) = 0.96 * tap weight
-__device__ __forceinline__ void add_texel_grad(const DiffGradPtrs& gr, const MatSample& ms, V3 g) {
-    if (!gr.tex || !ms.tex) return;
-    const float gc[3] = {g.x * 0.96f, g.y * 0.96f, g.z * 0.96f};
-    for (int t = 0; t < 4; ++t) {
-        if (ms.tw[t] == 0.0f) continue;
-        for (int ch = 0; ch < 3; ++ch)
-            if (gc[ch] != 0.0f) atomicAdd(gr.tex + ms.tap[t] * 3 + ch, gc[ch] * ms.tw[t]);
+// Sum of three addends per key over runs of consecutive lanes with the same key, then one
+// atomic per run and channel from the run's last lane.  Lanes that add nothing take a key of
+// their own, so they end runs; a key that recurs apart adds more than once, which is still
+// exact.  The samples of one pixel are neighbouring lanes and mostly hit the same texels, so
+// this removes most of the adjoint's per-lane texel atomics (which wait on each other at one
+// address).  Every lane of the wave calls it.
+__device__ __forceinline__ void wave_run_add3(float* base, uint32_t key, V3 v, bool pred) {
+    const uint32_t lane = lane_id();
+    const uint32_t kk = pred ? key : (0x80000000u | lane);  // texel indices stay below 2^31
+    const uint32_t kprev = __shfl_up(kk, 1);
+    const uint64_t starts = __ballot(lane == 0 || kprev != kk);
+    const uint32_t run = static_cast<uint32_t>(__popcll(starts & ((2ull << lane) - 1ull)));
+    float x = pred ? v.x : 0.0f, y = pred ? v.y : 0.0f, z = pred ? v.z : 0.0f;
+    for (uint32_t off = 1; off < 64; off <<= 1) {  // segmented inclusive scan over the runs
+        const uint32_t ro = __shfl_up(run, off);
+        const float xo = __shfl_up(x, off), yo = __shfl_up(y, off), zo = __shfl_up(z, off);
+        if (lane >= off && ro == run) {
+            x += xo;
+            y += yo;
+            z += zo;
+        }
     }
+    const uint32_t rnext = __shfl_down(run, 1);
+    if (pred && (lane == 63 || rnext != run)) {
+        if (x != 0.0f) atomicAdd(base + key * 3 + 0, x);
+        if (y != 0.0f) atomicAdd(base + key * 3 + 1, y);
+        if (z != 0.0f) atomicAdd(base + key * 3 + 2, z);
+    }
+}
+
+// a lane's texel-gradient addend of this bounce: the taps and weights of the vertex's base
+// colour texture and d(loss)/d(base colour), added to the texels at the loop's end (the base
+// colour is b' = 0.96 * texel + 0.04, so d/d(texel) = 0.96 * tap weight)
+struct TexelGrad {
+    uint32_t tap[4];
+    float tw[4];
+    V3 g;
+    bool on;
+};
+__device__ __forceinline__ void set_texel_grad(TexelGrad& tg, const DiffGradPtrs& gr, const MatSample& ms, V3 g) {
+    if (!gr.tex || !ms.tex) return;
+    for (int t = 0; t < 4; ++t) {
+        tg.tap[t] = ms.tap[t];
+        tg.tw[t] = ms.tw[t];
+    }
+    tg.g = g;
+    tg.on = true;
 }
 
 // ------------------------------------------------------------------ environment samples
@@ -1049,6 +1087,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
         float g_r = 0.0f, g_m = 0.0f;
         V3 g_e = mk(0.0f);
         bool gvalid = false;
+        TexelGrad tgr{};
         if (valid) {
             p = k == 0 ? idx : qin[idx];
             // the render this path belongs to (batched: its job's settings)
@@ -1113,7 +1152,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                     if (pm != 0xFFFFFFFFu) {
                         MatSample pms;
                         eval_material<2>(S, dp, pm, pv.x, pv.y, pms);
-                        add_texel_grad(gr, pms, mul(dl, mul(trad, sk)));
+                        set_texel_grad(tgr, gr, pms, mul(dl, mul(trad, sk)));
                     }
                 }
             } else if (kDiff == 0 && !kBatch && kExt && c.cache && !c.cache_after && k == c.cache_depth &&
@@ -1402,7 +1441,7 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
                             g_base = add(g_base, mk(wsum * sc.base));
                         }
                     }
-                    add_texel_grad(gr, ms, g_base);
+                    set_texel_grad(tgr, gr, ms, g_base);
                     b.adj_s[p] = f4(ssum, 0.0f);
                     b.adj_prev[p] = make_float4(tu, tv, __uint_as_float(ms.tex ? mi : 0xFFFFFFFFu), 0.0f);
                 }
@@ -1437,6 +1476,11 @@ __device__ __forceinline__ void shade_body(const DevScene& S, const ChunkParams&
             }
         }
         if (kDiff == 2) {
+            if (gr.tex) {  // every lane of the wave is here
+                const V3 gc = scale(tgr.g, 0.96f);
+                for (int t = 0; t < 4; ++t)
+                    wave_run_add3(gr.tex, tgr.tap[t], scale(gc, tgr.tw[t]), tgr.on && tgr.tw[t] != 0.0f);
+            }
             if (grad_in_lds(gr)) {
                 if (gvalid) {
                     float* sg = grad_lds() + gmat * 5u;
