@@ -941,8 +941,10 @@ constexpr uint32_t kDimNee = 0x9E000000u;  // dimensions of the training paths' 
 // surface where 2 (h.v)(h.n) <= cos(theta_o), i.e. where
 // sin(2t) sin(theta_o) cos(phi) <= -cos(theta_o) cos(2t): a fraction 1 - acos(c) / pi of the
 // azimuths, c = the right side over sin(2t) sin(theta_o).  Midpoint rule over u1, in double.
-__global__ void k_pbelow_table(float* table) {
-    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+// One wave per entry (launched with 64 threads per block, one block per entry): each lane
+// takes every 64th midpoint and the wave adds the lanes' sums.
+__global__ __launch_bounds__(64) void k_pbelow_table(float* table) {
+    const int e = static_cast<int>(blockIdx.x);
     if (e >= kPbCos * kPbRough) return;
     const int ri = e / kPbCos, ci = e % kPbCos;
     const double co = static_cast<double>(ci) / (kPbCos - 1), r = static_cast<double>(ri) / (kPbRough - 1);
@@ -950,7 +952,7 @@ __global__ void k_pbelow_table(float* table) {
     const double a = fmax(r * r, 0.0025), a2 = a * a;
     constexpr int n = 8192;
     double below = 0.0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += 64) {
         const double u = (i + 0.5) / n;
         const double t2 = a2 * u / fmax(1.0 - u, 1e-4);
         const double ct = 1.0 / sqrt(1.0 + t2), st = sqrt(fmax(0.0, 1.0 - ct * ct));
@@ -958,7 +960,66 @@ __global__ void k_pbelow_table(float* table) {
         if (den > 0.0) below += 1.0 - acos(fmin(1.0, fmax(-1.0, num / den))) / 3.14159265358979323846;
         else below += (num >= 0.0) ? 1.0 : 0.0;
     }
-    table[ri * kPbCos + ci] = static_cast<float>(below / n);
+    for (int off = 32; off > 0; off >>= 1) below += __shfl_xor(below, off);
+    if (threadIdx.x == 0) table[ri * kPbCos + ci] = static_cast<float>(below / n);
+}
+
+// The environment's sampling tables (the luminance-weighted 2-D CDF of env_sample/env_pdf):
+// block j builds row j's conditional CDF, cond[j * (W + 1) + i] for i = 0..W, from the weights
+// (lum(i, j) + blend * (lum(i + 1, j) + lum(i, j + 1) + lum(i + 1, j + 1))) * sin(theta_j), with
+// a contiguous run of texels per thread and a scan of the threads' sums, in double; rows[j] =
+// the row's total.  k_env_marginal then scans the rows into marg[0..H] and total[0].
+__device__ __forceinline__ double env_lum(const float4* env, uint32_t W, uint32_t H, uint32_t i, uint32_t j) {
+    const float4 t = env[static_cast<size_t>(min(j, H - 1)) * W + min(i, W - 1)];
+    const double m = (static_cast<double>(t.x) + t.y + t.z) / 3.0;
+    return (m > 0.0 && isfinite(m)) ? m : 0.0;
+}
+__global__ __launch_bounds__(256) void k_env_rows(const float4* env, uint32_t W, uint32_t H, double blend,
+                                                  float* cond, double* rows) {
+    __shared__ double part[256];
+    const uint32_t j = blockIdx.x;
+    const uint32_t per = (W + 255) / 256;
+    const uint32_t i0 = min(threadIdx.x * per, W), i1 = min(i0 + per, W);
+    const double sth = sin(3.14159265358979323846 * (j + 0.5) / H);
+    double sum = 0.0;
+    for (uint32_t i = i0; i < i1; ++i)
+        sum += (env_lum(env, W, H, i, j) + blend * (env_lum(env, W, H, i + 1, j) + env_lum(env, W, H, i, j + 1) +
+                                                    env_lum(env, W, H, i + 1, j + 1))) * sth;
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {  // inclusive scan of the threads' sums
+        const double v = threadIdx.x >= off ? part[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const double total = part[255];
+    float* row = cond + static_cast<size_t>(j) * (W + 1);
+    double cum = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0.0;
+    for (uint32_t i = i0; i < i1; ++i) {
+        cum += (env_lum(env, W, H, i, j) + blend * (env_lum(env, W, H, i + 1, j) + env_lum(env, W, H, i, j + 1) +
+                                                    env_lum(env, W, H, i + 1, j + 1))) * sth;
+        row[i + 1] = total > 0.0 ? static_cast<float>(cum / total) : static_cast<float>(i + 1) / W;
+    }
+    if (threadIdx.x == 0) {
+        row[0] = 0.0f;
+        rows[j] = total;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) row[W] = 1.0f;
+}
+__global__ void k_env_marginal(const double* rows, uint32_t H, float* marg, double* total) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double t = 0.0;
+    for (uint32_t j = 0; j < H; ++j) t += rows[j];
+    double acc = 0.0;
+    marg[0] = 0.0f;
+    for (uint32_t j = 0; j < H; ++j) {
+        acc += rows[j];
+        marg[j + 1] = t > 0.0 ? static_cast<float>(acc / t) : 0.0f;
+    }
+    marg[H] = 1.0f;
+    total[0] = t;
 }
 
 __device__ __forceinline__ float pbelow(const float* tab, float co, float r) {
@@ -3123,50 +3184,27 @@ int Renderer::setup_env_sampling() {
     int rc;
     float* tab = nullptr;
     if ((rc = alloc(owned_, &tab, static_cast<size_t>(kPbCos) * kPbRough))) return rc;
-    hipLaunchKernelGGL(k_pbelow_table, dim3((kPbCos * kPbRough + 255) / 256), dim3(256), 0, stream_, tab);
+    hipLaunchKernelGGL(k_pbelow_table, dim3(kPbCos * kPbRough), dim3(64), 0, stream_, tab);
     HIP_TRY(hipGetLastError());
     params_.pbelow = tab;
     params_.env_pc = 0.0f;
     const DevScene& D = scene_->dev;
     if (!D.has_env || !D.env_w || !D.env_h) return MT_OK;
     const uint32_t W = D.env_w, H = D.env_h;
-    std::vector<float4> env(static_cast<size_t>(W) * H);
-    HIP_TRY(hipMemcpy(env.data(), D.env, sizeof(float4) * env.size(), hipMemcpyDeviceToHost));
-    auto lum = [&](uint32_t i, uint32_t j) {
-        const float4 t = env[static_cast<size_t>(std::min(j, H - 1)) * W + std::min(i, W - 1)];
-        const double m = (static_cast<double>(t.x) + t.y + t.z) / 3.0;
-        return (m > 0.0 && std::isfinite(m)) ? m : 0.0;
-    };
     const double blend = D.env_filter == MT_FILTER_NEAREST ? 0.0 : 1.0 / std::min(W, H);
-    std::vector<float> marg(H + 1), cond(static_cast<size_t>(H) * (W + 1));
-    std::vector<double> rows(H), cum(W + 1);
-    double total = 0.0;
-    for (uint32_t j = 0; j < H; ++j) {
-        const double sth = std::sin(3.14159265358979323846 * (j + 0.5) / H);
-        cum[0] = 0.0;
-        for (uint32_t i = 0; i < W; ++i) {
-            const double w = (lum(i, j) + blend * (lum(i + 1, j) + lum(i, j + 1) + lum(i + 1, j + 1))) * sth;
-            cum[i + 1] = cum[i] + w;
-        }
-        float* row = cond.data() + static_cast<size_t>(j) * (W + 1);
-        for (uint32_t i = 0; i <= W; ++i)
-            row[i] = cum[W] > 0.0 ? static_cast<float>(cum[i] / cum[W]) : static_cast<float>(i) / W;
-        row[W] = 1.0f;
-        rows[j] = cum[W];
-        total += cum[W];
-    }
-    if (!(total > 0.0)) return MT_OK;  // a black environment: nothing to sample
-    double acc = 0.0;
-    marg[0] = 0.0f;
-    for (uint32_t j = 0; j < H; ++j) {
-        acc += rows[j];
-        marg[j + 1] = static_cast<float>(acc / total);
-    }
-    marg[H] = 1.0f;
     float *dm = nullptr, *dc = nullptr;
-    if ((rc = alloc(owned_, &dm, marg.size())) || (rc = alloc(owned_, &dc, cond.size()))) return rc;
-    HIP_TRY(hipMemcpy(dm, marg.data(), sizeof(float) * marg.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(dc, cond.data(), sizeof(float) * cond.size(), hipMemcpyHostToDevice));
+    double* drows = nullptr;
+    if ((rc = alloc(owned_, &dm, static_cast<size_t>(H) + 1)) ||
+        (rc = alloc(owned_, &dc, static_cast<size_t>(H) * (W + 1))) ||
+        (rc = alloc(owned_, &drows, static_cast<size_t>(H) + 1)))
+        return rc;
+    hipLaunchKernelGGL(k_env_rows, dim3(H), dim3(256), 0, stream_, D.env, W, H, blend, dc, drows);
+    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(64), 0, stream_, drows, H, dm, drows + H);
+    HIP_TRY(hipGetLastError());
+    double total = 0.0;
+    HIP_TRY(hipMemcpyAsync(&total, drows + H, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    if (!(total > 0.0)) return MT_OK;  // a black environment: nothing to sample
     params_.env_marg = dm;
     params_.env_cond = dc;
     params_.env_pc = scene_->light_count ? 0.5f : 1.0f;
