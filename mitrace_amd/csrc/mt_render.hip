@@ -46,10 +46,11 @@ constexpr uint32_t kRefillMinFast = 48;
 constexpr uint32_t kSetupMin = 12;   // lanes needing an instance set-up that trigger one
 constexpr uint32_t kGrab = 256;      // queue positions a traversal wave takes per atomic
 constexpr uint32_t kGrabFast = 1024;
-// the same two settings for the reference traversal of scenes whose records exceed the
-// last-level cache
+// the same settings, and the set-up thresholds of both traversals, for scenes whose records
+// exceed the last-level cache
 constexpr uint32_t kRefillMinLarge = 24;
 constexpr uint32_t kGrabLarge = 64;
+constexpr uint32_t kSetupMinLarge = 4, kSetupMinLargeFast = 48;
 constexpr uint32_t kDiffChunks = 2, kDiffSlots = 2;  // differentiable renders: chunks and slots
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -2859,9 +2860,18 @@ int Renderer::setup(mt_scene* scene, const mt_camera_desc& cam, const mt_setting
         // few and close together in queue order (measured on the large terrain scene: refill at
         // 24 idle lanes and batches of 64 positions, 17.0 Msamples/s, against 16.4 with the
         // settings of scenes that fit, which the sphere grid prefers: 700 against 665; the fast
-        // traversal's short rays keep their settings there: 334 against 324 with these)
+        // traversal's short rays keep their settings there: 334 against 324 with these), and
+        // its set-up steps (the instance tree's walk and instance entries, 4.8 per ray) run as
+        // soon as 4 lanes want one (set-up threshold 2, 4, 8, 12, 20: 19.0, 18.9-19.1, 17.8,
+        // 16.8, 15.0 Msamples/s)
         c.grab = std::max<uint32_t>(64u, std::min<uint32_t>(1u << 20, env_u32("MT_GRAB", kGrabLarge)));
         c.refill_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_REFILL_MIN", kRefillMinLarge)));
+        c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMinLarge)));
+    } else if (beyond_llc) {
+        // and the fast traversal there, whose set-up steps are cheap next to its short node
+        // walks, gathers more lanes for each (threshold 12, 20, 32, 48, 64: 345, 367, 382, 386,
+        // 364 Msamples/s)
+        c.setup_min = std::min<uint32_t>(64u, std::max<uint32_t>(1u, env_u32("MT_SETUP_MIN", kSetupMinLargeFast)));
     }
     c.stack_cap = cap;
     c.inv_spp = 1.0f / static_cast<float>(st.samples);
